@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Benchmark of the Bloch scan hot path on one MI355X.
+
+Workload (one "step" = one full scan of it, inputs resident in HBM):
+  3-D balanced SSFP, 128 x 128 x 64 voxels x 4 isochromats (uniform +-pi
+  intravoxel spread), T1/T2/PD/dB0 seeded as the single-slice case, 8
+  Gaussian receive coils, 45 deg hard rf with alternating phase, TR 5 ms
+  on a 10 us grid (500 steps), 128 x 64 phase encodes x 128 readout
+  samples = 8192 TRs = 4.096e6 steps = 1.72e13 isochromat-steps.
+
+Prints one JSON line with isochromat-steps/s, the forward kernel's
+roofline under the algorithmic-bytes convention (per isochromat-step:
+24 B state read+write, 4 B dWRnd, 36/S B per-voxel maps, 8*Rx/S B of coil
+maps on ADC steps) and the fp32 CPU oracle timed on the first TRs of the
+same phantom on this host's cores.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "isochromat-steps/s + % HBM roofline; batched speedup; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+FP32_FMA_PEAK_TFLOPS = 78.6  # non-packed fp32 FMA (packed fp32 is disabled in this build)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(spins: int, rx: int, adc_fraction: float) -> float:
+    """Bytes per isochromat-step a step-at-a-time engine moves (see module doc)."""
+    return 24 + 4 + 36.0 / spins + adc_fraction * 8.0 * rx / spins
+
+
+def cpu_baseline(n_tr: int, threads: int) -> dict:
+    import oracle
+    from mrilab_amd import workloads
+    sc = workloads.bssfp3d(n_tr=n_tr)
+    iso_steps = sc.n_iso * sc.n_steps()
+    oracle.set_threads(threads, "f32")
+    t0 = time.perf_counter()
+    oracle.scan(sc, "f32")
+    dt = time.perf_counter() - t0
+    oracle.set_threads(0, "f32")
+    return {"value": iso_steps / dt, "unit": "isochromat-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {n_tr} TRs of the same phantom and sequence ({iso_steps:.3g} isochromat-steps, "
+                      f"{dt:.1f} s), fp32 restatement of DoScanAtCPU, OpenMP over slices"}
+
+
+def traffic_from_profile() -> float | None:
+    """HBM bytes per forward launch measured by rocprofv3 (profiles/)."""
+    path = os.path.join(ROOT, "profiles", "bloch_fwd_bssfp3d_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return float(json.load(f)["bytes_per_launch"])
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--tr", type=int, default=None, help="truncate the scan to its first TRs (profiling)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-tr", type=int, default=2)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this engine runs the scan on one MI355X and has no "
+              "multi-GPU path; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from mrilab_amd import engine, workloads
+
+    log("building the workload")
+    t0 = time.perf_counter()
+    sc = workloads.bssfp3d(n_tr=args.tr)
+    log(f"{sc.name}: {sc.n_iso} isochromats, {sc.n_steps()} steps ({time.perf_counter() - t0:.1f} s)")
+    opts = engine.make_options(timing=1)
+    plan = engine.Plan(*sc.structs, options=opts)
+    info = plan.info()
+    log(f"plan: {info['n_steps']} steps, {info['n_adc']} ADC samples, {info['n_launches']} launches, "
+        f"{info['n_cached_steps']} cached free steps, {info['n_rf_steps']} rf steps")
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        plan.run()
+        plan.sync()
+        log(f"warmup {w + 1}/{args.warmup}: {time.perf_counter() - t:.2f} s")
+    plan.sync()
+    fwd_ms = red_ms = 0.0
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        plan.run()
+        plan.sync()
+        inf = plan.info()
+        fwd_ms += inf["fwd_kernel_ms"]
+        red_ms += inf["reduce_kernel_ms"]
+        log(f"step {k + 1}/{args.steps}: kernels {inf['fwd_kernel_ms']:.1f} + {inf['reduce_kernel_ms']:.1f} ms")
+    plan.sync()
+    elapsed = time.perf_counter() - t_start
+    iso_steps = info["n_iso"] * info["n_steps"]
+    value = iso_steps * args.steps / elapsed
+    spins, rx = int(sc.VObj["SpinNum"]), int(sc.VCoi["RxCoilNum"])
+    ab = algorithmic_bytes(spins, rx, info["n_adc"] / info["n_steps"])
+    n_launch = info["n_launches"]
+    avg_launch_s = fwd_ms / 1e3 / (args.steps * n_launch)
+    ab_per_launch = iso_steps / n_launch * ab
+    achieved = ab_per_launch / avg_launch_s / 1e9
+    traffic = traffic_from_profile()
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "isochromat-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "3-D bSSFP 128x128x64 voxels x 4 isochromats, 8 Rx coils, 45 deg hard rf, "
+                               "TR 5 ms = 500 x 10 us steps, 128x64 phase encodes x 128 readout",
+                   "isochromats": info["n_iso"], "steps": info["n_steps"], "adc_samples": info["n_adc"],
+                   "launches_per_step": n_launch, "precision_mode": "reference"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "convention": f"algorithmic bytes {ab:.2f} B per isochromat-step",
+                     "kernel": "bloch_fwd_kernel<4,8>", "avg_launch_ms": avg_launch_s * 1e3},
+        "kernel_ms_per_step": {"bloch_fwd": fwd_ms / args.steps, "signal_reduce": red_ms / args.steps},
+    }
+    plan.close()
+    if not args.no_cpu_baseline:
+        threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        log(f"cpu baseline: fp32 oracle on the first {args.cpu_sample_tr} TRs, {threads} threads")
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_tr, threads)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

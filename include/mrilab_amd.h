@@ -1,0 +1,206 @@
+/*
+ * mrilab_amd.h -- C ABI of the MI355X Bloch simulation engine.
+ *
+ * The engine replaces MRiLab's scan MEX functions. Those take no MEX
+ * arguments and exchange everything through seven MATLAB global structs
+ * (VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig). This ABI passes the same
+ * fields, under the same names, as plain C structs of pointers and sizes.
+ * A MEX gateway fills them with mxGetData() pointers and calls one
+ * entry point. Arrays keep MATLAB's column-major layout:
+ *   per-voxel maps   [R, C, Z]              flat index r + R*c + R*C*z
+ *   per-spin arrays  [R, C, Z, SpinNum, TypeNum]
+ *   per-type maps    [R, C, Z, TypeNum]
+ *   coil maps        [R, C, Z, CoilNum]
+ *
+ * Ownership: the library never allocates caller-visible memory. Outputs
+ * are written in place into caller buffers, as the reference mutates the
+ * MATLAB-owned arrays after mxUnshareArray (DoScanAtCPU.cpp:118-124).
+ * Device memory lives inside the library (per call, or in an mrs_plan).
+ *
+ * Errors: every entry point returns an mrs_status; mrs_last_error()
+ * gives a message for the calling thread. Nothing throws across the ABI.
+ * Threading: calls on different plans may run on different threads; one
+ * plan is not re-entrant. Callbacks run on the calling thread.
+ */
+#ifndef MRILAB_AMD_H
+#define MRILAB_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRS_ABI_VERSION 1
+
+typedef enum mrs_status {
+    MRS_OK = 0,
+    MRS_ERR_ARG = 1,          /* NULL pointer, bad size or inconsistent dims */
+    MRS_ERR_DEVICE = 2,       /* HIP runtime error or invalid GPUIndex */
+    MRS_ERR_CALLBACK = 3,     /* an Ext callback returned nonzero */
+    MRS_ERR_UNSUPPORTED = 4,  /* outside the engine's limits (coil count ...) */
+    MRS_ERR_NOMEM = 5         /* device or host allocation failed */
+} mrs_status;
+
+/* ---- VObj: phantom (DoScanAtCPU.cpp:127-135, DoGMScanAtGPU.cu K/TypeFlag) */
+typedef struct mrs_vobj {
+    double Gyro;              /* rad/s/T */
+    int32_t SpinNum;          /* isochromats per voxel */
+    int32_t TypeNum;          /* pools / species */
+    int32_t dims[3];          /* R, C, Z of VObj.Mz (Z = 1 for a 2-D object) */
+    float *Mx, *My, *Mz;      /* [R,C,Z,SpinNum,TypeNum], in: start, out: end */
+    const float *Rho;         /* [R,C,Z,TypeNum] proton density */
+    const float *T1, *T2;     /* [R,C,Z,TypeNum] seconds */
+    const float *K;           /* multi-pool only, layout per entry point */
+    const double *TypeFlag;   /* multi-pool only: [TypeNum], 0 free, 1 bound */
+} mrs_vobj;
+
+/* ---- VMag: field maps (DoScanAtCPU.cpp:137-141) */
+typedef struct mrs_vmag {
+    const float *dB0;         /* [R,C,Z] Tesla */
+    const float *dWRnd;       /* [R,C,Z,SpinNum,TypeNum] rad/s */
+    const float *Gzgrid, *Gygrid, *Gxgrid;  /* [R,C,Z] metres */
+} mrs_vmag;
+
+/* ---- VCoi: coils (DoScanAtCPU.cpp:143-150) */
+typedef struct mrs_vcoi {
+    const float *TxCoilmg, *TxCoilpe;  /* [R,C,Z,TxCoilNum] B1+ magnitude, phase */
+    const float *RxCoilx, *RxCoily;    /* [R,C,Z,RxCoilNum] B1- x, y */
+    int32_t TxCoilNum, RxCoilNum;
+    double TxCoilDefault, RxCoilDefault; /* nonzero: uniform coil */
+} mrs_vcoi;
+
+/* ---- VCtl: controls (DoScanAtCPU.cpp:152-155, Plugin_rfRef.m:7-11) */
+typedef struct mrs_vctl {
+    const double *CS;         /* [TypeNum] chemical shift, Hz */
+    int32_t TRNum;
+    int32_t RunMode;          /* 0 imaging, 1 per-step magnetization capture */
+    int32_t MaxThreadNum;     /* accepted for compatibility; unused on GPU */
+    int32_t GPUIndex;
+    int32_t MultiTransmit;    /* 0 = 'off', 1 = 'on' */
+    int32_t MasterTxCoil;     /* 1-based, used by Ext 7 when MultiTransmit on */
+} mrs_vctl;
+
+/* ---- VSeq: sequence lines (DoScanAtCPU.cpp:157-175) */
+typedef struct mrs_vseq {
+    const double *utsLine, *tsLine;
+    const double *rfAmpLine, *rfPhaseLine, *rfFreqLine, *rfCoilLine;
+    const double *GzAmpLine, *GyAmpLine, *GxAmpLine;
+    const double *ADCLine, *ExtLine;
+    const double *flagsLine;  /* [6, n_ts] column-major: rf,Gz,Gy,Gx,ADC,Ext */
+    int64_t n_ts, n_uts, n_rf, n_Gz, n_Gy, n_Gx, n_ADC, n_Ext;
+} mrs_vseq;
+
+/* ---- VVar: decoder state, read at start and written back at the end
+ * (DoScanAtCPU.cpp:177-199, DoPreScan.m:517-547, Plugin_LockK.m) */
+typedef struct mrs_vvar {
+    double t, dt;
+    double *rfAmp, *rfPhase, *rfFreq;   /* [TxCoilNum] held rf values */
+    double rfCoil, rfRef;
+    double GzAmp, GyAmp, GxAmp, ADC, Ext;
+    double Kz, Ky, Kx;                  /* running k-space position, 1/m */
+    double KzKeeper, KyKeeper, KxKeeper;
+    double gpuFetch;
+    int32_t utsi, rfi, Gzi, Gyi, Gxi, ADCi, Exti, TRCount;
+} mrs_vvar;
+
+/* ---- VSig: outputs (DoScanAtCPU.cpp:201-210) */
+typedef struct mrs_vsig {
+    double *Sx, *Sy;          /* [SignalNum*TypeNum*RxCoilNum], index
+                                 t*Rx*N + c*N + n; accumulated with += */
+    double *Kz, *Ky, *Kx;     /* [SignalNum], accumulated with += */
+    int32_t SignalNum;
+    float *Mx, *My, *Mz;      /* RunMode 1 only: [R,C,Z,S,Ty, n_uts] */
+    double *Muts;             /* RunMode 1 only: [n_uts] */
+} mrs_vsig;
+
+/* ---- Ext callbacks (DoScanAtCPU.cpp:356-409, DoScanAtGPU.cu:565-659)
+ * Ext 1-4 (k reset/reverse/lock/release), 6 (ideal spoiler) and 7 (rfRef)
+ * are executed by the engine itself. Any other nonzero Ext code (5 timer,
+ * 8 motion, 9 real-time recon, negative user codes) calls ext() if it is
+ * set, and is a no-op otherwise. Before ext(), gpu_fetch() decides whether
+ * the magnetization is copied to VObj.Mx/My/Mz on the host first and back
+ * to the device afterwards (DoGPUFetch.m). Both return 0 on success. */
+typedef struct mrs_callbacks {
+    int (*gpu_fetch)(void *user, double ext, int *fetch);
+    int (*ext)(void *user, double ext, mrs_vvar *vvar, mrs_vobj *vobj);
+    void *user;
+} mrs_callbacks;
+
+/* ---- Engine options; NULL means defaults */
+typedef struct mrs_options {
+    int32_t block_size;       /* threads per block, 0 = auto */
+    int32_t spins_per_thread; /* 0 = auto */
+    int32_t max_adc_per_launch; /* ADC samples per kernel launch, 0 = auto */
+    int32_t multipool_scheme; /* 0 = operator split as DoGMScanAtGPU,
+                                 1 = exact FP64 matrix exponential */
+    int32_t precision_mode;   /* 0 = reference: relaxation factors rounded
+                                 as BlochKernel.h:348-368 rounds them;
+                                 1 = accurate: expm1-based relaxation */
+    int32_t timing;           /* 1 = time every kernel launch with HIP events
+                                 on the plan stream (mrs_plan_info) */
+    int32_t reserved[2];
+} mrs_options;
+
+/* Thread-local message of the last failed call on this thread. */
+const char *mrs_last_error(void);
+int32_t mrs_abi_version(void);
+/* Number of visible HIP devices, or a negative value on error. */
+int32_t mrs_device_count(void);
+
+/* Single-pool scan with DoScanAtCPU semantics, run on the GPU.
+ * Replaces: mexFunction, Lib/src/engine/CPUEngine/DoScanAtCPU.cpp:70 */
+mrs_status mrs_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag,
+                           const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                           const mrs_vseq *vseq, mrs_vvar *vvar,
+                           mrs_vsig *vsig, const mrs_callbacks *cb,
+                           const mrs_options *opt);
+
+/* Single-pool scan, GPU entry point.
+ * Replaces: mexFunction, Lib/src/engine/GPUEngine/DoScanAtGPU.cu:76 */
+mrs_status mrs_DoScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
+                           const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                           const mrs_vseq *vseq, mrs_vvar *vvar,
+                           mrs_vsig *vsig, const mrs_callbacks *cb,
+                           const mrs_options *opt);
+
+/* ---- Plan API: decode once, keep maps and step records resident in HBM,
+ * run many times. mrs_DoScanAtGPU is plan_create + plan_run + plan_fetch +
+ * plan_destroy. No reference counterpart (the reference re-uploads per
+ * call); used by benchmarks and by callers that rerun one scan. */
+typedef struct mrs_plan mrs_plan;
+
+mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag,
+                           const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                           const mrs_vseq *vseq, const mrs_vvar *vvar,
+                           const mrs_vsig *vsig, const mrs_options *opt,
+                           mrs_plan **out);
+/* Resets device magnetization to the plan's initial state, zeroes the
+ * device signal, and runs every step on the plan's stream; asynchronous. */
+mrs_status mrs_plan_run(mrs_plan *plan);
+mrs_status mrs_plan_sync(mrs_plan *plan);
+/* Copies final magnetization, signal and k-space into caller buffers
+ * (VObj.Mx/My/Mz, VSig.Sx/Sy/Kx/Ky/Kz with +=) and the final decoder
+ * state into vvar. Synchronizes first. */
+mrs_status mrs_plan_fetch(mrs_plan *plan, mrs_vobj *vobj, mrs_vvar *vvar,
+                          mrs_vsig *vsig);
+/* Statistics of the decoded plan. */
+typedef struct mrs_plan_info {
+    int64_t n_steps;          /* decoded steps with dt > 0 */
+    int64_t n_adc;            /* ADC samples */
+    int64_t n_iso;            /* isochromats = R*C*Z*SpinNum*TypeNum */
+    int64_t n_launches;       /* kernel launches per run */
+    int64_t n_rf_steps;       /* steps with rf on */
+    int64_t n_cached_steps;   /* free steps reusing the previous rotation */
+    int32_t block_size, spins_per_thread, grid_blocks;
+    float   last_run_ms;      /* device time of the last run, 0 if unknown */
+    double  fwd_kernel_ms;    /* timing = 1: summed Bloch-kernel time of the last run */
+    double  reduce_kernel_ms; /* timing = 1: summed signal-reduction time */
+} mrs_plan_info;
+mrs_status mrs_plan_get_info(const mrs_plan *plan, mrs_plan_info *info);
+void mrs_plan_destroy(mrs_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MRILAB_AMD_H */

@@ -1,0 +1,252 @@
+"""ctypes mirror of include/mrilab_amd.h.
+
+The structs carry the field names of MRiLab's global scan structs
+(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig). ``pack_scan`` turns Python
+mappings of numpy arrays (one per MATLAB struct) into these structs after
+checking every dtype against the contract of Src/Main/DoDataTypeConv.m:19-138.
+Where the reference MEX would silently reinterpret a wrongly typed array,
+``pack_scan`` raises TypeError naming the field.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Any, Mapping
+
+import numpy as np
+
+F32P = C.POINTER(C.c_float)
+F64P = C.POINTER(C.c_double)
+
+
+class VObjC(C.Structure):
+    _fields_ = [("Gyro", C.c_double), ("SpinNum", C.c_int32), ("TypeNum", C.c_int32),
+                ("dims", C.c_int32 * 3),
+                ("Mx", F32P), ("My", F32P), ("Mz", F32P),
+                ("Rho", F32P), ("T1", F32P), ("T2", F32P),
+                ("K", F32P), ("TypeFlag", F64P)]
+
+
+class VMagC(C.Structure):
+    _fields_ = [("dB0", F32P), ("dWRnd", F32P), ("Gzgrid", F32P), ("Gygrid", F32P),
+                ("Gxgrid", F32P)]
+
+
+class VCoiC(C.Structure):
+    _fields_ = [("TxCoilmg", F32P), ("TxCoilpe", F32P), ("RxCoilx", F32P), ("RxCoily", F32P),
+                ("TxCoilNum", C.c_int32), ("RxCoilNum", C.c_int32),
+                ("TxCoilDefault", C.c_double), ("RxCoilDefault", C.c_double)]
+
+
+class VCtlC(C.Structure):
+    _fields_ = [("CS", F64P), ("TRNum", C.c_int32), ("RunMode", C.c_int32),
+                ("MaxThreadNum", C.c_int32), ("GPUIndex", C.c_int32),
+                ("MultiTransmit", C.c_int32), ("MasterTxCoil", C.c_int32)]
+
+
+class VSeqC(C.Structure):
+    _fields_ = [(n, F64P) for n in ("utsLine", "tsLine", "rfAmpLine", "rfPhaseLine",
+                                    "rfFreqLine", "rfCoilLine", "GzAmpLine", "GyAmpLine",
+                                    "GxAmpLine", "ADCLine", "ExtLine", "flagsLine")] + \
+              [(n, C.c_int64) for n in ("n_ts", "n_uts", "n_rf", "n_Gz", "n_Gy", "n_Gx",
+                                        "n_ADC", "n_Ext")]
+
+
+class VVarC(C.Structure):
+    _fields_ = [("t", C.c_double), ("dt", C.c_double),
+                ("rfAmp", F64P), ("rfPhase", F64P), ("rfFreq", F64P),
+                ("rfCoil", C.c_double), ("rfRef", C.c_double),
+                ("GzAmp", C.c_double), ("GyAmp", C.c_double), ("GxAmp", C.c_double),
+                ("ADC", C.c_double), ("Ext", C.c_double),
+                ("Kz", C.c_double), ("Ky", C.c_double), ("Kx", C.c_double),
+                ("KzKeeper", C.c_double), ("KyKeeper", C.c_double), ("KxKeeper", C.c_double),
+                ("gpuFetch", C.c_double)] + \
+              [(n, C.c_int32) for n in ("utsi", "rfi", "Gzi", "Gyi", "Gxi", "ADCi", "Exti",
+                                        "TRCount")]
+
+
+class VSigC(C.Structure):
+    _fields_ = [("Sx", F64P), ("Sy", F64P), ("Kz", F64P), ("Ky", F64P), ("Kx", F64P),
+                ("SignalNum", C.c_int32),
+                ("Mx", F32P), ("My", F32P), ("Mz", F32P), ("Muts", F64P)]
+
+
+GPU_FETCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_int))
+EXT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(VVarC), C.POINTER(VObjC))
+
+
+class CallbacksC(C.Structure):
+    _fields_ = [("gpu_fetch", GPU_FETCH_FN), ("ext", EXT_FN), ("user", C.c_void_p)]
+
+
+class OptionsC(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("spins_per_thread", C.c_int32),
+                ("max_adc_per_launch", C.c_int32), ("multipool_scheme", C.c_int32),
+                ("precision_mode", C.c_int32), ("timing", C.c_int32), ("reserved", C.c_int32 * 2)]
+
+
+class PlanInfoC(C.Structure):
+    _fields_ = [("n_steps", C.c_int64), ("n_adc", C.c_int64), ("n_iso", C.c_int64),
+                ("n_launches", C.c_int64), ("n_rf_steps", C.c_int64),
+                ("n_cached_steps", C.c_int64),
+                ("block_size", C.c_int32), ("spins_per_thread", C.c_int32),
+                ("grid_blocks", C.c_int32), ("last_run_ms", C.c_float),
+                ("fwd_kernel_ms", C.c_double), ("reduce_kernel_ms", C.c_double)]
+
+
+# ---------------------------------------------------------------------------
+# dtype contract, Src/Main/DoDataTypeConv.m
+# ---------------------------------------------------------------------------
+_VVAR_F64 = ("t", "dt", "rfCoil", "rfRef", "GzAmp", "GyAmp", "GxAmp", "ADC", "Ext",
+             "Kz", "Ky", "Kx", "gpuFetch")
+_VVAR_I32 = ("utsi", "rfi", "Gzi", "Gyi", "Gxi", "ADCi", "Exti", "TRCount")
+
+
+def _req(struct: Mapping[str, Any], sname: str, field: str, dtype, size: int | None = None,
+         writable: bool = False) -> np.ndarray:
+    if field not in struct:
+        raise KeyError(f"{sname}.{field} is missing")
+    a = struct[field]
+    if not isinstance(a, np.ndarray):
+        raise TypeError(f"{sname}.{field} must be a numpy array of {np.dtype(dtype).name}")
+    if a.dtype != np.dtype(dtype):
+        raise TypeError(f"{sname}.{field} has dtype {a.dtype}, expected {np.dtype(dtype).name}")
+    if not (a.flags.f_contiguous or a.flags.c_contiguous and a.ndim <= 1):
+        raise TypeError(f"{sname}.{field} must be contiguous in column-major (MATLAB) order")
+    if size is not None and a.size != size:
+        raise ValueError(f"{sname}.{field} has {a.size} elements, expected {size}")
+    if writable and not a.flags.writeable:
+        raise ValueError(f"{sname}.{field} must be writable")
+    return a
+
+
+def _scalar(struct, sname, field, dtype):
+    if field not in struct:
+        raise KeyError(f"{sname}.{field} is missing")
+    v = struct[field]
+    a = np.asarray(v)
+    if isinstance(v, np.ndarray) or isinstance(v, np.generic):
+        if a.dtype != np.dtype(dtype):
+            raise TypeError(f"{sname}.{field} has dtype {a.dtype}, expected {np.dtype(dtype).name}")
+    if a.size != 1:
+        raise ValueError(f"{sname}.{field} must be a scalar")
+    return a.reshape(-1)[0].item()
+
+
+def _p(a: np.ndarray | None, typ):
+    if a is None:
+        return C.cast(None, typ)
+    return a.ctypes.data_as(typ)
+
+
+class Packed:
+    """The ABI structs plus references that keep the numpy buffers alive."""
+
+    def __init__(self):
+        self.keep: list[Any] = []
+
+
+def pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool: bool = False) -> Packed:
+    P = Packed()
+    Mz = _req(VObj, "VObj", "Mz", np.float32, writable=True)
+    shp = Mz.shape + (1,) * (5 - Mz.ndim)
+    R, Cc, Z = int(shp[0]), int(shp[1]), int(shp[2])
+    S = int(_scalar(VObj, "VObj", "SpinNum", np.int32))
+    Ty = int(_scalar(VObj, "VObj", "TypeNum", np.int32))
+    nvox = R * Cc * Z
+    if Mz.size != nvox * S * Ty:
+        raise ValueError(f"VObj.Mz has {Mz.size} elements, expected R*C*Z*SpinNum*TypeNum = {nvox * S * Ty}")
+    Mx = _req(VObj, "VObj", "Mx", np.float32, Mz.size, writable=True)
+    My = _req(VObj, "VObj", "My", np.float32, Mz.size, writable=True)
+    Rho = _req(VObj, "VObj", "Rho", np.float32, nvox * Ty)
+    T1 = _req(VObj, "VObj", "T1", np.float32, nvox * Ty)
+    T2 = _req(VObj, "VObj", "T2", np.float32, nvox * Ty)
+    K = TF = None
+    if multipool:
+        K = _req(VObj, "VObj", "K", np.float32)
+        TF = _req(VObj, "VObj", "TypeFlag", np.float64, Ty)
+    vobj = VObjC(Gyro=float(_scalar(VObj, "VObj", "Gyro", np.float64)), SpinNum=S, TypeNum=Ty)
+    vobj.dims[0], vobj.dims[1], vobj.dims[2] = R, Cc, Z
+    vobj.Mx, vobj.My, vobj.Mz = _p(Mx, F32P), _p(My, F32P), _p(Mz, F32P)
+    vobj.Rho, vobj.T1, vobj.T2 = _p(Rho, F32P), _p(T1, F32P), _p(T2, F32P)
+    vobj.K, vobj.TypeFlag = _p(K, F32P), _p(TF, F64P)
+
+    vmag = VMagC(dB0=_p(_req(VMag, "VMag", "dB0", np.float32, nvox), F32P),
+                 dWRnd=_p(_req(VMag, "VMag", "dWRnd", np.float32, nvox * S * Ty), F32P),
+                 Gzgrid=_p(_req(VMag, "VMag", "Gzgrid", np.float32, nvox), F32P),
+                 Gygrid=_p(_req(VMag, "VMag", "Gygrid", np.float32, nvox), F32P),
+                 Gxgrid=_p(_req(VMag, "VMag", "Gxgrid", np.float32, nvox), F32P))
+
+    Tx = int(_scalar(VCoi, "VCoi", "TxCoilNum", np.int32))
+    Rx = int(_scalar(VCoi, "VCoi", "RxCoilNum", np.int32))
+    vcoi = VCoiC(TxCoilmg=_p(_req(VCoi, "VCoi", "TxCoilmg", np.float32, nvox * Tx), F32P),
+                 TxCoilpe=_p(_req(VCoi, "VCoi", "TxCoilpe", np.float32, nvox * Tx), F32P),
+                 RxCoilx=_p(_req(VCoi, "VCoi", "RxCoilx", np.float32, nvox * Rx), F32P),
+                 RxCoily=_p(_req(VCoi, "VCoi", "RxCoily", np.float32, nvox * Rx), F32P),
+                 TxCoilNum=Tx, RxCoilNum=Rx,
+                 TxCoilDefault=float(_scalar(VCoi, "VCoi", "TxCoilDefault", np.float64)),
+                 RxCoilDefault=float(_scalar(VCoi, "VCoi", "RxCoilDefault", np.float64)))
+
+    mt = VCtl.get("MultiTransmit", "off")
+    vctl = VCtlC(CS=_p(_req(VCtl, "VCtl", "CS", np.float64, Ty), F64P),
+                 TRNum=int(_scalar(VCtl, "VCtl", "TRNum", np.int32)),
+                 RunMode=int(_scalar(VCtl, "VCtl", "RunMode", np.int32)),
+                 MaxThreadNum=int(_scalar(VCtl, "VCtl", "MaxThreadNum", np.int32)),
+                 GPUIndex=int(VCtl.get("GPUIndex", np.int32(0))),
+                 MultiTransmit=1 if mt == "on" else 0,
+                 MasterTxCoil=int(VCtl.get("MasterTxCoil", 1)))
+
+    seq_fields = {}
+    for f in ("utsLine", "tsLine", "rfAmpLine", "rfPhaseLine", "rfFreqLine", "rfCoilLine",
+              "GzAmpLine", "GyAmpLine", "GxAmpLine", "ADCLine", "ExtLine", "flagsLine"):
+        seq_fields[f] = _req(VSeq, "VSeq", f, np.float64)
+    n_ts = seq_fields["tsLine"].size
+    if seq_fields["flagsLine"].size != 6 * n_ts:
+        raise ValueError("VSeq.flagsLine must be 6 x numel(tsLine)")
+    if seq_fields["rfCoilLine"].size != seq_fields["rfAmpLine"].size:
+        raise ValueError("VSeq.rfCoilLine and rfAmpLine differ in length")
+    vseq = VSeqC(**{f: _p(a, F64P) for f, a in seq_fields.items()},
+                 n_ts=n_ts, n_uts=seq_fields["utsLine"].size, n_rf=seq_fields["rfAmpLine"].size,
+                 n_Gz=seq_fields["GzAmpLine"].size, n_Gy=seq_fields["GyAmpLine"].size,
+                 n_Gx=seq_fields["GxAmpLine"].size, n_ADC=seq_fields["ADCLine"].size,
+                 n_Ext=seq_fields["ExtLine"].size)
+
+    rfA = _req(VVar, "VVar", "rfAmp", np.float64, Tx, writable=True)
+    rfP = _req(VVar, "VVar", "rfPhase", np.float64, Tx, writable=True)
+    rfF = _req(VVar, "VVar", "rfFreq", np.float64, Tx, writable=True)
+    vvar = VVarC(rfAmp=_p(rfA, F64P), rfPhase=_p(rfP, F64P), rfFreq=_p(rfF, F64P))
+    for f in _VVAR_F64:
+        if f == "gpuFetch":
+            vvar.gpuFetch = float(VVar.get(f, 0.0))
+        else:
+            setattr(vvar, f, float(_scalar(VVar, "VVar", f, np.float64)))
+    for f in ("KzKeeper", "KyKeeper", "KxKeeper"):
+        setattr(vvar, f, float(VVar.get(f, 0.0)))
+    for f in _VVAR_I32:
+        setattr(vvar, f, int(_scalar(VVar, "VVar", f, np.int32)))
+
+    N = int(_scalar(VSig, "VSig", "SignalNum", np.int32))
+    Sx = _req(VSig, "VSig", "Sx", np.float64, N * Ty * Rx, writable=True)
+    Sy = _req(VSig, "VSig", "Sy", np.float64, N * Ty * Rx, writable=True)
+    vsig = VSigC(Sx=_p(Sx, F64P), Sy=_p(Sy, F64P),
+                 Kz=_p(_req(VSig, "VSig", "Kz", np.float64, N, writable=True), F64P),
+                 Ky=_p(_req(VSig, "VSig", "Ky", np.float64, N, writable=True), F64P),
+                 Kx=_p(_req(VSig, "VSig", "Kx", np.float64, N, writable=True), F64P),
+                 SignalNum=N)
+    if vctl.RunMode == 1:
+        nuts = seq_fields["utsLine"].size
+        vsig.Mx = _p(_req(VSig, "VSig", "Mx", np.float32, Mz.size * nuts, writable=True), F32P)
+        vsig.My = _p(_req(VSig, "VSig", "My", np.float32, Mz.size * nuts, writable=True), F32P)
+        vsig.Mz = _p(_req(VSig, "VSig", "Mz", np.float32, Mz.size * nuts, writable=True), F32P)
+        vsig.Muts = _p(_req(VSig, "VSig", "Muts", np.float64, nuts, writable=True), F64P)
+    P.vobj, P.vmag, P.vcoi, P.vctl, P.vseq, P.vvar, P.vsig = vobj, vmag, vcoi, vctl, vseq, vvar, vsig
+    P.keep += [VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, seq_fields, K, TF]
+    return P
+
+
+def unpack_vvar(P: Packed, VVar) -> None:
+    """Write the scalar decoder state back into the VVar mapping, as the
+    reference leaves it in the MATLAB global after the MEX returns."""
+    for f in _VVAR_F64 + ("KzKeeper", "KyKeeper", "KxKeeper"):
+        VVar[f] = np.float64(getattr(P.vvar, f))
+    for f in _VVAR_I32:
+        VVar[f] = np.int32(getattr(P.vvar, f))

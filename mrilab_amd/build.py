@@ -1,0 +1,56 @@
+"""Builds the HIP engine library in-tree with hipcc for gfx950.
+
+``python -m mrilab_amd.build`` (or ``build()``) compiles mrilab_amd/csrc/*
+into mrilab_amd/libmrilab_amd.so. Packed fp32 instructions are disabled
+(``-packed-fp32-ops``) for every HIP translation unit.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmrilab_amd.so")
+ARCH = os.environ.get("MRS_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["engine.hip"]
+CXX_SOURCES = ["decode.cpp"]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found; the engine needs ROCm's hipcc to build")
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    objs = []
+    common = ["-O3", "-fPIC", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
+              "-Wall", "-Wno-unused-result"]
+    for src in HIP_SOURCES + CXX_SOURCES:
+        obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+               "-packed-fp32-ops", *common, "-c", os.path.join(CSRC, src), "-o", obj]
+        if src in CXX_SOURCES:
+            cmd = [hipcc(), *common, "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

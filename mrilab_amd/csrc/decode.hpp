@@ -1,0 +1,84 @@
+// Host-side event decoder: turns MRiLab's VSeq lines into a stream of
+// fixed-size step records that the GPU kernels read wave-uniformly.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "mrilab_amd.h"
+
+namespace mrs {
+
+// Record flag bits (word 5 of a record).
+enum : uint32_t {
+    REC_ADC = 1u << 0,      // take an ADC sample before the step
+    REC_RF = 1u << 1,       // rf is on during the step
+    REC_SPOIL = 1u << 2,    // zero Mx, My before anything else (Ext 6)
+    REC_SAME = 1u << 3,     // free step with the same dt and fields as the previous step
+    REC_CAPTURE = 1u << 4,  // RunMode 1: copy the state after the step
+    REC_NOSTEP = 1u << 5,   // no precession or relaxation (dt <= 0 event carrier)
+};
+
+// Record layout, 32-bit words:
+//   0 dt (s)            1 gamma*Gz (rad/s/m)  2 gamma*Gy   3 gamma*Gx
+//   4 2*pi*rf frequency summed over active Tx coils (rad/s)
+//   5 flags
+//   6+2c, 7+2c  gamma*A_c*cos(phi_c), gamma*A_c*sin(phi_c) for Tx coil c
+// The stride is rounded up to an even number of words.
+constexpr int REC_HEAD = 6;
+inline int rec_stride(int tx) { return (REC_HEAD + 2 * tx + 1) & ~1; }
+
+struct Decoded {
+    int tx = 1;
+    int stride = 8;
+    std::vector<uint32_t> rec;       // n_rec * stride words
+    std::vector<double> adc_ref;     // rfRef at each ADC sample
+    std::vector<double> kx, ky, kz;  // k-space at each ADC sample
+    std::vector<int32_t> cap_slot;   // per record (RunMode 1), else empty
+    std::vector<double> muts;        // RunMode 1: VSig.Muts values by utsi
+    int64_t n_steps = 0, n_rf = 0, n_same = 0;
+    int64_t n_rec() const { return stride ? (int64_t)rec.size() / stride : 0; }
+};
+
+// Stop reasons of Decoder::run.
+enum DecodeStop { DECODE_END = 0, DECODE_CALLBACK = 1 };
+
+// The event loop of DoScanAtCPU.cpp:246-553 without the Bloch step: it
+// advances the VVar state through the VSeq lines exactly as the reference
+// does and emits one record per time point with dt > 0.
+class Decoder {
+public:
+    Decoder(const mrs_vobj *vobj, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+            const mrs_vseq *vseq, const mrs_vvar *vvar_in, bool callbacks);
+    // Decodes until the end of the sequence or, when callbacks are set, an
+    // Ext code the engine does not execute itself. On DECODE_CALLBACK the
+    // pending code is ext_code(); the caller runs the callback on vvar()
+    // and calls run() again.
+    DecodeStop run(Decoded &out);
+    double ext_code() const { return ext_pending_; }
+    mrs_vvar &vvar() { return v_; }
+    // Pending Ext 6 not yet attached to a record; flush_tail() emits a
+    // no-step record carrying it (end of sequence).
+    void flush_tail(Decoded &out);
+
+private:
+    void emit(Decoded &out, bool step, bool adc, bool capture);
+    const mrs_vobj *vobj_;
+    const mrs_vcoi *vcoi_;
+    const mrs_vctl *vctl_;
+    const mrs_vseq *vseq_;
+    bool callbacks_;
+    mrs_vvar v_;
+    std::vector<double> rfAmp_, rfPhase_, rfFreq_;
+    int64_t i_ = 0;
+    bool spoil_pending_ = false;
+    bool capture_pending_ = false;
+    double ext_pending_ = 0;
+    bool resume_after_ext_ = false;
+    double flag_[6] = {0, 0, 0, 0, 0, 0};
+    // previous step record fields for REC_SAME
+    bool prev_free_ = false;
+    uint32_t prev_key_[4] = {0, 0, 0, 0};
+};
+
+}  // namespace mrs

@@ -1,0 +1,1139 @@
+// MI355X Bloch scan engine: forward kernel, signal reduction, plans and
+// the C ABI of include/mrilab_amd.h.
+//
+// Design (gfx950, 64-wide wavefronts):
+//  * One thread owns one voxel of one type and SPT of its isochromats; the
+//    magnetization stays in VGPRs for every step of a launch. Maps are read
+//    once per launch from the MRiLab column-major layout, which is already
+//    structure-of-arrays, so every load and store is coalesced.
+//  * Step records are wave-uniform. They are read through the scalar data
+//    cache (address space 4), so record fields land in SGPRs and the record
+//    flags drive scalar branches; no LDS or VGPRs are spent on them.
+//  * Free precession with the same dt and fields as the previous step reuses
+//    the per-spin rotation (cos, sin); relaxation factors are recomputed only
+//    when dt changes.
+//  * At an ADC step each thread forms sum_spins(Mx + iMy) * conj(C_c) for its
+//    coils; a transposed butterfly inside the wavefront leaves one coil value
+//    per lane group, the waves of a block add theirs in LDS, and each block
+//    writes one partial per sample. A second kernel sums the block partials
+//    of each sample in a fixed order (compensated fp32 per lane, fp64 across
+//    lanes), demodulates by rfRef and adds into Sx/Sy. No atomics: results
+//    are bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "decode.hpp"
+#include "mrilab_amd.h"
+
+namespace mrs {
+
+constexpr int kMaxTypes = 16;
+constexpr int kRedBatch = 16;   // ADC samples buffered in LDS before a flush
+constexpr double kPi = 3.14159265359;  // DoScanAtCPU.cpp:63
+
+typedef const __attribute__((address_space(4))) uint32_t *crec_t;
+
+struct FwdParams {
+    float *Mx, *My, *Mz;
+    const float *dWRnd, *Rho, *T1, *T2, *dB0, *gx, *gy, *gz;
+    const float *txmg, *txpe, *rxx, *rxy;
+    float *partial;                 // [adc_local][nblk][NV]
+    float *capx, *capy, *capz;      // RunMode 1 capture [slot][ntot]
+    const int32_t *cap_slot;        // per record
+    int64_t nvox;
+    int S, Ty, Tx, Rx, ngroups;
+    int rx_default, precision;
+    int stride;
+    float gyro;
+    float cs[kMaxTypes];            // 2*pi*CS per type, rounded as BlochKernel.h:42
+};
+
+// sin and cos of an fp32 angle, correctly rounded to fp32 (up to rare
+// near-midpoint cases): the reduction by pi/2 (two-part constant) and the
+// Taylor polynomials of sin and cos on [-pi/4, pi/4] (truncation error
+// below 3e-14) run in fp64, which on gfx950 issues at the fp32 FMA rate
+// when packed fp32 is off. MRiLab's CPU engine takes sin/cos from a
+// 24-bit-accurate vector library; a 1.5 ulp fp32 sincos drifts visibly
+// from it over 1e5 cached rotations, this one does not.
+__device__ __forceinline__ void sincos_cr(float xf, float *sp, float *cp) {
+    const double x = (double)xf;
+    double k = rint(x * 0.63661977236758134308);        // 2/pi
+    if (fabs(k) > 1.0e6) {                               // far outside the step angles seen in practice
+        double s, c;
+        sincos(x, &s, &c);
+        *sp = (float)s;
+        *cp = (float)c;
+        return;
+    }
+    double r = fma(-k, 1.5707963267948966192, x);        // pi/2, high part
+    r = fma(-k, 6.1232339957367660e-17, r);              // pi/2, low part
+    const double r2 = r * r;
+    const double sp_ = r + r * r2 * (-1.0 / 6 + r2 * (1.0 / 120 + r2 * (-1.0 / 5040 + r2 * (1.0 / 362880 +
+                       r2 * (-1.0 / 39916800 + r2 * (1.0 / 6227020800.0))))));
+    const double cp_ = 1.0 + r2 * (-0.5 + r2 * (1.0 / 24 + r2 * (-1.0 / 720 + r2 * (1.0 / 40320 +
+                       r2 * (-1.0 / 3628800 + r2 * (1.0 / 479001600.0 + r2 * (-1.0 / 87178291200.0)))))));
+    const int q = (int)(((long long)k) & 3);
+    double s = sp_, c = cp_;
+    if (q == 1) { s = cp_; c = -sp_; }
+    else if (q == 2) { s = -sp_; c = -cp_; }
+    else if (q == 3) { s = -cp_; c = sp_; }
+    *sp = (float)s;
+    *cp = (float)c;
+}
+
+// Lane exchange for one butterfly stage across lane bit O. For O = 32 and
+// 16, gfx950's v_permlane{32,16}_swap exchange whole rows; for O <= 8, DPP
+// row_mirror / row_half_mirror / quad_perm pair each lane with one whose bit
+// O differs (not always lane ^ O, but the same pair in both directions).
+template <int O>
+__device__ __forceinline__ float dpp_partner(float x) {
+    constexpr int ctrl = O == 8 ? 0x140 : O == 4 ? 0x141 : O == 2 ? 0x4E : 0xB1;
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), ctrl, 0xF, 0xF, true));
+}
+
+// Transposed butterfly: N values per lane, 64 lanes. Each stage halves the
+// number of values a lane keeps while pairing lanes across lane bit O (the
+// lane with bit O set keeps the upper half); when one value is left the
+// remaining stages sum plainly. Afterwards lane l holds the wave total of
+// value l >> (6 - log2 N). With the row swaps a pair costs one swap and
+// one add: after swap(X = v[i], Y = v[i+H]) the two outputs hold, in every
+// lane, exactly the two addends of the value that lane keeps.
+template <int N, int O>
+struct WaveSum {
+    static __device__ __forceinline__ void run(float *v, int lane) {
+        if constexpr (O >= 1) {
+            if constexpr (N > 1) {
+                constexpr int H = N / 2;
+                if constexpr (O >= 16) {
+#pragma unroll
+                    for (int i = 0; i < H; i++) {
+                        const auto r = O == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]), __float_as_uint(v[i + H]), false, false)
+                                               : __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]), __float_as_uint(v[i + H]), false, false);
+                        v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+                    }
+                } else {
+                    const bool up = (lane & O) != 0;
+#pragma unroll
+                    for (int i = 0; i < H; i++) {
+                        const float lo = v[i] + dpp_partner<O>(v[i]);
+                        const float hi = v[i + H] + dpp_partner<O>(v[i + H]);
+                        v[i] = up ? hi : lo;
+                    }
+                }
+                WaveSum<H, O / 2>::run(v, lane);
+            } else {
+                if constexpr (O >= 16) {
+                    const auto r = O == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(v[0]), __float_as_uint(v[0]), false, false)
+                                           : __builtin_amdgcn_permlane16_swap(__float_as_uint(v[0]), __float_as_uint(v[0]), false, false);
+                    // r[0] holds the partner's value in the upper rows, r[1] in the lower rows
+                    v[0] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+                } else {
+                    v[0] += dpp_partner<O>(v[0]);
+                }
+                WaveSum<1, O / 2>::run(v, lane);
+            }
+        }
+    }
+};
+
+// Mz relaxation exactly as BlochKernel.h:363-368 rounds it: the product and
+// the sum are rounded separately. Near equilibrium the per-step change is
+// below half an ulp of Mz and fp32 recovery stalls; where it stalls depends
+// on this rounding, so it must not be fused into an FMA.
+__device__ __forceinline__ float relax_mz_ref(float mz, float e1, float m0e) {
+#pragma clang fp contract(off)
+    const float t = mz * e1;
+    return t + m0e;
+}
+
+template <int SPT, int NRX>
+__global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g,
+                                                        int64_t r0, int64_t r1) {
+    constexpr int NV = 2 * NRX;
+    extern __shared__ float red[];  // [kRedBatch][nwaves][NV]
+    const crec_t rec = (crec_t)rec_g;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int type = blockIdx.y / p.ngroups;
+    const int grp = blockIdx.y - type * p.ngroups;
+    const int64_t nvox = p.nvox;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = v < nvox;
+    const int64_t vv = valid ? v : nvox - 1;
+    const int64_t tv = vv + (int64_t)type * nvox;
+    const int nblk = gridDim.x * gridDim.y;
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+
+    const float x = p.gx[vv], y = p.gy[vv], z = p.gz[vv];
+    const float rho = p.Rho[tv], t1 = p.T1[tv], t2 = p.T2[tv];
+    const float w0 = p.dB0[vv] * p.gyro;  // BlochKernel.h:38
+    const float cst = p.cs[type];
+
+    float mx[SPT], my[SPT], mz[SPT], wb[SPT], cc[SPT], ss[SPT];
+    int64_t sidx[SPT];
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        sidx[s] = vv + nvox * ((int64_t)(grp * SPT + s) + (int64_t)p.S * type);
+        mx[s] = p.Mx[sidx[s]];
+        my[s] = p.My[sidx[s]];
+        mz[s] = p.Mz[sidx[s]];
+        wb[s] = (w0 + p.dWRnd[sidx[s]]) + cst;  // BlochKernel.h:38-42
+        cc[s] = 1.f;
+        ss[s] = 0.f;
+    }
+    float crx[NRX], cry[NRX];
+#pragma unroll
+    for (int c = 0; c < NRX; c++) {
+        if (p.rx_default) {
+            crx[c] = 1.f;
+            cry[c] = 0.f;
+        } else if (c < p.Rx) {
+            crx[c] = p.rxx[vv + (int64_t)c * nvox];
+            cry[c] = p.rxy[vv + (int64_t)c * nvox];
+        } else {
+            crx[c] = 0.f;
+            cry[c] = 0.f;
+        }
+    }
+    // single-Tx B1+ as a complex weight mg*exp(i*pe)
+    float wtx_r = 0.f, wtx_i = 0.f;
+    if (p.Tx == 1) {
+        float sp, cp;
+        sincos_cr(p.txpe[vv], &sp, &cp);
+        const float mg = p.txmg[vv];
+        wtx_r = mg * cp;
+        wtx_i = mg * sp;
+    }
+    const float m0s = rho / (float)p.S;  // BlochKernel.h:365
+
+    bool have_rot = false;
+    float last_dt = -1.f;
+    float e1 = 1.f, e2 = 1.f, m0e = 0.f;   // reference mode
+    float d1 = 0.f, d2 = 0.f;              // accurate mode
+    int slot = 0;
+    int adc_batch0 = 0;
+
+    const bool fast_ok = p.precision == 0;
+    // record r is decoded from registers loaded one iteration earlier
+    crec_t q = rec + r0 * p.stride;
+    uint32_t n_flags = r0 < r1 ? q[5] : 0u;
+    float n_dt = r0 < r1 ? __uint_as_float(q[0]) : 0.f;
+    float n_gz = r0 < r1 ? __uint_as_float(q[1]) : 0.f;
+    float n_gy = r0 < r1 ? __uint_as_float(q[2]) : 0.f;
+    float n_gx = r0 < r1 ? __uint_as_float(q[3]) : 0.f;
+    float n_wrf = r0 < r1 ? __uint_as_float(q[4]) : 0.f;
+    float n_br = r0 < r1 ? __uint_as_float(q[REC_HEAD]) : 0.f;
+    float n_bi = r0 < r1 ? __uint_as_float(q[REC_HEAD + 1]) : 0.f;
+
+    for (int64_t r = r0; r < r1; ++r) {
+        const uint32_t flags = n_flags;
+        const float dt = n_dt, ggz = n_gz, ggy = n_gy, ggx = n_gx, wrf = n_wrf, br0 = n_br, bi0 = n_bi;
+        q = rec + r * p.stride;
+        if (r + 1 < r1) {
+            const crec_t qn = q + p.stride;
+            n_flags = qn[5];
+            n_dt = __uint_as_float(qn[0]);
+            n_gz = __uint_as_float(qn[1]);
+            n_gy = __uint_as_float(qn[2]);
+            n_gx = __uint_as_float(qn[3]);
+            n_wrf = __uint_as_float(qn[4]);
+            n_br = __uint_as_float(qn[REC_HEAD]);
+            n_bi = __uint_as_float(qn[REC_HEAD + 1]);
+        }
+        // common case: free precession with the previous step's rotation and dt
+        if ((flags & (REC_ADC | REC_RF | REC_SPOIL | REC_CAPTURE | REC_NOSTEP | REC_SAME)) == REC_SAME &&
+            have_rot && dt == last_dt && fast_ok) {
+#pragma unroll
+            for (int s = 0; s < SPT; s++) {
+                const float ax = mx[s], ay = my[s];
+                mx[s] = (ax * cc[s] + ay * ss[s]) * e2;
+                my[s] = (ay * cc[s] - ax * ss[s]) * e2;
+                mz[s] = relax_mz_ref(mz[s], e1, m0e);
+            }
+            continue;
+        }
+        if (flags & REC_SPOIL) {  // Plugin_IdealSpoiler.m:10-11
+#pragma unroll
+            for (int s = 0; s < SPT; s++) {
+                mx[s] = 0.f;
+                my[s] = 0.f;
+            }
+        }
+        if (flags & REC_ADC) {  // DoScanAtCPU.cpp:445-491, before the step
+            float sx = 0.f, sy = 0.f;
+#pragma unroll
+            for (int s = 0; s < SPT; s++) {
+                sx += mx[s];
+                sy += my[s];
+            }
+            if (!valid) {
+                sx = 0.f;
+                sy = 0.f;
+            }
+            float val[NV];
+#pragma unroll
+            for (int c = 0; c < NRX; c++) {
+                val[2 * c] = sx * crx[c] + sy * cry[c];
+                val[2 * c + 1] = sy * crx[c] - sx * cry[c];
+            }
+            WaveSum<NV, 32>::run(val, lane);
+            constexpr int G = 64 / NV;  // lanes per value after the butterfly
+            if ((lane & (G - 1)) == 0) red[(slot * nwaves + wave) * NV + lane / G] = val[0];
+            if (++slot == kRedBatch) {
+                __syncthreads();
+                for (int i = threadIdx.x; i < kRedBatch * NV; i += blockDim.x) {
+                    const int sl = i / NV, j = i - sl * NV;
+                    float acc = 0.f;
+                    for (int w = 0; w < nwaves; w++) acc += red[(sl * nwaves + w) * NV + j];
+                    p.partial[((int64_t)(adc_batch0 + sl) * nblk + blk) * NV + j] = acc;
+                }
+                __syncthreads();
+                adc_batch0 += kRedBatch;
+                slot = 0;
+            }
+        }
+        if (!(flags & REC_NOSTEP)) {
+            if (dt != last_dt) {
+                last_dt = dt;
+                if (p.precision == 0) {
+                    // BlochKernel.h:348-368 in fp32: E = exp(max(-dt/T, -200)),
+                    // with exp rounded correctly from fp64
+                    float a2 = (1.0f / t2) * dt;
+                    a2 = -a2;
+                    if (a2 < -200.f) a2 = -200.f;
+                    float a1 = (1.0f / t1) * dt;
+                    a1 = -a1;
+                    if (a1 < -200.f) a1 = -200.f;
+                    e2 = (float)exp((double)a2);
+                    e1 = (float)exp((double)a1);
+                    m0e = m0s * (1.0f - e1);
+                } else {
+                    d2 = (float)(-expm1(-(double)dt / (double)t2));
+                    d1 = (float)(-expm1(-(double)dt / (double)t1));
+                }
+            }
+            const float gv = z * ggz + y * ggy + x * ggx;  // BlochKernel.h:44-51
+            if (flags & REC_RF) {
+                float bx, by;
+                if (p.Tx == 1) {
+                    bx = wtx_r * br0 - wtx_i * bi0;
+                    by = wtx_r * bi0 + wtx_i * br0;
+                } else {
+                    bx = 0.f;
+                    by = 0.f;
+                    for (int c = 0; c < p.Tx; c++) {
+                        const float br = __uint_as_float(q[REC_HEAD + 2 * c]);
+                        const float bi = __uint_as_float(q[REC_HEAD + 2 * c + 1]);
+                        float sp, cp;
+                        sincos_cr(p.txpe[vv + (int64_t)c * nvox], &sp, &cp);
+                        const float mg = p.txmg[vv + (int64_t)c * nvox];
+                        bx += mg * (cp * br - sp * bi);
+                        by += mg * (cp * bi + sp * br);
+                    }
+                }
+                const float b2 = bx * bx + by * by;
+#pragma unroll
+                for (int s = 0; s < SPT; s++) {
+                    // rotation by |Omega|*dt about -Omega, Omega = (bx, by, w):
+                    // the matrix of BlochKernel.h:146-306 in Rodrigues form
+                    const float w = (wb[s] + gv) + wrf;
+                    const float om2 = b2 + w * w;
+                    if (om2 > 0.f) {
+                        const float om = sqrtf(om2);
+                        const float inv = 1.0f / om;
+                        const float nx = bx * inv, ny = by * inv, nz = w * inv;
+                        float sn, cs;
+                        sincos_cr(om * dt, &sn, &cs);
+                        const float ax = mx[s], ay = my[s], az = mz[s];
+                        const float k = (1.f - cs) * (nx * ax + ny * ay + nz * az);
+                        mx[s] = ax * cs - (ny * az - nz * ay) * sn + nx * k;
+                        my[s] = ay * cs - (nz * ax - nx * az) * sn + ny * k;
+                        mz[s] = az * cs - (nx * ay - ny * ax) * sn + nz * k;
+                    }
+                }
+            } else {
+                if (!((flags & REC_SAME) && have_rot)) {
+#pragma unroll
+                    for (int s = 0; s < SPT; s++) sincos_cr((wb[s] + gv) * dt, &ss[s], &cc[s]);
+                    have_rot = true;
+                }
+#pragma unroll
+                for (int s = 0; s < SPT; s++) {  // BlochKernel.h:314-343
+                    const float ax = mx[s], ay = my[s];
+                    mx[s] = ax * cc[s] + ay * ss[s];
+                    my[s] = ay * cc[s] - ax * ss[s];
+                }
+            }
+            if (p.precision == 0) {
+#pragma unroll
+                for (int s = 0; s < SPT; s++) {
+                    mx[s] = mx[s] * e2;
+                    my[s] = my[s] * e2;
+                    mz[s] = relax_mz_ref(mz[s], e1, m0e);
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < SPT; s++) {
+                    mx[s] = mx[s] - mx[s] * d2;
+                    my[s] = my[s] - my[s] * d2;
+                    mz[s] = mz[s] + (m0s - mz[s]) * d1;
+                }
+            }
+        }
+        if ((flags & REC_CAPTURE) && valid) {  // DoScanAtCPU.cpp:544-552
+            const int64_t base = (int64_t)p.cap_slot[r] * nvox * p.S * p.Ty;
+#pragma unroll
+            for (int s = 0; s < SPT; s++) {
+                p.capx[base + sidx[s]] = mx[s];
+                p.capy[base + sidx[s]] = my[s];
+                p.capz[base + sidx[s]] = mz[s];
+            }
+        }
+    }
+    if (slot > 0) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < slot * NV; i += blockDim.x) {
+            const int sl = i / NV, j = i - sl * NV;
+            float acc = 0.f;
+            for (int w = 0; w < nwaves; w++) acc += red[(sl * nwaves + w) * NV + j];
+            p.partial[((int64_t)(adc_batch0 + sl) * nblk + blk) * NV + j] = acc;
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < SPT; s++) {
+            p.Mx[sidx[s]] = mx[s];
+            p.My[sidx[s]] = my[s];
+            p.Mz[sidx[s]] = mz[s];
+        }
+    }
+}
+
+// Neumaier-compensated fp32 accumulation of block partials per lane, then
+// an fp64 butterfly across the wave; one wave per (sample, type).
+template <int NV>
+__global__ __launch_bounds__(64) void signal_reduce_kernel(const float *__restrict__ partial, int nblk,
+                                                           int blk_per_type, int Rx, int rx_default,
+                                                           const double *__restrict__ adc_ref,
+                                                           int64_t adc0, int64_t N, double *Sx,
+                                                           double *Sy) {
+    const int a = blockIdx.x;
+    const int t = blockIdx.y;
+    const int lane = threadIdx.x;
+    float sum[NV], comp[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        sum[j] = 0.f;
+        comp[j] = 0.f;
+    }
+    const int b0 = t * blk_per_type;
+    for (int b = lane; b < blk_per_type; b += 64) {
+        const float *src = partial + ((int64_t)a * nblk + b0 + b) * NV;
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            const float xv = src[j];
+            const float s = sum[j] + xv;
+            if (fabsf(sum[j]) >= fabsf(xv))
+                comp[j] += (sum[j] - s) + xv;
+            else
+                comp[j] += (xv - s) + sum[j];
+            sum[j] = s;
+        }
+    }
+    double tot[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        double d = (double)sum[j] + (double)comp[j];
+        for (int o = 32; o >= 1; o >>= 1) d += __shfl_xor(d, o);
+        tot[j] = d;
+    }
+    if (lane == 0) {
+        const double ref = adc_ref[adc0 + a];
+        const double cr = cos(-ref), sr = sin(-ref);
+        const int64_t n = adc0 + a;
+        for (int c = 0; c < Rx; c++) {
+            const int cc = rx_default ? 0 : c;
+            double re = tot[2 * cc], im = tot[2 * cc + 1];
+            if (ref != 0) {  // DoScanAtCPU.cpp:472-479: multiply by exp(-i*rfRef)
+                const double r2 = re * cr - im * sr;
+                const double i2 = re * sr + im * cr;
+                re = r2;
+                im = i2;
+            }
+            Sx[(int64_t)t * Rx * N + (int64_t)c * N + n] += re;
+            Sy[(int64_t)t * Rx * N + (int64_t)c * N + n] += im;
+        }
+    }
+}
+
+}  // namespace mrs
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+using namespace mrs;
+
+static thread_local std::string g_err;
+
+static mrs_status fail(mrs_status st, const std::string &msg) {
+    g_err = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(e_ == hipErrorOutOfMemory ? MRS_ERR_NOMEM : MRS_ERR_DEVICE,             \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+typedef void (*fwd_fn)(FwdParams, const uint32_t *, int64_t, int64_t);
+
+template <int SPT, int NRX>
+static void launch_fwd(dim3 grid, dim3 block, size_t lds, hipStream_t st, const FwdParams &p,
+                       const uint32_t *rec, int64_t r0, int64_t r1) {
+    bloch_fwd_kernel<SPT, NRX><<<grid, block, lds, st>>>(p, rec, r0, r1);
+}
+
+typedef void (*launch_fn)(dim3, dim3, size_t, hipStream_t, const FwdParams &, const uint32_t *, int64_t,
+                          int64_t);
+
+template <int SPT>
+static launch_fn pick_nrx(int nrx) {
+    switch (nrx) {
+        case 1: return launch_fwd<SPT, 1>;
+        case 2: return launch_fwd<SPT, 2>;
+        case 4: return launch_fwd<SPT, 4>;
+        case 8: return launch_fwd<SPT, 8>;
+        case 16: return launch_fwd<SPT, 16>;
+        case 32: return launch_fwd<SPT, 32>;
+    }
+    return nullptr;
+}
+
+static launch_fn pick_fwd(int spt, int nrx) {
+    switch (spt) {
+        case 1: return pick_nrx<1>(nrx);
+        case 2: return pick_nrx<2>(nrx);
+        case 4: return pick_nrx<4>(nrx);
+        case 8: return pick_nrx<8>(nrx);
+    }
+    return nullptr;
+}
+
+static void launch_reduce(int nv, dim3 grid, hipStream_t st, const float *partial, int nblk, int bpt, int Rx,
+                          int rxd, const double *ref, int64_t adc0, int64_t N, double *Sx, double *Sy) {
+    switch (nv) {
+        case 2: signal_reduce_kernel<2><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
+        case 4: signal_reduce_kernel<4><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
+        case 8: signal_reduce_kernel<8><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
+        case 16: signal_reduce_kernel<16><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
+        case 32: signal_reduce_kernel<32><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
+        case 64: signal_reduce_kernel<64><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct mrs_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int R = 0, C = 0, Z = 0, S = 0, Ty = 0, Tx = 0, Rx = 0;
+    int64_t nvox = 0, ntot = 0;
+    int rx_default = 1, precision = 0, run_mode = 0;
+    int spt = 1, nrx = 1, block = 256, ngroups = 1, gx = 1, nblk = 1, nv = 2;
+    int64_t max_adc = 0;
+    int64_t signal_num = 0;
+    float gyro = 0.f;
+    float cs[kMaxTypes] = {0};
+    // device memory
+    DevBuf M0[3], M[3], dWRnd, Rho, T1, T2, dB0, gxg, gyg, gzg, txmg, txpe, rxx, rxy;
+    DevBuf rec, cap_slot, adc_ref, partial, Sx, Sy, cap[3];
+    size_t rec_cap_words = 0, adc_cap = 0;
+    int64_t n_uts = 0;
+    // decoded program
+    Decoded dec;
+    std::vector<int64_t> launch_r0, launch_r1, launch_adc0, launch_nadc;
+    mrs_vvar vvar_final;
+    std::vector<double> rf_final[3];
+    bool ran = false;
+    float last_ms = 0.f;
+    int timing = 0;
+    std::vector<hipEvent_t> tev;  // per launch: fwd begin, fwd end, reduce end
+    double fwd_ms = 0.0, red_ms = 0.0;
+    ~mrs_plan() {
+        for (auto e : tev) (void)hipEventDestroy(e);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+template <class T>
+static hipError_t dalloc(DevBuf &b, size_t n) {
+    if (b.p) {
+        (void)hipFree(b.p);
+        b.p = nullptr;
+    }
+    if (n == 0) n = 1;
+    return hipMalloc(&b.p, n * sizeof(T));
+}
+
+static mrs_status upload(DevBuf &b, const void *src, size_t bytes, hipStream_t st) {
+    HIP_TRY(hipMalloc(&b.p, bytes ? bytes : 4));
+    if (bytes) HIP_TRY(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+    return MRS_OK;
+}
+
+static mrs_status check_args(const mrs_vobj *vo, const mrs_vmag *vm, const mrs_vcoi *vc, const mrs_vctl *vt,
+                             const mrs_vseq *vs, const mrs_vvar *vv, const mrs_vsig *vg) {
+    if (!vo || !vm || !vc || !vt || !vs || !vv || !vg) return fail(MRS_ERR_ARG, "NULL struct pointer");
+    if (!vo->Mx || !vo->My || !vo->Mz || !vo->Rho || !vo->T1 || !vo->T2)
+        return fail(MRS_ERR_ARG, "VObj array pointer is NULL");
+    if (!vm->dB0 || !vm->dWRnd || !vm->Gxgrid || !vm->Gygrid || !vm->Gzgrid)
+        return fail(MRS_ERR_ARG, "VMag array pointer is NULL");
+    if (!vc->TxCoilmg || !vc->TxCoilpe || !vc->RxCoilx || !vc->RxCoily)
+        return fail(MRS_ERR_ARG, "VCoi array pointer is NULL");
+    if (vo->dims[0] <= 0 || vo->dims[1] <= 0 || vo->dims[2] <= 0 || vo->SpinNum <= 0 || vo->TypeNum <= 0)
+        return fail(MRS_ERR_ARG, "VObj dims, SpinNum and TypeNum must be positive");
+    if (vo->TypeNum > kMaxTypes) return fail(MRS_ERR_UNSUPPORTED, "TypeNum above 16");
+    if (vc->TxCoilNum <= 0 || vc->RxCoilNum <= 0) return fail(MRS_ERR_ARG, "coil counts must be positive");
+    if (vc->RxCoilDefault == 0 && vc->RxCoilNum > 32) return fail(MRS_ERR_UNSUPPORTED, "more than 32 receive coils");
+    if (!vt->CS) return fail(MRS_ERR_ARG, "VCtl.CS is NULL");
+    if (!vs->utsLine || !vs->tsLine || !vs->flagsLine || !vs->rfAmpLine || !vs->rfPhaseLine || !vs->rfFreqLine ||
+        !vs->rfCoilLine || !vs->GzAmpLine || !vs->GyAmpLine || !vs->GxAmpLine || !vs->ADCLine || !vs->ExtLine)
+        return fail(MRS_ERR_ARG, "VSeq line pointer is NULL");
+    if (vs->n_ts < 2 || vs->n_uts < 1) return fail(MRS_ERR_ARG, "VSeq.tsLine needs at least two entries");
+    if (!vv->rfAmp || !vv->rfPhase || !vv->rfFreq) return fail(MRS_ERR_ARG, "VVar rf arrays are NULL");
+    if (vt->RunMode == 1 && (!vg->Mx || !vg->My || !vg->Mz || !vg->Muts))
+        return fail(MRS_ERR_ARG, "RunMode 1 needs VSig.Mx/My/Mz/Muts");
+    if (vt->MultiTransmit && (vt->MasterTxCoil < 1 || vt->MasterTxCoil > vc->TxCoilNum))
+        return fail(MRS_ERR_ARG, "VCtl.MasterTxCoil outside 1..TxCoilNum");
+    return MRS_OK;
+}
+
+// Splits [0, n_rec) into launches of at most max_adc ADC samples.
+static void partition(mrs_plan *P) {
+    P->launch_r0.clear();
+    P->launch_r1.clear();
+    P->launch_adc0.clear();
+    P->launch_nadc.clear();
+    const int64_t n = P->dec.n_rec();
+    const int stride = P->dec.stride;
+    int64_t r0 = 0, nadc = 0, adc0 = 0;
+    for (int64_t r = 0; r < n; r++) {
+        const bool adc = (P->dec.rec[r * stride + 5] & REC_ADC) != 0;
+        if (adc && nadc == P->max_adc) {
+            P->launch_r0.push_back(r0);
+            P->launch_r1.push_back(r);
+            P->launch_adc0.push_back(adc0);
+            P->launch_nadc.push_back(nadc);
+            adc0 += nadc;
+            r0 = r;
+            nadc = 0;
+        }
+        if (adc) nadc++;
+    }
+    if (n > r0) {
+        P->launch_r0.push_back(r0);
+        P->launch_r1.push_back(n);
+        P->launch_adc0.push_back(adc0);
+        P->launch_nadc.push_back(nadc);
+    }
+}
+
+static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm, const mrs_vcoi *vc,
+                             const mrs_vctl *vt, const mrs_vseq *vs, const mrs_vsig *vg, const mrs_options *opt) {
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (vt->GPUIndex < 0 || vt->GPUIndex >= ndev)
+        return fail(MRS_ERR_DEVICE, "VCtl.GPUIndex " + std::to_string(vt->GPUIndex) + " is not a visible device");
+    P->device = vt->GPUIndex;
+    HIP_TRY(hipSetDevice(P->device));
+    HIP_TRY(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&P->ev0));
+    HIP_TRY(hipEventCreate(&P->ev1));
+    P->R = vo->dims[0];
+    P->C = vo->dims[1];
+    P->Z = vo->dims[2];
+    P->S = vo->SpinNum;
+    P->Ty = vo->TypeNum;
+    P->Tx = vc->TxCoilNum;
+    P->Rx = vc->RxCoilNum;
+    P->nvox = (int64_t)P->R * P->C * P->Z;
+    P->ntot = P->nvox * P->S * P->Ty;
+    P->rx_default = vc->RxCoilDefault != 0;
+    P->precision = opt ? opt->precision_mode : 0;
+    P->timing = opt ? opt->timing : 0;
+    P->run_mode = vt->RunMode;
+    P->signal_num = vg->SignalNum;
+    P->n_uts = vs->n_uts;
+    P->gyro = (float)vo->Gyro;
+    for (int t = 0; t < P->Ty; t++)
+        P->cs[t] = (float)((double)((float)vt->CS[t] * 2.0f) * kPi);  // BlochKernel.h:42
+    // spins per thread: largest of 8, 4, 2, 1 dividing SpinNum
+    int spt = opt && opt->spins_per_thread > 0 ? opt->spins_per_thread : 8;
+    while (spt > 1 && (P->S % spt != 0 || spt > P->S)) spt >>= 1;
+    if (spt != 1 && spt != 2 && spt != 4 && spt != 8) return fail(MRS_ERR_ARG, "spins_per_thread must be 1, 2, 4 or 8");
+    if (P->S % spt) return fail(MRS_ERR_ARG, "spins_per_thread must divide SpinNum");
+    P->spt = spt;
+    P->ngroups = P->S / spt;
+    int block = opt && opt->block_size > 0 ? opt->block_size : (P->nvox >= (int64_t)256 * 512 ? 256 : 64);
+    if (block % 64 || block > 256) return fail(MRS_ERR_ARG, "block_size must be 64, 128, 192 or 256");
+    P->block = block;
+    P->gx = (int)((P->nvox + block - 1) / block);
+    if ((int64_t)P->Ty * P->ngroups > 65535) return fail(MRS_ERR_UNSUPPORTED, "TypeNum*SpinNum/spins_per_thread too large");
+    P->nblk = P->gx * P->Ty * P->ngroups;
+    int nrx = 1;
+    const int rxn = P->rx_default ? 1 : P->Rx;
+    while (nrx < rxn) nrx <<= 1;
+    P->nrx = nrx;
+    P->nv = 2 * nrx;
+    const size_t budget = (size_t)512 << 20;  // partial-buffer budget
+    int64_t max_adc = (int64_t)(budget / ((size_t)P->nblk * P->nv * sizeof(float)));
+    max_adc = std::max<int64_t>(kRedBatch, std::min<int64_t>(max_adc, 65536));
+    if (opt && opt->max_adc_per_launch > 0) max_adc = opt->max_adc_per_launch;
+    P->max_adc = max_adc;
+    if (!pick_fwd(P->spt, P->nrx)) return fail(MRS_ERR_UNSUPPORTED, "no kernel for this coil count");
+
+    hipStream_t st = P->stream;
+    const size_t vb = (size_t)P->nvox * sizeof(float);
+    mrs_status rc;
+    const float *src[3] = {vo->Mx, vo->My, vo->Mz};
+    for (int k = 0; k < 3; k++) {
+        if ((rc = upload(P->M0[k], src[k], P->ntot * sizeof(float), st))) return rc;
+        HIP_TRY(hipMalloc(&P->M[k].p, P->ntot * sizeof(float)));
+    }
+    if ((rc = upload(P->dWRnd, vm->dWRnd, P->ntot * sizeof(float), st))) return rc;
+    if ((rc = upload(P->Rho, vo->Rho, vb * P->Ty, st))) return rc;
+    if ((rc = upload(P->T1, vo->T1, vb * P->Ty, st))) return rc;
+    if ((rc = upload(P->T2, vo->T2, vb * P->Ty, st))) return rc;
+    if ((rc = upload(P->dB0, vm->dB0, vb, st))) return rc;
+    if ((rc = upload(P->gxg, vm->Gxgrid, vb, st))) return rc;
+    if ((rc = upload(P->gyg, vm->Gygrid, vb, st))) return rc;
+    if ((rc = upload(P->gzg, vm->Gzgrid, vb, st))) return rc;
+    if ((rc = upload(P->txmg, vc->TxCoilmg, vb * P->Tx, st))) return rc;
+    if ((rc = upload(P->txpe, vc->TxCoilpe, vb * P->Tx, st))) return rc;
+    if (!P->rx_default) {
+        if ((rc = upload(P->rxx, vc->RxCoilx, vb * P->Rx, st))) return rc;
+        if ((rc = upload(P->rxy, vc->RxCoily, vb * P->Rx, st))) return rc;
+    } else {
+        HIP_TRY(hipMalloc(&P->rxx.p, 4));
+        HIP_TRY(hipMalloc(&P->rxy.p, 4));
+    }
+    const size_t nsig = (size_t)std::max<int64_t>(1, P->signal_num) * P->Ty * P->Rx;
+    HIP_TRY(hipMalloc(&P->Sx.p, nsig * sizeof(double)));
+    HIP_TRY(hipMalloc(&P->Sy.p, nsig * sizeof(double)));
+    if (P->run_mode == 1) {
+        const size_t ncap = (size_t)P->ntot * P->n_uts;
+        if (ncap * 3 * sizeof(float) > ((size_t)64 << 30))
+            return fail(MRS_ERR_UNSUPPORTED, "RunMode 1 capture above 64 GiB of device memory");
+        for (int k = 0; k < 3; k++) HIP_TRY(hipMalloc(&P->cap[k].p, ncap * sizeof(float)));
+    }
+    return MRS_OK;
+}
+
+// Uploads the decoded records of P->dec (replacing earlier ones).
+static mrs_status upload_program(mrs_plan *P) {
+    hipStream_t st = P->stream;
+    const size_t words = P->dec.rec.size();
+    if (words > P->rec_cap_words || !P->rec.p) {
+        HIP_TRY(hipStreamSynchronize(st));
+        if (P->rec.p) (void)hipFree(P->rec.p);
+        P->rec.p = nullptr;
+        HIP_TRY(hipMalloc(&P->rec.p, std::max<size_t>(words, 8) * 4));
+        P->rec_cap_words = std::max<size_t>(words, 8);
+        if (P->run_mode == 1) {
+            if (P->cap_slot.p) (void)hipFree(P->cap_slot.p);
+            P->cap_slot.p = nullptr;
+            HIP_TRY(hipMalloc(&P->cap_slot.p, std::max<size_t>(P->dec.cap_slot.size(), 1) * 4));
+        }
+    }
+    if (words) HIP_TRY(hipMemcpyAsync(P->rec.p, P->dec.rec.data(), words * 4, hipMemcpyHostToDevice, st));
+    if (P->run_mode == 1 && !P->dec.cap_slot.empty())
+        HIP_TRY(hipMemcpyAsync(P->cap_slot.p, P->dec.cap_slot.data(), P->dec.cap_slot.size() * 4,
+                               hipMemcpyHostToDevice, st));
+    const size_t nadc = P->dec.adc_ref.size();
+    if (nadc > P->adc_cap || !P->adc_ref.p) {
+        HIP_TRY(hipStreamSynchronize(st));
+        if (P->adc_ref.p) (void)hipFree(P->adc_ref.p);
+        P->adc_ref.p = nullptr;
+        HIP_TRY(hipMalloc(&P->adc_ref.p, std::max<size_t>(nadc, 1) * sizeof(double)));
+        P->adc_cap = std::max<size_t>(nadc, 1);
+    }
+    if (nadc)
+        HIP_TRY(hipMemcpyAsync(P->adc_ref.p, P->dec.adc_ref.data(), nadc * sizeof(double), hipMemcpyHostToDevice,
+                               st));
+    partition(P);
+    int64_t maxn = 0;
+    for (auto n : P->launch_nadc) maxn = std::max(maxn, n);
+    const size_t need = (size_t)std::max<int64_t>(maxn, 1) * P->nblk * P->nv;
+    if (!P->partial.p || need > (size_t)P->max_adc * P->nblk * P->nv) {
+        if (P->partial.p) {
+            HIP_TRY(hipStreamSynchronize(st));
+            (void)hipFree(P->partial.p);
+            P->partial.p = nullptr;
+        }
+        HIP_TRY(hipMalloc(&P->partial.p, (size_t)std::max<int64_t>(P->max_adc, 1) * P->nblk * P->nv * sizeof(float)));
+    }
+    return MRS_OK;
+}
+
+// Runs every launch of the uploaded program on the plan stream.
+// adc_base: index of the program's first ADC sample in the scan.
+static mrs_status execute(mrs_plan *P, int64_t adc_base) {
+    FwdParams fp;
+    std::memset(&fp, 0, sizeof fp);
+    fp.Mx = (float *)P->M[0].p;
+    fp.My = (float *)P->M[1].p;
+    fp.Mz = (float *)P->M[2].p;
+    fp.dWRnd = (const float *)P->dWRnd.p;
+    fp.Rho = (const float *)P->Rho.p;
+    fp.T1 = (const float *)P->T1.p;
+    fp.T2 = (const float *)P->T2.p;
+    fp.dB0 = (const float *)P->dB0.p;
+    fp.gx = (const float *)P->gxg.p;
+    fp.gy = (const float *)P->gyg.p;
+    fp.gz = (const float *)P->gzg.p;
+    fp.txmg = (const float *)P->txmg.p;
+    fp.txpe = (const float *)P->txpe.p;
+    fp.rxx = (const float *)P->rxx.p;
+    fp.rxy = (const float *)P->rxy.p;
+    fp.partial = (float *)P->partial.p;
+    fp.capx = (float *)P->cap[0].p;
+    fp.capy = (float *)P->cap[1].p;
+    fp.capz = (float *)P->cap[2].p;
+    fp.cap_slot = (const int32_t *)P->cap_slot.p;
+    fp.nvox = P->nvox;
+    fp.S = P->S;
+    fp.Ty = P->Ty;
+    fp.Tx = P->Tx;
+    fp.Rx = P->Rx;
+    fp.ngroups = P->ngroups;
+    fp.rx_default = P->rx_default;
+    fp.precision = P->precision;
+    fp.stride = P->dec.stride;
+    fp.gyro = P->gyro;
+    for (int t = 0; t < P->Ty; t++) fp.cs[t] = P->cs[t];
+    launch_fn fn = pick_fwd(P->spt, P->nrx);
+    const dim3 grid(P->gx, P->Ty * P->ngroups), block(P->block);
+    const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
+    const uint32_t *rec = (const uint32_t *)P->rec.p;
+    const size_t nl = P->launch_r0.size();
+    if (P->timing) {
+        while (P->tev.size() < 3 * nl) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            P->tev.push_back(e);
+        }
+    }
+    for (size_t l = 0; l < nl; l++) {
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l], P->stream));
+        fn(grid, block, lds, P->stream, fp, rec, P->launch_r0[l], P->launch_r1[l]);
+        HIP_TRY(hipGetLastError());
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 1], P->stream));
+        const int64_t na = P->launch_nadc[l];
+        if (na > 0) {
+            launch_reduce(P->nv, dim3((unsigned)na, P->Ty), P->stream, (const float *)P->partial.p, P->nblk,
+                          P->gx * P->ngroups, P->Rx, P->rx_default, (const double *)P->adc_ref.p,
+                          P->launch_adc0[l], P->signal_num, (double *)P->Sx.p, (double *)P->Sy.p);
+            HIP_TRY(hipGetLastError());
+        }
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 2], P->stream));
+    }
+    (void)adc_base;
+    return MRS_OK;
+}
+
+static mrs_status reset_state(mrs_plan *P) {
+    hipStream_t st = P->stream;
+    for (int k = 0; k < 3; k++)
+        HIP_TRY(hipMemcpyAsync(P->M[k].p, P->M0[k].p, P->ntot * sizeof(float), hipMemcpyDeviceToDevice, st));
+    const size_t nsig = (size_t)std::max<int64_t>(1, P->signal_num) * P->Ty * P->Rx;
+    HIP_TRY(hipMemsetAsync(P->Sx.p, 0, nsig * sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(P->Sy.p, 0, nsig * sizeof(double), st));
+    return MRS_OK;
+}
+
+extern "C" {
+
+const char *mrs_last_error(void) { return g_err.c_str(); }
+int32_t mrs_abi_version(void) { return MRS_ABI_VERSION; }
+int32_t mrs_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return -1;
+    return n;
+}
+
+mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                           const mrs_vseq *vseq, const mrs_vvar *vvar, const mrs_vsig *vsig,
+                           const mrs_options *opt, mrs_plan **out) {
+    if (!out) return fail(MRS_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    mrs_status rc = check_args(vobj, vmag, vcoi, vctl, vseq, vvar, vsig);
+    if (rc) return rc;
+    mrs_plan *P = new (std::nothrow) mrs_plan();
+    if (!P) return fail(MRS_ERR_NOMEM, "host allocation failed");
+    if ((rc = plan_setup(P, vobj, vmag, vcoi, vctl, vseq, vsig, opt))) {
+        delete P;
+        return rc;
+    }
+    P->dec.tx = P->Tx;
+    P->dec.stride = rec_stride(P->Tx);
+    Decoder dec(vobj, vcoi, vctl, vseq, vvar, false);
+    dec.run(P->dec);
+    dec.flush_tail(P->dec);
+    if ((int64_t)P->dec.adc_ref.size() > P->signal_num) {
+        delete P;
+        return fail(MRS_ERR_ARG, "VSig.SignalNum is smaller than the number of ADC samples");
+    }
+    P->vvar_final = dec.vvar();
+    P->rf_final[0].assign(dec.vvar().rfAmp, dec.vvar().rfAmp + P->Tx);
+    P->rf_final[1].assign(dec.vvar().rfPhase, dec.vvar().rfPhase + P->Tx);
+    P->rf_final[2].assign(dec.vvar().rfFreq, dec.vvar().rfFreq + P->Tx);
+    if ((rc = upload_program(P))) {
+        delete P;
+        return rc;
+    }
+    if (hipStreamSynchronize(P->stream) != hipSuccess) {
+        delete P;
+        return fail(MRS_ERR_DEVICE, "upload failed");
+    }
+    *out = P;
+    return MRS_OK;
+}
+
+mrs_status mrs_plan_run(mrs_plan *P) {
+    if (!P) return fail(MRS_ERR_ARG, "plan is NULL");
+    HIP_TRY(hipSetDevice(P->device));
+    mrs_status rc = reset_state(P);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(P->ev0, P->stream));
+    if ((rc = execute(P, 0))) return rc;
+    HIP_TRY(hipEventRecord(P->ev1, P->stream));
+    P->ran = true;
+    return MRS_OK;
+}
+
+mrs_status mrs_plan_sync(mrs_plan *P) {
+    if (!P) return fail(MRS_ERR_ARG, "plan is NULL");
+    HIP_TRY(hipSetDevice(P->device));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    if (P->ran) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, P->ev0, P->ev1) == hipSuccess) P->last_ms = ms;
+        if (P->timing) {
+            double f = 0.0, r = 0.0;
+            for (size_t l = 0; l < P->launch_r0.size() && 3 * l + 2 < P->tev.size(); l++) {
+                float a = 0.f, b = 0.f;
+                if (hipEventElapsedTime(&a, P->tev[3 * l], P->tev[3 * l + 1]) == hipSuccess) f += a;
+                if (hipEventElapsedTime(&b, P->tev[3 * l + 1], P->tev[3 * l + 2]) == hipSuccess) r += b;
+            }
+            P->fwd_ms = f;
+            P->red_ms = r;
+        }
+        P->ran = false;
+    }
+    return MRS_OK;
+}
+
+static void copy_vvar_out(const mrs_vvar &src, const std::vector<double> *rf, int tx, mrs_vvar *dst) {
+    double *a = dst->rfAmp, *p = dst->rfPhase, *f = dst->rfFreq;
+    *dst = src;
+    dst->rfAmp = a;
+    dst->rfPhase = p;
+    dst->rfFreq = f;
+    for (int c = 0; c < tx; c++) {
+        a[c] = rf[0][c];
+        p[c] = rf[1][c];
+        f[c] = rf[2][c];
+    }
+}
+
+mrs_status mrs_plan_fetch(mrs_plan *P, mrs_vobj *vobj, mrs_vvar *vvar, mrs_vsig *vsig) {
+    if (!P) return fail(MRS_ERR_ARG, "plan is NULL");
+    mrs_status rc = mrs_plan_sync(P);
+    if (rc) return rc;
+    if (vobj) {
+        float *dst[3] = {vobj->Mx, vobj->My, vobj->Mz};
+        for (int k = 0; k < 3; k++)
+            HIP_TRY(hipMemcpy(dst[k], P->M[k].p, P->ntot * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    if (vsig) {
+        const size_t nsig = (size_t)P->signal_num * P->Ty * P->Rx;
+        std::vector<double> hx(nsig), hy(nsig);
+        if (nsig) {
+            HIP_TRY(hipMemcpy(hx.data(), P->Sx.p, nsig * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(hy.data(), P->Sy.p, nsig * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        for (size_t i = 0; i < nsig; i++) {
+            vsig->Sx[i] += hx[i];
+            vsig->Sy[i] += hy[i];
+        }
+        for (size_t n = 0; n < P->dec.kx.size(); n++) {
+            vsig->Kz[n] += P->dec.kz[n];
+            vsig->Ky[n] += P->dec.ky[n];
+            vsig->Kx[n] += P->dec.kx[n];
+        }
+        if (P->run_mode == 1) {
+            const size_t ncap = (size_t)P->ntot * P->n_uts;
+            float *dst[3] = {vsig->Mx, vsig->My, vsig->Mz};
+            // slots never written keep the caller's values: copy only captured slots
+            std::vector<char> hit(P->n_uts, 0);
+            for (auto s : P->dec.cap_slot)
+                if (s >= 0) hit[s] = 1;
+            for (int k = 0; k < 3; k++) {
+                std::vector<float> tmp(ncap);
+                HIP_TRY(hipMemcpy(tmp.data(), P->cap[k].p, ncap * sizeof(float), hipMemcpyDeviceToHost));
+                for (int64_t u = 0; u < P->n_uts; u++)
+                    if (hit[u]) std::memcpy(dst[k] + u * P->ntot, tmp.data() + u * P->ntot, P->ntot * sizeof(float));
+            }
+            for (int64_t u = 0; u < P->n_uts; u++)
+                if (hit[u]) vsig->Muts[u] = P->dec.muts[u];
+        }
+    }
+    if (vvar) copy_vvar_out(P->vvar_final, P->rf_final, P->Tx, vvar);
+    return MRS_OK;
+}
+
+mrs_status mrs_plan_get_info(const mrs_plan *P, mrs_plan_info *info) {
+    if (!P || !info) return fail(MRS_ERR_ARG, "NULL argument");
+    info->n_steps = P->dec.n_steps;
+    info->n_adc = (int64_t)P->dec.adc_ref.size();
+    info->n_iso = P->ntot;
+    info->n_launches = (int64_t)P->launch_r0.size();
+    info->n_rf_steps = P->dec.n_rf;
+    info->n_cached_steps = P->dec.n_same;
+    info->block_size = P->block;
+    info->spins_per_thread = P->spt;
+    info->grid_blocks = P->nblk;
+    info->last_run_ms = P->last_ms;
+    info->fwd_kernel_ms = P->fwd_ms;
+    info->reduce_kernel_ms = P->red_ms;
+    return MRS_OK;
+}
+
+void mrs_plan_destroy(mrs_plan *P) {
+    if (!P) return;
+    (void)hipSetDevice(P->device);
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    delete P;
+}
+
+// One call with callbacks: decode until an Ext code that needs the caller,
+// run the steps decoded so far, hand over to the callback, continue.
+static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
+                                      const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar,
+                                      mrs_vsig *vsig, const mrs_callbacks *cb, const mrs_options *opt) {
+    mrs_plan *P = new (std::nothrow) mrs_plan();
+    if (!P) return fail(MRS_ERR_NOMEM, "host allocation failed");
+    struct Guard {
+        mrs_plan *p;
+        ~Guard() { mrs_plan_destroy(p); }
+    } guard{P};
+    mrs_status rc = plan_setup(P, vobj, vmag, vcoi, vctl, vseq, vsig, opt);
+    if (rc) return rc;
+    if ((rc = reset_state(P))) return rc;
+    Decoder dec(vobj, vcoi, vctl, vseq, vvar, true);
+    Decoded all;  // accumulated k-space, muts and capture slots across chunks
+    int64_t adc_base = 0;
+    for (;;) {
+        P->dec = Decoded();
+        P->dec.tx = P->Tx;
+        P->dec.stride = rec_stride(P->Tx);
+        if (!all.muts.empty()) P->dec.muts = all.muts;
+        const DecodeStop stop = dec.run(P->dec);
+        if (stop == DECODE_END) dec.flush_tail(P->dec);
+        if (adc_base + (int64_t)P->dec.adc_ref.size() > P->signal_num)
+            return fail(MRS_ERR_ARG, "VSig.SignalNum is smaller than the number of ADC samples");
+        if ((rc = upload_program(P))) return rc;
+        // the launches of this chunk write samples adc_base + [0, n)
+        for (auto &a : P->launch_adc0) a += adc_base;
+        {
+            // adc_ref indices are chunk-local; shift the table by uploading it at its offset
+            std::vector<double> full(adc_base + P->dec.adc_ref.size());
+            std::copy(P->dec.adc_ref.begin(), P->dec.adc_ref.end(), full.begin() + adc_base);
+            if (P->adc_ref.p) (void)hipFree(P->adc_ref.p);
+            P->adc_ref.p = nullptr;
+            HIP_TRY(hipMalloc(&P->adc_ref.p, std::max<size_t>(full.size(), 1) * sizeof(double)));
+            if (!full.empty())
+                HIP_TRY(hipMemcpy(P->adc_ref.p, full.data(), full.size() * sizeof(double), hipMemcpyHostToDevice));
+            P->adc_cap = 0;
+        }
+        if ((rc = execute(P, adc_base))) return rc;
+        HIP_TRY(hipStreamSynchronize(P->stream));
+        adc_base += (int64_t)P->dec.adc_ref.size();
+        all.kx.insert(all.kx.end(), P->dec.kx.begin(), P->dec.kx.end());
+        all.ky.insert(all.ky.end(), P->dec.ky.begin(), P->dec.ky.end());
+        all.kz.insert(all.kz.end(), P->dec.kz.begin(), P->dec.kz.end());
+        all.cap_slot.insert(all.cap_slot.end(), P->dec.cap_slot.begin(), P->dec.cap_slot.end());
+        all.muts = P->dec.muts;
+        if (stop == DECODE_END) break;
+        // callback: DoGPUFetch.m decides whether the state travels to the host
+        const double ext = dec.ext_code();
+        int fetch = 0;
+        if (cb->gpu_fetch && cb->gpu_fetch(cb->user, ext, &fetch) != 0)
+            return fail(MRS_ERR_CALLBACK, "gpu_fetch callback failed");
+        if (fetch) {
+            float *dst[3] = {vobj->Mx, vobj->My, vobj->Mz};
+            for (int k = 0; k < 3; k++)
+                HIP_TRY(hipMemcpy(dst[k], P->M[k].p, P->ntot * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        if (cb->ext && cb->ext(cb->user, ext, &dec.vvar(), vobj) != 0)
+            return fail(MRS_ERR_CALLBACK, "Extended process encounters ERROR!");
+        if (fetch) {
+            const float *src[3] = {vobj->Mx, vobj->My, vobj->Mz};
+            for (int k = 0; k < 3; k++)
+                HIP_TRY(hipMemcpy(P->M[k].p, src[k], P->ntot * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+    // outputs
+    P->dec.kx.swap(all.kx);
+    P->dec.ky.swap(all.ky);
+    P->dec.kz.swap(all.kz);
+    P->dec.cap_slot.swap(all.cap_slot);
+    P->dec.muts.swap(all.muts);
+    P->vvar_final = dec.vvar();
+    P->rf_final[0].assign(dec.vvar().rfAmp, dec.vvar().rfAmp + P->Tx);
+    P->rf_final[1].assign(dec.vvar().rfPhase, dec.vvar().rfPhase + P->Tx);
+    P->rf_final[2].assign(dec.vvar().rfFreq, dec.vvar().rfFreq + P->Tx);
+    P->ran = false;
+    return mrs_plan_fetch(P, vobj, vvar, vsig);
+}
+
+mrs_status mrs_DoScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                           const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
+                           const mrs_options *opt) {
+    mrs_status rc = check_args(vobj, vmag, vcoi, vctl, vseq, vvar, vsig);
+    if (rc) return rc;
+    if (cb && (cb->ext || cb->gpu_fetch))
+        return scan_with_callbacks(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt);
+    mrs_plan *P = nullptr;
+    if ((rc = mrs_plan_create(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, opt, &P))) return rc;
+    rc = mrs_plan_run(P);
+    if (!rc) rc = mrs_plan_fetch(P, vobj, vvar, vsig);
+    mrs_plan_destroy(P);
+    return rc;
+}
+
+mrs_status mrs_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                           const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
+                           const mrs_options *opt) {
+    return mrs_DoScanAtGPU(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt);
+}
+
+}  // extern "C"

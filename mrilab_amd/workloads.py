@@ -1,0 +1,340 @@
+"""Seeded synthetic scans in MRiLab's struct layout.
+
+Each builder returns a :class:`Scan` holding the seven structs the scan
+engine reads (VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig), laid out and typed
+as Src/Main/DoPreScan.m and Src/Main/DoDataTypeConv.m leave them:
+
+* arrays are Fortran-ordered [R, C, Z, SpinNum, TypeNum];
+* Mz starts at Rho / SpinNum, Mx = My = 0 (DoPreScan.m:409-415);
+* positions follow DoPreScan.m's meshgrid: Gxgrid varies along columns,
+  Gygrid along rows, Gzgrid along slices, in metres;
+* VVar starts at zero (DoPreScan.m:517-547) with TRCount = 1
+  (DoDataTypeConv.m:125);
+* VSig.Sx/Sy have SignalNum*TypeNum*RxCoilNum entries
+  (DoDataTypeConv.m:20-30).
+
+Workloads (sizes are parameters so tests can shrink them):
+
+* :func:`gre2d` -- spoiled 2-D gradient echo, sinc excitation, uniform 10 us
+  grid (the single-slice parity case).
+* :func:`bssfp3d` -- 3-D balanced SSFP, hard pulse with alternating phase,
+  rfRef demodulation, several receive coils.
+"""
+from __future__ import annotations
+
+import copy
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import psd
+from .psd import GYRO
+
+F = "F"
+
+
+@dataclass
+class Scan:
+    VObj: dict
+    VMag: dict
+    VCoi: dict
+    VCtl: dict
+    VSeq: dict
+    VVar: dict
+    VSig: dict
+    name: str = ""
+
+    def copy(self) -> "Scan":
+        return copy.deepcopy(self)
+
+    @property
+    def structs(self):
+        return (self.VObj, self.VMag, self.VCoi, self.VCtl, self.VSeq, self.VVar, self.VSig)
+
+    @property
+    def n_iso(self) -> int:
+        return int(self.VObj["Mz"].size)
+
+    def n_steps(self) -> int:
+        """Decoded steps with dt > 0 (one per unique time point that is
+        followed by a later point of the same TR)."""
+        uts = self.VSeq["utsLine"]
+        return int(np.count_nonzero(np.diff(uts) > 0))
+
+
+def tissue_maps(shape, rng, n_types=1):
+    """T1 ~ U[0.3, 2.0] s, T2 ~ U[0.02, 0.3] s clipped below T1,
+    PD ~ U[0, 1], dB0 = 2*pi*N(0, 20 Hz)/gamma T."""
+    full = tuple(shape) + (n_types,)
+    T1 = rng.uniform(0.3, 2.0, full)
+    T2 = np.minimum(rng.uniform(0.02, 0.3, full), 0.99 * T1)
+    Rho = rng.uniform(0.0, 1.0, full)
+    dB0 = 2 * np.pi * rng.normal(0.0, 20.0, tuple(shape)) / GYRO
+    return Rho, T1, T2, dB0
+
+
+def make_structs(Rho, T1, T2, dB0, res, spin_num=1, dWRnd=None, cs=None,
+                 rx=None, tx=None, seq=None, tr_num=1, run_mode=0, max_threads=64,
+                 multitransmit=False, master_tx=1):
+    """Assemble the seven structs from maps (all [R,C,Z(,Ty)] arrays)."""
+    R, C_, Z = dB0.shape
+    Ty = Rho.shape[3]
+    S = int(spin_num)
+    f32 = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float32))
+    xv = (np.arange(C_) - C_ // 2) * res[0]
+    yv = (np.arange(R) - R // 2) * res[1]
+    zv = (np.arange(Z) - Z // 2) * res[2]
+    Gx, Gy, Gz = np.meshgrid(xv, yv, zv, indexing="xy")  # MATLAB meshgrid: x along columns
+    Mz = np.zeros((R, C_, Z, S, Ty))
+    for t in range(Ty):
+        Mz[..., t] = np.repeat((Rho[..., t].astype(np.float32).astype(np.float64) / S)[..., None], S, axis=3)
+    VObj = {"Gyro": np.float64(GYRO), "SpinNum": np.int32(S), "TypeNum": np.int32(Ty),
+            "Mx": f32(np.zeros_like(Mz)), "My": f32(np.zeros_like(Mz)), "Mz": f32(Mz),
+            "Rho": f32(Rho), "T1": f32(T1), "T2": f32(T2),
+            "ChemShift": np.float64(0.0)}
+    if dWRnd is None:
+        dWRnd = np.zeros((R, C_, Z, S, Ty))
+    VMag = {"dB0": f32(dB0), "dWRnd": f32(dWRnd), "Gzgrid": f32(Gz), "Gygrid": f32(Gy),
+            "Gxgrid": f32(Gx), "FRange": np.ones((R, C_, Z))}
+    if tx is None:
+        VCoi = {"TxCoilmg": f32(np.ones((R, C_, Z))), "TxCoilpe": f32(np.zeros((R, C_, Z))),
+                "TxCoilNum": np.int32(1), "TxCoilDefault": np.float64(1.0)}
+    else:
+        mg, pe = tx
+        VCoi = {"TxCoilmg": f32(mg), "TxCoilpe": f32(pe), "TxCoilNum": np.int32(mg.shape[3] if mg.ndim == 4 else 1),
+                "TxCoilDefault": np.float64(0.0)}
+    if rx is None:
+        VCoi.update({"RxCoilx": f32(np.ones((R, C_, Z))), "RxCoily": f32(np.zeros((R, C_, Z))),
+                     "RxCoilNum": np.int32(1), "RxCoilDefault": np.float64(1.0)})
+    else:
+        cx, cy = rx
+        VCoi.update({"RxCoilx": f32(cx), "RxCoily": f32(cy), "RxCoilNum": np.int32(cx.shape[3]),
+                     "RxCoilDefault": np.float64(0.0)})
+    if cs is None:
+        cs = np.zeros(Ty)
+    VCtl = {"CS": np.asarray(cs, dtype=np.float64), "TRNum": np.int32(tr_num),
+            "RunMode": np.int32(run_mode), "MaxThreadNum": np.int32(max_threads),
+            "GPUIndex": np.int32(0), "MultiTransmit": "on" if multitransmit else "off",
+            "MasterTxCoil": int(master_tx)}
+    Tx = int(VCoi["TxCoilNum"])
+    VVar = {"rfAmp": np.zeros(Tx), "rfPhase": np.zeros(Tx), "rfFreq": np.zeros(Tx),
+            "rfCoil": np.float64(0), "rfRef": np.float64(0), "GzAmp": np.float64(0),
+            "GyAmp": np.float64(0), "GxAmp": np.float64(0), "ADC": np.float64(0),
+            "Ext": np.float64(0), "t": np.float64(0), "dt": np.float64(0),
+            "Kz": np.float64(0), "Ky": np.float64(0), "Kx": np.float64(0),
+            "KzKeeper": np.float64(0), "KyKeeper": np.float64(0), "KxKeeper": np.float64(0),
+            "gpuFetch": np.float64(0)}
+    for k in ("rfi", "Gzi", "Gyi", "Gxi", "ADCi", "Exti", "utsi"):
+        VVar[k] = np.int32(0)
+    VVar["TRCount"] = np.int32(1)
+    N = int(np.count_nonzero(seq["ADCLine"] == 1))
+    Rx = int(VCoi["RxCoilNum"])
+    VSig = {"Sx": np.zeros(N * Ty * Rx), "Sy": np.zeros(N * Ty * Rx), "Kz": np.zeros(N),
+            "Ky": np.zeros(N), "Kx": np.zeros(N), "SignalNum": np.int32(N)}
+    if run_mode == 1:
+        nuts = seq["utsLine"].size
+        shape6 = (R, C_, Z, S, Ty, nuts)
+        VSig["Mx"] = np.zeros(shape6, dtype=np.float32, order=F)
+        VSig["My"] = np.zeros(shape6, dtype=np.float32, order=F)
+        VSig["Mz"] = np.zeros(shape6, dtype=np.float32, order=F)
+        VSig["Muts"] = np.zeros(nuts)
+    else:
+        VSig["Mx"] = VSig["My"] = VSig["Mz"] = np.zeros(1, dtype=np.float32)
+        VSig["Muts"] = np.zeros(1)
+    return Scan(VObj, VMag, VCoi, VCtl, seq, VVar, VSig)
+
+
+# --------------------------------------------------------------------------
+# Sequences on an exact time grid
+# --------------------------------------------------------------------------
+
+def _trap_breaks(k0, k1, ramp, amp):
+    """Breakpoints (grid indices) of a trapezoid starting at k0, ending at k1."""
+    return [k0, k0 + ramp, k1 - ramp, k1], [0.0, amp, amp, 0.0]
+
+
+def _pwl_on_grid(segments, n):
+    """Sum of trapezoids, each (k0, k1, ramp, amp), sampled at grid 0..n."""
+    g = np.zeros(n + 1)
+    kk = np.arange(n + 1)
+    for k0, k1, ramp, amp in segments:
+        bt, bg = _trap_breaks(k0, k1, ramp, amp)
+        g += np.interp(kk, bt, bg, left=0.0, right=0.0)
+    return g
+
+
+def _held_area_grid(g, dt):
+    a = np.where(np.abs(g[:-1]) <= np.abs(g[1:]), g[:-1], g[1:])
+    return np.sum(a) * dt
+
+
+def gre2d(n_pe=64, n_ro=64, fov=0.256, tr_steps=1000, te_steps=500, dt=10e-6, fa=30.0,
+          rf_steps=100, tbp=4.0, seed=0, spoil=True, spins=1) -> Scan:
+    """Spoiled 2-D gradient echo on a uniform grid of ``dt``.
+
+    64x64x1 phantom, 1 isochromat per voxel, TR = tr_steps*dt (10 ms),
+    TE = te_steps*dt (5 ms), sinc excitation (TBW 4, 1 ms), n_pe phase
+    encodes of n_ro samples, one uniform receive coil. Every grid point
+    carries a gradient sample, so utsLine is the uniform grid.
+    """
+    rng = np.random.default_rng(seed)
+    shape = (n_pe, n_ro, 1)
+    Rho, T1, T2, dB0 = tissue_maps(shape, rng)
+    res = (fov / n_ro, fov / n_pe, 5e-3)
+    gbar = GYRO / (2 * np.pi)
+    dk_x, dk_y = 1.0 / fov, 1.0 / fov
+    g_ro = dk_x / (gbar * dt)
+    n = tr_steps
+    k_ro0 = te_steps - n_ro // 2            # first ADC grid point
+    k_ro1 = k_ro0 + n_ro - 1                # last ADC grid point
+    ramp = 10
+    ro_seg = (k_ro0 - 4 - ramp, k_ro1 + 4 + ramp, ramp, g_ro)
+    pre_k0, pre_k1 = rf_steps + 10, rf_steps + 110
+    # prephaser amplitude such that the held k-space is 0 at TE
+    g_ro_w = _pwl_on_grid([ro_seg], n)
+    area_ro_to_te = _held_area_grid(g_ro_w[: te_steps + 1], dt)
+    g_unit = _pwl_on_grid([(pre_k0, pre_k1, ramp, 1.0)], n)
+    area_unit = _held_area_grid(g_unit, dt)
+    g_pre = -area_ro_to_te / area_unit
+    gx = _pwl_on_grid([(pre_k0, pre_k1, ramp, g_pre), ro_seg], n)
+    gz_ss = _pwl_on_grid([(0, rf_steps, 5, 2e-3), (rf_steps + 5, rf_steps + 55, 5, -2e-3 * 0.5 * (rf_steps - 5) / 45)], n)
+    times = np.arange(n + 1) * dt
+    trs = []
+    for j in range(n_pe):
+        ky = (j - n_pe // 2) * dk_y
+        gy = _pwl_on_grid([(pre_k0, pre_k1, ramp, ky / (gbar * area_unit))], n)
+        tr = psd.TR(duration=n * dt)
+        tr.add_rf(psd.rf_sinc(times[: rf_steps + 1], fa, tbp=tbp))
+        tr.add_gz(times, gz_ss)
+        tr.add_gy(times, gy)
+        tr.add_gx(times, gx)
+        adc_k = np.arange(k_ro0 - 1, k_ro1 + 2)
+        adc_v = np.ones(adc_k.size); adc_v[0] = adc_v[-1] = 0
+        tr.add_adc(times[adc_k], adc_v)
+        e, et = psd.ext_bit(0, dt, 0)
+        tr.add_ext(times[[rf_steps, rf_steps + 1, rf_steps + 2]], [0, 1, 0])     # reset k (Ext 1)
+        if spoil:
+            k = n - 50
+            tr.add_ext(times[[k, k + 1, k + 2]], [0, 6, 0])                     # ideal spoiler (Ext 6)
+        trs.append(tr.finalize())
+    seq = psd.concat(trs)
+    dWRnd = None
+    if spins > 1:
+        q = np.linspace(0.01, 0.99, spins)
+        dWRnd = np.zeros(shape + (spins, 1))
+        for s in range(spins):
+            dWRnd[..., s, 0] = 50.0 * np.tan(np.pi * (q[s] - 0.5))   # DoPreScan.m:381-390 with 1/T2*-1/T2 = 50/s
+    sc = make_structs(Rho, T1, T2, dB0, res, spin_num=spins, dWRnd=dWRnd, seq=seq, tr_num=n_pe)
+    sc.name = f"gre2d_{n_pe}x{n_ro}_S{spins}"
+    return sc
+
+
+def gaussian_coils(shape, res, n_coils, rng, sigma_frac=0.4):
+    """Receive coils: Gaussian blobs (sigma = sigma_frac * FOV) centred on a
+    ring of radius FOV/2 in the central slice, each with a random phase."""
+    R, C_, Z = shape
+    xv = (np.arange(C_) - C_ // 2) * res[0]
+    yv = (np.arange(R) - R // 2) * res[1]
+    zv = (np.arange(Z) - Z // 2) * res[2]
+    X, Y, Zg = np.meshgrid(xv, yv, zv, indexing="xy")
+    fov = max(C_ * res[0], R * res[1])
+    sig = sigma_frac * fov
+    cx = np.zeros(shape + (n_coils,))
+    cy = np.zeros(shape + (n_coils,))
+    for c in range(n_coils):
+        ang = 2 * np.pi * c / n_coils
+        x0, y0 = 0.5 * fov * np.cos(ang), 0.5 * fov * np.sin(ang)
+        mag = np.exp(-((X - x0) ** 2 + (Y - y0) ** 2 + Zg ** 2) / (2 * sig ** 2))
+        ph = rng.uniform(-np.pi, np.pi)
+        cx[..., c] = mag * np.cos(ph)
+        cy[..., c] = mag * np.sin(ph)
+    return cx, cy
+
+
+def bssfp3d(shape=(128, 128, 64), n_ro=128, n_pe_y=None, n_pe_z=None, spins=4, n_coils=8,
+            tr_steps=500, rf_steps=10, dt=10e-6, fa=45.0, seed=1, n_tr=None,
+            fov=(0.256, 0.256, 0.128)) -> Scan:
+    """3-D balanced SSFP on a uniform grid of ``dt``.
+
+    Phantom [R, C, Z] with ``spins`` isochromats per voxel spread uniformly
+    over +-pi of intravoxel dephasing per TR; hard ``fa`` pulse of
+    ``rf_steps`` grid steps whose phase alternates 0/pi; Ext 7 at the pulse
+    centre sets rfRef so the signal is demodulated; n_pe_y x n_pe_z phase
+    encodes (default: rows x slices) of ``n_ro`` samples at TE = TR/2;
+    ``n_coils`` Gaussian receive coils. ``n_tr`` truncates the scan to its
+    first TRs (for tests and bounded CPU baselines).
+    """
+    rng = np.random.default_rng(seed)
+    R, C_, Z = shape
+    n_pe_y = R if n_pe_y is None else n_pe_y
+    n_pe_z = Z if n_pe_z is None else n_pe_z
+    Rho, T1, T2, dB0 = tissue_maps(shape, rng)
+    res = (fov[0] / C_, fov[1] / R, fov[2] / Z)
+    gbar = GYRO / (2 * np.pi)
+    n = tr_steps
+    te = n // 2
+    TR = n * dt
+    ramp = 10
+    k_ro0 = te - n_ro // 2
+    k_ro1 = k_ro0 + n_ro - 1
+    g_ro = (1.0 / fov[0]) / (gbar * dt)
+    ro_seg = (k_ro0 - 2 - ramp, k_ro1 + 2 + ramp, ramp, g_ro)
+    pre = (rf_steps + 5, k_ro0 - 2 - ramp - 2)          # prephase window
+    post = (k_ro1 + 2 + ramp + 2, n - 5)                 # rewind window
+    g_ro_w = _pwl_on_grid([ro_seg], n)
+    area_ro_to_te = _held_area_grid(g_ro_w[: te + 1], dt)
+    area_ro_tot = _held_area_grid(g_ro_w, dt)
+    area_pre_unit = _held_area_grid(_pwl_on_grid([(pre[0], pre[1], ramp, 1.0)], n), dt)
+    area_post_unit = _held_area_grid(_pwl_on_grid([(post[0], post[1], ramp, 1.0)], n), dt)
+    g_pre = -area_ro_to_te / area_pre_unit
+    g_post = -(area_ro_tot - area_ro_to_te) / area_post_unit
+    gx = _pwl_on_grid([(pre[0], pre[1], ramp, g_pre), ro_seg, (post[0], post[1], ramp, g_post)], n)
+    times = np.arange(n + 1) * dt
+    total = n_pe_y * n_pe_z
+    count = total if n_tr is None else min(n_tr, total)
+    # Every TR has the same event times; only the phase-encode amplitudes
+    # and the rf phase change. Finalize one template TR (unit phase-encode
+    # amplitude, rf phase pi) and scale its lines per TR.
+    unit_y = _pwl_on_grid([(pre[0], pre[1], ramp, 1.0), (post[0], post[1], ramp, -area_pre_unit / area_post_unit)], n)
+    tr = psd.TR(duration=TR)
+    tr.add_rf(psd.rf_rect(times[: rf_steps + 1], fa, phase=np.pi))
+    tr.add_gz(times, unit_y)
+    tr.add_gy(times, unit_y)
+    tr.add_gx(times, gx)
+    adc_k = np.arange(k_ro0 - 1, k_ro1 + 2)
+    adc_v = np.ones(adc_k.size); adc_v[0] = adc_v[-1] = 0
+    tr.add_adc(times[adc_k], adc_v)
+    kc = rf_steps // 2
+    tr.add_ext(times[[kc - 1, kc, kc + 1]], [0, 7, 0])   # rfRef at the pulse centre (Ext 7)
+    tpl = tr.finalize()
+    it = np.arange(count)
+    jz, jy = np.divmod(it, n_pe_y)
+    ay = (jy - n_pe_y // 2) / fov[1] / (gbar * area_pre_unit)
+    az = (jz - n_pe_z // 2) / fov[2] / (gbar * area_pre_unit)
+    par = (it % 2).astype(np.float64)          # rf phase 0 / pi alternates
+    tile = lambda a: np.tile(a, count)
+    seq = {
+        "rfAmpLine": tile(tpl["rfAmp"]),
+        "rfPhaseLine": (par[:, None] * tpl["rfPhase"][None, :]).ravel(),
+        "rfFreqLine": tile(tpl["rfFreq"]),
+        "rfCoilLine": np.ones(count * tpl["rfCoil"].size),
+        "GzAmpLine": (az[:, None] * tpl["Gz"][None, :]).ravel(),
+        "GyAmpLine": (ay[:, None] * tpl["Gy"][None, :]).ravel(),
+        "GxAmpLine": tile(tpl["Gx"]),
+        "ADCLine": tile(tpl["ADC"]),
+        "ExtLine": tile(tpl["Ext"]),
+        "utsLine": tile(tpl["uts"]),
+        "tsLine": tile(tpl["ts"]),
+        "flagsLine": np.asfortranarray(np.tile(tpl["flags"], (1, count))),
+    }
+    for k_, v_ in seq.items():
+        seq[k_] = np.asfortranarray(v_, dtype=np.float64)
+    # uniform +-pi intravoxel spread per TR
+    dWRnd = np.zeros(shape + (spins, 1))
+    for s in range(spins):
+        dWRnd[..., s, 0] = (-np.pi + 2 * np.pi * (s + 0.5) / spins) / TR
+    cx, cy = gaussian_coils(shape, res, n_coils, rng)
+    sc = make_structs(Rho, T1, T2, dB0, res, spin_num=spins, dWRnd=dWRnd, seq=seq,
+                      rx=(cx, cy), tr_num=count)
+    sc.name = f"bssfp3d_{R}x{C_}x{Z}_S{spins}_Rx{n_coils}_TR{count}"
+    return sc

@@ -1,0 +1,474 @@
+/*
+ * bloch_oracle.c -- CPU oracle of the single-pool Bloch scan.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker or the timed CPU baseline, never as part of the product path.
+ *
+ * It restates MRiLab's CPU engine:
+ *   event loop   Lib/src/engine/CPUEngine/DoScanAtCPU.cpp:246-553
+ *   Bloch step   Lib/src/engine/CPUEngine/BlochKernel.h:7-397
+ *   Ext plugins  Src/Main/DoExtPlugin.m:11-33, Src/Plugin/Plugin_*.m
+ * The same source is built twice:
+ *   ORC_REAL=float  -> liborc_f32.so: every vector pass of BlochKernel.h is
+ *                      one rounded fp32 operation in the reference's order
+ *                      (libm in place of IPP's A24 functions; exp rounded
+ *                      correctly from fp64).
+ *   ORC_REAL=double -> liborc_f64.so: the same discrete scheme in fp64.
+ * The reference itself cannot be built here (it needs mex.h and Intel IPP,
+ * neither of which exists in this environment).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "../include/mrilab_amd.h"
+
+#ifndef ORC_REAL
+#define ORC_REAL float
+#endif
+typedef ORC_REAL real;
+
+#define ORC_PI 3.14159265359 /* DoScanAtCPU.cpp:63 */
+
+static int g_threads = 0; /* 0: min(slices, MaxThreadNum) like the reference */
+static int g_relax = 0;   /* 0: reference order; 1: expm1 form (error study only) */
+
+void orc_set_threads(int n) { g_threads = n; }
+void orc_set_relax_mode(int m) { g_relax = m; }
+
+int orc_real_size(void) { return (int)sizeof(real); }
+
+static real r_sin(real x) { return sizeof(real) == 4 ? (real)sinf((float)x) : (real)sin((double)x); }
+static real r_cos(real x) { return sizeof(real) == 4 ? (real)cosf((float)x) : (real)cos((double)x); }
+static real r_atan(real x) { return sizeof(real) == 4 ? (real)atanf((float)x) : (real)atan((double)x); }
+static real r_atan2(real y, real x) { return sizeof(real) == 4 ? (real)atan2f((float)y, (float)x) : (real)atan2((double)y, (double)x); }
+static real r_sqrt(real x) { return sizeof(real) == 4 ? (real)sqrtf((float)x) : (real)sqrt((double)x); }
+/* exp of the relaxation factors: fp32 build rounds exp correctly (from
+ * fp64), the canonical fp32 exp; IPP's ippsExp_32f_I is not available and
+ * any accurate fp32 exp agrees with it except near rounding midpoints. */
+static real r_exp(real x) { return (real)exp((double)x); }
+
+/*
+ * One Bloch step for the voxels of one slice of one (type, spin).
+ * BlochKernel.h:7-397; every statement below is one elementwise IPP pass
+ * of the reference, applied per element.
+ */
+static void orc_bloch_slice(real Gyro, const double *CS, int SpinNum,
+                            const float *Rho, const float *T1, const float *T2,
+                            float *Mz, float *My, float *Mx,
+                            const float *dB0, const float *dWRnd,
+                            const float *Gzgrid, const float *Gygrid, const float *Gxgrid,
+                            const float *TxCoilmg, const float *TxCoilpe,
+                            real dt, const double *rfAmp, const double *rfPhase, const double *rfFreq,
+                            real GzAmp, real GyAmp, real GxAmp,
+                            int Typei, int64_t SpinMxNum, int64_t SpinMxSliceNum, int TxCoilNum,
+                            real *stMx, real *stMy, real *stMz)
+{
+    real rfAmpSum = 0;
+    for (int i = 0; i < TxCoilNum; i++)
+        rfAmpSum += (real)fabs((double)(real)rfAmp[i]);          /* :54-56 */
+    const real csterm = (real)((double)((real)CS[Typei] * 2) * ORC_PI); /* :42 */
+    const real gzg = GzAmp * Gyro, gyg = GyAmp * Gyro, gxg = GxAmp * Gyro;
+
+    for (int64_t v = 0; v < SpinMxNum; v++) {
+        real mx = stMx ? stMx[v] : (real)Mx[v];
+        real my = stMy ? stMy[v] : (real)My[v];
+        real mz = stMz ? stMz[v] : (real)Mz[v];
+        real nmx, nmy, nmz = mz;
+        /* E1 off-resonance, :38-51 */
+        real dW = (real)dB0[v] * Gyro;
+        dW = dW + (real)dWRnd[v];
+        dW = dW + csterm;
+        real buf = (real)Gzgrid[v] * gzg; dW = dW + buf;
+        buf = (real)Gygrid[v] * gyg; dW = dW + buf;
+        buf = (real)Gxgrid[v] * gxg; dW = dW + buf;
+
+        if (rfAmpSum != 0) {
+            real b1, b2;
+            if (TxCoilNum == 1) {                                   /* :58-72 */
+                real A = (real)rfAmp[0], P = (real)rfPhase[0], F = (real)rfFreq[0];
+                if (A < 0) { A = (real)fabs((double)A); P = (real)((double)P + ORC_PI); }
+                b1 = (real)TxCoilmg[v] * A;
+                b2 = (real)TxCoilpe[v] + P;
+                dW = dW + (real)((double)(F * 2) * ORC_PI);
+                b1 = b1 * Gyro;
+                b2 = b2 * -1;
+            } else {                                                /* :73-121 */
+                real b5 = 0, b6 = 0;
+                for (int i = 0; i < TxCoilNum; i++) {
+                    real A = (real)rfAmp[i], P = (real)rfPhase[i], F = (real)rfFreq[i];
+                    if (A != 0) {
+                        if (A < 0) { A = (real)fabs((double)A); P = (real)((double)P + ORC_PI); }
+                        real m = (real)TxCoilmg[v + i * SpinMxSliceNum * SpinMxNum] * A;
+                        real ph = (real)TxCoilpe[v + i * SpinMxSliceNum * SpinMxNum] + P;
+                        real re = m * r_cos(ph), im = m * r_sin(ph);   /* PolarToCart */
+                        b5 = b5 + re; b6 = b6 + im;
+                        dW = dW + (real)((double)(F * 2) * ORC_PI);
+                    }
+                }
+                b1 = r_sqrt(b5 * b5 + b6 * b6);                     /* CartToPolar */
+                b2 = r_atan2(b6, b5);
+                b1 = b1 * Gyro;
+                b2 = b2 * -1;
+            }
+            /* beta = atan(dW / b1), alpha = sqrt(dW^2 + b1^2) * dt, :124-133 */
+            real q = dW / b1;
+            real beta = r_atan(q);
+            real s2 = dW * dW; b1 = b1 * b1; s2 = b1 + s2; s2 = r_sqrt(s2);
+            real alpha = s2 * dt;
+            real sA = r_sin(alpha), sB = r_sin(beta), cA = r_cos(alpha), cB = r_cos(beta);
+            real sP = r_sin(b2), msP = sP * -1, cP = r_cos(b2), mcP = cP * -1;
+            real t1, t2, t3, t4, acc;
+            /* Mx, :146-196 */
+            t1 = sB * cA; t1 = mcP * t1; t3 = sA * sP; t1 = t3 + t1; t1 = sB * t1; t1 = t1 * -1;
+            t3 = cB * cB; t3 = cP * t3; t1 = t3 + t1; t3 = t1;
+            t1 = cP * t1; t3 = sP * t3;
+            t2 = sA * sB; t2 = cP * t2; t4 = cA * sP; t2 = t4 + t2; t4 = t2;
+            t2 = sP * t2; t4 = mcP * t4;
+            t2 = t1 + t2; t2 = mx * t2;
+            t4 = t3 + t4; t4 = my * t4; t4 = t4 * -1;
+            acc = t2 + t4;
+            t1 = sB * cB; t1 = cP * t1;
+            t2 = sB * cA; t2 = mcP * t2;
+            t3 = sA * sP;
+            t2 = t3 + t2; t2 = cB * t2;
+            t1 = t2 + t1; t1 = mz * t1;
+            nmx = acc + t1;
+            /* My, :205-254 */
+            t1 = cA * sB; t1 = sP * t1; t3 = sA * cP; t1 = t3 + t1; t1 = sB * t1;
+            t3 = cB * cB; t3 = sP * t3; t1 = t3 + t1; t3 = t1;
+            t1 = sP * t1; t3 = cP * t3;
+            t2 = sA * sB; t2 = msP * t2; t4 = cA * cP; t2 = t4 + t2; t4 = t2;
+            t2 = cP * t2; t4 = msP * t4;
+            t2 = t1 + t2; t2 = my * t2;
+            t4 = t3 + t4; t4 = mx * t4; t4 = t4 * -1;
+            acc = t2 + t4;
+            t1 = cB * sB; t1 = msP * t1;
+            t2 = cA * sB; t2 = sP * t2;
+            t3 = sA * cP;
+            t2 = t3 + t2; t2 = cB * t2;
+            t1 = t2 + t1; t1 = mz * t1;
+            nmy = acc + t1;
+            /* Mz, :263-299 */
+            t1 = cA * cB; t1 = sB * t1; t1 = t1 * -1;
+            t3 = cB * sB; t1 = t3 + t1; t3 = t1;
+            t1 = cP * t1; t3 = sP * t3;
+            t2 = cB * sA; t2 = msP * t2;
+            t4 = cB * sA; t4 = cP * t4;
+            t2 = t1 + t2; t2 = mx * t2;
+            t4 = t3 + t4; t4 = my * t4; t4 = t4 * -1;
+            acc = t2 + t4;
+            t1 = sB * sB;
+            t2 = cB * cB; t2 = cA * t2;
+            t1 = t2 + t1; t1 = mz * t1;
+            nmz = acc + t1;
+        } else {
+            /* free precession, :314-343 */
+            real alpha = dW * dt;
+            real sA = r_sin(alpha), cA = r_cos(alpha);
+            real t1 = mx * cA, t2 = my * sA;
+            nmx = t1 + t2;
+            real t3 = my * cA, t4 = mx * sA; t4 = t4 * -1;
+            nmy = t3 + t4;
+        }
+        /* relaxation, :346-368 */
+        real e2 = (real)1 / (real)T2[v]; e2 = e2 * dt; e2 = e2 * -1;
+        if (e2 < -200) e2 = -200;
+        e2 = r_exp(e2);
+        real e1 = (real)1 / (real)T1[v]; e1 = e1 * dt; e1 = e1 * -1;
+        if (e1 < -200) e1 = -200;
+        e1 = r_exp(e1);
+        if (g_relax == 0) {
+            nmx = e2 * nmx;
+            nmy = e2 * nmy;
+            real r3 = e1 * -1; r3 = r3 + 1;
+            real r4 = (real)Rho[v] / (real)SpinNum;
+            r3 = r4 * r3;
+            r4 = nmz * e1;
+            nmz = r3 + r4;
+        } else {
+            /* M' = M - M*d with d = -expm1(-dt/T) computed without the
+             * cancellation of 1 - exp(-dt/T); not the reference's order. */
+            real d2 = (real)-expm1(-(double)dt / (double)T2[v]);
+            real d1 = (real)-expm1(-(double)dt / (double)T1[v]);
+            real m0 = (real)Rho[v] / (real)SpinNum;
+            nmx = nmx - nmx * d2;
+            nmy = nmy - nmy * d2;
+            nmz = nmz + (m0 - nmz) * d1;
+        }
+        if (stMx) { stMx[v] = nmx; stMy[v] = nmy; stMz[v] = nmz; }
+        else { Mx[v] = (float)nmx; My[v] = (float)nmy; Mz[v] = (float)nmz; }
+    }
+}
+
+/* fp32 sum in 16 interleaved partial sums, like a vectorised ippsSum_32f
+ * with ippAlgHintFast; fp64 build: plain fp64 sum. */
+static double orc_sum(const real *x, int64_t n)
+{
+    if (sizeof(real) == 4) {
+        float p[16] = {0};
+        for (int64_t i = 0; i < n; i++) p[i & 15] += (float)x[i];
+        for (int w = 8; w >= 1; w >>= 1)
+            for (int i = 0; i < w; i++) p[i] += p[i + w];
+        return (double)p[0];
+    }
+    double s = 0;
+    for (int64_t i = 0; i < n; i++) s += (double)x[i];
+    return s;
+}
+
+typedef struct orc_ctx {
+    mrs_vobj *vobj; const mrs_vmag *vmag; const mrs_vcoi *vcoi; const mrs_vctl *vctl;
+    const mrs_vseq *vseq; mrs_vvar *vvar; mrs_vsig *vsig; const mrs_callbacks *cb;
+    int64_t nvox;   /* R*C*Z */
+    int64_t nxy;    /* R*C */
+    int64_t nz;
+    real *wMx, *wMy, *wMz; /* fp64 build keeps the state in double */
+} orc_ctx;
+
+/* Ext plugins executed natively: Src/Main/DoExtPlugin.m:11-33 */
+static int orc_ext(orc_ctx *c, double ext)
+{
+    mrs_vvar *V = c->vvar;
+    int64_t ntot = c->nvox * c->vobj->SpinNum * c->vobj->TypeNum;
+    switch ((int)ext) {
+    case 1: V->Kz = 0; V->Ky = 0; V->Kx = 0; return 0;              /* Plugin_ResetK.m:7-9 */
+    case 2: V->Kz = -V->Kz; V->Ky = -V->Ky; V->Kx = -V->Kx; return 0; /* Plugin_ReverseK.m */
+    case 3: V->KzKeeper = V->Kz; V->KyKeeper = V->Ky; V->KxKeeper = V->Kx; return 0; /* LockK */
+    case 4: V->Kz = V->KzKeeper; V->Ky = V->KyKeeper; V->Kx = V->KxKeeper; return 0; /* ReleaseK */
+    case 6:                                                       /* Plugin_IdealSpoiler.m:10-11 */
+        for (int64_t i = 0; i < ntot; i++) {
+            if (c->wMx) { c->wMx[i] = 0; c->wMy[i] = 0; }
+            c->vobj->Mx[i] = 0; c->vobj->My[i] = 0;
+        }
+        return 0;
+    case 7:                                                       /* Plugin_rfRef.m:7-12 */
+        if (c->vctl->MultiTransmit == 0) V->rfRef = V->rfPhase[0];
+        else V->rfRef = V->rfPhase[c->vctl->MasterTxCoil - 1];
+        return 0;
+    default:
+        if (c->cb && c->cb->ext) {
+            if (c->wMx)
+                for (int64_t i = 0; i < ntot; i++) {
+                    c->vobj->Mx[i] = (float)c->wMx[i]; c->vobj->My[i] = (float)c->wMy[i];
+                    c->vobj->Mz[i] = (float)c->wMz[i];
+                }
+            int rc = c->cb->ext(c->cb->user, ext, V, c->vobj);
+            if (c->wMx)
+                for (int64_t i = 0; i < ntot; i++) {
+                    c->wMx[i] = c->vobj->Mx[i]; c->wMy[i] = c->vobj->My[i]; c->wMz[i] = c->vobj->Mz[i];
+                }
+            return rc;
+        }
+        return 0;
+    }
+}
+
+static void orc_capture(orc_ctx *c)
+{
+    /* RunMode 1 copy of the full state into VSig.M*[utsi], DoScanAtCPU.cpp:433-435,546-548 */
+    int64_t ntot = c->nvox * c->vobj->SpinNum * c->vobj->TypeNum;
+    int64_t off = (int64_t)c->vvar->utsi * ntot;
+    for (int64_t i = 0; i < ntot; i++) {
+        c->vsig->Mx[off + i] = c->wMx ? (float)c->wMx[i] : c->vobj->Mx[i];
+        c->vsig->My[off + i] = c->wMy ? (float)c->wMy[i] : c->vobj->My[i];
+        c->vsig->Mz[off + i] = c->wMz ? (float)c->wMz[i] : c->vobj->Mz[i];
+    }
+}
+
+/*
+ * The scan: DoScanAtCPU.cpp:246-553 with the MEX globals replaced by the
+ * ABI structs. Returns 0 on success, MRS_ERR_CALLBACK if a callback failed.
+ */
+int orc_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
+                    const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar,
+                    mrs_vsig *vsig, const mrs_callbacks *cb)
+{
+    orc_ctx C;
+    memset(&C, 0, sizeof C);
+    C.vobj = vobj; C.vmag = vmag; C.vcoi = vcoi; C.vctl = vctl; C.vseq = vseq;
+    C.vvar = vvar; C.vsig = vsig; C.cb = cb;
+    const int64_t SpinMxNum = (int64_t)vobj->dims[0] * vobj->dims[1];
+    const int64_t SpinMxSliceNum = vobj->dims[2] > 0 ? vobj->dims[2] : 1;
+    C.nxy = SpinMxNum; C.nz = SpinMxSliceNum; C.nvox = SpinMxNum * SpinMxSliceNum;
+    const int S = vobj->SpinNum, Ty = vobj->TypeNum, Tx = vcoi->TxCoilNum, Rx = vcoi->RxCoilNum;
+    const int64_t ntot = C.nvox * S * Ty;
+    const int64_t SignalNum = vsig->SignalNum;
+    const real Gyro = (real)vobj->Gyro;
+
+    if (sizeof(real) == 8) {       /* fp64 state */
+        C.wMx = (real *)malloc(sizeof(real) * ntot);
+        C.wMy = (real *)malloc(sizeof(real) * ntot);
+        C.wMz = (real *)malloc(sizeof(real) * ntot);
+        for (int64_t i = 0; i < ntot; i++) { C.wMx[i] = vobj->Mx[i]; C.wMy[i] = vobj->My[i]; C.wMz[i] = vobj->Mz[i]; }
+    }
+    real *buf1 = (real *)malloc(sizeof(real) * C.nvox);
+    real *buf3 = (real *)malloc(sizeof(real) * C.nvox);
+
+    int nthreads = g_threads > 0 ? g_threads
+                 : (SpinMxSliceNum < vctl->MaxThreadNum ? (int)SpinMxSliceNum : vctl->MaxThreadNum); /* :225-228 */
+    if (nthreads < 1) nthreads = 1;
+
+    const double *tsLine = vseq->tsLine, *flagsLine = vseq->flagsLine, *utsLine = vseq->utsLine;
+    const int64_t MaxStep = vseq->n_ts, MaxutsStep = vseq->n_uts, MaxrfStep = vseq->n_rf;
+    const int64_t MaxGzStep = vseq->n_Gz, MaxGyStep = vseq->n_Gy, MaxGxStep = vseq->n_Gx;
+    int64_t i = 0, Signali = 0;
+    double flag[6];
+    int status = 0;
+
+    while (i < MaxStep - 1) {                                        /* :246 */
+        for (int k = 0; k < 6; k++) flag[k] = 0;
+        if (tsLine[i] != tsLine[i + 1]) {                           /* :254-284 */
+            for (int k = 0; k < 6; k++) flag[k] += flagsLine[i * 6 + k];
+            i++;
+        } else {
+            for (int k = 0; k < 6; k++) flag[k] += flagsLine[i * 6 + k];
+            while (tsLine[i] == tsLine[i + 1]) {
+                for (int k = 0; k < 6; k++) flag[k] += flagsLine[(i + 1) * 6 + k];
+                i++;
+                if (i == MaxStep - 1) break;
+            }
+            i++;
+        }
+        /* :287-289 */
+        vvar->t = utsLine[vvar->utsi];
+        int64_t un = vvar->utsi + 1 < MaxutsStep - 1 ? vvar->utsi + 1 : MaxutsStep - 1;
+        vvar->dt = utsLine[un] - utsLine[vvar->utsi];
+        vvar->utsi = (int32_t)un;
+
+        if (flag[0] >= 1) {                                          /* :291-320 */
+            for (int64_t j = 0; j < flag[0]; j++) {
+                vvar->rfCoil = vseq->rfCoilLine[vvar->rfi];
+                int TxCoili = (int)vvar->rfCoil;
+                int64_t s = vvar->rfi + 1;
+                while (s < MaxrfStep) {
+                    if (vvar->rfCoil == vseq->rfCoilLine[s]) {
+                        const double *A = vseq->rfAmpLine, *P = vseq->rfPhaseLine, *F = vseq->rfFreqLine;
+                        vvar->rfAmp[TxCoili - 1] = fabs(A[vvar->rfi]) <= fabs(A[s]) ? A[vvar->rfi] : A[s];
+                        vvar->rfPhase[TxCoili - 1] = fabs(P[vvar->rfi]) <= fabs(P[s]) ? P[vvar->rfi] : P[s];
+                        vvar->rfFreq[TxCoili - 1] = fabs(F[vvar->rfi]) <= fabs(F[s]) ? F[vvar->rfi] : F[s];
+                        break;
+                    }
+                    s++;
+                }
+                vvar->rfi++;
+            }
+        }
+        if (flag[1] == 1) {                                          /* :322-329 */
+            const double *L = vseq->GzAmpLine; int64_t n1 = vvar->Gzi + 1 < MaxGzStep - 1 ? vvar->Gzi + 1 : MaxGzStep - 1;
+            vvar->GzAmp = fabs(L[vvar->Gzi]) <= fabs(L[n1]) ? L[vvar->Gzi] : L[vvar->Gzi + 1];
+            vvar->Gzi++;
+        }
+        if (flag[2] == 1) {                                          /* :331-338 */
+            const double *L = vseq->GyAmpLine; int64_t n1 = vvar->Gyi + 1 < MaxGyStep - 1 ? vvar->Gyi + 1 : MaxGyStep - 1;
+            vvar->GyAmp = fabs(L[vvar->Gyi]) <= fabs(L[n1]) ? L[vvar->Gyi] : L[vvar->Gyi + 1];
+            vvar->Gyi++;
+        }
+        if (flag[3] == 1) {                                          /* :340-347 */
+            const double *L = vseq->GxAmpLine; int64_t n1 = vvar->Gxi + 1 < MaxGxStep - 1 ? vvar->Gxi + 1 : MaxGxStep - 1;
+            vvar->GxAmp = fabs(L[vvar->Gxi]) <= fabs(L[n1]) ? L[vvar->Gxi] : L[vvar->Gxi + 1];
+            vvar->Gxi++;
+        }
+        vvar->ADC = 0;                                               /* :349-354 */
+        if (flag[4] == 1) { vvar->ADC = vseq->ADCLine[vvar->ADCi]; vvar->ADCi++; }
+        if (flag[5] == 1) {                                          /* :356-409 */
+            vvar->Ext = vseq->ExtLine[vvar->Exti];
+            if (vvar->Ext != 0) {
+                if (orc_ext(&C, vvar->Ext)) { status = MRS_ERR_CALLBACK; goto done; }
+            }
+            vvar->Exti++;
+        }
+        if (flag[0] + flag[1] + flag[2] + flag[3] + flag[4] + flag[5] == 0) { /* :411-420 */
+            for (int k = 0; k < Tx; k++) { vvar->rfAmp[k] = 0; vvar->rfPhase[k] = 0; vvar->rfFreq[k] = 0; }
+            vvar->GzAmp = 0; vvar->GyAmp = 0; vvar->GxAmp = 0; vvar->ADC = 0; vvar->Ext = 0;
+        }
+        if (vvar->dt == 0) continue;                                 /* :423-425 */
+        if (vvar->dt < 0) {                                          /* :426-440 */
+            vvar->TRCount++;
+            if (vctl->RunMode == 1) {
+                orc_capture(&C);
+                vsig->Muts[vvar->utsi] = vsig->Muts[vvar->utsi - 1];
+            }
+            continue;
+        }
+        for (int Typei = 0; Typei < Ty; Typei++) {
+            for (int Spini = 0; Spini < S; Spini++) {
+                const int64_t base = (int64_t)Typei * S * C.nvox + (int64_t)Spini * C.nvox;
+                if (vvar->ADC == 1) {                                /* :445-491 */
+                    for (int c = 0; c < Rx; c++) {
+                        const float *Cx = vcoi->RxCoilx + (int64_t)c * C.nvox, *Cy = vcoi->RxCoily + (int64_t)c * C.nvox;
+                        const real cr = (real)cos(-vvar->rfRef), msr = (real)-sin(-vvar->rfRef);
+                        const real sr = (real)sin(-vvar->rfRef);
+                        for (int64_t v = 0; v < C.nvox; v++) {
+                            real mx = C.wMx ? C.wMx[base + v] : (real)vobj->Mx[base + v];
+                            real my = C.wMy ? C.wMy[base + v] : (real)vobj->My[base + v];
+                            real b1, b2, b3, b4;
+                            if (vcoi->RxCoilDefault == 0) {
+                                b1 = mx * (real)Cx[v]; b2 = my * (real)Cy[v]; b3 = b1 + b2;
+                                b1 = mx * (real)Cy[v]; b1 = b1 * -1; b2 = my * (real)Cx[v]; b4 = b1 + b2;
+                                b1 = b3; b2 = b4;
+                            } else {
+                                b1 = mx; b2 = my; b3 = mx; b4 = my;
+                            }
+                            if (vvar->rfRef != 0) {
+                                b1 = b1 * cr; b2 = b2 * msr; b1 = b2 + b1;
+                                b3 = b3 * sr; b4 = b4 * cr; b3 = b4 + b3;
+                            } else {
+                                b3 = b4;
+                            }
+                            buf1[v] = b1; buf3[v] = b3;
+                        }
+                        vsig->Sx[(int64_t)Typei * Rx * SignalNum + (int64_t)c * SignalNum + Signali] += orc_sum(buf1, C.nvox);
+                        vsig->Sy[(int64_t)Typei * Rx * SignalNum + (int64_t)c * SignalNum + Signali] += orc_sum(buf3, C.nvox);
+                    }
+                }
+                /* Bloch step over slices, OpenMP as :494-527 */
+                const real rdt = (real)vvar->dt;
+                const real gz = (real)vvar->GzAmp, gy = (real)vvar->GyAmp, gx = (real)vvar->GxAmp;
+                int64_t Slicei;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+                for (Slicei = 0; Slicei < SpinMxSliceNum; Slicei++) {
+                    const int64_t so = Slicei * SpinMxNum;
+                    const int64_t po = (int64_t)Typei * C.nvox + so;
+                    orc_bloch_slice(Gyro, vctl->CS, S,
+                                    vobj->Rho + po, vobj->T1 + po, vobj->T2 + po,
+                                    vobj->Mz + base + so, vobj->My + base + so, vobj->Mx + base + so,
+                                    vmag->dB0 + so, vmag->dWRnd + base + so,
+                                    vmag->Gzgrid + so, vmag->Gygrid + so, vmag->Gxgrid + so,
+                                    vcoi->TxCoilmg + so, vcoi->TxCoilpe + so,
+                                    rdt, vvar->rfAmp, vvar->rfPhase, vvar->rfFreq, gz, gy, gx,
+                                    Typei, SpinMxNum, SpinMxSliceNum, Tx,
+                                    C.wMx ? C.wMx + base + so : NULL,
+                                    C.wMy ? C.wMy + base + so : NULL,
+                                    C.wMz ? C.wMz + base + so : NULL);
+                }
+            }
+        }
+        if (vvar->ADC == 1) {                                        /* :531-537 */
+            vsig->Kz[Signali] += vvar->Kz;
+            vsig->Ky[Signali] += vvar->Ky;
+            vsig->Kx[Signali] += vvar->Kx;
+            Signali++;
+        }
+        vvar->Kz += vvar->GzAmp * vvar->dt * (vobj->Gyro / (2 * ORC_PI)); /* :540-542 */
+        vvar->Ky += vvar->GyAmp * vvar->dt * (vobj->Gyro / (2 * ORC_PI));
+        vvar->Kx += vvar->GxAmp * vvar->dt * (vobj->Gyro / (2 * ORC_PI));
+        if (vctl->RunMode == 1) {                                    /* :544-552 */
+            orc_capture(&C);
+            vsig->Muts[vvar->utsi] = vsig->Muts[vvar->utsi - 1] + vvar->dt;
+        }
+    }
+done:
+    if (C.wMx) {
+        for (int64_t q = 0; q < ntot; q++) {
+            vobj->Mx[q] = (float)C.wMx[q]; vobj->My[q] = (float)C.wMy[q]; vobj->Mz[q] = (float)C.wMz[q];
+        }
+        free(C.wMx); free(C.wMy); free(C.wMz);
+    }
+    free(buf1); free(buf3);
+    return status;
+}
