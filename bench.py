@@ -73,6 +73,8 @@ def main() -> int:
     ap.add_argument("--tr", type=int, default=None, help="truncate the scan to its first TRs (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-tr", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -81,6 +83,16 @@ def main() -> int:
         return 2
 
     from mrilab_amd import engine, workloads
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 256 MiB device-to-device")
+        del src, dst
 
     log("building the workload")
     t0 = time.perf_counter()

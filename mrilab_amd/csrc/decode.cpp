@@ -80,6 +80,9 @@ void Decoder::emit(Decoded &out, bool step, bool adc, bool capture) {
             std::memcpy(prev_key_, w, sizeof prev_key_);
         }
         w[4] = fbits(wrf);
+        if (!have_prev_dt_ || prev_dt_ != w[0]) flags |= REC_NEWDT;
+        have_prev_dt_ = true;
+        prev_dt_ = w[0];
         out.n_steps++;
     }
     if (adc) {
@@ -94,6 +97,30 @@ void Decoder::emit(Decoded &out, bool step, bool adc, bool capture) {
         out.cap_slot.push_back(capture ? v_.utsi : -1);
     }
     w[5] = flags;
+}
+
+void compress_runs(Decoded &d) {
+    if (!d.cap_slot.empty()) return;
+    const int st = d.stride;
+    const int64_t n = d.n_rec();
+    const uint32_t plain = REC_ADC | REC_RF | REC_SPOIL | REC_CAPTURE | REC_NOSTEP | REC_NEWDT;
+    int64_t out = -1;  // index of the last kept record
+    bool open = false; // the last kept record accepts repeats
+    for (int64_t r = 0; r < n; r++) {
+        const uint32_t *w = d.rec.data() + r * st;
+        const uint32_t f = w[5];
+        if (open && (f & REC_SAME) && !(f & plain)) {
+            uint32_t *k = d.rec.data() + out * st;
+            const uint32_t rep = (k[5] >> REC_REP_SHIFT) + 1;
+            k[5] = (k[5] & ((1u << REC_REP_SHIFT) - 1)) | (rep << REC_REP_SHIFT);
+            if (rep == REC_REP_MAX) open = false;
+            continue;
+        }
+        out++;
+        if (out != r) std::memmove(d.rec.data() + out * st, w, st * sizeof(uint32_t));
+        open = !(f & (REC_RF | REC_NOSTEP | REC_CAPTURE));
+    }
+    d.rec.resize((size_t)(out + 1) * st);
 }
 
 void Decoder::flush_tail(Decoded &out) {

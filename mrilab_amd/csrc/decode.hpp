@@ -17,7 +17,13 @@ enum : uint32_t {
     REC_SAME = 1u << 3,     // free step with the same dt and fields as the previous step
     REC_CAPTURE = 1u << 4,  // RunMode 1: copy the state after the step
     REC_NOSTEP = 1u << 5,   // no precession or relaxation (dt <= 0 event carrier)
+    REC_NEWDT = 1u << 6,    // dt differs from the previous step's
 };
+// Bits 8..31 of the flags word: number of further plain free steps with
+// this record's dt and fields (no ADC, rf, spoiler or capture) that follow
+// it; compress_runs() folds such steps into the record before them.
+constexpr int REC_REP_SHIFT = 8;
+constexpr uint32_t REC_REP_MAX = (1u << 24) - 1;
 
 // Record layout, 32-bit words:
 //   0 dt (s)            1 gamma*Gz (rad/s/m)  2 gamma*Gy   3 gamma*Gx
@@ -39,6 +45,11 @@ struct Decoded {
     int64_t n_steps = 0, n_rf = 0, n_same = 0;
     int64_t n_rec() const { return stride ? (int64_t)rec.size() / stride : 0; }
 };
+
+// Folds runs of plain cached free steps into the record before them
+// (repeat count in the flags word). Not applied when records carry
+// capture slots (RunMode 1).
+void compress_runs(Decoded &d);
 
 // Stop reasons of Decoder::run.
 enum DecodeStop { DECODE_END = 0, DECODE_CALLBACK = 1 };
@@ -72,12 +83,13 @@ private:
     std::vector<double> rfAmp_, rfPhase_, rfFreq_;
     int64_t i_ = 0;
     bool spoil_pending_ = false;
-    bool capture_pending_ = false;
     double ext_pending_ = 0;
     bool resume_after_ext_ = false;
     double flag_[6] = {0, 0, 0, 0, 0, 0};
     // previous step record fields for REC_SAME
     bool prev_free_ = false;
+    bool have_prev_dt_ = false;
+    uint32_t prev_dt_ = 0;
     uint32_t prev_key_[4] = {0, 0, 0, 0};
 };
 

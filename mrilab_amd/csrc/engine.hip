@@ -152,12 +152,37 @@ __device__ __forceinline__ float relax_mz_ref(float mz, float e1, float m0e) {
     return t + m0e;
 }
 
-template <int SPT, int NRX>
+// One step of free precession + relaxation with a cached rotation (c, s)
+// (BlochKernel.h:314-368); PREC selects the relaxation form.
+template <int SPT, int PREC>
+__device__ __forceinline__ void free_step(float *mx, float *my, float *mz, const float *cc, const float *ss,
+                                          float e1, float e2, float m0e, float d1, float d2, float m0s) {
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        const float ax = mx[s], ay = my[s];
+        const float rx = ax * cc[s] + ay * ss[s];
+        const float ry = ay * cc[s] - ax * ss[s];
+        if constexpr (PREC == 0) {
+            mx[s] = rx * e2;
+            my[s] = ry * e2;
+            mz[s] = relax_mz_ref(mz[s], e1, m0e);
+        } else {
+            mx[s] = rx - rx * d2;
+            my[s] = ry - ry * d2;
+            mz[s] = mz[s] + (m0s - mz[s]) * d1;
+        }
+    }
+}
+
+// GEN = false: single Tx coil and no RunMode-1 capture (the common case);
+// GEN = true: any Tx coil count and capture.
+template <int SPT, int NRX, bool GEN>
 __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g,
-                                                        int64_t r0, int64_t r1) {
+                                                        int r0, int r1) {
     constexpr int NV = 2 * NRX;
     extern __shared__ float red[];  // [kRedBatch][nwaves][NV]
     const crec_t rec = (crec_t)rec_g;
+    const int stride = GEN ? p.stride : 8;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nwaves = blockDim.x >> 6;
@@ -204,7 +229,7 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
     }
     // single-Tx B1+ as a complex weight mg*exp(i*pe)
     float wtx_r = 0.f, wtx_i = 0.f;
-    if (p.Tx == 1) {
+    if (!GEN || p.Tx == 1) {
         float sp, cp;
         sincos_cr(p.txpe[vv], &sp, &cp);
         const float mg = p.txmg[vv];
@@ -213,52 +238,15 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
     }
     const float m0s = rho / (float)p.S;  // BlochKernel.h:365
 
-    bool have_rot = false;
-    float last_dt = -1.f;
-    float e1 = 1.f, e2 = 1.f, m0e = 0.f;   // reference mode
-    float d1 = 0.f, d2 = 0.f;              // accurate mode
+    bool have_rot = false, have_e = false;
+    float e1 = 1.f, e2 = 1.f, m0e = 0.f;   // reference relaxation
+    float d1 = 0.f, d2 = 0.f;              // accurate relaxation
     int slot = 0;
     int adc_batch0 = 0;
 
-    const bool fast_ok = p.precision == 0;
-    // record r is decoded from registers loaded one iteration earlier
-    crec_t q = rec + r0 * p.stride;
-    uint32_t n_flags = r0 < r1 ? q[5] : 0u;
-    float n_dt = r0 < r1 ? __uint_as_float(q[0]) : 0.f;
-    float n_gz = r0 < r1 ? __uint_as_float(q[1]) : 0.f;
-    float n_gy = r0 < r1 ? __uint_as_float(q[2]) : 0.f;
-    float n_gx = r0 < r1 ? __uint_as_float(q[3]) : 0.f;
-    float n_wrf = r0 < r1 ? __uint_as_float(q[4]) : 0.f;
-    float n_br = r0 < r1 ? __uint_as_float(q[REC_HEAD]) : 0.f;
-    float n_bi = r0 < r1 ? __uint_as_float(q[REC_HEAD + 1]) : 0.f;
-
-    for (int64_t r = r0; r < r1; ++r) {
-        const uint32_t flags = n_flags;
-        const float dt = n_dt, ggz = n_gz, ggy = n_gy, ggx = n_gx, wrf = n_wrf, br0 = n_br, bi0 = n_bi;
-        q = rec + r * p.stride;
-        if (r + 1 < r1) {
-            const crec_t qn = q + p.stride;
-            n_flags = qn[5];
-            n_dt = __uint_as_float(qn[0]);
-            n_gz = __uint_as_float(qn[1]);
-            n_gy = __uint_as_float(qn[2]);
-            n_gx = __uint_as_float(qn[3]);
-            n_wrf = __uint_as_float(qn[4]);
-            n_br = __uint_as_float(qn[REC_HEAD]);
-            n_bi = __uint_as_float(qn[REC_HEAD + 1]);
-        }
-        // common case: free precession with the previous step's rotation and dt
-        if ((flags & (REC_ADC | REC_RF | REC_SPOIL | REC_CAPTURE | REC_NOSTEP | REC_SAME)) == REC_SAME &&
-            have_rot && dt == last_dt && fast_ok) {
-#pragma unroll
-            for (int s = 0; s < SPT; s++) {
-                const float ax = mx[s], ay = my[s];
-                mx[s] = (ax * cc[s] + ay * ss[s]) * e2;
-                my[s] = (ay * cc[s] - ax * ss[s]) * e2;
-                mz[s] = relax_mz_ref(mz[s], e1, m0e);
-            }
-            continue;
-        }
+    for (int r = r0; r < r1; ++r) {
+        const crec_t q = rec + r * stride;
+        const uint32_t flags = q[5];
         if (flags & REC_SPOIL) {  // Plugin_IdealSpoiler.m:10-11
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
@@ -300,8 +288,9 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
             }
         }
         if (!(flags & REC_NOSTEP)) {
-            if (dt != last_dt) {
-                last_dt = dt;
+            const float dt = __uint_as_float(q[0]);
+            if ((flags & REC_NEWDT) || !have_e) {
+                have_e = true;
                 if (p.precision == 0) {
                     // BlochKernel.h:348-368 in fp32: E = exp(max(-dt/T, -200)),
                     // with exp rounded correctly from fp64
@@ -319,12 +308,14 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
                     d1 = (float)(-expm1(-(double)dt / (double)t1));
                 }
             }
-            const float gv = z * ggz + y * ggy + x * ggx;  // BlochKernel.h:44-51
+            const float gv = z * __uint_as_float(q[1]) + y * __uint_as_float(q[2]) + x * __uint_as_float(q[3]);
             if (flags & REC_RF) {
+                const float wrf = __uint_as_float(q[4]);
                 float bx, by;
-                if (p.Tx == 1) {
-                    bx = wtx_r * br0 - wtx_i * bi0;
-                    by = wtx_r * bi0 + wtx_i * br0;
+                if (!GEN || p.Tx == 1) {
+                    const float br = __uint_as_float(q[REC_HEAD]), bi = __uint_as_float(q[REC_HEAD + 1]);
+                    bx = wtx_r * br - wtx_i * bi;
+                    by = wtx_r * bi + wtx_i * br;
                 } else {
                     bx = 0.f;
                     by = 0.f;
@@ -358,36 +349,37 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
                         mz[s] = az * cs - (nx * ay - ny * ax) * sn + nz * k;
                     }
                 }
+                if (p.precision == 0) {
+#pragma unroll
+                    for (int s = 0; s < SPT; s++) {
+                        mx[s] = mx[s] * e2;
+                        my[s] = my[s] * e2;
+                        mz[s] = relax_mz_ref(mz[s], e1, m0e);
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < SPT; s++) {
+                        mx[s] = mx[s] - mx[s] * d2;
+                        my[s] = my[s] - my[s] * d2;
+                        mz[s] = mz[s] + (m0s - mz[s]) * d1;
+                    }
+                }
             } else {
                 if (!((flags & REC_SAME) && have_rot)) {
 #pragma unroll
                     for (int s = 0; s < SPT; s++) sincos_cr((wb[s] + gv) * dt, &ss[s], &cc[s]);
                     have_rot = true;
                 }
-#pragma unroll
-                for (int s = 0; s < SPT; s++) {  // BlochKernel.h:314-343
-                    const float ax = mx[s], ay = my[s];
-                    mx[s] = ax * cc[s] + ay * ss[s];
-                    my[s] = ay * cc[s] - ax * ss[s];
-                }
-            }
-            if (p.precision == 0) {
-#pragma unroll
-                for (int s = 0; s < SPT; s++) {
-                    mx[s] = mx[s] * e2;
-                    my[s] = my[s] * e2;
-                    mz[s] = relax_mz_ref(mz[s], e1, m0e);
-                }
-            } else {
-#pragma unroll
-                for (int s = 0; s < SPT; s++) {
-                    mx[s] = mx[s] - mx[s] * d2;
-                    my[s] = my[s] - my[s] * d2;
-                    mz[s] = mz[s] + (m0s - mz[s]) * d1;
+                // this step, then the folded repeats of it
+                const int n = 1 + (int)(flags >> REC_REP_SHIFT);
+                if (p.precision == 0) {
+                    for (int k = 0; k < n; k++) free_step<SPT, 0>(mx, my, mz, cc, ss, e1, e2, m0e, d1, d2, m0s);
+                } else {
+                    for (int k = 0; k < n; k++) free_step<SPT, 1>(mx, my, mz, cc, ss, e1, e2, m0e, d1, d2, m0s);
                 }
             }
         }
-        if ((flags & REC_CAPTURE) && valid) {  // DoScanAtCPU.cpp:544-552
+        if (GEN && (flags & REC_CAPTURE) && valid) {  // DoScanAtCPU.cpp:544-552
             const int64_t base = (int64_t)p.cap_slot[r] * nvox * p.S * p.Ty;
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
@@ -497,36 +489,40 @@ static mrs_status fail(mrs_status st, const std::string &msg) {
 
 typedef void (*fwd_fn)(FwdParams, const uint32_t *, int64_t, int64_t);
 
-template <int SPT, int NRX>
+template <int SPT, int NRX, bool GEN>
 static void launch_fwd(dim3 grid, dim3 block, size_t lds, hipStream_t st, const FwdParams &p,
-                       const uint32_t *rec, int64_t r0, int64_t r1) {
-    bloch_fwd_kernel<SPT, NRX><<<grid, block, lds, st>>>(p, rec, r0, r1);
+                       const uint32_t *rec, int r0, int r1) {
+    bloch_fwd_kernel<SPT, NRX, GEN><<<grid, block, lds, st>>>(p, rec, r0, r1);
 }
 
-typedef void (*launch_fn)(dim3, dim3, size_t, hipStream_t, const FwdParams &, const uint32_t *, int64_t,
-                          int64_t);
+typedef void (*launch_fn)(dim3, dim3, size_t, hipStream_t, const FwdParams &, const uint32_t *, int, int);
 
-template <int SPT>
+template <int SPT, bool GEN>
 static launch_fn pick_nrx(int nrx) {
     switch (nrx) {
-        case 1: return launch_fwd<SPT, 1>;
-        case 2: return launch_fwd<SPT, 2>;
-        case 4: return launch_fwd<SPT, 4>;
-        case 8: return launch_fwd<SPT, 8>;
-        case 16: return launch_fwd<SPT, 16>;
-        case 32: return launch_fwd<SPT, 32>;
+        case 1: return launch_fwd<SPT, 1, GEN>;
+        case 2: return launch_fwd<SPT, 2, GEN>;
+        case 4: return launch_fwd<SPT, 4, GEN>;
+        case 8: return launch_fwd<SPT, 8, GEN>;
+        case 16: return launch_fwd<SPT, 16, GEN>;
+        case 32: return launch_fwd<SPT, 32, GEN>;
     }
     return nullptr;
 }
 
-static launch_fn pick_fwd(int spt, int nrx) {
+template <bool GEN>
+static launch_fn pick_spt(int spt, int nrx) {
     switch (spt) {
-        case 1: return pick_nrx<1>(nrx);
-        case 2: return pick_nrx<2>(nrx);
-        case 4: return pick_nrx<4>(nrx);
-        case 8: return pick_nrx<8>(nrx);
+        case 1: return pick_nrx<1, GEN>(nrx);
+        case 2: return pick_nrx<2, GEN>(nrx);
+        case 4: return pick_nrx<4, GEN>(nrx);
+        case 8: return pick_nrx<8, GEN>(nrx);
     }
     return nullptr;
+}
+
+static launch_fn pick_fwd(int spt, int nrx, bool gen) {
+    return gen ? pick_spt<true>(spt, nrx) : pick_spt<false>(spt, nrx);
 }
 
 static void launch_reduce(int nv, dim3 grid, hipStream_t st, const float *partial, int nblk, int bpt, int Rx,
@@ -708,7 +704,7 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     max_adc = std::max<int64_t>(kRedBatch, std::min<int64_t>(max_adc, 65536));
     if (opt && opt->max_adc_per_launch > 0) max_adc = opt->max_adc_per_launch;
     P->max_adc = max_adc;
-    if (!pick_fwd(P->spt, P->nrx)) return fail(MRS_ERR_UNSUPPORTED, "no kernel for this coil count");
+    if (!pick_fwd(P->spt, P->nrx, true)) return fail(MRS_ERR_UNSUPPORTED, "no kernel for this coil count");
 
     hipStream_t st = P->stream;
     const size_t vb = (size_t)P->nvox * sizeof(float);
@@ -751,6 +747,8 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
 static mrs_status upload_program(mrs_plan *P) {
     hipStream_t st = P->stream;
     const size_t words = P->dec.rec.size();
+    if (words >= ((size_t)1 << 31))
+        return fail(MRS_ERR_UNSUPPORTED, "more than 2^31 record words in one program");
     if (words > P->rec_cap_words || !P->rec.p) {
         HIP_TRY(hipStreamSynchronize(st));
         if (P->rec.p) (void)hipFree(P->rec.p);
@@ -829,7 +827,7 @@ static mrs_status execute(mrs_plan *P, int64_t adc_base) {
     fp.stride = P->dec.stride;
     fp.gyro = P->gyro;
     for (int t = 0; t < P->Ty; t++) fp.cs[t] = P->cs[t];
-    launch_fn fn = pick_fwd(P->spt, P->nrx);
+    launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1);
     const dim3 grid(P->gx, P->Ty * P->ngroups), block(P->block);
     const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
     const uint32_t *rec = (const uint32_t *)P->rec.p;
@@ -843,7 +841,7 @@ static mrs_status execute(mrs_plan *P, int64_t adc_base) {
     }
     for (size_t l = 0; l < nl; l++) {
         if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l], P->stream));
-        fn(grid, block, lds, P->stream, fp, rec, P->launch_r0[l], P->launch_r1[l]);
+        fn(grid, block, lds, P->stream, fp, rec, (int)P->launch_r0[l], (int)P->launch_r1[l]);
         HIP_TRY(hipGetLastError());
         if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 1], P->stream));
         const int64_t na = P->launch_nadc[l];
@@ -897,6 +895,7 @@ mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs
     Decoder dec(vobj, vcoi, vctl, vseq, vvar, false);
     dec.run(P->dec);
     dec.flush_tail(P->dec);
+    compress_runs(P->dec);
     if ((int64_t)P->dec.adc_ref.size() > P->signal_num) {
         delete P;
         return fail(MRS_ERR_ARG, "VSig.SignalNum is smaller than the number of ADC samples");
@@ -1058,6 +1057,7 @@ static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, cons
         if (!all.muts.empty()) P->dec.muts = all.muts;
         const DecodeStop stop = dec.run(P->dec);
         if (stop == DECODE_END) dec.flush_tail(P->dec);
+        compress_runs(P->dec);
         if (adc_base + (int64_t)P->dec.adc_ref.size() > P->signal_num)
             return fail(MRS_ERR_ARG, "VSig.SignalNum is smaller than the number of ADC samples");
         if ((rc = upload_program(P))) return rc;

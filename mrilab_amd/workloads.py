@@ -144,6 +144,21 @@ def make_structs(Rho, T1, T2, dB0, res, spin_num=1, dWRnd=None, cs=None,
     return Scan(VObj, VMag, VCoi, VCtl, seq, VVar, VSig)
 
 
+def with_run_mode1(sc: Scan) -> Scan:
+    """A copy of ``sc`` with VCtl.RunMode = 1 and the per-step capture arrays
+    VSig.Mx/My/Mz [R,C,Z,S,Ty,n_uts] and VSig.Muts [n_uts] (DoPreScan.m:417-420)."""
+    out = sc.copy()
+    out.VCtl["RunMode"] = np.int32(1)
+    nuts = out.VSeq["utsLine"].size
+    shape6 = tuple(out.VObj["Mz"].shape) + (1,) * (5 - out.VObj["Mz"].ndim) + (nuts,)
+    for k in ("Mx", "My", "Mz"):
+        a = np.zeros(shape6, dtype=np.float32, order=F)
+        a[..., 0] = out.VObj[k].reshape(shape6[:5], order="F")
+        out.VSig[k] = a
+    out.VSig["Muts"] = np.zeros(nuts)
+    return out
+
+
 # --------------------------------------------------------------------------
 # Sequences on an exact time grid
 # --------------------------------------------------------------------------
