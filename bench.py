@@ -14,6 +14,16 @@ roofline under the algorithmic-bytes convention (per isochromat-step:
 maps on ADC steps) and the fp32 CPU oracle timed on the first TRs of the
 same phantom on this host's cores.
 
+The kernel keeps the state in registers for a whole launch, so the bytes
+it really moves (rocprofv3 FETCH_SIZE + WRITE_SIZE, stored in
+profiles/bloch_fwd_bssfp3d_traffic.json) are a small fraction of the
+algorithmic bytes and the algorithmic-byte rate can exceed HBM peak. The
+line therefore also carries the measured HBM rate and the VALU roofline:
+algorithmic fp32 flops per isochromat-step = 10 (cached free-precession
+rotation 6, transverse and longitudinal relaxation 4) + 35 more on rf steps
+(Rodrigues rotation; plus sqrt, reciprocal and sin/cos) + 8 per receive
+coil on ADC steps (complex product with the coil map and accumulation).
+
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 from __future__ import annotations
@@ -29,7 +39,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "isochromat-steps/s + % HBM roofline; batched speedup; adjoint/forward time"
 HBM_PEAK_GBS = 8000.0
-FP32_FMA_PEAK_TFLOPS = 78.6  # non-packed fp32 FMA (packed fp32 is disabled in this build)
+FP32_PEAK_TFLOPS = 157.3           # MI355X fp32 vector peak (packed)
+FP32_NONPACKED_PEAK_TFLOPS = 78.6  # non-packed fp32 FMA rate; this build disables packed fp32
 
 
 def log(msg: str) -> None:
@@ -39,6 +50,11 @@ def log(msg: str) -> None:
 def algorithmic_bytes(spins: int, rx: int, adc_fraction: float) -> float:
     """Bytes per isochromat-step a step-at-a-time engine moves (see module doc)."""
     return 24 + 4 + 36.0 / spins + adc_fraction * 8.0 * rx / spins
+
+
+def algorithmic_flops(rf_fraction: float, adc_fraction: float, rx: int) -> float:
+    """fp32 flops per isochromat-step (see module doc)."""
+    return 10.0 + 35.0 * rf_fraction + 8.0 * rx * adc_fraction
 
 
 def cpu_baseline(n_tr: int, threads: int) -> dict:
@@ -129,6 +145,8 @@ def main() -> int:
     ab_per_launch = iso_steps / n_launch * ab
     achieved = ab_per_launch / avg_launch_s / 1e9
     traffic = traffic_from_profile()
+    flop = algorithmic_flops(info["n_rf_steps"] / info["n_steps"], info["n_adc"] / info["n_steps"], rx)
+    tflops = iso_steps / n_launch * flop / avg_launch_s / 1e12
     line = {
         "metric": METRIC,
         "value": value,
@@ -149,7 +167,12 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "convention": f"algorithmic bytes {ab:.2f} B per isochromat-step",
-                     "kernel": "bloch_fwd_kernel<4,8>", "avg_launch_ms": avg_launch_s * 1e3},
+                     "kernel": "bloch_fwd_kernel<4,8>", "avg_launch_ms": avg_launch_s * 1e3,
+                     "measured_hbm_GBs": None if traffic is None else traffic / avg_launch_s / 1e9,
+                     "valu": {"achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": tflops / FP32_PEAK_TFLOPS,
+                              "frac_nonpacked": tflops / FP32_NONPACKED_PEAK_TFLOPS,
+                              "flop_per_isochromat_step": flop}},
         "kernel_ms_per_step": {"bloch_fwd": fwd_ms / args.steps, "signal_reduce": red_ms / args.steps},
     }
     plan.close()

@@ -139,7 +139,10 @@ typedef struct mrs_options {
                                  1 = accurate: expm1-based relaxation */
     int32_t timing;           /* 1 = time every kernel launch with HIP events
                                  on the plan stream (mrs_plan_info) */
-    int32_t reserved[2];
+    int32_t model;            /* 0 = single-pool engine, 1 = generalized
+                                 multi-pool engine (VObj.K, VObj.TypeFlag);
+                                 set by the entry point */
+    int32_t reserved[1];
 } mrs_options;
 
 /* Thread-local message of the last failed call on this thread. */
@@ -163,6 +166,18 @@ mrs_status mrs_DoScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
                            const mrs_vseq *vseq, mrs_vvar *vvar,
                            mrs_vsig *vsig, const mrs_callbacks *cb,
                            const mrs_options *opt);
+
+/* Generalized multi-pool scan (free and bound pools with exchange, MT
+ * saturation), TypeNum 2..4. VObj.K is [R,C,Z,TypeNum,TypeNum] with
+ * K(:,:,:,a,b) the rate from pool a into pool b; VObj.TypeFlag is 0 for a
+ * free pool and 1 for a bound pool. mrs_options.multipool_scheme picks the
+ * reference's operator split (0) or the fp64 matrix exponential (1).
+ * Replaces: mexFunction, Lib/src/engine/GPUEngine/DoGMScanAtGPU.cu:76 */
+mrs_status mrs_DoGMScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
+                             const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                             const mrs_vseq *vseq, mrs_vvar *vvar,
+                             mrs_vsig *vsig, const mrs_callbacks *cb,
+                             const mrs_options *opt);
 
 /* ---- Plan API: decode once, keep maps and step records resident in HBM,
  * run many times. mrs_DoScanAtGPU is plan_create + plan_run + plan_fetch +

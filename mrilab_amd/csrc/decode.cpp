@@ -65,8 +65,13 @@ void Decoder::emit(Decoded &out, bool step, bool adc, bool capture) {
                 }
                 wrf += (float)((double)(Fq * 2.0f) * kPi);  // BlochKernel.h:69
                 const double b = (double)gyro * (double)A;
-                w[REC_HEAD + 2 * c] = fbits((float)(b * std::cos((double)P)));
-                w[REC_HEAD + 2 * c + 1] = fbits((float)(b * std::sin((double)P)));
+                const int cw = out.coil_words;
+                w[REC_HEAD + cw * c] = fbits((float)(b * std::cos((double)P)));
+                w[REC_HEAD + cw * c + 1] = fbits((float)(b * std::sin((double)P)));
+                if (cw == 4) {  // bound-pool saturation inputs (BlochKernelGM.cuh:325-327)
+                    w[REC_HEAD + cw * c + 2] = fbits(A);
+                    w[REC_HEAD + cw * c + 3] = fbits(Fq);
+                }
             }
             prev_free_ = false;
         } else {
@@ -100,27 +105,38 @@ void Decoder::emit(Decoded &out, bool step, bool adc, bool capture) {
 }
 
 void compress_runs(Decoded &d) {
-    if (!d.cap_slot.empty()) return;
     const int st = d.stride;
-    const int64_t n = d.n_rec();
-    const uint32_t plain = REC_ADC | REC_RF | REC_SPOIL | REC_CAPTURE | REC_NOSTEP | REC_NEWDT;
-    int64_t out = -1;  // index of the last kept record
-    bool open = false; // the last kept record accepts repeats
-    for (int64_t r = 0; r < n; r++) {
-        const uint32_t *w = d.rec.data() + r * st;
-        const uint32_t f = w[5];
-        if (open && (f & REC_SAME) && !(f & plain)) {
-            uint32_t *k = d.rec.data() + out * st;
-            const uint32_t rep = (k[5] >> REC_REP_SHIFT) + 1;
-            k[5] = (k[5] & ((1u << REC_REP_SHIFT) - 1)) | (rep << REC_REP_SHIFT);
-            if (rep == REC_REP_MAX) open = false;
-            continue;
+    if (d.cap_slot.empty()) {
+        const int64_t n = d.n_rec();
+        const uint32_t plain = REC_ADC | REC_RF | REC_SPOIL | REC_CAPTURE | REC_NOSTEP | REC_NEWDT;
+        int64_t out = -1;  // index of the last kept record
+        bool open = false; // the last kept record accepts repeats
+        for (int64_t r = 0; r < n; r++) {
+            const uint32_t *w = d.rec.data() + r * st;
+            const uint32_t f = w[5];
+            if (open && (f & REC_SAME) && !(f & plain)) {
+                uint32_t *k = d.rec.data() + out * st;
+                const uint32_t rep = (k[5] >> REC_REP_SHIFT) + 1;
+                k[5] = (k[5] & ((1u << REC_REP_SHIFT) - 1)) | (rep << REC_REP_SHIFT);
+                if (rep == REC_REP_MAX) open = false;
+                continue;
+            }
+            out++;
+            if (out != r) std::memmove(d.rec.data() + out * st, w, st * sizeof(uint32_t));
+            open = !(f & (REC_RF | REC_NOSTEP | REC_CAPTURE));
         }
-        out++;
-        if (out != r) std::memmove(d.rec.data() + out * st, w, st * sizeof(uint32_t));
-        open = !(f & (REC_RF | REC_NOSTEP | REC_CAPTURE));
+        d.rec.resize((size_t)(out + 1) * st);
     }
-    d.rec.resize((size_t)(out + 1) * st);
+    // single-use rotations: scan backwards remembering whether the next
+    // step record reuses the rotation of the one before it
+    bool next_same = false;
+    for (int64_t r = d.n_rec() - 1; r >= 0; r--) {
+        uint32_t &f = d.rec[(size_t)r * st + 5];
+        if (f & REC_NOSTEP) continue;
+        const bool free_step = !(f & REC_RF);
+        if (free_step && (f >> REC_REP_SHIFT) == 0 && !next_same) f |= REC_ONCE;
+        next_same = (f & REC_SAME) != 0;
+    }
 }
 
 void Decoder::flush_tail(Decoded &out) {

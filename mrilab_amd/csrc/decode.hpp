@@ -18,6 +18,7 @@ enum : uint32_t {
     REC_CAPTURE = 1u << 4,  // RunMode 1: copy the state after the step
     REC_NOSTEP = 1u << 5,   // no precession or relaxation (dt <= 0 event carrier)
     REC_NEWDT = 1u << 6,    // dt differs from the previous step's
+    REC_ONCE = 1u << 7,     // free step whose rotation no later step reuses
 };
 // Bits 8..31 of the flags word: number of further plain free steps with
 // this record's dt and fields (no ADC, rf, spoiler or capture) that follow
@@ -29,13 +30,15 @@ constexpr uint32_t REC_REP_MAX = (1u << 24) - 1;
 //   0 dt (s)            1 gamma*Gz (rad/s/m)  2 gamma*Gy   3 gamma*Gx
 //   4 2*pi*rf frequency summed over active Tx coils (rad/s)
 //   5 flags
-//   6+2c, 7+2c  gamma*A_c*cos(phi_c), gamma*A_c*sin(phi_c) for Tx coil c
+//   then per Tx coil c (coil_words = 2, or 4 for the multi-pool engine):
+//     gamma*A_c*cos(phi_c), gamma*A_c*sin(phi_c) [, |A_c| (T), rf frequency_c (Hz)]
 // The stride is rounded up to an even number of words.
 constexpr int REC_HEAD = 6;
-inline int rec_stride(int tx) { return (REC_HEAD + 2 * tx + 1) & ~1; }
+inline int rec_stride(int tx, int coil_words = 2) { return (REC_HEAD + coil_words * tx + 1) & ~1; }
 
 struct Decoded {
     int tx = 1;
+    int coil_words = 2;
     int stride = 8;
     std::vector<uint32_t> rec;       // n_rec * stride words
     std::vector<double> adc_ref;     // rfRef at each ADC sample
@@ -47,8 +50,9 @@ struct Decoded {
 };
 
 // Folds runs of plain cached free steps into the record before them
-// (repeat count in the flags word). Not applied when records carry
-// capture slots (RunMode 1).
+// (repeat count in the flags word; not when records carry RunMode-1
+// capture slots), then marks free-step records whose rotation is used by
+// that step only (REC_ONCE: no repeats and the next step is not REC_SAME).
 void compress_runs(Decoded &d);
 
 // Stop reasons of Decoder::run.

@@ -29,15 +29,14 @@
 #include <vector>
 
 #include "decode.hpp"
+#include "device_common.hpp"
 #include "mrilab_amd.h"
+#include "multipool.hpp"
 
 namespace mrs {
 
 constexpr int kMaxTypes = 16;
-constexpr int kRedBatch = 16;   // ADC samples buffered in LDS before a flush
 constexpr double kPi = 3.14159265359;  // DoScanAtCPU.cpp:63
-
-typedef const __attribute__((address_space(4))) uint32_t *crec_t;
 
 struct FwdParams {
     float *Mx, *My, *Mz;
@@ -53,104 +52,6 @@ struct FwdParams {
     float gyro;
     float cs[kMaxTypes];            // 2*pi*CS per type, rounded as BlochKernel.h:42
 };
-
-// sin and cos of an fp32 angle, correctly rounded to fp32 (up to rare
-// near-midpoint cases): the reduction by pi/2 (two-part constant) and the
-// Taylor polynomials of sin and cos on [-pi/4, pi/4] (truncation error
-// below 3e-14) run in fp64, which on gfx950 issues at the fp32 FMA rate
-// when packed fp32 is off. MRiLab's CPU engine takes sin/cos from a
-// 24-bit-accurate vector library; a 1.5 ulp fp32 sincos drifts visibly
-// from it over 1e5 cached rotations, this one does not.
-__device__ __forceinline__ void sincos_cr(float xf, float *sp, float *cp) {
-    const double x = (double)xf;
-    double k = rint(x * 0.63661977236758134308);        // 2/pi
-    if (fabs(k) > 1.0e6) {                               // far outside the step angles seen in practice
-        double s, c;
-        sincos(x, &s, &c);
-        *sp = (float)s;
-        *cp = (float)c;
-        return;
-    }
-    double r = fma(-k, 1.5707963267948966192, x);        // pi/2, high part
-    r = fma(-k, 6.1232339957367660e-17, r);              // pi/2, low part
-    const double r2 = r * r;
-    const double sp_ = r + r * r2 * (-1.0 / 6 + r2 * (1.0 / 120 + r2 * (-1.0 / 5040 + r2 * (1.0 / 362880 +
-                       r2 * (-1.0 / 39916800 + r2 * (1.0 / 6227020800.0))))));
-    const double cp_ = 1.0 + r2 * (-0.5 + r2 * (1.0 / 24 + r2 * (-1.0 / 720 + r2 * (1.0 / 40320 +
-                       r2 * (-1.0 / 3628800 + r2 * (1.0 / 479001600.0 + r2 * (-1.0 / 87178291200.0)))))));
-    const int q = (int)(((long long)k) & 3);
-    double s = sp_, c = cp_;
-    if (q == 1) { s = cp_; c = -sp_; }
-    else if (q == 2) { s = -sp_; c = -cp_; }
-    else if (q == 3) { s = -cp_; c = sp_; }
-    *sp = (float)s;
-    *cp = (float)c;
-}
-
-// Lane exchange for one butterfly stage across lane bit O. For O = 32 and
-// 16, gfx950's v_permlane{32,16}_swap exchange whole rows; for O <= 8, DPP
-// row_mirror / row_half_mirror / quad_perm pair each lane with one whose bit
-// O differs (not always lane ^ O, but the same pair in both directions).
-template <int O>
-__device__ __forceinline__ float dpp_partner(float x) {
-    constexpr int ctrl = O == 8 ? 0x140 : O == 4 ? 0x141 : O == 2 ? 0x4E : 0xB1;
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), ctrl, 0xF, 0xF, true));
-}
-
-// Transposed butterfly: N values per lane, 64 lanes. Each stage halves the
-// number of values a lane keeps while pairing lanes across lane bit O (the
-// lane with bit O set keeps the upper half); when one value is left the
-// remaining stages sum plainly. Afterwards lane l holds the wave total of
-// value l >> (6 - log2 N). With the row swaps a pair costs one swap and
-// one add: after swap(X = v[i], Y = v[i+H]) the two outputs hold, in every
-// lane, exactly the two addends of the value that lane keeps.
-template <int N, int O>
-struct WaveSum {
-    static __device__ __forceinline__ void run(float *v, int lane) {
-        if constexpr (O >= 1) {
-            if constexpr (N > 1) {
-                constexpr int H = N / 2;
-                if constexpr (O >= 16) {
-#pragma unroll
-                    for (int i = 0; i < H; i++) {
-                        const auto r = O == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]), __float_as_uint(v[i + H]), false, false)
-                                               : __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]), __float_as_uint(v[i + H]), false, false);
-                        v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
-                    }
-                } else {
-                    const bool up = (lane & O) != 0;
-#pragma unroll
-                    for (int i = 0; i < H; i++) {
-                        const float lo = v[i] + dpp_partner<O>(v[i]);
-                        const float hi = v[i + H] + dpp_partner<O>(v[i + H]);
-                        v[i] = up ? hi : lo;
-                    }
-                }
-                WaveSum<H, O / 2>::run(v, lane);
-            } else {
-                if constexpr (O >= 16) {
-                    const auto r = O == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(v[0]), __float_as_uint(v[0]), false, false)
-                                           : __builtin_amdgcn_permlane16_swap(__float_as_uint(v[0]), __float_as_uint(v[0]), false, false);
-                    // r[0] holds the partner's value in the upper rows, r[1] in the lower rows
-                    v[0] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
-                } else {
-                    v[0] += dpp_partner<O>(v[0]);
-                }
-                WaveSum<1, O / 2>::run(v, lane);
-            }
-        }
-    }
-};
-
-// Mz relaxation exactly as BlochKernel.h:363-368 rounds it: the product and
-// the sum are rounded separately. Near equilibrium the per-step change is
-// below half an ulp of Mz and fp32 recovery stalls; where it stalls depends
-// on this rounding, so it must not be fused into an FMA.
-__device__ __forceinline__ float relax_mz_ref(float mz, float e1, float m0e) {
-#pragma clang fp contract(off)
-    const float t = mz * e1;
-    return t + m0e;
-}
 
 // One step of free precession + relaxation with a cached rotation (c, s)
 // (BlochKernel.h:314-368); PREC selects the relaxation form.
@@ -175,8 +76,9 @@ __device__ __forceinline__ void free_step(float *mx, float *my, float *mz, const
 }
 
 // GEN = false: single Tx coil and no RunMode-1 capture (the common case);
-// GEN = true: any Tx coil count and capture.
-template <int SPT, int NRX, bool GEN>
+// GEN = true: any Tx coil count and capture. PREC: relaxation form
+// (0 reference rounding, 1 accurate).
+template <int SPT, int NRX, bool GEN, int PREC>
 __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g,
                                                         int r0, int r1) {
     constexpr int NV = 2 * NRX;
@@ -248,6 +150,9 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
         const crec_t q = rec + r * stride;
         const uint32_t flags = q[5];
         if (flags & REC_SPOIL) {  // Plugin_IdealSpoiler.m:10-11
+            // a real (scalar) branch: the spoiler is rare, and a select
+            // here would cost 2*SPT VALU on every record
+            __asm__ volatile("");
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
                 mx[s] = 0.f;
@@ -276,12 +181,7 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
             if ((lane & (G - 1)) == 0) red[(slot * nwaves + wave) * NV + lane / G] = val[0];
             if (++slot == kRedBatch) {
                 __syncthreads();
-                for (int i = threadIdx.x; i < kRedBatch * NV; i += blockDim.x) {
-                    const int sl = i / NV, j = i - sl * NV;
-                    float acc = 0.f;
-                    for (int w = 0; w < nwaves; w++) acc += red[(sl * nwaves + w) * NV + j];
-                    p.partial[((int64_t)(adc_batch0 + sl) * nblk + blk) * NV + j] = acc;
-                }
+                flush_partials(red, kRedBatch, nwaves, NV, p.partial, adc_batch0, nblk, blk, NV);
                 __syncthreads();
                 adc_batch0 += kRedBatch;
                 slot = 0;
@@ -291,7 +191,7 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
             const float dt = __uint_as_float(q[0]);
             if ((flags & REC_NEWDT) || !have_e) {
                 have_e = true;
-                if (p.precision == 0) {
+                if constexpr (PREC == 0) {
                     // BlochKernel.h:348-368 in fp32: E = exp(max(-dt/T, -200)),
                     // with exp rounded correctly from fp64
                     float a2 = (1.0f / t2) * dt;
@@ -349,7 +249,7 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
                         mz[s] = az * cs - (nx * ay - ny * ax) * sn + nz * k;
                     }
                 }
-                if (p.precision == 0) {
+                if constexpr (PREC == 0) {
 #pragma unroll
                     for (int s = 0; s < SPT; s++) {
                         mx[s] = mx[s] * e2;
@@ -365,18 +265,19 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
                     }
                 }
             } else {
-                if (!((flags & REC_SAME) && have_rot)) {
+                if (flags & REC_ONCE) {
+                    // rotation used by this step only
+#pragma unroll
+                    for (int s = 0; s < SPT; s++) sincos_fast((wb[s] + gv) * dt, &ss[s], &cc[s]);
+                    have_rot = false;
+                } else if (!((flags & REC_SAME) && have_rot)) {
 #pragma unroll
                     for (int s = 0; s < SPT; s++) sincos_cr((wb[s] + gv) * dt, &ss[s], &cc[s]);
                     have_rot = true;
                 }
                 // this step, then the folded repeats of it
                 const int n = 1 + (int)(flags >> REC_REP_SHIFT);
-                if (p.precision == 0) {
-                    for (int k = 0; k < n; k++) free_step<SPT, 0>(mx, my, mz, cc, ss, e1, e2, m0e, d1, d2, m0s);
-                } else {
-                    for (int k = 0; k < n; k++) free_step<SPT, 1>(mx, my, mz, cc, ss, e1, e2, m0e, d1, d2, m0s);
-                }
+                for (int k = 0; k < n; k++) free_step<SPT, PREC>(mx, my, mz, cc, ss, e1, e2, m0e, d1, d2, m0s);
             }
         }
         if (GEN && (flags & REC_CAPTURE) && valid) {  // DoScanAtCPU.cpp:544-552
@@ -391,12 +292,7 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
     }
     if (slot > 0) {
         __syncthreads();
-        for (int i = threadIdx.x; i < slot * NV; i += blockDim.x) {
-            const int sl = i / NV, j = i - sl * NV;
-            float acc = 0.f;
-            for (int w = 0; w < nwaves; w++) acc += red[(sl * nwaves + w) * NV + j];
-            p.partial[((int64_t)(adc_batch0 + sl) * nblk + blk) * NV + j] = acc;
-        }
+        flush_partials(red, slot, nwaves, NV, p.partial, adc_batch0, nblk, blk, NV);
     }
     if (valid) {
 #pragma unroll
@@ -409,13 +305,15 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
 }
 
 // Neumaier-compensated fp32 accumulation of block partials per lane, then
-// an fp64 butterfly across the wave; one wave per (sample, type).
+// an fp64 butterfly across the wave; one wave per (sample, type). The
+// partials of type t are blocks [t*bstep, t*bstep + bcount) at value offset
+// t*vstep within each block's vstride values.
 template <int NV>
-__global__ __launch_bounds__(64) void signal_reduce_kernel(const float *__restrict__ partial, int nblk,
-                                                           int blk_per_type, int Rx, int rx_default,
-                                                           const double *__restrict__ adc_ref,
-                                                           int64_t adc0, int64_t N, double *Sx,
-                                                           double *Sy) {
+__global__ __launch_bounds__(64) void signal_reduce_kernel(const float *__restrict__ partial, int nblk, int bstep,
+                                                           int bcount, int vstride, int vstep, int Rx,
+                                                           int rx_default, const double *__restrict__ adc_ref,
+                                                           int64_t adc0, int64_t N, double *Sx, double *Sy) {
+    __shared__ double tot[NV];
     const int a = blockIdx.x;
     const int t = blockIdx.y;
     const int lane = threadIdx.x;
@@ -425,9 +323,10 @@ __global__ __launch_bounds__(64) void signal_reduce_kernel(const float *__restri
         sum[j] = 0.f;
         comp[j] = 0.f;
     }
-    const int b0 = t * blk_per_type;
-    for (int b = lane; b < blk_per_type; b += 64) {
-        const float *src = partial + ((int64_t)a * nblk + b0 + b) * NV;
+    const int b0 = t * bstep;
+    const int voff = t * vstep;
+    for (int b = lane; b < bcount; b += 64) {
+        const float *src = partial + ((int64_t)a * nblk + b0 + b) * vstride + voff;
 #pragma unroll
         for (int j = 0; j < NV; j++) {
             const float xv = src[j];
@@ -439,29 +338,27 @@ __global__ __launch_bounds__(64) void signal_reduce_kernel(const float *__restri
             sum[j] = s;
         }
     }
-    double tot[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++) {
         double d = (double)sum[j] + (double)comp[j];
         for (int o = 32; o >= 1; o >>= 1) d += __shfl_xor(d, o);
-        tot[j] = d;
+        if (lane == j) tot[j] = d;
     }
-    if (lane == 0) {
-        const double ref = adc_ref[adc0 + a];
-        const double cr = cos(-ref), sr = sin(-ref);
-        const int64_t n = adc0 + a;
-        for (int c = 0; c < Rx; c++) {
-            const int cc = rx_default ? 0 : c;
-            double re = tot[2 * cc], im = tot[2 * cc + 1];
-            if (ref != 0) {  // DoScanAtCPU.cpp:472-479: multiply by exp(-i*rfRef)
-                const double r2 = re * cr - im * sr;
-                const double i2 = re * sr + im * cr;
-                re = r2;
-                im = i2;
-            }
-            Sx[(int64_t)t * Rx * N + (int64_t)c * N + n] += re;
-            Sy[(int64_t)t * Rx * N + (int64_t)c * N + n] += im;
+    __syncthreads();
+    const double ref = adc_ref[adc0 + a];
+    const double cr = cos(-ref), sr = sin(-ref);
+    const int64_t n = adc0 + a;
+    for (int c = lane; c < Rx; c += 64) {
+        const int cc = rx_default ? 0 : c;
+        double re = tot[2 * cc], im = tot[2 * cc + 1];
+        if (ref != 0) {  // DoScanAtCPU.cpp:472-479: multiply by exp(-i*rfRef)
+            const double r2 = re * cr - im * sr;
+            const double i2 = re * sr + im * cr;
+            re = r2;
+            im = i2;
         }
+        Sx[(int64_t)t * Rx * N + (int64_t)c * N + n] += re;
+        Sy[(int64_t)t * Rx * N + (int64_t)c * N + n] += im;
     }
 }
 
@@ -489,52 +386,61 @@ static mrs_status fail(mrs_status st, const std::string &msg) {
 
 typedef void (*fwd_fn)(FwdParams, const uint32_t *, int64_t, int64_t);
 
-template <int SPT, int NRX, bool GEN>
+template <int SPT, int NRX, bool GEN, int PREC>
 static void launch_fwd(dim3 grid, dim3 block, size_t lds, hipStream_t st, const FwdParams &p,
                        const uint32_t *rec, int r0, int r1) {
-    bloch_fwd_kernel<SPT, NRX, GEN><<<grid, block, lds, st>>>(p, rec, r0, r1);
+    bloch_fwd_kernel<SPT, NRX, GEN, PREC><<<grid, block, lds, st>>>(p, rec, r0, r1);
 }
 
 typedef void (*launch_fn)(dim3, dim3, size_t, hipStream_t, const FwdParams &, const uint32_t *, int, int);
 
-template <int SPT, bool GEN>
+template <int SPT, bool GEN, int PREC>
 static launch_fn pick_nrx(int nrx) {
     switch (nrx) {
-        case 1: return launch_fwd<SPT, 1, GEN>;
-        case 2: return launch_fwd<SPT, 2, GEN>;
-        case 4: return launch_fwd<SPT, 4, GEN>;
-        case 8: return launch_fwd<SPT, 8, GEN>;
-        case 16: return launch_fwd<SPT, 16, GEN>;
-        case 32: return launch_fwd<SPT, 32, GEN>;
+        case 1: return launch_fwd<SPT, 1, GEN, PREC>;
+        case 2: return launch_fwd<SPT, 2, GEN, PREC>;
+        case 4: return launch_fwd<SPT, 4, GEN, PREC>;
+        case 8: return launch_fwd<SPT, 8, GEN, PREC>;
+        case 16: return launch_fwd<SPT, 16, GEN, PREC>;
+        case 32: return launch_fwd<SPT, 32, GEN, PREC>;
     }
     return nullptr;
 }
 
-template <bool GEN>
+template <bool GEN, int PREC>
 static launch_fn pick_spt(int spt, int nrx) {
     switch (spt) {
-        case 1: return pick_nrx<1, GEN>(nrx);
-        case 2: return pick_nrx<2, GEN>(nrx);
-        case 4: return pick_nrx<4, GEN>(nrx);
-        case 8: return pick_nrx<8, GEN>(nrx);
+        case 1: return pick_nrx<1, GEN, PREC>(nrx);
+        case 2: return pick_nrx<2, GEN, PREC>(nrx);
+        case 4: return pick_nrx<4, GEN, PREC>(nrx);
+        case 8: return pick_nrx<8, GEN, PREC>(nrx);
     }
     return nullptr;
 }
 
-static launch_fn pick_fwd(int spt, int nrx, bool gen) {
-    return gen ? pick_spt<true>(spt, nrx) : pick_spt<false>(spt, nrx);
+// Forward kernel for spt spins per thread, nrx (padded) receive coils,
+// the general variant (multi-Tx, capture) or not, and relaxation form prec.
+static launch_fn pick_fwd(int spt, int nrx, bool gen, int prec) {
+    if (prec)
+        return gen ? pick_spt<true, 1>(spt, nrx) : pick_spt<false, 1>(spt, nrx);
+    return gen ? pick_spt<true, 0>(spt, nrx) : pick_spt<false, 0>(spt, nrx);
 }
 
-static void launch_reduce(int nv, dim3 grid, hipStream_t st, const float *partial, int nblk, int bpt, int Rx,
-                          int rxd, const double *ref, int64_t adc0, int64_t N, double *Sx, double *Sy) {
+void launch_signal_reduce(int nv, dim3 grid, hipStream_t st, const float *partial, int nblk, int bstep,
+                          int bcount, int vstride, int vstep, int Rx, int rxd, const double *ref, int64_t adc0,
+                          int64_t N, double *Sx, double *Sy) {
+#define MRS_RED(NV_)                                                                                         \
+    signal_reduce_kernel<NV_><<<grid, 64, 0, st>>>(partial, nblk, bstep, bcount, vstride, vstep, Rx, rxd, ref, \
+                                                   adc0, N, Sx, Sy)
     switch (nv) {
-        case 2: signal_reduce_kernel<2><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
-        case 4: signal_reduce_kernel<4><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
-        case 8: signal_reduce_kernel<8><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
-        case 16: signal_reduce_kernel<16><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
-        case 32: signal_reduce_kernel<32><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
-        case 64: signal_reduce_kernel<64><<<grid, 64, 0, st>>>(partial, nblk, bpt, Rx, rxd, ref, adc0, N, Sx, Sy); break;
+        case 2: MRS_RED(2); break;
+        case 4: MRS_RED(4); break;
+        case 8: MRS_RED(8); break;
+        case 16: MRS_RED(16); break;
+        case 32: MRS_RED(32); break;
+        case 64: MRS_RED(64); break;
     }
+#undef MRS_RED
 }
 
 struct DevBuf {
@@ -569,6 +475,9 @@ struct mrs_plan {
     bool ran = false;
     float last_ms = 0.f;
     int timing = 0;
+    int model = 0, scheme = 0;        // 1: generalized multi-pool engine
+    DevBuf K;
+    int flag[kGmMaxTypes] = {0};
     std::vector<hipEvent_t> tev;  // per launch: fwd begin, fwd end, reduce end
     double fwd_ms = 0.0, red_ms = 0.0;
     ~mrs_plan() {
@@ -675,6 +584,14 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     P->rx_default = vc->RxCoilDefault != 0;
     P->precision = opt ? opt->precision_mode : 0;
     P->timing = opt ? opt->timing : 0;
+    P->model = opt ? opt->model : 0;
+    P->scheme = opt ? opt->multipool_scheme : 0;
+    if (P->model == 1) {
+        if (!vo->K || !vo->TypeFlag) return fail(MRS_ERR_ARG, "multi-pool scan needs VObj.K and VObj.TypeFlag");
+        if (vo->TypeNum < 2 || vo->TypeNum > kGmMaxTypes)
+            return fail(MRS_ERR_UNSUPPORTED, "multi-pool engine supports TypeNum 2..4");
+        for (int t = 0; t < vo->TypeNum; t++) P->flag[t] = (int)vo->TypeFlag[t];
+    }
     P->run_mode = vt->RunMode;
     P->signal_num = vg->SignalNum;
     P->n_uts = vs->n_uts;
@@ -686,25 +603,29 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     while (spt > 1 && (P->S % spt != 0 || spt > P->S)) spt >>= 1;
     if (spt != 1 && spt != 2 && spt != 4 && spt != 8) return fail(MRS_ERR_ARG, "spins_per_thread must be 1, 2, 4 or 8");
     if (P->S % spt) return fail(MRS_ERR_ARG, "spins_per_thread must divide SpinNum");
+    if (P->model == 1) spt = 1;
     P->spt = spt;
     P->ngroups = P->S / spt;
     int block = opt && opt->block_size > 0 ? opt->block_size : (P->nvox >= (int64_t)256 * 512 ? 256 : 64);
     if (block % 64 || block > 256) return fail(MRS_ERR_ARG, "block_size must be 64, 128, 192 or 256");
     P->block = block;
     P->gx = (int)((P->nvox + block - 1) / block);
-    if ((int64_t)P->Ty * P->ngroups > 65535) return fail(MRS_ERR_UNSUPPORTED, "TypeNum*SpinNum/spins_per_thread too large");
-    P->nblk = P->gx * P->Ty * P->ngroups;
+    const int gy = P->model == 1 ? P->ngroups : P->Ty * P->ngroups;
+    if (gy > 65535) return fail(MRS_ERR_UNSUPPORTED, "TypeNum*SpinNum/spins_per_thread too large");
+    P->nblk = P->gx * gy;
     int nrx = 1;
     const int rxn = P->rx_default ? 1 : P->Rx;
     while (nrx < rxn) nrx <<= 1;
     P->nrx = nrx;
-    P->nv = 2 * nrx;
+    P->nv = P->model == 1 ? gm_values_per_isochromat(P->Ty, nrx) : 2 * nrx;
+    if (P->model == 1 && (nrx > 8 || !pick_gm(P->Ty, nrx, P->scheme)))
+        return fail(MRS_ERR_UNSUPPORTED, "multi-pool engine supports up to 8 receive coils");
     const size_t budget = (size_t)512 << 20;  // partial-buffer budget
     int64_t max_adc = (int64_t)(budget / ((size_t)P->nblk * P->nv * sizeof(float)));
     max_adc = std::max<int64_t>(kRedBatch, std::min<int64_t>(max_adc, 65536));
     if (opt && opt->max_adc_per_launch > 0) max_adc = opt->max_adc_per_launch;
     P->max_adc = max_adc;
-    if (!pick_fwd(P->spt, P->nrx, true)) return fail(MRS_ERR_UNSUPPORTED, "no kernel for this coil count");
+    if (!pick_fwd(P->spt, P->nrx, true, 0)) return fail(MRS_ERR_UNSUPPORTED, "no kernel for this coil count");
 
     hipStream_t st = P->stream;
     const size_t vb = (size_t)P->nvox * sizeof(float);
@@ -715,6 +636,7 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
         HIP_TRY(hipMalloc(&P->M[k].p, P->ntot * sizeof(float)));
     }
     if ((rc = upload(P->dWRnd, vm->dWRnd, P->ntot * sizeof(float), st))) return rc;
+    if (P->model == 1 && (rc = upload(P->K, vo->K, vb * P->Ty * P->Ty, st))) return rc;
     if ((rc = upload(P->Rho, vo->Rho, vb * P->Ty, st))) return rc;
     if ((rc = upload(P->T1, vo->T1, vb * P->Ty, st))) return rc;
     if ((rc = upload(P->T2, vo->T2, vb * P->Ty, st))) return rc;
@@ -791,6 +713,69 @@ static mrs_status upload_program(mrs_plan *P) {
     return MRS_OK;
 }
 
+// Multi-pool launches: one thread per (voxel, isochromat), all pools.
+static mrs_status execute_gm(mrs_plan *P) {
+    GmParams gp;
+    std::memset(&gp, 0, sizeof gp);
+    gp.Mx = (float *)P->M[0].p;
+    gp.My = (float *)P->M[1].p;
+    gp.Mz = (float *)P->M[2].p;
+    gp.dWRnd = (const float *)P->dWRnd.p;
+    gp.Rho = (const float *)P->Rho.p;
+    gp.T1 = (const float *)P->T1.p;
+    gp.T2 = (const float *)P->T2.p;
+    gp.K = (const float *)P->K.p;
+    gp.dB0 = (const float *)P->dB0.p;
+    gp.gx = (const float *)P->gxg.p;
+    gp.gy = (const float *)P->gyg.p;
+    gp.gz = (const float *)P->gzg.p;
+    gp.txmg = (const float *)P->txmg.p;
+    gp.txpe = (const float *)P->txpe.p;
+    gp.rxx = (const float *)P->rxx.p;
+    gp.rxy = (const float *)P->rxy.p;
+    gp.partial = (float *)P->partial.p;
+    gp.nvox = P->nvox;
+    gp.S = P->S;
+    gp.Ty = P->Ty;
+    gp.Tx = P->Tx;
+    gp.Rx = P->Rx;
+    gp.rx_default = P->rx_default;
+    gp.stride = P->dec.stride;
+    gp.gyro = P->gyro;
+    for (int t = 0; t < P->Ty; t++) {
+        gp.cs[t] = P->cs[t];
+        gp.flag[t] = P->flag[t];
+    }
+    gm_launch_fn fn = pick_gm(P->Ty, P->nrx, P->scheme);
+    const dim3 grid(P->gx, P->ngroups), block(P->block);
+    const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
+    const uint32_t *rec = (const uint32_t *)P->rec.p;
+    const size_t nl = P->launch_r0.size();
+    if (P->timing) {
+        while (P->tev.size() < 3 * nl) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            P->tev.push_back(e);
+        }
+    }
+    for (size_t l = 0; l < nl; l++) {
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l], P->stream));
+        fn(grid, block, lds, P->stream, gp, rec, (int)P->launch_r0[l], (int)P->launch_r1[l]);
+        HIP_TRY(hipGetLastError());
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 1], P->stream));
+        const int64_t na = P->launch_nadc[l];
+        if (na > 0) {
+            launch_signal_reduce(2 * P->nrx, dim3((unsigned)na, P->Ty), P->stream, (const float *)P->partial.p,
+                                 P->nblk, 0, P->nblk, P->nv, 2 * P->nrx, P->Rx, P->rx_default,
+                                 (const double *)P->adc_ref.p, P->launch_adc0[l], P->signal_num, (double *)P->Sx.p,
+                                 (double *)P->Sy.p);
+            HIP_TRY(hipGetLastError());
+        }
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 2], P->stream));
+    }
+    return MRS_OK;
+}
+
 // Runs every launch of the uploaded program on the plan stream.
 // adc_base: index of the program's first ADC sample in the scan.
 static mrs_status execute(mrs_plan *P, int64_t adc_base) {
@@ -827,7 +812,8 @@ static mrs_status execute(mrs_plan *P, int64_t adc_base) {
     fp.stride = P->dec.stride;
     fp.gyro = P->gyro;
     for (int t = 0; t < P->Ty; t++) fp.cs[t] = P->cs[t];
-    launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1);
+    if (P->model == 1) return execute_gm(P);
+    launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1, P->precision != 0);
     const dim3 grid(P->gx, P->Ty * P->ngroups), block(P->block);
     const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
     const uint32_t *rec = (const uint32_t *)P->rec.p;
@@ -846,9 +832,10 @@ static mrs_status execute(mrs_plan *P, int64_t adc_base) {
         if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 1], P->stream));
         const int64_t na = P->launch_nadc[l];
         if (na > 0) {
-            launch_reduce(P->nv, dim3((unsigned)na, P->Ty), P->stream, (const float *)P->partial.p, P->nblk,
-                          P->gx * P->ngroups, P->Rx, P->rx_default, (const double *)P->adc_ref.p,
-                          P->launch_adc0[l], P->signal_num, (double *)P->Sx.p, (double *)P->Sy.p);
+            launch_signal_reduce(P->nv, dim3((unsigned)na, P->Ty), P->stream, (const float *)P->partial.p, P->nblk,
+                                 P->gx * P->ngroups, P->gx * P->ngroups, P->nv, 0, P->Rx, P->rx_default,
+                                 (const double *)P->adc_ref.p, P->launch_adc0[l], P->signal_num, (double *)P->Sx.p,
+                                 (double *)P->Sy.p);
             HIP_TRY(hipGetLastError());
         }
         if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 2], P->stream));
@@ -891,7 +878,8 @@ mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs
         return rc;
     }
     P->dec.tx = P->Tx;
-    P->dec.stride = rec_stride(P->Tx);
+    P->dec.coil_words = P->model == 1 ? 4 : 2;
+    P->dec.stride = rec_stride(P->Tx, P->dec.coil_words);
     Decoder dec(vobj, vcoi, vctl, vseq, vvar, false);
     dec.run(P->dec);
     dec.flush_tail(P->dec);
@@ -1053,7 +1041,8 @@ static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, cons
     for (;;) {
         P->dec = Decoded();
         P->dec.tx = P->Tx;
-        P->dec.stride = rec_stride(P->Tx);
+        P->dec.coil_words = P->model == 1 ? 4 : 2;
+        P->dec.stride = rec_stride(P->Tx, P->dec.coil_words);
         if (!all.muts.empty()) P->dec.muts = all.muts;
         const DecodeStop stop = dec.run(P->dec);
         if (stop == DECODE_END) dec.flush_tail(P->dec);
@@ -1128,6 +1117,16 @@ mrs_status mrs_DoScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi 
     if (!rc) rc = mrs_plan_fetch(P, vobj, vvar, vsig);
     mrs_plan_destroy(P);
     return rc;
+}
+
+mrs_status mrs_DoGMScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                             const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
+                             const mrs_options *opt) {
+    mrs_options o;
+    std::memset(&o, 0, sizeof o);
+    if (opt) o = *opt;
+    o.model = 1;
+    return mrs_DoScanAtGPU(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, &o);
 }
 
 mrs_status mrs_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,

@@ -1,0 +1,447 @@
+// Multi-pool (generalized model) Bloch kernel for gfx950.
+//
+// MRiLab's GM engine (Lib/src/engine/GPUEngine/BlochKernelGM.cuh:6-487)
+// evolves TypeNum pools per isochromat: free pools (TypeFlag 0) rotate,
+// relax and exchange; bound pools (TypeFlag 1) carry only Mz and are
+// saturated by rf through a super-Lorentzian line. Two schemes:
+//   SCHEME 0: the reference's first-order operator split in fp32, in its
+//             statement order (rotate, relax, then exchange with the
+//             start-of-step magnetization of the other pools);
+//   SCHEME 1: the exact solution of the same linear Bloch-McConnell system
+//             over each held step, in fp64: the Taylor series of the action
+//             of exp(A*dt) on the state, with substeps of h*|A| <= 0.5 and the
+//             structured generator applied directly (3x3 rotation-relaxation
+//             blocks, scalar bound-pool rows, diagonal exchange couplings).
+// One thread owns one isochromat of one voxel with all its pools; all pool
+// state stays in registers (fp64 for SCHEME 1) across the launch.
+#include "multipool.hpp"
+
+#include "decode.hpp"
+#include "device_common.hpp"
+
+namespace mrs {
+
+__device__ __forceinline__ float expf_cr(float x) { return (float)exp((double)x); }
+
+// Bound-pool lineshape G(dw) as BlochKernelGM.cuh:339-344 evaluates it: a
+// 1000-point Riemann sum over u = 0, 0.001, ... (u accumulated by repeated
+// addition); T is float for the split scheme and double for the exact one.
+template <typename T>
+__device__ T superlorentzian(double dw, double t2b) {
+    T u = 0, G = 0;
+    const T c = (T)sqrt(2.0 / 3.14159265359);
+    const T w = (T)dw, t2 = (T)t2b;
+    for (int k = 0; k < 1000; k++) {
+        const T den = (T)3 * u * u - (T)1;
+        const T b = (w * t2) / den;
+        T e;
+        if constexpr (sizeof(T) == 4)
+            e = expf_cr(-2 * b * b);
+        else
+            e = exp(-2 * b * b);
+        G += (c * (t2 / fabs(den)) * e) * (T)0.001;
+        u += (T)0.001;
+    }
+    return G;
+}
+
+template <int TY, typename T>
+struct GmState {
+    T mx[TY], my[TY], mz[TY];
+};
+
+template <int TY, int NRX, int SCHEME>
+__global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_t *__restrict__ rec_g, int r0,
+                                                       int r1) {
+    using T = typename std::conditional<SCHEME == 0, float, double>::type;
+    constexpr int NV = 2 * NRX;
+    constexpr int NVT = (NV * TY <= 2) ? 2 : (NV * TY <= 4) ? 4 : (NV * TY <= 8) ? 8 : (NV * TY <= 16) ? 16
+                                                                 : (NV * TY <= 32) ? 32 : 64;
+    static_assert(NV * TY <= 64, "too many signal values per isochromat");
+    extern __shared__ float red[];  // [kRedBatch][nwaves][NVT]
+    const crec_t rec = (crec_t)rec_g;
+    const int cw = 4;
+    const int stride = p.stride;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int64_t nvox = p.nvox;
+    const int s = blockIdx.y;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = v < nvox;
+    const int64_t vv = valid ? v : nvox - 1;
+    const int nblk = gridDim.x * gridDim.y;
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+
+    const float x = p.gx[vv], y = p.gy[vv], z = p.gz[vv];
+    const float dB0 = p.dB0[vv];
+    const float w0 = dB0 * p.gyro;
+    GmState<TY, T> st;
+    int64_t sidx[TY];
+    float rho[TY], t1[TY], t2[TY], wb[TY];
+    bool skip[TY];
+    float kmat[TY][TY];  // kmat[t][q]: rate from pool t into pool q
+#pragma unroll
+    for (int t = 0; t < TY; t++) {
+        sidx[t] = vv + nvox * ((int64_t)s + (int64_t)p.S * t);
+        st.mx[t] = p.Mx[sidx[t]];
+        st.my[t] = p.My[sidx[t]];
+        st.mz[t] = p.Mz[sidx[t]];
+        const int64_t tv = vv + (int64_t)t * nvox;
+        rho[t] = p.Rho[tv];
+        t1[t] = p.T1[tv];
+        t2[t] = p.T2[tv];
+        skip[t] = t1[t] == 0.f || t2[t] == 0.f || rho[t] == 0.f;  // BlochKernelGM.cuh:176
+        wb[t] = (w0 + p.dWRnd[sidx[t]]) + p.cs[t];
+#pragma unroll
+        for (int q = 0; q < TY; q++) kmat[t][q] = q == t ? 0.f : p.K[vv + nvox * ((int64_t)q * TY + t)];
+    }
+    float crx[NRX], cry[NRX];
+#pragma unroll
+    for (int c = 0; c < NRX; c++) {
+        if (p.rx_default) {
+            crx[c] = 1.f;
+            cry[c] = 0.f;
+        } else if (c < p.Rx) {
+            crx[c] = p.rxx[vv + (int64_t)c * nvox];
+            cry[c] = p.rxy[vv + (int64_t)c * nvox];
+        } else {
+            crx[c] = 0.f;
+            cry[c] = 0.f;
+        }
+    }
+    // per-dt factors of the split scheme
+    float e1[TY], e2[TY], m0[TY], kout[TY], fin[TY][TY];
+#pragma unroll
+    for (int t = 0; t < TY; t++) {
+        e1[t] = e2[t] = kout[t] = 1.f;
+        m0[t] = 0.f;
+#pragma unroll
+        for (int q = 0; q < TY; q++) fin[t][q] = 0.f;
+    }
+    // lineshape cache per pool (key: rf frequency, T2 of the bound pool)
+    float gkey[TY];
+    double gval[TY];
+    bool gok[TY];
+#pragma unroll
+    for (int t = 0; t < TY; t++) gok[t] = false;
+    bool have_e = false;
+    int slot = 0;
+    int adc_batch0 = 0;
+
+    for (int r = r0; r < r1; ++r) {
+        const crec_t q = rec + r * stride;
+        const uint32_t flags = q[5];
+        if (flags & REC_SPOIL) {  // Plugin_IdealSpoiler.m:10-11 zeroes every pool
+#pragma unroll
+            for (int t = 0; t < TY; t++) {
+                st.mx[t] = 0;
+                st.my[t] = 0;
+            }
+        }
+        if (flags & REC_ADC) {  // free pools only (BlochKernelGM.cuh:192)
+            float val[NVT];
+#pragma unroll
+            for (int j = 0; j < NVT; j++) val[j] = 0.f;
+#pragma unroll
+            for (int t = 0; t < TY; t++) {
+                if (p.flag[t] != 0) continue;
+                const float sx = (valid && !skip[t]) ? (float)st.mx[t] : 0.f;
+                const float sy = (valid && !skip[t]) ? (float)st.my[t] : 0.f;
+#pragma unroll
+                for (int c = 0; c < NRX; c++) {
+                    val[t * NV + 2 * c] = sx * crx[c] + sy * cry[c];
+                    val[t * NV + 2 * c + 1] = sy * crx[c] - sx * cry[c];
+                }
+            }
+            WaveSum<NVT, 32>::run(val, lane);
+            constexpr int G = 64 / NVT;
+            if ((lane & (G - 1)) == 0) red[(slot * nwaves + wave) * NVT + lane / G] = val[0];
+            if (++slot == kRedBatch) {
+                __syncthreads();
+                flush_partials(red, kRedBatch, nwaves, NVT, p.partial, adc_batch0, nblk, blk, NVT);
+                __syncthreads();
+                adc_batch0 += kRedBatch;
+                slot = 0;
+            }
+        }
+        if (flags & REC_NOSTEP) continue;
+        const float dt = __uint_as_float(q[0]);
+        const bool rf = (flags & REC_RF) != 0;
+        const float gv = z * __uint_as_float(q[1]) + y * __uint_as_float(q[2]) + x * __uint_as_float(q[3]);
+        const float wrf = __uint_as_float(q[4]);
+        // effective B1 (gamma-scaled) and bound-pool saturation rates
+        float bx = 0.f, by = 0.f;
+        float wsat[TY];
+#pragma unroll
+        for (int t = 0; t < TY; t++) wsat[t] = 0.f;
+        if (rf) {
+            double wsum = 0.0;
+            int nact = 0;
+            for (int c = 0; c < p.Tx; c++) {
+                const float br = __uint_as_float(q[REC_HEAD + cw * c]);
+                const float bi = __uint_as_float(q[REC_HEAD + cw * c + 1]);
+                const float A = __uint_as_float(q[REC_HEAD + cw * c + 2]);
+                const float F = __uint_as_float(q[REC_HEAD + cw * c + 3]);
+                if (A == 0.f) continue;
+                float sp, cp;
+                sincos_cr(p.txpe[vv + (int64_t)c * nvox], &sp, &cp);
+                const float mg = p.txmg[vv + (int64_t)c * nvox];
+                bx += mg * (cp * br - sp * bi);
+                by += mg * (cp * bi + sp * br);
+                // BlochKernelGM.cuh:323-380: W = pi*(gamma*B1)^2*G, averaged over active coils
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    if (p.flag[t] != 1 || skip[t]) continue;
+                    if (!gok[t] || gkey[t] != F) {
+                        gkey[t] = F;
+                        gok[t] = true;
+                        gval[t] = (double)superlorentzian<T>(2 * 3.14159265359 * (double)F + (double)dB0 * p.gyro,
+                                                             (double)t2[t]);
+                    }
+                    const float bg = (mg * A) * p.gyro;
+                    if (SCHEME == 0)
+                        wsat[t] += (float)(3.14159265359f * bg * bg * (float)gval[t]);
+                    else
+                        wsat[t] += (float)(3.14159265359 * (double)bg * bg * gval[t]);
+                }
+                wsum += 1.0;
+                nact++;
+            }
+            if (nact > 1) {
+#pragma unroll
+                for (int t = 0; t < TY; t++) wsat[t] = wsat[t] / (float)nact;
+            }
+            (void)wsum;
+        }
+        const int nrep = 1 + (int)(flags >> REC_REP_SHIFT);
+        if constexpr (SCHEME == 0) {
+#pragma clang fp contract(off)
+            if ((flags & REC_NEWDT) || !have_e) {
+                have_e = true;
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    e1[t] = expf_cr(-dt / t1[t]);
+                    e2[t] = expf_cr(-dt / t2[t]);
+                    m0[t] = (rho[t] * (1.f - e1[t])) / (float)p.S;
+                    float ko = 1.f;
+#pragma unroll
+                    for (int qq = 0; qq < TY; qq++) {
+                        if (qq == t) continue;
+                        if (kmat[t][qq] != 0.f) ko = ko * expf_cr(-dt * kmat[t][qq]);
+                        fin[t][qq] = kmat[qq][t] != 0.f ? 1.f - expf_cr(-dt * kmat[qq][t]) : 0.f;
+                    }
+                    kout[t] = ko;
+                }
+            }
+            for (int k = 0; k < nrep; k++) {
+                float Bx[TY], By[TY], Bz[TY];
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    Bx[t] = st.mx[t];
+                    By[t] = st.my[t];
+                    Bz[t] = st.mz[t];
+                }
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    if (skip[t] || (p.flag[t] != 0 && p.flag[t] != 1)) continue;
+                    float ax = Bx[t], ay = By[t], az = Bz[t];
+                    if (p.flag[t] == 0) {
+                        const float w = rf ? (wb[t] + gv) + wrf : wb[t] + gv;
+                        if (rf) {
+                            const float om2 = (bx * bx + by * by) + w * w;
+                            if (om2 > 0.f) {
+                                const float om = sqrtf(om2);
+                                const float inv = 1.0f / om;
+                                const float nx = bx * inv, ny = by * inv, nz = w * inv;
+                                float sn, cs;
+                                sincos_cr(om * dt, &sn, &cs);
+                                const float kk = (1.f - cs) * (nx * ax + ny * ay + nz * az);
+                                const float rx = ax * cs - (ny * az - nz * ay) * sn + nx * kk;
+                                const float ry = ay * cs - (nz * ax - nx * az) * sn + ny * kk;
+                                const float rz = az * cs - (nx * ay - ny * ax) * sn + nz * kk;
+                                ax = rx;
+                                ay = ry;
+                                az = rz;
+                            }
+                        } else {
+                            float sn, cs;
+                            sincos_cr(w * dt, &sn, &cs);
+                            const float rx = ax * cs + ay * sn;
+                            const float ry = ay * cs - ax * sn;
+                            ax = rx;
+                            ay = ry;
+                        }
+                        ax = ax * e2[t];
+                        ay = ay * e2[t];
+                    } else if (rf) {
+                        az = az * expf_cr(-wsat[t] * dt);
+                    }
+                    az = az * e1[t];
+                    float kx = 0.f, ky = 0.f, kz = 0.f;
+#pragma unroll
+                    for (int qq = 0; qq < TY; qq++) {
+                        if (qq == t || fin[t][qq] == 0.f) continue;
+                        if (p.flag[t] == 0) {
+                            kx = kx + fin[t][qq] * Bx[qq];
+                            ky = ky + fin[t][qq] * By[qq];
+                        }
+                        kz = kz + fin[t][qq] * Bz[qq];
+                    }
+                    if (p.flag[t] == 0) {
+                        st.mx[t] = ax * kout[t] + kx;
+                        st.my[t] = ay * kout[t] + ky;
+                    }
+                    st.mz[t] = az * kout[t] + kz + m0[t];
+                }
+            }
+        } else {
+            // exact step: x <- exp(A*dt) x with the affine M0 term
+            double om[TY], r1d[TY], r2d[TY], kod[TY], aff[TY];
+            double nrm = 0.0;
+            const double bxd = bx, byd = by;
+#pragma unroll
+            for (int t = 0; t < TY; t++) {
+                om[t] = 0;
+                r1d[t] = 0;
+                r2d[t] = 0;
+                kod[t] = 0;
+                aff[t] = 0;
+                if (skip[t]) continue;
+                double ko = 0.0, kin = 0.0;
+#pragma unroll
+                for (int qq = 0; qq < TY; qq++) {
+                    ko += kmat[t][qq];
+                    kin += kmat[qq][t];
+                }
+                kod[t] = ko;
+                r1d[t] = 1.0 / (double)t1[t];
+                r2d[t] = 1.0 / (double)t2[t];
+                aff[t] = r1d[t] * (double)rho[t] / p.S;
+                double rowmax;
+                if (p.flag[t] == 0) {
+                    om[t] = (double)wb[t] + (double)gv + (rf ? (double)wrf : 0.0);
+                    rowmax = fabs(om[t]) + fabs(bxd) + fabs(byd) + r1d[t] + r2d[t] + ko + kin;
+                } else {
+                    r2d[t] = 0.0;
+                    rowmax = (double)wsat[t] + r1d[t] + ko + kin;
+                }
+                nrm = fmax(nrm, rowmax);
+            }
+            int nsub = (int)ceil(nrm * (double)dt * 2.0);
+            if (nsub < 1) nsub = 1;
+            const double h = (double)dt / nsub;
+            auto apply = [&](const double *in, double *out) {
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    double sx = 0, sy = 0, sz = 0;
+#pragma unroll
+                    for (int qq = 0; qq < TY; qq++) {
+                        if (qq == t) continue;
+                        const double kk = kmat[qq][t];
+                        sx += kk * in[3 * qq];
+                        sy += kk * in[3 * qq + 1];
+                        sz += kk * in[3 * qq + 2];
+                    }
+                    const double ix = in[3 * t], iy = in[3 * t + 1], iz = in[3 * t + 2];
+                    if (skip[t]) {
+                        out[3 * t] = out[3 * t + 1] = out[3 * t + 2] = 0.0;
+                    } else if (p.flag[t] == 0) {
+                        out[3 * t] = om[t] * iy - byd * iz - (r2d[t] + kod[t]) * ix + sx;
+                        out[3 * t + 1] = bxd * iz - om[t] * ix - (r2d[t] + kod[t]) * iy + sy;
+                        out[3 * t + 2] = byd * ix - bxd * iy - (r1d[t] + kod[t]) * iz + sz;
+                    } else {
+                        out[3 * t] = 0.0;
+                        out[3 * t + 1] = 0.0;
+                        out[3 * t + 2] = -((double)wsat[t] + r1d[t] + kod[t]) * iz + sz;
+                    }
+                }
+            };
+            for (int k = 0; k < nrep; k++) {
+                for (int sub = 0; sub < nsub; sub++) {
+                    double xv[3 * TY], term[3 * TY], nt[3 * TY];
+#pragma unroll
+                    for (int t = 0; t < TY; t++) {
+                        xv[3 * t] = st.mx[t];
+                        xv[3 * t + 1] = st.my[t];
+                        xv[3 * t + 2] = st.mz[t];
+                    }
+                    apply(xv, nt);
+#pragma unroll
+                    for (int t = 0; t < TY; t++) nt[3 * t + 2] += skip[t] ? 0.0 : aff[t];
+#pragma unroll
+                    for (int i = 0; i < 3 * TY; i++) {
+                        term[i] = h * nt[i];
+                        xv[i] += term[i];
+                    }
+                    for (int kk = 2; kk <= 14; kk++) {
+                        apply(term, nt);
+                        const double f = h / kk;
+#pragma unroll
+                        for (int i = 0; i < 3 * TY; i++) {
+                            term[i] = f * nt[i];
+                            xv[i] += term[i];
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < TY; t++) {
+                        st.mx[t] = xv[3 * t];
+                        st.my[t] = xv[3 * t + 1];
+                        st.mz[t] = xv[3 * t + 2];
+                    }
+                }
+            }
+        }
+    }
+    if (slot > 0) {
+        __syncthreads();
+        flush_partials(red, slot, nwaves, NVT, p.partial, adc_batch0, nblk, blk, NVT);
+    }
+    if (valid) {
+#pragma unroll
+        for (int t = 0; t < TY; t++) {
+            p.Mx[sidx[t]] = (float)st.mx[t];
+            p.My[sidx[t]] = (float)st.my[t];
+            p.Mz[sidx[t]] = (float)st.mz[t];
+        }
+    }
+}
+
+template <int TY, int NRX, int SCHEME>
+static void launch_t(dim3 g, dim3 b, size_t lds, hipStream_t st, const GmParams &p, const uint32_t *rec, int r0,
+                     int r1) {
+    bloch_gm_kernel<TY, NRX, SCHEME><<<g, b, lds, st>>>(p, rec, r0, r1);
+}
+
+template <int TY, int NRX>
+static gm_launch_fn pick_scheme(int scheme) {
+    return scheme ? launch_t<TY, NRX, 1> : launch_t<TY, NRX, 0>;
+}
+
+template <int TY>
+static gm_launch_fn pick_nrx(int nrx, int scheme) {
+    switch (nrx) {
+        case 1: return pick_scheme<TY, 1>(scheme);
+        case 2: return pick_scheme<TY, 2>(scheme);
+        case 4: return pick_scheme<TY, 4>(scheme);
+        case 8: return pick_scheme<TY, 8>(scheme);
+    }
+    return nullptr;
+}
+
+gm_launch_fn pick_gm(int ty, int nrx, int scheme) {
+    switch (ty) {
+        case 2: return pick_nrx<2>(nrx, scheme);
+        case 3: return pick_nrx<3>(nrx, scheme);
+        case 4: return pick_nrx<4>(nrx, scheme);
+    }
+    return nullptr;
+}
+
+int gm_values_per_isochromat(int ty, int nrx) {
+    int n = 2;
+    while (n < 2 * nrx * ty) n <<= 1;
+    return n;
+}
+
+}  // namespace mrs

@@ -39,7 +39,7 @@ STATUS = {0: "MRS_OK", 1: "MRS_ERR_ARG", 2: "MRS_ERR_DEVICE", 3: "MRS_ERR_CALLBA
 
 # every symbol declared in include/mrilab_amd.h
 EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanAtCPU",
-           "mrs_DoScanAtGPU", "mrs_plan_create", "mrs_plan_run", "mrs_plan_sync",
+           "mrs_DoScanAtGPU", "mrs_DoGMScanAtGPU", "mrs_plan_create", "mrs_plan_run", "mrs_plan_sync",
            "mrs_plan_fetch", "mrs_plan_get_info", "mrs_plan_destroy")
 
 
@@ -61,7 +61,7 @@ def load() -> C.CDLL:
         P = C.POINTER
         scan_args = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC), P(abi.VSeqC),
                      P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC), P(abi.OptionsC)]
-        for fn in ("mrs_DoScanAtCPU", "mrs_DoScanAtGPU"):
+        for fn in ("mrs_DoScanAtCPU", "mrs_DoScanAtGPU", "mrs_DoGMScanAtGPU"):
             getattr(lib, fn).argtypes = scan_args
             getattr(lib, fn).restype = C.c_int
         lib.mrs_last_error.restype = C.c_char_p
@@ -90,10 +90,10 @@ def _check(rc: int) -> None:
 
 
 def make_options(block_size=0, spins_per_thread=0, max_adc_per_launch=0, multipool_scheme=0,
-                 precision_mode=0, timing=0) -> abi.OptionsC:
+                 precision_mode=0, timing=0, model=0) -> abi.OptionsC:
     return abi.OptionsC(block_size=block_size, spins_per_thread=spins_per_thread,
                         max_adc_per_launch=max_adc_per_launch, multipool_scheme=multipool_scheme,
-                        precision_mode=precision_mode, timing=timing)
+                        precision_mode=precision_mode, timing=timing, model=model)
 
 
 def _callbacks(ext: Optional[Callable], gpu_fetch: Optional[Callable]):
@@ -125,9 +125,9 @@ def _callbacks(ext: Optional[Callable], gpu_fetch: Optional[Callable]):
 
 
 def _scan(fn: str, VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=None,
-          options: Optional[abi.OptionsC] = None) -> None:
+          options: Optional[abi.OptionsC] = None, multipool: bool = False) -> None:
     lib = load()
-    P = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig)
+    P = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool=multipool)
     cb, keep = _callbacks(ext, gpu_fetch)
     rc = getattr(lib, fn)(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi), C.byref(P.vctl),
                           C.byref(P.vseq), C.byref(P.vvar), C.byref(P.vsig),
@@ -143,6 +143,13 @@ def DoScanAtGPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=No
     _scan("mrs_DoScanAtGPU", VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext, gpu_fetch, options)
 
 
+def DoGMScanAtGPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=None, options=None):
+    """Multi-pool scan on the GPU (DoGMScanAtGPU.cu contract): VObj carries K
+    [R,C,Z,Ty,Ty] (K(a,b) = rate a -> b) and TypeFlag (0 free, 1 bound)."""
+    _scan("mrs_DoGMScanAtGPU", VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext, gpu_fetch, options,
+          multipool=True)
+
+
 def DoScanAtCPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=None, options=None):
     """The DoScanAtCPU entry point (DoScanAtCPU.cpp:70 contract), executed on the GPU."""
     _scan("mrs_DoScanAtCPU", VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext, gpu_fetch, options)
@@ -153,7 +160,8 @@ class Plan:
 
     def __init__(self, VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, options=None):
         self._lib = load()
-        self._packed = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig)
+        mp = options is not None and options.model == 1
+        self._packed = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool=mp)
         self._h = C.c_void_p()
         P = self._packed
         _check(self._lib.mrs_plan_create(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi),

@@ -19,6 +19,9 @@ Workloads (sizes are parameters so tests can shrink them):
   grid (the single-slice parity case).
 * :func:`bssfp3d` -- 3-D balanced SSFP, hard pulse with alternating phase,
   rfRef demodulation, several receive coils.
+* :func:`mt_cest` -- three-pool exchange (free water, myelin water, MT bound
+  pool with a super-Lorentzian line) under Gaussian saturation trains at a
+  series of off-resonance offsets (a Z-spectrum), for the multi-pool engine.
 """
 from __future__ import annotations
 
@@ -352,4 +355,80 @@ def bssfp3d(shape=(128, 128, 64), n_ro=128, n_pe_y=None, n_pe_z=None, spins=4, n
     sc = make_structs(Rho, T1, T2, dB0, res, spin_num=spins, dWRnd=dWRnd, seq=seq,
                       rx=(cx, cy), tr_num=count)
     sc.name = f"bssfp3d_{R}x{C_}x{Z}_S{spins}_Rx{n_coils}_TR{count}"
+    return sc
+
+
+def mt_cest(shape=(256, 256, 1), n_offsets=40, ppm_range=50.0, b0=3.0, sat_time=2.0, pulse=0.02,
+            gap=0.02, sat_fa=300.0, dt=10e-6, seed=2, t2b=10e-6, spins=1, n_coils=1) -> Scan:
+    """Three-pool exchange with MT and CEST-style saturation.
+
+    Pools: 0 free water, 1 myelin water (free), 2 MT bound pool
+    (TypeFlag 1, T2b = 10 us, super-Lorentzian line). Pool fractions come
+    from Dirichlet(8, 1, 1); exchange rates out of the larger pool are
+    U[1, 30]/s and the reverse rates follow detailed balance. For each of
+    ``n_offsets`` offsets (linspace over +-ppm_range ppm at ``b0`` tesla) one
+    TR holds a ``sat_time`` train of Gaussian pulses (``pulse`` long,
+    ``gap`` apart, ``sat_fa`` degrees each) at that offset, an ideal spoiler
+    (Ext 6), a 90 deg hard readout pulse on resonance and one ADC sample.
+    K is [R,C,Z,3,3]: K(a,b) is the rate from pool a into pool b.
+    ``spins`` > 1 spreads each voxel's isochromats over +-25 Hz (dWRnd, same
+    for every pool); ``n_coils`` > 1 adds Gaussian receive coils.
+    """
+    rng = np.random.default_rng(seed)
+    R, C_, Z = shape
+    f = rng.dirichlet([8.0, 1.0, 1.0], size=shape)               # [R,C,Z,3]
+    Rho = f.copy()
+    T1 = np.empty(shape + (3,))
+    T2 = np.empty(shape + (3,))
+    T1[..., 0] = rng.uniform(0.8, 2.0, shape); T2[..., 0] = rng.uniform(0.05, 0.3, shape)
+    T1[..., 1] = rng.uniform(0.3, 0.6, shape); T2[..., 1] = rng.uniform(0.01, 0.03, shape)
+    T1[..., 2] = 1.0;                          T2[..., 2] = t2b
+    dB0 = 2 * np.pi * rng.normal(0.0, 20.0, shape) / GYRO
+    K = np.zeros(shape + (3, 3))
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        k_ab = rng.uniform(1.0, 30.0, shape)                      # a -> b
+        K[..., a, b] = k_ab
+        K[..., b, a] = k_ab * f[..., a] / f[..., b]               # detailed balance
+    gbar = GYRO / (2 * np.pi)
+    offsets = np.linspace(-ppm_range, ppm_range, n_offsets) * 1e-6 * gbar * b0   # Hz
+    n_pulse = int(round(pulse / dt))
+    n_gap = int(round(gap / dt))
+    period = n_pulse + n_gap
+    n_sat = int(round(sat_time / dt)) // period * period
+    n_read = 40
+    n = n_sat + n_read
+    times = np.arange(n + 1) * dt
+    tpl_pulse = psd.rf_gaussian(times[: n_pulse + 1], sat_fa)
+    trs = []
+    for off in offsets:
+        tr = psd.TR(duration=n * dt)
+        for k in range(n_sat // period):
+            t0 = k * period
+            tr.add_rf(psd.RF(times[t0: t0 + n_pulse + 1], tpl_pulse.amp, tpl_pulse.phase,
+                             np.where(tpl_pulse.amp != 0, off, 0.0), tpl_pulse.coil))
+        rd = psd.rf_rect(times[n_sat + 4: n_sat + 4 + 11], 90.0)
+        tr.add_rf(rd)
+        zero = np.zeros(n + 1)
+        tr.add_gz(times[[0, n]], [0, 0])
+        tr.add_gy(times[[0, n]], [0, 0])
+        tr.add_gx(times, zero)
+        k_adc = n_sat + 4 + 11 + 2
+        tr.add_adc(times[[k_adc - 1, k_adc, k_adc + 1]], [0, 1, 0])
+        tr.add_ext(times[[n_sat, n_sat + 1, n_sat + 2]], [0, 6, 0])          # ideal spoiler before readout
+        tr.add_ext(times[[n - 4, n - 3, n - 2]], [0, 6, 0])                  # and after it
+        trs.append(tr.finalize())
+    seq = psd.concat(trs)
+    res = (2e-3, 2e-3, 5e-3)
+    dWRnd = np.zeros(shape + (spins, 3))
+    for q in range(spins):
+        dWRnd[..., q, :] = 2 * np.pi * 25.0 * (2 * (q + 0.5) / spins - 1)
+    rx = gaussian_coils(shape, res, n_coils, rng) if n_coils > 1 else None
+    sc = make_structs(Rho, T1, T2, dB0, res, spin_num=spins, dWRnd=dWRnd, rx=rx, seq=seq,
+                      tr_num=n_offsets)
+    sc.VObj["K"] = np.asfortranarray(K.astype(np.float32))
+    sc.VObj["TypeFlag"] = np.array([0.0, 0.0, 1.0])
+    sc.VCtl["GM_Flag"] = "on"
+    sc.VCtl["CS"] = np.zeros(3)
+    sc.name = f"mt_cest_{R}x{C_}x{Z}_off{n_offsets}_steps{n}"
+    sc.offsets_hz = offsets
     return sc

@@ -39,6 +39,11 @@ def _lib(precision: str) -> C.CDLL:
         lib.orc_DoScanAtCPU.argtypes = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC),
                                         P(abi.VSeqC), P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC)]
         lib.orc_DoScanAtCPU.restype = C.c_int
+        lib.orc_DoGMScan.argtypes = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC),
+                                     P(abi.VSeqC), P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC), C.c_int]
+        lib.orc_DoGMScan.restype = C.c_int
+        lib.orc_superlorentzian.argtypes = [C.c_double, C.c_double]
+        lib.orc_superlorentzian.restype = C.c_double
         lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_set_relax_mode.argtypes = [C.c_int]
         _libs[precision] = lib
@@ -67,3 +72,22 @@ def scan(sc, precision: str = "f64", callbacks=None, inplace: bool = False):
         raise RuntimeError(f"oracle scan failed with status {rc}")
     abi.unpack_vvar(P, out.VVar)
     return out
+
+
+def gm_scan(sc, precision: str = "f64", scheme: int = 0, inplace: bool = False):
+    """Multi-pool scan (MRiLab's GM model): scheme 0 = operator split as
+    BlochKernelGM.cuh, 1 = fp64 matrix exponential of the same generator."""
+    out = sc if inplace else sc.copy()
+    lib = _lib(precision)
+    P = abi.pack_scan(*out.structs, multipool=True)
+    rc = lib.orc_DoGMScan(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi), C.byref(P.vctl),
+                          C.byref(P.vseq), C.byref(P.vvar), C.byref(P.vsig), None, int(scheme))
+    if rc != 0:
+        raise RuntimeError(f"oracle GM scan failed with status {rc}")
+    abi.unpack_vvar(P, out.VVar)
+    return out
+
+
+def superlorentzian(dw: float, t2b: float, precision: str = "f64") -> float:
+    """The bound-pool lineshape G(dw) as the GM kernel's Riemann sum computes it."""
+    return float(_lib(precision).orc_superlorentzian(float(dw), float(t2b)))

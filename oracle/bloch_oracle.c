@@ -52,6 +52,67 @@ static real r_sqrt(real x) { return sizeof(real) == 4 ? (real)sqrtf((float)x) : 
  * any accurate fp32 exp agrees with it except near rounding midpoints. */
 static real r_exp(real x) { return (real)exp((double)x); }
 
+/* The rf rotation of BlochKernel.h:124-306 for one isochromat: b1 is the
+ * gamma-scaled B1 magnitude, b2 the negated phase -(pe + phase), dW the
+ * off-resonance including the rf frequency; each statement is one IPP pass. */
+static void orc_rf_rotate(real dW, real b1, real b2, real dt, real mx, real my, real mz,
+                          real *omx, real *omy, real *omz)
+{
+    real nmx, nmy, nmz;
+    /* beta = atan(dW / b1), alpha = sqrt(dW^2 + b1^2) * dt, :124-133 */
+    real q = dW / b1;
+    real beta = r_atan(q);
+    real s2 = dW * dW; b1 = b1 * b1; s2 = b1 + s2; s2 = r_sqrt(s2);
+    real alpha = s2 * dt;
+    real sA = r_sin(alpha), sB = r_sin(beta), cA = r_cos(alpha), cB = r_cos(beta);
+    real sP = r_sin(b2), msP = sP * -1, cP = r_cos(b2), mcP = cP * -1;
+    real t1, t2, t3, t4, acc;
+    /* Mx, :146-196 */
+    t1 = sB * cA; t1 = mcP * t1; t3 = sA * sP; t1 = t3 + t1; t1 = sB * t1; t1 = t1 * -1;
+    t3 = cB * cB; t3 = cP * t3; t1 = t3 + t1; t3 = t1;
+    t1 = cP * t1; t3 = sP * t3;
+    t2 = sA * sB; t2 = cP * t2; t4 = cA * sP; t2 = t4 + t2; t4 = t2;
+    t2 = sP * t2; t4 = mcP * t4;
+    t2 = t1 + t2; t2 = mx * t2;
+    t4 = t3 + t4; t4 = my * t4; t4 = t4 * -1;
+    acc = t2 + t4;
+    t1 = sB * cB; t1 = cP * t1;
+    t2 = sB * cA; t2 = mcP * t2;
+    t3 = sA * sP;
+    t2 = t3 + t2; t2 = cB * t2;
+    t1 = t2 + t1; t1 = mz * t1;
+    nmx = acc + t1;
+    /* My, :205-254 */
+    t1 = cA * sB; t1 = sP * t1; t3 = sA * cP; t1 = t3 + t1; t1 = sB * t1;
+    t3 = cB * cB; t3 = sP * t3; t1 = t3 + t1; t3 = t1;
+    t1 = sP * t1; t3 = cP * t3;
+    t2 = sA * sB; t2 = msP * t2; t4 = cA * cP; t2 = t4 + t2; t4 = t2;
+    t2 = cP * t2; t4 = msP * t4;
+    t2 = t1 + t2; t2 = my * t2;
+    t4 = t3 + t4; t4 = mx * t4; t4 = t4 * -1;
+    acc = t2 + t4;
+    t1 = cB * sB; t1 = msP * t1;
+    t2 = cA * sB; t2 = sP * t2;
+    t3 = sA * cP;
+    t2 = t3 + t2; t2 = cB * t2;
+    t1 = t2 + t1; t1 = mz * t1;
+    nmy = acc + t1;
+    /* Mz, :263-299 */
+    t1 = cA * cB; t1 = sB * t1; t1 = t1 * -1;
+    t3 = cB * sB; t1 = t3 + t1; t3 = t1;
+    t1 = cP * t1; t3 = sP * t3;
+    t2 = cB * sA; t2 = msP * t2;
+    t4 = cB * sA; t4 = cP * t4;
+    t2 = t1 + t2; t2 = mx * t2;
+    t4 = t3 + t4; t4 = my * t4; t4 = t4 * -1;
+    acc = t2 + t4;
+    t1 = sB * sB;
+    t2 = cB * cB; t2 = cA * t2;
+    t1 = t2 + t1; t1 = mz * t1;
+    nmz = acc + t1;
+    *omx = nmx; *omy = nmy; *omz = nmz;
+}
+
 /*
  * One Bloch step for the voxels of one slice of one (type, spin).
  * BlochKernel.h:7-397; every statement below is one elementwise IPP pass
@@ -115,57 +176,7 @@ static void orc_bloch_slice(real Gyro, const double *CS, int SpinNum,
                 b1 = b1 * Gyro;
                 b2 = b2 * -1;
             }
-            /* beta = atan(dW / b1), alpha = sqrt(dW^2 + b1^2) * dt, :124-133 */
-            real q = dW / b1;
-            real beta = r_atan(q);
-            real s2 = dW * dW; b1 = b1 * b1; s2 = b1 + s2; s2 = r_sqrt(s2);
-            real alpha = s2 * dt;
-            real sA = r_sin(alpha), sB = r_sin(beta), cA = r_cos(alpha), cB = r_cos(beta);
-            real sP = r_sin(b2), msP = sP * -1, cP = r_cos(b2), mcP = cP * -1;
-            real t1, t2, t3, t4, acc;
-            /* Mx, :146-196 */
-            t1 = sB * cA; t1 = mcP * t1; t3 = sA * sP; t1 = t3 + t1; t1 = sB * t1; t1 = t1 * -1;
-            t3 = cB * cB; t3 = cP * t3; t1 = t3 + t1; t3 = t1;
-            t1 = cP * t1; t3 = sP * t3;
-            t2 = sA * sB; t2 = cP * t2; t4 = cA * sP; t2 = t4 + t2; t4 = t2;
-            t2 = sP * t2; t4 = mcP * t4;
-            t2 = t1 + t2; t2 = mx * t2;
-            t4 = t3 + t4; t4 = my * t4; t4 = t4 * -1;
-            acc = t2 + t4;
-            t1 = sB * cB; t1 = cP * t1;
-            t2 = sB * cA; t2 = mcP * t2;
-            t3 = sA * sP;
-            t2 = t3 + t2; t2 = cB * t2;
-            t1 = t2 + t1; t1 = mz * t1;
-            nmx = acc + t1;
-            /* My, :205-254 */
-            t1 = cA * sB; t1 = sP * t1; t3 = sA * cP; t1 = t3 + t1; t1 = sB * t1;
-            t3 = cB * cB; t3 = sP * t3; t1 = t3 + t1; t3 = t1;
-            t1 = sP * t1; t3 = cP * t3;
-            t2 = sA * sB; t2 = msP * t2; t4 = cA * cP; t2 = t4 + t2; t4 = t2;
-            t2 = cP * t2; t4 = msP * t4;
-            t2 = t1 + t2; t2 = my * t2;
-            t4 = t3 + t4; t4 = mx * t4; t4 = t4 * -1;
-            acc = t2 + t4;
-            t1 = cB * sB; t1 = msP * t1;
-            t2 = cA * sB; t2 = sP * t2;
-            t3 = sA * cP;
-            t2 = t3 + t2; t2 = cB * t2;
-            t1 = t2 + t1; t1 = mz * t1;
-            nmy = acc + t1;
-            /* Mz, :263-299 */
-            t1 = cA * cB; t1 = sB * t1; t1 = t1 * -1;
-            t3 = cB * sB; t1 = t3 + t1; t3 = t1;
-            t1 = cP * t1; t3 = sP * t3;
-            t2 = cB * sA; t2 = msP * t2;
-            t4 = cB * sA; t4 = cP * t4;
-            t2 = t1 + t2; t2 = mx * t2;
-            t4 = t3 + t4; t4 = my * t4; t4 = t4 * -1;
-            acc = t2 + t4;
-            t1 = sB * sB;
-            t2 = cB * cB; t2 = cA * t2;
-            t1 = t2 + t1; t1 = mz * t1;
-            nmz = acc + t1;
+            orc_rf_rotate(dW, b1, b2, dt, mx, my, mz, &nmx, &nmy, &nmz);
         } else {
             /* free precession, :314-343 */
             real alpha = dW * dt;
@@ -230,6 +241,330 @@ typedef struct orc_ctx {
     real *wMx, *wMy, *wMz; /* fp64 build keeps the state in double */
 } orc_ctx;
 
+/* ======================================================================
+ * Generalized multi-pool model (MRiLab's GM engine,
+ * Lib/src/engine/GPUEngine/BlochKernelGM.cuh:110-487). Pools t with
+ * TypeFlag 0 are free (3 components), TypeFlag 1 bound (Mz only).
+ * K is [R,C,Z,Ty,Ty]: plane p*Ty + t is the rate from pool t into pool p
+ * (BlochKernelGM.cuh:417 "go away", :423 "come in").
+ * ==================================================================== */
+
+/* Super-Lorentzian lineshape of the bound pool as BlochKernelGM.cuh:339-344
+ * evaluates it: a 1000-point Riemann sum over u = 0, 0.001, ..., with u
+ * accumulated by repeated addition. dw = 2*pi*rfFreq + gamma*dB0 (rad/s). */
+static double orc_sl_lineshape(double dw, double T2b)
+{
+    real u = 0, G = 0;
+    const real c = (real)sqrt(2.0 / ORC_PI);
+    for (int p = 0; p < 1000; p++) {
+        real den = (real)3 * u * u - 1;
+        real b = ((real)dw * (real)T2b) / den;
+        G += (c * ((real)T2b / (real)fabs((double)den)) * r_exp(-2 * b * b)) * (real)0.001;
+        u += (real)0.001;
+    }
+    return (double)G;
+}
+
+double orc_superlorentzian(double dw, double T2b) { return orc_sl_lineshape(dw, T2b); }
+
+typedef struct gm_cache {   /* per (voxel, spin, pool): lineshape cache key and value */
+    double key_f, key_t2, G;
+    int valid;
+} gm_cache;
+
+/* Saturation rate W of a bound pool (BlochKernelGM.cuh:323-380). */
+static double orc_gm_W(const orc_ctx *C, int64_t v, int t, double T2b, const double *rfAmp,
+                       const double *rfFreq, gm_cache *gc)
+{
+    const int Tx = C->vcoi->TxCoilNum;
+    const double gyro = (real)C->vobj->Gyro;
+    const double dB0 = C->vmag->dB0[v];
+    if (Tx == 1) {
+        real A = (real)fabs((double)(real)rfAmp[0]);
+        real F = (real)rfFreq[0];
+        real b1 = (real)C->vcoi->TxCoilmg[v] * A;
+        if (!gc->valid || gc->key_f != F || gc->key_t2 != T2b) {
+            gc->key_f = F; gc->key_t2 = T2b; gc->valid = 1;
+            gc->G = orc_sl_lineshape(2 * ORC_PI * F + dB0 * gyro, T2b);
+        }
+        real bg = b1 * (real)gyro;
+        return (double)((real)ORC_PI * bg * bg * (real)gc->G);
+    }
+    double W = 0; int n = 0;
+    for (int c = 0; c < Tx; c++) {
+        real A = (real)rfAmp[c], F = (real)rfFreq[c];
+        if (A == 0) continue;
+        A = (real)fabs((double)A);
+        real b1 = (real)C->vcoi->TxCoilmg[v + (int64_t)c * C->nvox] * A;
+        if (!gc->valid || gc->key_f != F || gc->key_t2 != T2b) {
+            gc->key_f = F; gc->key_t2 = T2b; gc->valid = 1;
+            gc->G = orc_sl_lineshape(2 * ORC_PI * F + dB0 * gyro, T2b);
+        }
+        real bg = b1 * (real)gyro;
+        W += (double)((real)ORC_PI * bg * bg * (real)gc->G);
+        n++;
+    }
+    return n ? W / n : 0.0;
+}
+
+/* Effective field of a free pool on an rf step: gamma*B1 magnitude and the
+ * negated phase -(pe + phase) as BlochKernel.h:58-121 forms them, plus the
+ * rf frequency term added to dW. Returns 0 if rf is off. */
+static int orc_rf_field(const orc_ctx *C, int64_t v, const double *rfAmp, const double *rfPhase,
+                        const double *rfFreq, real *b1o, real *b2o, real *dW)
+{
+    const int Tx = C->vcoi->TxCoilNum;
+    const real Gyro = (real)C->vobj->Gyro;
+    real rfAmpSum = 0;
+    for (int i = 0; i < Tx; i++) rfAmpSum += (real)fabs((double)(real)rfAmp[i]);
+    if (rfAmpSum == 0) return 0;
+    real b1, b2;
+    if (Tx == 1) {
+        real A = (real)rfAmp[0], P = (real)rfPhase[0], F = (real)rfFreq[0];
+        if (A < 0) { A = (real)fabs((double)A); P = (real)((double)P + ORC_PI); }
+        b1 = (real)C->vcoi->TxCoilmg[v] * A;
+        b2 = (real)C->vcoi->TxCoilpe[v] + P;
+        *dW = *dW + (real)((double)(F * 2) * ORC_PI);
+    } else {
+        real b5 = 0, b6 = 0;
+        for (int i = 0; i < Tx; i++) {
+            real A = (real)rfAmp[i], P = (real)rfPhase[i], F = (real)rfFreq[i];
+            if (A == 0) continue;
+            if (A < 0) { A = (real)fabs((double)A); P = (real)((double)P + ORC_PI); }
+            real m = (real)C->vcoi->TxCoilmg[v + (int64_t)i * C->nvox] * A;
+            real ph = (real)C->vcoi->TxCoilpe[v + (int64_t)i * C->nvox] + P;
+            b5 = b5 + m * r_cos(ph); b6 = b6 + m * r_sin(ph);
+            *dW = *dW + (real)((double)(F * 2) * ORC_PI);
+        }
+        b1 = r_sqrt(b5 * b5 + b6 * b6);
+        b2 = r_atan2(b6, b5);
+    }
+    *b1o = b1 * Gyro;
+    *b2o = b2 * -1;
+    return 1;
+}
+
+/* Off-resonance of a free pool (BlochKernelGM.cuh:242-245). */
+static real orc_gm_dW(const orc_ctx *C, int64_t v, int64_t si, int t, real gz, real gy, real gx)
+{
+    const real Gyro = (real)C->vobj->Gyro;
+    real dW = (real)C->vmag->dB0[v] * Gyro;
+    dW = dW + (real)C->vmag->dWRnd[si];
+    dW = dW + (real)((double)((real)C->vctl->CS[t] * 2) * ORC_PI);
+    dW = dW + (real)C->vmag->Gzgrid[v] * (gz * Gyro);
+    dW = dW + (real)C->vmag->Gygrid[v] * (gy * Gyro);
+    dW = dW + (real)C->vmag->Gxgrid[v] * (gx * Gyro);
+    return dW;
+}
+
+#define GM_MAXT 8
+
+static int orc_gm_skip(const orc_ctx *C, int64_t v, int t)
+{
+    const int64_t tv = v + (int64_t)t * C->nvox;
+    return C->vobj->T2[tv] == 0 || C->vobj->T1[tv] == 0 || C->vobj->Rho[tv] == 0;  /* :176 */
+}
+
+/* One operator-split step of all pools of one (voxel, spin):
+ * BlochKernelGM.cuh:159-476. */
+static void orc_gm_split_voxel(orc_ctx *C, int64_t v, int s, real dt, real gz, real gy, real gx,
+                               const double *rfAmp, const double *rfPhase, const double *rfFreq,
+                               gm_cache *gcs)
+{
+    const int Ty = C->vobj->TypeNum, S = C->vobj->SpinNum;
+    const int64_t nvox = C->nvox;
+    real Bx[GM_MAXT], By[GM_MAXT], Bz[GM_MAXT];
+    int64_t si[GM_MAXT];
+    for (int t = 0; t < Ty; t++) {
+        si[t] = v + nvox * ((int64_t)s + (int64_t)S * t);
+        Bx[t] = C->wMx ? C->wMx[si[t]] : (real)C->vobj->Mx[si[t]];
+        By[t] = C->wMy ? C->wMy[si[t]] : (real)C->vobj->My[si[t]];
+        Bz[t] = C->wMz ? C->wMz[si[t]] : (real)C->vobj->Mz[si[t]];
+    }
+    for (int t = 0; t < Ty; t++) {
+        if (orc_gm_skip(C, v, t)) continue;
+        const int64_t tv = v + (int64_t)t * nvox;
+        const real Rho = C->vobj->Rho[tv], T1 = C->vobj->T1[tv], T2 = C->vobj->T2[tv];
+        real mx = Bx[t], my = By[t], mz = Bz[t];
+        real bx = mx, by = my, bz = mz;
+        const int flag = (int)C->vobj->TypeFlag[t];
+        if (flag == 0) {
+            real dW = orc_gm_dW(C, v, si[t], t, gz, gy, gx);
+            real b1, b2;
+            if (orc_rf_field(C, v, rfAmp, rfPhase, rfFreq, &b1, &b2, &dW)) {
+                orc_rf_rotate(dW, b1, b2, dt, mx, my, mz, &bx, &by, &bz);
+            } else {
+                real alpha = dW * dt, sA = r_sin(alpha), cA = r_cos(alpha);
+                bx = mx * cA + my * sA;
+                by = my * cA - mx * sA;
+                bz = mz;
+            }
+        } else if (flag == 1) {
+            real rfAmpSum = 0;
+            for (int i = 0; i < C->vcoi->TxCoilNum; i++) rfAmpSum += (real)fabs((double)(real)rfAmp[i]);
+            if (rfAmpSum != 0) {
+                real W = (real)orc_gm_W(C, v, t, T2, rfAmp, rfFreq, &gcs[s * Ty + t]);
+                bz = mz * r_exp(-W * dt);
+            } else {
+                bz = mz;
+            }
+        } else {
+            continue;
+        }
+        /* relaxation (:397-405, :443-447) */
+        real e1 = r_exp(-dt / T1);
+        real m0 = (Rho * (1 - e1)) / (real)S;
+        if (flag == 0) {
+            real e2 = r_exp(-dt / T2);
+            bx = bx * e2;
+            by = by * e2;
+        }
+        bz = bz * e1;
+        /* exchange with start-of-step values of the other pools (:408-433) */
+        real kout = 1, kx = 0, ky = 0, kz = 0;
+        for (int p = 0; p < Ty; p++) {
+            if (p == t) continue;
+            real ko = C->vobj->K[v + nvox * ((int64_t)p * Ty + t)];
+            if (ko != 0) kout = kout * r_exp(-dt * ko);
+            real ki = C->vobj->K[v + nvox * ((int64_t)t * Ty + p)];
+            if (ki != 0) {
+                real f = 1 - r_exp(-dt * ki);
+                if (flag == 0) { kx = kx + f * Bx[p]; ky = ky + f * By[p]; }
+                kz = kz + f * Bz[p];
+            }
+        }
+        if (flag == 0) {
+            mx = bx * kout + kx;
+            my = by * kout + ky;
+        }
+        mz = bz * kout + kz + m0;
+        if (C->wMx) {
+            if (flag == 0) { C->wMx[si[t]] = mx; C->wMy[si[t]] = my; }
+            C->wMz[si[t]] = mz;
+        } else {
+            if (flag == 0) { C->vobj->Mx[si[t]] = (float)mx; C->vobj->My[si[t]] = (float)my; }
+            C->vobj->Mz[si[t]] = (float)mz;
+        }
+    }
+}
+
+/* x <- expm(A*dt) x for an n x n generator (row-major), in fp64: Taylor
+ * series of the action on x with substeps of h*|A|_1 <= 0.5. */
+static void orc_expm_action(int n, const double *A, double dt, double *x)
+{
+    double nrm = 0;
+    for (int j = 0; j < n; j++) {
+        double cs = 0;
+        for (int i = 0; i < n; i++) cs += fabs(A[i * n + j]);
+        if (cs > nrm) nrm = cs;
+    }
+    int sub = (int)ceil(nrm * dt / 0.5);
+    if (sub < 1) sub = 1;
+    const double h = dt / sub;
+    double term[4 * GM_MAXT], nt[4 * GM_MAXT], y[4 * GM_MAXT];
+    for (int st = 0; st < sub; st++) {
+        for (int i = 0; i < n; i++) { term[i] = x[i]; y[i] = x[i]; }
+        for (int k = 1; k <= 40; k++) {
+            double tn = 0, yn = 0;
+            for (int i = 0; i < n; i++) {
+                double acc = 0;
+                for (int j = 0; j < n; j++) acc += A[i * n + j] * term[j];
+                nt[i] = acc * h / k;
+            }
+            for (int i = 0; i < n; i++) { term[i] = nt[i]; y[i] += nt[i]; tn += fabs(nt[i]); yn += fabs(y[i]); }
+            if (tn <= 1e-18 * (yn + 1e-300)) break;
+        }
+        for (int i = 0; i < n; i++) x[i] = y[i];
+    }
+}
+
+/* One step of all pools of one (voxel, spin) as the exact solution of the
+ * linear Bloch-McConnell system the split scheme approximates: state
+ * (Mx,My,Mz per pool, 1), generator = rotation M x Omega_t, relaxation,
+ * saturation W of bound pools, exchange k_{t->p} = K[p*Ty+t]. */
+static void orc_gm_expm_voxel(orc_ctx *C, int64_t v, int s, double dt, double gz, double gy, double gx,
+                              const double *rfAmp, const double *rfPhase, const double *rfFreq,
+                              gm_cache *gcs)
+{
+    const int Ty = C->vobj->TypeNum, S = C->vobj->SpinNum;
+    const int64_t nvox = C->nvox;
+    const int n = 3 * Ty + 1;
+    double A[(3 * GM_MAXT + 1) * (3 * GM_MAXT + 1)];
+    double x[3 * GM_MAXT + 1];
+    int64_t si[GM_MAXT];
+    memset(A, 0, sizeof(double) * n * n);
+    const double gyro = (double)(real)C->vobj->Gyro;
+    for (int t = 0; t < Ty; t++) {
+        si[t] = v + nvox * ((int64_t)s + (int64_t)S * t);
+        x[3 * t] = C->wMx ? (double)C->wMx[si[t]] : C->vobj->Mx[si[t]];
+        x[3 * t + 1] = C->wMy ? (double)C->wMy[si[t]] : C->vobj->My[si[t]];
+        x[3 * t + 2] = C->wMz ? (double)C->wMz[si[t]] : C->vobj->Mz[si[t]];
+    }
+    x[n - 1] = 1.0;
+    /* rf field (same for all free pools) */
+    const int Tx = C->vcoi->TxCoilNum;
+    double rfsum = 0;
+    for (int i = 0; i < Tx; i++) rfsum += fabs(rfAmp[i]);
+    double bxr = 0, byr = 0, wrf = 0;
+    if (rfsum != 0) {
+        for (int i = 0; i < Tx; i++) {
+            double Aa = rfAmp[i], P = rfPhase[i];
+            if (Aa == 0) continue;
+            if (Aa < 0) { Aa = -Aa; P += ORC_PI; }
+            double m = (double)C->vcoi->TxCoilmg[v + (int64_t)i * nvox] * Aa * gyro;
+            double ph = (double)C->vcoi->TxCoilpe[v + (int64_t)i * nvox] + P;
+            bxr += m * cos(ph); byr += m * sin(ph);
+            wrf += 2 * ORC_PI * rfFreq[i];
+        }
+    }
+    for (int t = 0; t < Ty; t++) {
+        if (orc_gm_skip(C, v, t)) continue;
+        const int64_t tv = v + (int64_t)t * nvox;
+        const double Rho = C->vobj->Rho[tv], R1 = 1.0 / C->vobj->T1[tv], R2 = 1.0 / C->vobj->T2[tv];
+        const int flag = (int)C->vobj->TypeFlag[t];
+        double kout = 0;
+        for (int p = 0; p < Ty; p++)
+            if (p != t) kout += C->vobj->K[v + nvox * ((int64_t)p * Ty + t)];
+        const int z = 3 * t + 2;
+        if (flag == 0) {
+            const int xi = 3 * t, yi = 3 * t + 1;
+            double w = (double)C->vmag->dB0[v] * gyro + (double)C->vmag->dWRnd[si[t]] +
+                       2 * ORC_PI * C->vctl->CS[t] + gyro * ((double)C->vmag->Gzgrid[v] * gz +
+                       (double)C->vmag->Gygrid[v] * gy + (double)C->vmag->Gxgrid[v] * gx);
+            if (rfsum != 0) w += wrf;
+            /* dM/dt = M x Omega, Omega = (bx, by, w) */
+            A[xi * n + yi] += w;   A[xi * n + z] -= byr;
+            A[yi * n + z] += bxr;  A[yi * n + xi] -= w;
+            A[z * n + xi] += byr;  A[z * n + yi] -= bxr;
+            A[xi * n + xi] -= R2 + kout;
+            A[yi * n + yi] -= R2 + kout;
+            for (int p = 0; p < Ty; p++) {
+                if (p == t) continue;
+                double ki = C->vobj->K[v + nvox * ((int64_t)t * Ty + p)];
+                A[xi * n + 3 * p] += ki;
+                A[yi * n + 3 * p + 1] += ki;
+            }
+        } else if (flag == 1) {
+            double W = 0;
+            if (rfsum != 0) W = orc_gm_W(C, v, t, C->vobj->T2[tv], rfAmp, rfFreq, &gcs[s * Ty + t]);
+            A[z * n + z] -= W;
+        } else {
+            continue;
+        }
+        A[z * n + z] -= R1 + kout;
+        A[z * n + (n - 1)] += R1 * Rho / S;
+        for (int p = 0; p < Ty; p++) {
+            if (p == t) continue;
+            A[z * n + 3 * p + 2] += C->vobj->K[v + nvox * ((int64_t)t * Ty + p)];
+        }
+    }
+    orc_expm_action(n, A, dt, x);
+    for (int t = 0; t < Ty; t++) {
+        if (C->wMx) { C->wMx[si[t]] = x[3 * t]; C->wMy[si[t]] = x[3 * t + 1]; C->wMz[si[t]] = x[3 * t + 2]; }
+        else { C->vobj->Mx[si[t]] = (float)x[3 * t]; C->vobj->My[si[t]] = (float)x[3 * t + 1]; C->vobj->Mz[si[t]] = (float)x[3 * t + 2]; }
+    }
+}
+
 /* Ext plugins executed natively: Src/Main/DoExtPlugin.m:11-33 */
 static int orc_ext(orc_ctx *c, double ext)
 {
@@ -284,9 +619,9 @@ static void orc_capture(orc_ctx *c)
  * The scan: DoScanAtCPU.cpp:246-553 with the MEX globals replaced by the
  * ABI structs. Returns 0 on success, MRS_ERR_CALLBACK if a callback failed.
  */
-int orc_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
+static int orc_scan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                     const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar,
-                    mrs_vsig *vsig, const mrs_callbacks *cb)
+                    mrs_vsig *vsig, const mrs_callbacks *cb, int model)
 {
     orc_ctx C;
     memset(&C, 0, sizeof C);
@@ -308,6 +643,7 @@ int orc_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
     }
     real *buf1 = (real *)malloc(sizeof(real) * C.nvox);
     real *buf3 = (real *)malloc(sizeof(real) * C.nvox);
+    gm_cache *gcs = model ? (gm_cache *)calloc((size_t)C.nvox * S * Ty, sizeof(gm_cache)) : NULL;
 
     int nthreads = g_threads > 0 ? g_threads
                  : (SpinMxSliceNum < vctl->MaxThreadNum ? (int)SpinMxSliceNum : vctl->MaxThreadNum); /* :225-228 */
@@ -398,7 +734,9 @@ int orc_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
         for (int Typei = 0; Typei < Ty; Typei++) {
             for (int Spini = 0; Spini < S; Spini++) {
                 const int64_t base = (int64_t)Typei * S * C.nvox + (int64_t)Spini * C.nvox;
-                if (vvar->ADC == 1) {                                /* :445-491 */
+                /* the GM engine samples free pools only (BlochKernelGM.cuh:192) */
+                const int sample = vvar->ADC == 1 && (model == 0 || vobj->TypeFlag[Typei] == 0);
+                if (sample) {                                        /* :445-491 */
                     for (int c = 0; c < Rx; c++) {
                         const float *Cx = vcoi->RxCoilx + (int64_t)c * C.nvox, *Cy = vcoi->RxCoily + (int64_t)c * C.nvox;
                         const real cr = (real)cos(-vvar->rfRef), msr = (real)-sin(-vvar->rfRef);
@@ -420,12 +758,14 @@ int orc_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                             } else {
                                 b3 = b4;
                             }
+                            if (model && orc_gm_skip(&C, v, Typei)) { b1 = 0; b3 = 0; }
                             buf1[v] = b1; buf3[v] = b3;
                         }
                         vsig->Sx[(int64_t)Typei * Rx * SignalNum + (int64_t)c * SignalNum + Signali] += orc_sum(buf1, C.nvox);
                         vsig->Sy[(int64_t)Typei * Rx * SignalNum + (int64_t)c * SignalNum + Signali] += orc_sum(buf3, C.nvox);
                     }
                 }
+                if (model) continue;  /* multi-pool steps run per voxel below */
                 /* Bloch step over slices, OpenMP as :494-527 */
                 const real rdt = (real)vvar->dt;
                 const real gz = (real)vvar->GzAmp, gy = (real)vvar->GyAmp, gx = (real)vvar->GxAmp;
@@ -445,6 +785,22 @@ int orc_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                                     C.wMx ? C.wMx + base + so : NULL,
                                     C.wMy ? C.wMy + base + so : NULL,
                                     C.wMz ? C.wMz + base + so : NULL);
+                }
+            }
+        }
+        if (model) {
+            const real rdt = (real)vvar->dt;
+            const real gz = (real)vvar->GzAmp, gy = (real)vvar->GyAmp, gx = (real)vvar->GxAmp;
+            int64_t v;
+#pragma omp parallel for num_threads(g_threads > 0 ? g_threads : 1) schedule(static)
+            for (v = 0; v < C.nvox; v++) {
+                for (int sp = 0; sp < S; sp++) {
+                    gm_cache *gc = gcs + (size_t)v * S * Ty;
+                    if (model == 1)
+                        orc_gm_split_voxel(&C, v, sp, rdt, gz, gy, gx, vvar->rfAmp, vvar->rfPhase, vvar->rfFreq, gc);
+                    else
+                        orc_gm_expm_voxel(&C, v, sp, vvar->dt, vvar->GzAmp, vvar->GyAmp, vvar->GxAmp,
+                                          vvar->rfAmp, vvar->rfPhase, vvar->rfFreq, gc);
                 }
             }
         }
@@ -469,6 +825,24 @@ done:
         }
         free(C.wMx); free(C.wMy); free(C.wMz);
     }
-    free(buf1); free(buf3);
+    free(buf1); free(buf3); free(gcs);
     return status;
+}
+
+/* The single-pool scan of DoScanAtCPU.cpp:70. */
+int orc_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
+                    const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar,
+                    mrs_vsig *vsig, const mrs_callbacks *cb)
+{
+    return orc_scan(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, 0);
+}
+
+/* The multi-pool scan of DoGMScanAtGPU.cu (same decoder): scheme 0 is the
+ * GM kernel's operator split, scheme 1 the fp64 matrix exponential. */
+int orc_DoGMScan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
+                 const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar,
+                 mrs_vsig *vsig, const mrs_callbacks *cb, int scheme)
+{
+    if (!vobj->K || !vobj->TypeFlag || vobj->TypeNum > GM_MAXT) return MRS_ERR_ARG;
+    return orc_scan(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, scheme ? 2 : 1);
 }
