@@ -72,6 +72,93 @@ def cpu_baseline(n_tr: int, threads: int) -> dict:
                       f"{dt:.1f} s), fp32 restatement of DoScanAtCPU, OpenMP over slices"}
 
 
+def batched_section(n_sets: int, plain_sample: int, reps: int) -> dict:
+    """Batched uneven-length fingerprinting: n_sets single-voxel scans (64
+    isochromats, 500-1500 TRs) as one persistent launch replayed from a
+    captured graph, against plain per-scan DoScanAtGPU calls on a sample."""
+    import numpy as np
+    from mrilab_amd import engine, workloads
+    t0 = time.perf_counter()
+    scans = workloads.fingerprint_batch(n_sets=n_sets)
+    log(f"batched: {n_sets} fingerprint scans generated ({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    b = engine.Batch([sc.copy().structs for sc in scans], options=engine.make_options(timing=1))
+    t_create = time.perf_counter() - t0
+    info = b.info()
+    log(f"batched: create {t_create:.2f} s (decode {info['decode_ms']:.0f} ms, {info['n_candidates']} shapes "
+        f"tuned, best spt {info['spins_per_thread']} block {info['block_size']})")
+    b.run()
+    b.sync()
+    run_ms = []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+        b.sync()
+        run_ms.append(b.info()["last_run_ms"])
+    wall_run = (time.perf_counter() - t0) / reps
+    outs = [sc.copy() for sc in scans]
+    t0 = time.perf_counter()
+    b.fetch([o.structs for o in outs])
+    t_fetch = time.perf_counter() - t0
+    b.close()
+    # plain per-scan launches on a sample, end to end (what BatchSim.m does per scan)
+    idx = np.linspace(0, n_sets - 1, plain_sample).astype(int)
+    plain = []
+    errs = []
+    for i in idx:
+        o = scans[i].copy()
+        t0 = time.perf_counter()
+        engine.DoScanAtGPU(*o.structs)
+        plain.append(time.perf_counter() - t0)
+        a = o.VSig["Sx"] + 1j * o.VSig["Sy"]
+        c = outs[i].VSig["Sx"] + 1j * outs[i].VSig["Sy"]
+        errs.append(float(np.linalg.norm(c - a) / max(np.linalg.norm(a), 1e-30)))
+    plain_total = float(np.mean(plain)) * n_sets
+    batched_e2e = t_create + wall_run + t_fetch
+    iso_steps = info["n_iso_steps"]
+    return {"workload": f"{n_sets} single-voxel fingerprinting scans x 64 isochromats, 500-1500 TRs of 5 steps "
+                        "(hard pulse, TE 3 ms, TR 10-15 ms), flips 5-70 deg",
+            "isochromat_steps": iso_steps, "value": iso_steps / wall_run, "unit": "isochromat-steps/s",
+            "ms_per_batch_run": wall_run * 1e3, "device_ms_per_run": float(np.median(run_ms)),
+            "end_to_end_s": batched_e2e, "create_s": t_create, "decode_ms": info["decode_ms"],
+            "plain_per_scan_ms": float(np.mean(plain)) * 1e3, "plain_sample": int(plain_sample),
+            "plain_total_s_estimated": plain_total,
+            "speedup": plain_total / wall_run, "speedup_end_to_end": plain_total / batched_e2e,
+            "max_rel_l2_vs_plain": max(errs),
+            "shape": {"spins_per_thread": info["spins_per_thread"], "block_size": info["block_size"],
+                      "grid_blocks": info["grid_blocks"], "tiles": info["n_tiles"],
+                      "tuned_candidates": info["n_candidates"], "tune_best_ms": info["tune_best_ms"],
+                      "tune_worst_ms": info["tune_worst_ms"]}}
+
+
+def adjoint_section(reps: int) -> dict:
+    """Adjoint rf design: 128x128 slice x 32 isochromats, 2000 complex rf
+    samples at 4 us under a spiral-in gradient, checkpoint every 64 steps;
+    device time of the forward pass and of the backward (gradient) pass."""
+    import numpy as np
+    from mrilab_amd import engine, workloads
+    e = workloads.excitation()
+    adj = engine.Adjoint(e, checkpoint=64)
+    adj.forward(e.rf, e.T1, e.T2, e.dB0)
+    adj.backward()
+    fwd, bwd = [], []
+    for _ in range(reps):
+        adj.forward(e.rf, e.T1, e.T2, e.dB0)
+        fwd.append(adj.info()["forward_ms"])
+        adj.backward()
+        bwd.append(adj.info()["backward_ms"])
+    info = adj.info()
+    adj.close()
+    f, b = float(np.median(fwd)), float(np.median(bwd))
+    iso_steps = info["n_iso"] * info["n_steps"]
+    log(f"adjoint: forward {f:.2f} ms, backward {b:.2f} ms, ratio {b / f:.2f}")
+    return {"workload": "rf design, 128x128 x 32 isochromats, 2000 complex rf samples at 4 us, spiral-in gradient, "
+                        "L2 loss on the final Mxy, gradients w.r.t. rf, T1, T2, dB0, checkpoint K=64",
+            "forward_ms": f, "backward_ms": b, "adjoint_over_forward": b / f,
+            "forward_isochromat_steps_per_s": iso_steps / (f / 1e3), "checkpoints": info["n_checkpoints"],
+            "spins_per_thread": info["spins_per_thread"]}
+
+
 def traffic_from_profile() -> float | None:
     """HBM bytes per forward launch measured by rocprofv3 (profiles/)."""
     path = os.path.join(ROOT, "profiles", "bloch_fwd_bssfp3d_traffic.json")
@@ -89,6 +176,10 @@ def main() -> int:
     ap.add_argument("--tr", type=int, default=None, help="truncate the scan to its first TRs (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-tr", type=int, default=2)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched fingerprinting section")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint section")
+    ap.add_argument("--batch-sets", type=int, default=4096)
+    ap.add_argument("--batch-plain-sample", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -176,6 +267,10 @@ def main() -> int:
         "kernel_ms_per_step": {"bloch_fwd": fwd_ms / args.steps, "signal_reduce": red_ms / args.steps},
     }
     plan.close()
+    if not args.no_batched:
+        line["batched"] = batched_section(args.batch_sets, args.batch_plain_sample, 3)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_section(3)
     if not args.no_cpu_baseline:
         threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         log(f"cpu baseline: fp32 oracle on the first {args.cpu_sample_tr} TRs, {threads} threads")

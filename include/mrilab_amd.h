@@ -215,6 +215,94 @@ typedef struct mrs_plan_info {
 mrs_status mrs_plan_get_info(const mrs_plan *plan, mrs_plan_info *info);
 void mrs_plan_destroy(mrs_plan *plan);
 
+/* ---------------------------------------------------------------------------
+ * Batched scans. No reference counterpart: MRiLab's batch runner
+ * (Src/Main/BatchSim.m:129-166) loads saved scans and calls the engine
+ * (DoScanAtGPU.cu:76) on each in turn. Here n independent single-pool scans,
+ * each with its own phantom, coils and sequence of any length, run in one
+ * persistent kernel launch replayed through a captured hipGraph. Each scan's
+ * outputs are what mrs_DoScanAtGPU writes for it (up to fp32 summation
+ * order in the coil sums).
+ * Limits: TypeNum 1, TxCoilNum 1, RunMode 0, one GPUIndex for all scans, no
+ * Ext callbacks (Ext 1-4, 6 and 7 run natively, other codes are ignored).
+ * options: spins_per_thread / block_size 0 = chosen by the autotuner at
+ * create (every valid shape is timed on the device, the fastest is kept);
+ * precision_mode as for single scans; timing = 1 times each run.
+ * ------------------------------------------------------------------------- */
+typedef struct mrs_batch mrs_batch;
+mrs_status mrs_batch_create(int32_t n_scans, mrs_vobj *const *vobj, const mrs_vmag *const *vmag,
+                            const mrs_vcoi *const *vcoi, const mrs_vctl *const *vctl,
+                            const mrs_vseq *const *vseq, mrs_vvar *const *vvar, mrs_vsig *const *vsig,
+                            const mrs_options *options, mrs_batch **out);
+/* Enqueues one run of every scan (graph replay); asynchronous. */
+mrs_status mrs_batch_run(mrs_batch *batch);
+mrs_status mrs_batch_sync(mrs_batch *batch);
+/* Adds the last run's signals and k-space into each VSig, writes the final
+ * magnetization into each VObj and the final decoder state into each VVar
+ * (any of the three array arguments may be NULL). */
+mrs_status mrs_batch_fetch(mrs_batch *batch, mrs_vobj *const *vobj, mrs_vvar *const *vvar,
+                           mrs_vsig *const *vsig);
+typedef struct mrs_batch_info {
+    int64_t n_scans;
+    int64_t n_tiles;          /* wavefront work items (scan, 64*SPT isochromats) */
+    int64_t n_iso;            /* isochromats over all scans */
+    int64_t n_iso_steps;      /* sum over scans of isochromats x steps */
+    int64_t n_adc;            /* ADC samples over all scans */
+    int32_t spins_per_thread, block_size, grid_blocks;
+    int32_t tuned;            /* 1 if the autotuner chose spins_per_thread and block_size */
+    int32_t n_candidates;     /* shapes timed by the autotuner */
+    float   last_run_ms;      /* timing = 1: device time of the last run */
+    float   tune_best_ms, tune_worst_ms;
+    double  decode_ms;        /* host time spent decoding the sequences at create */
+} mrs_batch_info;
+mrs_status mrs_batch_get_info(const mrs_batch *batch, mrs_batch_info *info);
+void mrs_batch_destroy(mrs_batch *batch);
+/* create + run + fetch + destroy. */
+mrs_status mrs_DoScanBatchAtGPU(int32_t n_scans, mrs_vobj *const *vobj, const mrs_vmag *const *vmag,
+                                const mrs_vcoi *const *vcoi, const mrs_vctl *const *vctl,
+                                const mrs_vseq *const *vseq, mrs_vvar *const *vvar, mrs_vsig *const *vsig,
+                                const mrs_options *options);
+
+/* ---------------------------------------------------------------------------
+ * Adjoint rf design. No reference counterpart: MRiLab simulates only. The
+ * forward model is the engine's rf step (BlochKernel.h:146-306 rotation,
+ * BlochKernel.h:348-368 relaxation, reference fp32 rounding) over a fixed-dt
+ * excitation with complex rf rf[k] (T, [n_steps][2]) and gradients
+ * grad[k] (Gx, Gy, Gz in T/m) on the phantom of VObj/VMag/VCoi (single pool,
+ * one Tx coil; VObj.Mx/My/Mz is the initial state, VObj.Rho sets M0;
+ * VObj.T1/T2 and VMag.dB0 are replaced by the parameters of each call).
+ * Loss L = sum over voxels of |sum over isochromats (Mx + iMy) - target|^2.
+ * mrs_adjoint_backward returns dL/drf, dL/dT1, dL/dT2 and dL/ddB0 by a
+ * backward-in-time pass that keeps the forward state every `checkpoint`
+ * steps and recomputes in between.
+ * ------------------------------------------------------------------------- */
+typedef struct mrs_excitation {
+    int32_t n_steps;
+    int32_t checkpoint;       /* K; 0 = 64 */
+    double  dt;               /* s */
+    const float *grad;        /* [n_steps][3] Gx, Gy, Gz (T/m), host */
+    const float *target;      /* [nvox][2] target transverse magnetization, host */
+} mrs_excitation;
+typedef struct mrs_adjoint mrs_adjoint;
+mrs_status mrs_adjoint_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
+                              const mrs_excitation *exc, const mrs_options *options, mrs_adjoint **out);
+/* rf [n_steps][2] (T), T1/T2 [nvox] (s), dB0 [nvox] (T); device = 1 when the
+ * pointers are device memory of GPU 0, 0 for host memory. Writes the loss
+ * and, if mxy is not NULL, the voxel sums [nvox][2] (host, fp64). */
+mrs_status mrs_adjoint_forward(mrs_adjoint *adj, const float *rf, const float *T1, const float *T2,
+                               const float *dB0, int32_t device, double *loss, double *mxy);
+/* Gradients of the loss of the last forward pass; same memory convention. */
+mrs_status mrs_adjoint_backward(mrs_adjoint *adj, float *g_rf, float *g_T1, float *g_T2, float *g_dB0,
+                                int32_t device);
+typedef struct mrs_adjoint_info {
+    int64_t n_iso, n_steps;
+    int32_t checkpoint, n_checkpoints, spins_per_thread, block_size;
+    float   forward_ms;       /* device time of the last forward pass */
+    float   backward_ms;      /* device time of the last backward pass */
+} mrs_adjoint_info;
+mrs_status mrs_adjoint_get_info(const mrs_adjoint *adj, mrs_adjoint_info *info);
+void mrs_adjoint_destroy(mrs_adjoint *adj);
+
 #ifdef __cplusplus
 }
 #endif

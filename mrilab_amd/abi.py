@@ -94,6 +94,25 @@ class PlanInfoC(C.Structure):
                 ("fwd_kernel_ms", C.c_double), ("reduce_kernel_ms", C.c_double)]
 
 
+class BatchInfoC(C.Structure):
+    _fields_ = [("n_scans", C.c_int64), ("n_tiles", C.c_int64), ("n_iso", C.c_int64),
+                ("n_iso_steps", C.c_int64), ("n_adc", C.c_int64),
+                ("spins_per_thread", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32),
+                ("tuned", C.c_int32), ("n_candidates", C.c_int32), ("last_run_ms", C.c_float),
+                ("tune_best_ms", C.c_float), ("tune_worst_ms", C.c_float), ("decode_ms", C.c_double)]
+
+
+class ExcitationC(C.Structure):
+    _fields_ = [("n_steps", C.c_int32), ("checkpoint", C.c_int32), ("dt", C.c_double),
+                ("grad", C.POINTER(C.c_float)), ("target", C.POINTER(C.c_float))]
+
+
+class AdjointInfoC(C.Structure):
+    _fields_ = [("n_iso", C.c_int64), ("n_steps", C.c_int64), ("checkpoint", C.c_int32),
+                ("n_checkpoints", C.c_int32), ("spins_per_thread", C.c_int32), ("block_size", C.c_int32),
+                ("forward_ms", C.c_float), ("backward_ms", C.c_float)]
+
+
 # ---------------------------------------------------------------------------
 # dtype contract, Src/Main/DoDataTypeConv.m
 # ---------------------------------------------------------------------------

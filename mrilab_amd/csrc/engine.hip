@@ -31,7 +31,9 @@
 #include "decode.hpp"
 #include "device_common.hpp"
 #include "mrilab_amd.h"
+#include "host_common.hpp"
 #include "multipool.hpp"
+#include "spin_step.hpp"
 
 namespace mrs {
 
@@ -52,28 +54,6 @@ struct FwdParams {
     float gyro;
     float cs[kMaxTypes];            // 2*pi*CS per type, rounded as BlochKernel.h:42
 };
-
-// One step of free precession + relaxation with a cached rotation (c, s)
-// (BlochKernel.h:314-368); PREC selects the relaxation form.
-template <int SPT, int PREC>
-__device__ __forceinline__ void free_step(float *mx, float *my, float *mz, const float *cc, const float *ss,
-                                          float e1, float e2, float m0e, float d1, float d2, float m0s) {
-#pragma unroll
-    for (int s = 0; s < SPT; s++) {
-        const float ax = mx[s], ay = my[s];
-        const float rx = ax * cc[s] + ay * ss[s];
-        const float ry = ay * cc[s] - ax * ss[s];
-        if constexpr (PREC == 0) {
-            mx[s] = rx * e2;
-            my[s] = ry * e2;
-            mz[s] = relax_mz_ref(mz[s], e1, m0e);
-        } else {
-            mx[s] = rx - rx * d2;
-            my[s] = ry - ry * d2;
-            mz[s] = mz[s] + (m0s - mz[s]) * d1;
-        }
-    }
-}
 
 // GEN = false: single Tx coil and no RunMode-1 capture (the common case);
 // GEN = true: any Tx coil count and capture. PREC: relaxation form
@@ -98,22 +78,27 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
     const int nblk = gridDim.x * gridDim.y;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x;
 
-    const float x = p.gx[vv], y = p.gy[vv], z = p.gz[vv];
-    const float rho = p.Rho[tv], t1 = p.T1[tv], t2 = p.T2[tv];
+    VoxelConst k;
+    k.x = p.gx[vv];
+    k.y = p.gy[vv];
+    k.z = p.gz[vv];
+    k.t1 = p.T1[tv];
+    k.t2 = p.T2[tv];
+    k.m0s = p.Rho[tv] / (float)p.S;  // BlochKernel.h:365
     const float w0 = p.dB0[vv] * p.gyro;  // BlochKernel.h:38
     const float cst = p.cs[type];
 
-    float mx[SPT], my[SPT], mz[SPT], wb[SPT], cc[SPT], ss[SPT];
+    SpinState<SPT> st;
     int64_t sidx[SPT];
 #pragma unroll
     for (int s = 0; s < SPT; s++) {
         sidx[s] = vv + nvox * ((int64_t)(grp * SPT + s) + (int64_t)p.S * type);
-        mx[s] = p.Mx[sidx[s]];
-        my[s] = p.My[sidx[s]];
-        mz[s] = p.Mz[sidx[s]];
-        wb[s] = (w0 + p.dWRnd[sidx[s]]) + cst;  // BlochKernel.h:38-42
-        cc[s] = 1.f;
-        ss[s] = 0.f;
+        st.mx[s] = p.Mx[sidx[s]];
+        st.my[s] = p.My[sidx[s]];
+        st.mz[s] = p.Mz[sidx[s]];
+        st.wb[s] = (w0 + p.dWRnd[sidx[s]]) + cst;  // BlochKernel.h:38-42
+        st.cc[s] = 1.f;
+        st.ss[s] = 0.f;
     }
     float crx[NRX], cry[NRX];
 #pragma unroll
@@ -129,42 +114,33 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
             cry[c] = 0.f;
         }
     }
-    // single-Tx B1+ as a complex weight mg*exp(i*pe)
-    float wtx_r = 0.f, wtx_i = 0.f;
+    k.wtx_r = k.wtx_i = 0.f;
     if (!GEN || p.Tx == 1) {
         float sp, cp;
         sincos_cr(p.txpe[vv], &sp, &cp);
         const float mg = p.txmg[vv];
-        wtx_r = mg * cp;
-        wtx_i = mg * sp;
+        k.wtx_r = mg * cp;
+        k.wtx_i = mg * sp;
     }
-    const float m0s = rho / (float)p.S;  // BlochKernel.h:365
+    const TxMaps txm{p.txmg, p.txpe, vv, nvox, GEN ? p.Tx : 1};
 
-    bool have_rot = false, have_e = false;
-    float e1 = 1.f, e2 = 1.f, m0e = 0.f;   // reference relaxation
-    float d1 = 0.f, d2 = 0.f;              // accurate relaxation
     int slot = 0;
     int adc_batch0 = 0;
-
     for (int r = r0; r < r1; ++r) {
         const crec_t q = rec + r * stride;
         const uint32_t flags = q[5];
-        if (flags & REC_SPOIL) {  // Plugin_IdealSpoiler.m:10-11
-            // a real (scalar) branch: the spoiler is rare, and a select
-            // here would cost 2*SPT VALU on every record
+        if (flags & REC_SPOIL) {
+            // a real (scalar) branch: the spoiler is rare, and a select here
+            // would cost 2*SPT VALU on every record
             __asm__ volatile("");
-#pragma unroll
-            for (int s = 0; s < SPT; s++) {
-                mx[s] = 0.f;
-                my[s] = 0.f;
-            }
+            st.spoil();
         }
         if (flags & REC_ADC) {  // DoScanAtCPU.cpp:445-491, before the step
             float sx = 0.f, sy = 0.f;
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
-                sx += mx[s];
-                sy += my[s];
+                sx += st.mx[s];
+                sy += st.my[s];
             }
             if (!valid) {
                 sx = 0.f;
@@ -187,106 +163,14 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
                 slot = 0;
             }
         }
-        if (!(flags & REC_NOSTEP)) {
-            const float dt = __uint_as_float(q[0]);
-            if ((flags & REC_NEWDT) || !have_e) {
-                have_e = true;
-                if constexpr (PREC == 0) {
-                    // BlochKernel.h:348-368 in fp32: E = exp(max(-dt/T, -200)),
-                    // with exp rounded correctly from fp64
-                    float a2 = (1.0f / t2) * dt;
-                    a2 = -a2;
-                    if (a2 < -200.f) a2 = -200.f;
-                    float a1 = (1.0f / t1) * dt;
-                    a1 = -a1;
-                    if (a1 < -200.f) a1 = -200.f;
-                    e2 = (float)exp((double)a2);
-                    e1 = (float)exp((double)a1);
-                    m0e = m0s * (1.0f - e1);
-                } else {
-                    d2 = (float)(-expm1(-(double)dt / (double)t2));
-                    d1 = (float)(-expm1(-(double)dt / (double)t1));
-                }
-            }
-            const float gv = z * __uint_as_float(q[1]) + y * __uint_as_float(q[2]) + x * __uint_as_float(q[3]);
-            if (flags & REC_RF) {
-                const float wrf = __uint_as_float(q[4]);
-                float bx, by;
-                if (!GEN || p.Tx == 1) {
-                    const float br = __uint_as_float(q[REC_HEAD]), bi = __uint_as_float(q[REC_HEAD + 1]);
-                    bx = wtx_r * br - wtx_i * bi;
-                    by = wtx_r * bi + wtx_i * br;
-                } else {
-                    bx = 0.f;
-                    by = 0.f;
-                    for (int c = 0; c < p.Tx; c++) {
-                        const float br = __uint_as_float(q[REC_HEAD + 2 * c]);
-                        const float bi = __uint_as_float(q[REC_HEAD + 2 * c + 1]);
-                        float sp, cp;
-                        sincos_cr(p.txpe[vv + (int64_t)c * nvox], &sp, &cp);
-                        const float mg = p.txmg[vv + (int64_t)c * nvox];
-                        bx += mg * (cp * br - sp * bi);
-                        by += mg * (cp * bi + sp * br);
-                    }
-                }
-                const float b2 = bx * bx + by * by;
-#pragma unroll
-                for (int s = 0; s < SPT; s++) {
-                    // rotation by |Omega|*dt about -Omega, Omega = (bx, by, w):
-                    // the matrix of BlochKernel.h:146-306 in Rodrigues form
-                    const float w = (wb[s] + gv) + wrf;
-                    const float om2 = b2 + w * w;
-                    if (om2 > 0.f) {
-                        const float om = sqrtf(om2);
-                        const float inv = 1.0f / om;
-                        const float nx = bx * inv, ny = by * inv, nz = w * inv;
-                        float sn, cs;
-                        sincos_cr(om * dt, &sn, &cs);
-                        const float ax = mx[s], ay = my[s], az = mz[s];
-                        const float k = (1.f - cs) * (nx * ax + ny * ay + nz * az);
-                        mx[s] = ax * cs - (ny * az - nz * ay) * sn + nx * k;
-                        my[s] = ay * cs - (nz * ax - nx * az) * sn + ny * k;
-                        mz[s] = az * cs - (nx * ay - ny * ax) * sn + nz * k;
-                    }
-                }
-                if constexpr (PREC == 0) {
-#pragma unroll
-                    for (int s = 0; s < SPT; s++) {
-                        mx[s] = mx[s] * e2;
-                        my[s] = my[s] * e2;
-                        mz[s] = relax_mz_ref(mz[s], e1, m0e);
-                    }
-                } else {
-#pragma unroll
-                    for (int s = 0; s < SPT; s++) {
-                        mx[s] = mx[s] - mx[s] * d2;
-                        my[s] = my[s] - my[s] * d2;
-                        mz[s] = mz[s] + (m0s - mz[s]) * d1;
-                    }
-                }
-            } else {
-                if (flags & REC_ONCE) {
-                    // rotation used by this step only
-#pragma unroll
-                    for (int s = 0; s < SPT; s++) sincos_fast((wb[s] + gv) * dt, &ss[s], &cc[s]);
-                    have_rot = false;
-                } else if (!((flags & REC_SAME) && have_rot)) {
-#pragma unroll
-                    for (int s = 0; s < SPT; s++) sincos_cr((wb[s] + gv) * dt, &ss[s], &cc[s]);
-                    have_rot = true;
-                }
-                // this step, then the folded repeats of it
-                const int n = 1 + (int)(flags >> REC_REP_SHIFT);
-                for (int k = 0; k < n; k++) free_step<SPT, PREC>(mx, my, mz, cc, ss, e1, e2, m0e, d1, d2, m0s);
-            }
-        }
+        if (!(flags & REC_NOSTEP)) bloch_step<SPT, GEN, PREC>(st, k, q, flags, txm);
         if (GEN && (flags & REC_CAPTURE) && valid) {  // DoScanAtCPU.cpp:544-552
             const int64_t base = (int64_t)p.cap_slot[r] * nvox * p.S * p.Ty;
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
-                p.capx[base + sidx[s]] = mx[s];
-                p.capy[base + sidx[s]] = my[s];
-                p.capz[base + sidx[s]] = mz[s];
+                p.capx[base + sidx[s]] = st.mx[s];
+                p.capy[base + sidx[s]] = st.my[s];
+                p.capz[base + sidx[s]] = st.mz[s];
             }
         }
     }
@@ -297,12 +181,13 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
     if (valid) {
 #pragma unroll
         for (int s = 0; s < SPT; s++) {
-            p.Mx[sidx[s]] = mx[s];
-            p.My[sidx[s]] = my[s];
-            p.Mz[sidx[s]] = mz[s];
+            p.Mx[sidx[s]] = st.mx[s];
+            p.My[sidx[s]] = st.my[s];
+            p.Mz[sidx[s]] = st.mz[s];
         }
     }
 }
+
 
 // Neumaier-compensated fp32 accumulation of block partials per lane, then
 // an fp64 butterfly across the wave; one wave per (sample, type). The
@@ -371,18 +256,12 @@ using namespace mrs;
 
 static thread_local std::string g_err;
 
-static mrs_status fail(mrs_status st, const std::string &msg) {
+namespace mrs {
+mrs_status fail(mrs_status st, const std::string &msg) {
     g_err = msg;
     return st;
 }
-
-#define HIP_TRY(expr)                                                                           \
-    do {                                                                                        \
-        hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess)                                                                   \
-            return fail(e_ == hipErrorOutOfMemory ? MRS_ERR_NOMEM : MRS_ERR_DEVICE,             \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                     \
-    } while (0)
+}  // namespace mrs
 
 typedef void (*fwd_fn)(FwdParams, const uint32_t *, int64_t, int64_t);
 
@@ -504,7 +383,7 @@ static mrs_status upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s
     return MRS_OK;
 }
 
-static mrs_status check_args(const mrs_vobj *vo, const mrs_vmag *vm, const mrs_vcoi *vc, const mrs_vctl *vt,
+mrs_status mrs::check_args(const mrs_vobj *vo, const mrs_vmag *vm, const mrs_vcoi *vc, const mrs_vctl *vt,
                              const mrs_vseq *vs, const mrs_vvar *vv, const mrs_vsig *vg) {
     if (!vo || !vm || !vc || !vt || !vs || !vv || !vg) return fail(MRS_ERR_ARG, "NULL struct pointer");
     if (!vo->Mx || !vo->My || !vo->Mz || !vo->Rho || !vo->T1 || !vo->T2)
@@ -854,6 +733,19 @@ static mrs_status reset_state(mrs_plan *P) {
     return MRS_OK;
 }
 
+void mrs::copy_vvar_out(const mrs_vvar &src, const std::vector<double> *rf, int tx, mrs_vvar *dst) {
+    double *a = dst->rfAmp, *p = dst->rfPhase, *f = dst->rfFreq;
+    *dst = src;
+    dst->rfAmp = a;
+    dst->rfPhase = p;
+    dst->rfFreq = f;
+    for (int c = 0; c < tx; c++) {
+        a[c] = rf[0][c];
+        p[c] = rf[1][c];
+        f[c] = rf[2][c];
+    }
+}
+
 extern "C" {
 
 const char *mrs_last_error(void) { return g_err.c_str(); }
@@ -938,18 +830,6 @@ mrs_status mrs_plan_sync(mrs_plan *P) {
     return MRS_OK;
 }
 
-static void copy_vvar_out(const mrs_vvar &src, const std::vector<double> *rf, int tx, mrs_vvar *dst) {
-    double *a = dst->rfAmp, *p = dst->rfPhase, *f = dst->rfFreq;
-    *dst = src;
-    dst->rfAmp = a;
-    dst->rfPhase = p;
-    dst->rfFreq = f;
-    for (int c = 0; c < tx; c++) {
-        a[c] = rf[0][c];
-        p[c] = rf[1][c];
-        f[c] = rf[2][c];
-    }
-}
 
 mrs_status mrs_plan_fetch(mrs_plan *P, mrs_vobj *vobj, mrs_vvar *vvar, mrs_vsig *vsig) {
     if (!P) return fail(MRS_ERR_ARG, "plan is NULL");

@@ -1,0 +1,172 @@
+// Per-thread spin state and the Bloch step of one decoded record, shared by
+// the single-scan forward kernel and the batched kernel.
+//
+// A thread owns SPT isochromats of one voxel and type. bloch_step() applies
+// one step record (BlochKernel.h:7-397): rf rotation or free precession,
+// then T2/T1 relaxation, with the cached free-precession rotation reused for
+// REC_SAME steps and repeated for the folded run length of the record.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "decode.hpp"
+#include "device_common.hpp"
+
+namespace mrs {
+
+// Per-voxel inputs of a thread that stay constant over a launch.
+struct VoxelConst {
+    float x, y, z;        // voxel position (m)
+    float t1, t2;         // relaxation times (s)
+    float m0s;            // Rho / SpinNum (BlochKernel.h:365)
+    float wtx_r, wtx_i;   // single-Tx B1+ weight mg * exp(i * pe)
+};
+
+// Multi-Tx rf inputs (GEN kernels): per-coil B1+ maps of this voxel.
+struct TxMaps {
+    const float *mg, *pe;
+    int64_t v, nvox;
+    int tx;
+};
+
+template <int SPT>
+struct SpinState {
+    float mx[SPT], my[SPT], mz[SPT];
+    float wb[SPT];            // off-resonance without gradients (rad/s)
+    float cc[SPT], ss[SPT];   // cached free-precession rotation
+    float e1 = 1.f, e2 = 1.f, m0e = 0.f;  // reference relaxation factors
+    float d1 = 0.f, d2 = 0.f;             // accurate relaxation factors
+    bool have_rot = false, have_e = false;
+
+    __device__ __forceinline__ void spoil() {  // Plugin_IdealSpoiler.m:10-11
+#pragma unroll
+        for (int s = 0; s < SPT; s++) {
+            mx[s] = 0.f;
+            my[s] = 0.f;
+        }
+    }
+};
+
+// One step of free precession + relaxation with a cached rotation (c, s)
+// (BlochKernel.h:314-368); PREC selects the relaxation form.
+template <int SPT, int PREC>
+__device__ __forceinline__ void free_step(SpinState<SPT> &st, float m0s) {
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        const float ax = st.mx[s], ay = st.my[s];
+        const float rx = ax * st.cc[s] + ay * st.ss[s];
+        const float ry = ay * st.cc[s] - ax * st.ss[s];
+        if constexpr (PREC == 0) {
+            st.mx[s] = rx * st.e2;
+            st.my[s] = ry * st.e2;
+            st.mz[s] = relax_mz_ref(st.mz[s], st.e1, st.m0e);
+        } else {
+            st.mx[s] = rx - rx * st.d2;
+            st.my[s] = ry - ry * st.d2;
+            st.mz[s] = st.mz[s] + (m0s - st.mz[s]) * st.d1;
+        }
+    }
+}
+
+// Applies the step of record q (flags already read; REC_NOSTEP excluded by
+// the caller). GEN: any Tx coil count (TxMaps), else one Tx coil (VoxelConst
+// weight). PREC: 0 = the reference's fp32 relaxation rounding, 1 = accurate.
+template <int SPT, bool GEN, int PREC>
+__device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst &k, crec_t q, uint32_t flags,
+                                           const TxMaps &txm) {
+    const float dt = __uint_as_float(q[0]);
+    if ((flags & REC_NEWDT) || !st.have_e) {
+        st.have_e = true;
+        if constexpr (PREC == 0) {
+            // BlochKernel.h:348-368 in fp32: E = exp(max(-dt/T, -200)), with
+            // exp rounded correctly from fp64
+            float a2 = (1.0f / k.t2) * dt;
+            a2 = -a2;
+            if (a2 < -200.f) a2 = -200.f;
+            float a1 = (1.0f / k.t1) * dt;
+            a1 = -a1;
+            if (a1 < -200.f) a1 = -200.f;
+            st.e2 = (float)exp((double)a2);
+            st.e1 = (float)exp((double)a1);
+            st.m0e = k.m0s * (1.0f - st.e1);
+        } else {
+            st.d2 = (float)(-expm1(-(double)dt / (double)k.t2));
+            st.d1 = (float)(-expm1(-(double)dt / (double)k.t1));
+        }
+    }
+    const float gv = k.z * __uint_as_float(q[1]) + k.y * __uint_as_float(q[2]) + k.x * __uint_as_float(q[3]);
+    if (flags & REC_RF) {
+        const float wrf = __uint_as_float(q[4]);
+        float bx, by;
+        if (!GEN || txm.tx == 1) {
+            const float br = __uint_as_float(q[REC_HEAD]), bi = __uint_as_float(q[REC_HEAD + 1]);
+            bx = k.wtx_r * br - k.wtx_i * bi;
+            by = k.wtx_r * bi + k.wtx_i * br;
+        } else {
+            bx = 0.f;
+            by = 0.f;
+            for (int c = 0; c < txm.tx; c++) {
+                const float br = __uint_as_float(q[REC_HEAD + 2 * c]);
+                const float bi = __uint_as_float(q[REC_HEAD + 2 * c + 1]);
+                float sp, cp;
+                sincos_cr(txm.pe[txm.v + (int64_t)c * txm.nvox], &sp, &cp);
+                const float mg = txm.mg[txm.v + (int64_t)c * txm.nvox];
+                bx += mg * (cp * br - sp * bi);
+                by += mg * (cp * bi + sp * br);
+            }
+        }
+        const float b2 = bx * bx + by * by;
+#pragma unroll
+        for (int s = 0; s < SPT; s++) {
+            // rotation by |Omega|*dt about -Omega, Omega = (bx, by, w): the
+            // matrix of BlochKernel.h:146-306 in Rodrigues form
+            const float w = (st.wb[s] + gv) + wrf;
+            const float om2 = b2 + w * w;
+            if (om2 > 0.f) {
+                const float om = sqrtf(om2);
+                const float inv = 1.0f / om;
+                const float nx = bx * inv, ny = by * inv, nz = w * inv;
+                float sn, cs;
+                sincos_cr(om * dt, &sn, &cs);
+                const float ax = st.mx[s], ay = st.my[s], az = st.mz[s];
+                const float kk = (1.f - cs) * (nx * ax + ny * ay + nz * az);
+                st.mx[s] = ax * cs - (ny * az - nz * ay) * sn + nx * kk;
+                st.my[s] = ay * cs - (nz * ax - nx * az) * sn + ny * kk;
+                st.mz[s] = az * cs - (nx * ay - ny * ax) * sn + nz * kk;
+            }
+        }
+        if constexpr (PREC == 0) {
+#pragma unroll
+            for (int s = 0; s < SPT; s++) {
+                st.mx[s] = st.mx[s] * st.e2;
+                st.my[s] = st.my[s] * st.e2;
+                st.mz[s] = relax_mz_ref(st.mz[s], st.e1, st.m0e);
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < SPT; s++) {
+                st.mx[s] = st.mx[s] - st.mx[s] * st.d2;
+                st.my[s] = st.my[s] - st.my[s] * st.d2;
+                st.mz[s] = st.mz[s] + (k.m0s - st.mz[s]) * st.d1;
+            }
+        }
+    } else {
+        if (flags & REC_ONCE) {
+            // rotation used by this step only
+#pragma unroll
+            for (int s = 0; s < SPT; s++) sincos_fast((st.wb[s] + gv) * dt, &st.ss[s], &st.cc[s]);
+            st.have_rot = false;
+        } else if (!((flags & REC_SAME) && st.have_rot)) {
+#pragma unroll
+            for (int s = 0; s < SPT; s++) sincos_cr((st.wb[s] + gv) * dt, &st.ss[s], &st.cc[s]);
+            st.have_rot = true;
+        }
+        // this step, then the folded repeats of it
+        const int n = 1 + (int)(flags >> REC_REP_SHIFT);
+        for (int r = 0; r < n; r++) free_step<SPT, PREC>(st, k.m0s);
+    }
+}
+
+}  // namespace mrs

@@ -40,7 +40,10 @@ STATUS = {0: "MRS_OK", 1: "MRS_ERR_ARG", 2: "MRS_ERR_DEVICE", 3: "MRS_ERR_CALLBA
 # every symbol declared in include/mrilab_amd.h
 EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanAtCPU",
            "mrs_DoScanAtGPU", "mrs_DoGMScanAtGPU", "mrs_plan_create", "mrs_plan_run", "mrs_plan_sync",
-           "mrs_plan_fetch", "mrs_plan_get_info", "mrs_plan_destroy")
+           "mrs_plan_fetch", "mrs_plan_get_info", "mrs_plan_destroy", "mrs_batch_create", "mrs_batch_run",
+           "mrs_batch_sync", "mrs_batch_fetch", "mrs_batch_get_info", "mrs_batch_destroy",
+           "mrs_DoScanBatchAtGPU", "mrs_adjoint_create", "mrs_adjoint_forward", "mrs_adjoint_backward",
+           "mrs_adjoint_get_info", "mrs_adjoint_destroy")
 
 
 class MRSError(RuntimeError):
@@ -80,6 +83,35 @@ def load() -> C.CDLL:
         lib.mrs_plan_get_info.restype = C.c_int
         lib.mrs_plan_destroy.argtypes = [C.c_void_p]
         lib.mrs_plan_destroy.restype = None
+        PP = lambda t: P(P(t))  # noqa: E731  (array of struct pointers)
+        batch_structs = [PP(abi.VObjC), PP(abi.VMagC), PP(abi.VCoiC), PP(abi.VCtlC), PP(abi.VSeqC),
+                         PP(abi.VVarC), PP(abi.VSigC)]
+        lib.mrs_batch_create.argtypes = [C.c_int32] + batch_structs + [P(abi.OptionsC), P(C.c_void_p)]
+        lib.mrs_batch_create.restype = C.c_int
+        lib.mrs_DoScanBatchAtGPU.argtypes = [C.c_int32] + batch_structs + [P(abi.OptionsC)]
+        lib.mrs_DoScanBatchAtGPU.restype = C.c_int
+        for fn in ("mrs_batch_run", "mrs_batch_sync"):
+            getattr(lib, fn).argtypes = [C.c_void_p]
+            getattr(lib, fn).restype = C.c_int
+        lib.mrs_batch_fetch.argtypes = [C.c_void_p, PP(abi.VObjC), PP(abi.VVarC), PP(abi.VSigC)]
+        lib.mrs_batch_fetch.restype = C.c_int
+        lib.mrs_batch_get_info.argtypes = [C.c_void_p, P(abi.BatchInfoC)]
+        lib.mrs_batch_get_info.restype = C.c_int
+        lib.mrs_batch_destroy.argtypes = [C.c_void_p]
+        lib.mrs_batch_destroy.restype = None
+        lib.mrs_adjoint_create.argtypes = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.ExcitationC),
+                                           P(abi.OptionsC), P(C.c_void_p)]
+        lib.mrs_adjoint_create.restype = C.c_int
+        lib.mrs_adjoint_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_int32, P(C.c_double), P(C.c_double)]
+        lib.mrs_adjoint_forward.restype = C.c_int
+        lib.mrs_adjoint_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_int32]
+        lib.mrs_adjoint_backward.restype = C.c_int
+        lib.mrs_adjoint_get_info.argtypes = [C.c_void_p, P(abi.AdjointInfoC)]
+        lib.mrs_adjoint_get_info.restype = C.c_int
+        lib.mrs_adjoint_destroy.argtypes = [C.c_void_p]
+        lib.mrs_adjoint_destroy.restype = None
         _lib = lib
         return lib
 
@@ -199,6 +231,176 @@ class Plan:
     def close(self) -> None:
         if self._h:
             self._lib.mrs_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _ptr_array(ctype, items):
+    arr = (C.POINTER(ctype) * len(items))()
+    for i, it in enumerate(items):
+        arr[i] = C.pointer(it) if it is not None else C.POINTER(ctype)()
+    return arr
+
+
+class Batch:
+    """Many independent single-pool scans run as one persistent launch
+    (mrs_batch_*): ``scans`` is a sequence of (VObj, VMag, VCoi, VCtl, VSeq,
+    VVar, VSig) tuples. No reference counterpart; BatchSim.m:129-166 runs
+    saved scans one after another through the engine."""
+
+    def __init__(self, scans, options=None):
+        self._lib = load()
+        self._packed = [abi.pack_scan(*sc) for sc in scans]
+        self.n = len(self._packed)
+        P = self._packed
+        self._args = [_ptr_array(abi.VObjC, [p.vobj for p in P]), _ptr_array(abi.VMagC, [p.vmag for p in P]),
+                      _ptr_array(abi.VCoiC, [p.vcoi for p in P]), _ptr_array(abi.VCtlC, [p.vctl for p in P]),
+                      _ptr_array(abi.VSeqC, [p.vseq for p in P]), _ptr_array(abi.VVarC, [p.vvar for p in P]),
+                      _ptr_array(abi.VSigC, [p.vsig for p in P])]
+        self._h = C.c_void_p()
+        _check(self._lib.mrs_batch_create(C.c_int32(self.n), *self._args,
+                                          C.byref(options) if options is not None else None,
+                                          C.byref(self._h)))
+
+    def run(self) -> None:
+        _check(self._lib.mrs_batch_run(self._h))
+
+    def sync(self) -> None:
+        _check(self._lib.mrs_batch_sync(self._h))
+
+    def fetch(self, scans=None, vobj=True, vvar=True, vsig=True) -> None:
+        """Writes results into ``scans`` (default: the structs given at
+        construction), each a (VObj, ..., VSig) tuple of the same shapes."""
+        if scans is None:
+            Q = self._packed
+        else:
+            Q = [abi.pack_scan(*sc) for sc in scans]
+        a_obj = _ptr_array(abi.VObjC, [q.vobj for q in Q]) if vobj else None
+        a_var = _ptr_array(abi.VVarC, [q.vvar for q in Q]) if vvar else None
+        a_sig = _ptr_array(abi.VSigC, [q.vsig for q in Q]) if vsig else None
+        _check(self._lib.mrs_batch_fetch(self._h, a_obj, a_var, a_sig))
+        if vvar:
+            src = scans if scans is not None else [p.keep for p in self._packed]
+            for q, sc in zip(Q, src):
+                abi.unpack_vvar(q, sc[5])
+
+    def info(self) -> dict:
+        inf = abi.BatchInfoC()
+        _check(self._lib.mrs_batch_get_info(self._h, C.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in abi.BatchInfoC._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.mrs_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def DoScanBatchAtGPU(scans, options=None) -> None:
+    """Runs every scan of ``scans`` ((VObj, ..., VSig) tuples) in one launch
+    and mutates each tuple's structs as DoScanAtGPU would."""
+    with Batch(scans, options) as b:
+        b.run()
+        b.fetch()
+
+
+def _vec_ptr(a):
+    """(pointer, is_device, keep-alive) of a float32 numpy array or torch tensor."""
+    if hasattr(a, "data_ptr"):  # torch tensor
+        t = a.detach().contiguous()
+        if t.dtype != __import__("torch").float32:
+            raise TypeError("adjoint tensors must be float32")
+        return C.c_void_p(t.data_ptr()), 1 if t.is_cuda else 0, t
+    arr = np.ascontiguousarray(a, dtype=np.float32)
+    return C.c_void_p(arr.ctypes.data), 0, arr
+
+
+class Adjoint:
+    """rf-design problem on the GPU (mrs_adjoint_*): forward loss and its
+    gradients w.r.t. rf [n, 2], T1, T2 and dB0 [nvox]. No reference
+    counterpart. ``exc`` is a workloads.Excitation (or anything with the same
+    fields); parameters may be numpy arrays (host) or float32 torch tensors on
+    the GPU (device pointers)."""
+
+    def __init__(self, exc, checkpoint: int = 64, options=None):
+        self._lib = load()
+        S = int(exc.VObj["SpinNum"])
+        self._packed = abi.pack_scan(exc.VObj, exc.VMag, exc.VCoi, *exc.aux)
+        self.n = int(exc.grad.shape[0])
+        self.nvox = int(exc.target.shape[0])
+        self.spins = S
+        self._grad = np.ascontiguousarray(exc.grad, dtype=np.float32)
+        self._target = np.ascontiguousarray(exc.target, dtype=np.float32)
+        if self._grad.shape != (self.n, 3) or self._target.shape != (self.nvox, 2):
+            raise ValueError("grad must be [n, 3] and target [nvox, 2]")
+        ex = abi.ExcitationC(n_steps=self.n, checkpoint=int(checkpoint), dt=float(exc.dt),
+                             grad=self._grad.ctypes.data_as(C.POINTER(C.c_float)),
+                             target=self._target.ctypes.data_as(C.POINTER(C.c_float)))
+        self._h = C.c_void_p()
+        P = self._packed
+        _check(self._lib.mrs_adjoint_create(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi), C.byref(ex),
+                                            C.byref(options) if options is not None else None,
+                                            C.byref(self._h)))
+
+    def forward(self, rf, T1, T2, dB0, want_mxy: bool = False):
+        """Returns the loss (and the voxel sums [nvox, 2] if asked)."""
+        ptrs = [_vec_ptr(a) for a in (rf, T1, T2, dB0)]
+        dev = {p[1] for p in ptrs}
+        if len(dev) != 1:
+            raise ValueError("parameters must all be on the host or all on the GPU")
+        if dev == {1}:
+            import torch
+            torch.cuda.synchronize()
+        loss = C.c_double()
+        mxy = np.zeros((self.nvox, 2)) if want_mxy else None
+        _check(self._lib.mrs_adjoint_forward(self._h, ptrs[0][0], ptrs[1][0], ptrs[2][0], ptrs[3][0],
+                                             C.c_int32(dev.pop()), C.byref(loss),
+                                             mxy.ctypes.data_as(C.POINTER(C.c_double)) if want_mxy else None))
+        del ptrs
+        return (loss.value, mxy) if want_mxy else loss.value
+
+    def backward(self, out=None):
+        """Gradients (g_rf [n,2], g_T1, g_T2, g_dB0 [nvox]) of the last forward
+        pass, as numpy arrays, or written into ``out`` (four float32 torch
+        tensors on the GPU)."""
+        if out is None:
+            g = (np.zeros((self.n, 2), np.float32), np.zeros(self.nvox, np.float32),
+                 np.zeros(self.nvox, np.float32), np.zeros(self.nvox, np.float32))
+            _check(self._lib.mrs_adjoint_backward(self._h, *[C.c_void_p(a.ctypes.data) for a in g], C.c_int32(0)))
+            return g
+        _check(self._lib.mrs_adjoint_backward(self._h, *[C.c_void_p(t.data_ptr()) for t in out], C.c_int32(1)))
+        return out
+
+    def info(self) -> dict:
+        inf = abi.AdjointInfoC()
+        _check(self._lib.mrs_adjoint_get_info(self._h, C.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in abi.AdjointInfoC._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.mrs_adjoint_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

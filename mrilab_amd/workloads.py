@@ -432,3 +432,154 @@ def mt_cest(shape=(256, 256, 1), n_offsets=40, ppm_range=50.0, b0=3.0, sat_time=
     sc.name = f"mt_cest_{R}x{C_}x{Z}_off{n_offsets}_steps{n}"
     sc.offsets_hz = offsets
     return sc
+
+
+def _fingerprint_template(t_rf, te, delta):
+    """One finalized fingerprinting TR (hard pulse of 1 deg, ADC at TE) with
+    a placeholder duration; returns it and the placeholder."""
+    t_end = 0.0123456789
+    tr = psd.TR(duration=t_end)
+    t = np.array([0.0, delta, delta + t_rf, 2 * delta + t_rf])
+    amp = np.array([0.0, 1.0, 1.0, 0.0]) * (np.pi / 180.0) / (GYRO * t_rf)
+    tr.add_rf(psd.RF(t, amp, np.zeros(4), np.zeros(4), np.ones(4)))
+    tr.add_gz([0.0, t_end], [0, 0])
+    tr.add_gy([0.0, t_end], [0, 0])
+    tr.add_gx([0.0, t_end], [0, 0])
+    tr.add_adc([0.0, te, t_end], [0, 1, 0])
+    tr.add_ext([0.0, t_end], [0, 0])
+    return tr.finalize(), t_end
+
+
+def fingerprint_set(flips_deg, tr_s, T1, T2, dB0, b1, spins=64, t_rf=0.5e-3, te=3e-3, delta=10e-6,
+                    tpl=None) -> Scan:
+    """One single-voxel fingerprinting scan: a train of hard pulses with
+    flip ``flips_deg[k]`` and repetition time ``tr_s[k]``, one ADC sample at
+    TE per TR (5 steps per TR: pulse edges, pulse, to TE, to TR). The
+    ``spins`` isochromats spread uniformly over one cycle per 10 ms; B1+
+    scale ``b1`` enters as TxCoilmg."""
+    fl, t_end = tpl if tpl is not None else _fingerprint_template(t_rf, te, delta)
+    flips = np.asarray(flips_deg, dtype=np.float64)
+    trs = np.asarray(tr_s, dtype=np.float64)
+    n = flips.size
+    tsub = lambda a: np.where(a[None, :] == t_end, trs[:, None], a[None, :]).ravel()  # noqa: E731
+    seq = {"rfAmpLine": (fl["rfAmp"][None, :] * flips[:, None]).ravel(),
+           "rfPhaseLine": np.tile(fl["rfPhase"], n), "rfFreqLine": np.tile(fl["rfFreq"], n),
+           "rfCoilLine": np.ones(fl["rfCoil"].size * n),
+           "GzAmpLine": np.tile(fl["Gz"], n), "GyAmpLine": np.tile(fl["Gy"], n),
+           "GxAmpLine": np.tile(fl["Gx"], n), "ADCLine": np.tile(fl["ADC"], n),
+           "ExtLine": np.tile(fl["Ext"], n), "utsLine": tsub(fl["uts"]), "tsLine": tsub(fl["ts"]),
+           "flagsLine": np.tile(fl["flags"], (1, n))}
+    for k, v in seq.items():
+        seq[k] = np.asfortranarray(v, dtype=np.float64)
+    shape = (1, 1, 1)
+    dW = np.zeros(shape + (spins, 1))
+    dW[0, 0, 0, :, 0] = (-np.pi + 2 * np.pi * (np.arange(spins) + 0.5) / spins) / 0.010
+    sc = make_structs(np.ones(shape + (1,)), np.full(shape + (1,), T1), np.full(shape + (1,), T2),
+                      np.full(shape, dB0), (1e-3, 1e-3, 1e-3), spin_num=spins, dWRnd=dW, seq=seq, tr_num=n,
+                      tx=(np.full(shape, b1), np.zeros(shape)))
+    sc.name = f"fingerprint_{n}tr_S{spins}"
+    return sc
+
+
+def fingerprint_batch(n_sets=4096, spins=64, tr_min=500, tr_max=1500, seed=3, t_rf=0.5e-3, te=3e-3):
+    """Batched uneven-length fingerprinting: ``n_sets`` single-voxel scans
+    with T1~U[0.3,2] s, T2~U[0.02,0.3] s (< T1), dB0 ~ 2*pi*N(0,20 Hz)/gamma,
+    B1 scale ~U[0.7,1.3]; train lengths ~U[tr_min, tr_max] TRs, flips a
+    smoothed random walk in [5,70] deg, TR ~U[10,15] ms per TR."""
+    rng = np.random.default_rng(seed)
+    tpl = _fingerprint_template(t_rf, te, 10e-6)
+    out = []
+    for _ in range(n_sets):
+        n = int(rng.integers(tr_min, tr_max + 1))
+        walk = 5.0 + 65.0 * rng.random() + np.cumsum(rng.normal(0.0, 4.0, n))
+        walk = np.abs((walk - 5.0) % 130.0 - 65.0)  # reflect into [0, 65]
+        k = 25
+        flips = np.convolve(np.pad(walk, (k // 2, k - 1 - k // 2), mode="edge"), np.ones(k) / k, "valid")
+        flips = np.clip(70.0 - flips, 5.0, 70.0)
+        trs = rng.uniform(10e-3, 15e-3, n)
+        T1 = rng.uniform(0.3, 2.0)
+        T2 = min(rng.uniform(0.02, 0.3), 0.99 * T1)
+        dB0 = 2 * np.pi * rng.normal(0.0, 20.0) / GYRO
+        b1 = rng.uniform(0.7, 1.3)
+        out.append(fingerprint_set(flips, trs, T1, T2, dB0, b1, spins=spins, t_rf=t_rf, te=te, tpl=tpl))
+    return out
+
+
+@dataclass
+class Excitation:
+    """An rf-design problem for the adjoint: phantom structs, gradient
+    waveform [n, 3] (Gx, Gy, Gz, T/m), target [nvox, 2], a starting rf
+    [n, 2] (T) and the per-voxel parameters T1, T2 (s) and dB0 (T)."""
+    VObj: dict
+    VMag: dict
+    VCoi: dict
+    grad: np.ndarray
+    target: np.ndarray
+    rf: np.ndarray
+    T1: np.ndarray
+    T2: np.ndarray
+    dB0: np.ndarray
+    dt: float
+    name: str = ""
+    aux: tuple = ()   # VCtl, VSeq, VVar, VSig of a placeholder scan (struct packing only)
+
+    @property
+    def nvox(self) -> int:
+        return int(self.target.shape[0])
+
+
+def excitation(shape=(128, 128), spins=32, n_steps=2000, dt=4e-6, peak_b1=15e-6, fov=0.256, res_exc=0.02,
+               n_turns=8, flip=30.0, seed=4) -> Excitation:
+    """2-D selective excitation: a spiral-in gradient (k-space radius
+    1/(2*res_exc) down to 0 in ``n_turns`` turns over ``n_steps`` x ``dt``),
+    a smooth random complex starting rf of peak 2/3 * ``peak_b1``, and a
+    seeded target: transverse magnetization sin(flip) * PD inside a random
+    union of discs, 0 outside. Phantom maps as the single-slice case, with
+    ``spins`` isochromats per voxel on Lorentzian quantiles (50/s)."""
+    rng = np.random.default_rng(seed)
+    R, C_ = shape
+    shp = (R, C_, 1)
+    Rho, T1, T2, dB0 = tissue_maps(shp, rng)
+    res = (fov / C_, fov / R, 5e-3)
+    q = np.linspace(0.01, 0.99, spins)
+    dWRnd = np.zeros(shp + (spins, 1))
+    for s in range(spins):
+        dWRnd[..., s, 0] = 50.0 * np.tan(np.pi * (q[s] - 0.5))
+    # a one-TR placeholder sequence only to assemble the structs
+    tr = psd.TR(duration=1e-3)
+    tr.add_rf(psd.rf_rect(np.array([0.0, 1e-4, 2e-4, 3e-4]), 1.0))
+    for add in (tr.add_gz, tr.add_gy, tr.add_gx, tr.add_ext):
+        add([0.0, 1e-3], [0, 0])
+    tr.add_adc([0.0, 5e-4, 1e-3], [0, 1, 0])
+    sc = make_structs(Rho, T1, T2, dB0, res, spin_num=spins, dWRnd=dWRnd, seq=psd.concat([tr.finalize()]))
+    # spiral-in k-space trajectory and its gradient
+    t = (np.arange(n_steps + 1)) * dt
+    T = n_steps * dt
+    kmax = 1.0 / (2 * res_exc)
+    kr = kmax * (1 - t / T)
+    ph = 2 * np.pi * n_turns * t / T
+    kx, ky = kr * np.cos(ph), kr * np.sin(ph)
+    gbar = GYRO / (2 * np.pi)
+    grad = np.zeros((n_steps, 3))
+    grad[:, 0] = np.diff(kx) / dt / gbar
+    grad[:, 1] = np.diff(ky) / dt / gbar
+    # smooth random complex rf
+    n_knots = 24
+    knots = rng.normal(size=(n_knots, 2))
+    tk = np.linspace(0, n_steps - 1, n_knots)
+    rf = np.stack([np.interp(np.arange(n_steps), tk, knots[:, 0]), np.interp(np.arange(n_steps), tk, knots[:, 1])], 1)
+    rf *= (2.0 / 3.0) * peak_b1 / np.max(np.hypot(rf[:, 0], rf[:, 1]))
+    # target: union of discs
+    yy, xx = np.meshgrid(np.arange(R), np.arange(C_), indexing="ij")
+    mask = np.zeros((R, C_), bool)
+    for _ in range(4):
+        cy, cx = rng.uniform(0.25, 0.75) * R, rng.uniform(0.25, 0.75) * C_
+        rad = rng.uniform(0.08, 0.2) * min(R, C_)
+        mask |= (yy - cy) ** 2 + (xx - cx) ** 2 < rad ** 2
+    amp = np.sin(np.radians(flip)) * Rho[..., 0, 0].astype(np.float32) * mask
+    target = np.zeros((R * C_, 2), np.float32)
+    target[:, 1] = amp.ravel(order="F")
+    f32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float32))  # noqa: E731
+    return Excitation(sc.VObj, sc.VMag, sc.VCoi, f32(grad), f32(target), f32(rf),
+                      f32(T1.ravel(order="F")), f32(T2.ravel(order="F")), f32(dB0.ravel(order="F")), float(dt),
+                      name=f"excitation_{R}x{C_}_S{spins}_n{n_steps}", aux=(sc.VCtl, sc.VSeq, sc.VVar, sc.VSig))

@@ -91,3 +91,59 @@ def gm_scan(sc, precision: str = "f64", scheme: int = 0, inplace: bool = False):
 def superlorentzian(dw: float, t2b: float, precision: str = "f64") -> float:
     """The bound-pool lineshape G(dw) as the GM kernel's Riemann sum computes it."""
     return float(_lib(precision).orc_superlorentzian(float(dw), float(t2b)))
+
+
+class _ExcC(C.Structure):
+    _F = C.POINTER(C.c_float)
+    _fields_ = [("nvox", C.c_int32), ("S", C.c_int32), ("n", C.c_int32), ("dt", C.c_double),
+                ("gyro", C.c_double), ("x", _F), ("y", _F), ("z", _F), ("mg", _F), ("pe", _F),
+                ("dWRnd", _F), ("rho", _F), ("M0x", _F), ("M0y", _F), ("M0z", _F), ("grad", _F),
+                ("target", _F)]
+
+
+def _exc(exc):
+    import numpy as np
+    f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel(order="F"))  # noqa: E731
+    arrs = {"x": f(exc.VMag["Gxgrid"]), "y": f(exc.VMag["Gygrid"]), "z": f(exc.VMag["Gzgrid"]),
+            "mg": f(exc.VCoi["TxCoilmg"]), "pe": f(exc.VCoi["TxCoilpe"]), "dWRnd": f(exc.VMag["dWRnd"]),
+            "rho": f(exc.VObj["Rho"]), "M0x": f(exc.VObj["Mx"]), "M0y": f(exc.VObj["My"]),
+            "M0z": f(exc.VObj["Mz"]), "grad": np.ascontiguousarray(exc.grad, dtype=np.float32).ravel(),
+            "target": np.ascontiguousarray(exc.target, dtype=np.float32).ravel()}
+    P = C.POINTER(C.c_float)
+    s = _ExcC(nvox=int(exc.target.shape[0]), S=int(exc.VObj["SpinNum"]), n=int(exc.grad.shape[0]),
+              dt=float(exc.dt), gyro=float(exc.VObj["Gyro"]),
+              **{k: v.ctypes.data_as(P) for k, v in arrs.items()})
+    return s, arrs
+
+
+def _dptr(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def exc_loss(exc, rf, T1, T2, dB0, precision: str = "f64"):
+    """Loss and voxel sums [nvox, 2] of the excitation problem (no reference
+    counterpart; forward model of oracle/adjoint_oracle.c)."""
+    import numpy as np
+    lib = _lib(precision)
+    lib.orc_exc_loss.restype = C.c_double
+    s, keep = _exc(exc)
+    ps = [_dptr(a) for a in (rf, T1, T2, dB0)]
+    mxy = np.zeros((s.nvox, 2))
+    L = lib.orc_exc_loss(C.byref(s), *[p[1] for p in ps], mxy.ctypes.data_as(C.POINTER(C.c_double)))
+    del keep, ps
+    return L, mxy
+
+
+def exc_grad(exc, rf, T1, T2, dB0, precision: str = "f64"):
+    """Loss and reverse-mode gradients (g_rf [n,2], g_T1, g_T2, g_dB0)."""
+    import numpy as np
+    lib = _lib(precision)
+    lib.orc_exc_grad.restype = C.c_double
+    s, keep = _exc(exc)
+    ps = [_dptr(a) for a in (rf, T1, T2, dB0)]
+    g = [np.zeros(s.n * 2), np.zeros(s.nvox), np.zeros(s.nvox), np.zeros(s.nvox)]
+    L = lib.orc_exc_grad(C.byref(s), *[p[1] for p in ps], *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in g])
+    del keep, ps
+    return L, g[0].reshape(s.n, 2), g[1], g[2], g[3]

@@ -99,3 +99,35 @@ def test_bssfp_workload_shape_and_counts():
     # uniform +-pi spread of intravoxel frequencies over one TR
     w = sc.VMag["dWRnd"][0, 0, 0, :, 0] * 100 * 10e-6
     np.testing.assert_allclose(w, [-np.pi / 2, np.pi / 2], atol=1e-5)
+
+
+def test_fingerprint_batch_shape():
+    """Uneven trains: lengths in range, 5 steps and one ADC sample per TR,
+    flips in [5, 70] deg (area of the held pulse), TR in [10, 15] ms."""
+    scans = workloads.fingerprint_batch(n_sets=5, spins=64, tr_min=20, tr_max=40, seed=1)
+    lengths = set()
+    for sc in scans:
+        n = int(sc.VCtl["TRNum"])
+        lengths.add(n)
+        assert 20 <= n <= 40
+        assert sc.n_steps() == 5 * n
+        assert int(sc.VSig["SignalNum"]) == n
+        assert int(sc.VObj["SpinNum"]) == 64 and sc.n_iso == 64
+        amp = sc.VSeq["rfAmpLine"].reshape(n, -1)[:, 1]
+        fa = np.degrees(amp * psd.GYRO * 0.5e-3)
+        assert fa.min() >= 5 - 1e-9 and fa.max() <= 70 + 1e-9
+        uts = sc.VSeq["utsLine"].reshape(n, -1)
+        assert np.all((uts[:, -1] >= 10e-3) & (uts[:, -1] <= 15e-3))
+    assert len(lengths) > 1
+
+
+def test_batch_entry_fails_loudly_without_gpu():
+    """The batched entry point marshals its struct arrays and, with no
+    device visible, raises instead of falling back to the CPU."""
+    if not os.path.exists(engine.LIB_PATH):
+        pytest.skip("library not built")
+    if engine.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    scans = workloads.fingerprint_batch(n_sets=2, tr_min=5, tr_max=8)
+    with pytest.raises(engine.MRSError):
+        engine.DoScanBatchAtGPU([s.structs for s in scans])
