@@ -139,9 +139,10 @@ typedef struct mrs_options {
                                  1 = accurate: expm1-based relaxation */
     int32_t timing;           /* 1 = time every kernel launch with HIP events
                                  on the plan stream (mrs_plan_info) */
-    int32_t model;            /* 0 = single-pool engine, 1 = generalized
-                                 multi-pool engine (VObj.K, VObj.TypeFlag);
-                                 set by the entry point */
+    int32_t model;            /* 0 = single-pool engine; multi-pool engines
+                                 1 = GM, 2 = MT, 3 = CEST, 4 = ME (VObj.K in
+                                 that engine's layout); set by the entry
+                                 point, or by the caller of mrs_plan_create */
     int32_t reserved[1];
 } mrs_options;
 
@@ -174,6 +175,38 @@ mrs_status mrs_DoScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
  * reference's operator split (0) or the fp64 matrix exponential (1).
  * Replaces: mexFunction, Lib/src/engine/GPUEngine/DoGMScanAtGPU.cu:76 */
 mrs_status mrs_DoGMScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
+                             const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                             const mrs_vseq *vseq, mrs_vvar *vvar,
+                             mrs_vsig *vsig, const mrs_callbacks *cb,
+                             const mrs_options *opt);
+
+/* Two-pool magnetization transfer: pool 0 free water, pool 1 bound
+ * (super-Lorentzian saturation). VObj.K is [R,C,Z,4]: K(:,:,:,2) the rate
+ * water -> bound, K(:,:,:,3) bound -> water; water and bound exchange
+ * longitudinally only; signal is water only. TypeFlag is not read.
+ * Replaces: mexFunction, Lib/src/engine/GPUEngine/DoMTScanAtGPU.cu:76 */
+mrs_status mrs_DoMTScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
+                             const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                             const mrs_vseq *vseq, mrs_vvar *vvar,
+                             mrs_vsig *vsig, const mrs_callbacks *cb,
+                             const mrs_options *opt);
+
+/* CEST: pool 0 water, 1 MT bound pool, 2..TypeNum-1 CEST pools (TypeNum
+ * 3..4). VObj.K is [R,C,Z,2*(TypeNum-1)]: plane 1 water -> MT, plane TypeNum
+ * MT -> water, plane t water -> CEST pool t, plane t+TypeNum-1 CEST pool t
+ * -> water (1-based planes); signal is water only.
+ * Replaces: mexFunction, Lib/src/engine/GPUEngine/DoCESTScanAtGPU.cu:76 */
+mrs_status mrs_DoCESTScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
+                               const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                               const mrs_vseq *vseq, mrs_vvar *vvar,
+                               mrs_vsig *vsig, const mrs_callbacks *cb,
+                               const mrs_options *opt);
+
+/* Multi-exchange: every pool free, VObj.K [R,C,Z,TypeNum^2] with plane
+ * a*TypeNum + b (0-based) the rate from pool a into pool b (the transpose
+ * of the GM layout, diagonal applied); signal per pool.
+ * Replaces: mexFunction, Lib/src/engine/GPUEngine/DoMEScanAtGPU.cu:76 */
+mrs_status mrs_DoMEScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
                              const mrs_vcoi *vcoi, const mrs_vctl *vctl,
                              const mrs_vseq *vseq, mrs_vvar *vvar,
                              mrs_vsig *vsig, const mrs_callbacks *cb,

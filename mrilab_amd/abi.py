@@ -165,7 +165,12 @@ class Packed:
         self.keep: list[Any] = []
 
 
-def pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool: bool = False) -> Packed:
+def pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool: bool = False,
+              k_planes: int | None = None) -> Packed:
+    """Checks dtypes and sizes of the seven structs and returns the ABI
+    structs pointing at their buffers. multipool: pass VObj.K (with
+    ``k_planes`` planes of R*C*Z, default TypeNum^2) and VObj.TypeFlag
+    (optional for the MT, CEST and ME layouts)."""
     P = Packed()
     Mz = _req(VObj, "VObj", "Mz", np.float32, writable=True)
     shp = Mz.shape + (1,) * (5 - Mz.ndim)
@@ -182,8 +187,9 @@ def pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool: bool = False)
     T2 = _req(VObj, "VObj", "T2", np.float32, nvox * Ty)
     K = TF = None
     if multipool:
-        K = _req(VObj, "VObj", "K", np.float32)
-        TF = _req(VObj, "VObj", "TypeFlag", np.float64, Ty)
+        K = _req(VObj, "VObj", "K", np.float32, nvox * (Ty * Ty if k_planes is None else int(k_planes)))
+        if "TypeFlag" in VObj or k_planes is None:
+            TF = _req(VObj, "VObj", "TypeFlag", np.float64, Ty)
     vobj = VObjC(Gyro=float(_scalar(VObj, "VObj", "Gyro", np.float64)), SpinNum=S, TypeNum=Ty)
     vobj.dims[0], vobj.dims[1], vobj.dims[2] = R, Cc, Z
     vobj.Mx, vobj.My, vobj.Mz = _p(Mx, F32P), _p(My, F32P), _p(Mz, F32P)

@@ -11,6 +11,10 @@
 //   psi = -Omega dt, a = (1 - cos th)/th^2, b = (th - sin th)/th^3,
 //   dL/dT2 += (lx Mx' + ly My') e2 dt/T2^2, dL/dT1 += lz (Mz' - m0) e1 dt/T1^2.
 //
+//  * The longitudinal state is carried as the deviation d = Mz - m0 (m0 =
+//    Rho/SpinNum): relaxation is then d*e1 exactly and a rotation adds
+//    m0*(R z - z), so d keeps full fp32 relative precision near equilibrium,
+//    where dL/dT1 ~ lz*(Mz' - m0) would otherwise cancel.
 //  * Forward kernel: one thread per voxel x SPT isochromats; state in VGPRs;
 //    every K steps the state goes to a checkpoint buffer in HBM; at the end
 //    each thread writes its share of the voxel's Mxy.
@@ -56,7 +60,7 @@ struct ExcParams {
 };
 
 struct ExcVoxel {
-    float x, y, z, wr, wi, m0s, e1, e2, m0e, t1, t2;
+    float x, y, z, wr, wi, m0s, e1, e2, t1, t2;
 };
 
 __device__ __forceinline__ void exc_setup(const ExcParams &p, int64_t vv, ExcVoxel &k) {
@@ -70,7 +74,8 @@ __device__ __forceinline__ void exc_setup(const ExcParams &p, int64_t vv, ExcVox
     k.m0s = p.rho[vv] / (float)p.S;
     k.t1 = p.T1[vv];
     k.t2 = p.T2[vv];
-    // relaxation factors rounded as the engine's reference mode
+    // relaxation factors rounded as the engine's reference mode (the
+    // longitudinal relaxation itself is applied to the deviation Mz - m0)
     float a2 = (1.0f / k.t2) * p.dt;
     a2 = -a2;
     if (a2 < -200.f) a2 = -200.f;
@@ -79,7 +84,6 @@ __device__ __forceinline__ void exc_setup(const ExcParams &p, int64_t vv, ExcVox
     if (a1 < -200.f) a1 = -200.f;
     k.e2 = (float)exp((double)a2);
     k.e1 = (float)exp((double)a1);
-    k.m0e = k.m0s * (1.0f - k.e1);
 }
 
 // B1 (gamma-scaled, rotated by the Tx map) and the gradient off-resonance of step j
@@ -93,9 +97,12 @@ __device__ __forceinline__ void exc_fields(const ExcParams &p, const ExcVoxel &k
     by = k.wr * bi + k.wi * br;
 }
 
-// M <- R(-Omega dt) M; returns sin, cos of |Omega| dt (0, 1 without rotation)
-__device__ __forceinline__ void exc_rotate(float bx, float by, float w, float dt, float &mx, float &my, float &mz,
-                                           float &sn, float &cs) {
+// (Mx, My, d) <- R(-Omega dt) applied to M = (Mx, My, d + m0), with d the
+// deviation of Mz from m0; returns sin, cos of |Omega| dt (0, 1 without
+// rotation). R z - z = (-ny sn + nx nz k, nx sn + ny nz k, -k (nx^2 + ny^2)),
+// k = 1 - cos.
+__device__ __forceinline__ void exc_rotate(float bx, float by, float w, float dt, float m0, float &mx, float &my,
+                                           float &md, float &sn, float &cs) {
     const float om2 = (bx * bx + by * by) + w * w;
     sn = 0.f;
     cs = 1.f;
@@ -104,11 +111,12 @@ __device__ __forceinline__ void exc_rotate(float bx, float by, float w, float dt
         const float inv = 1.0f / om;
         const float nx = bx * inv, ny = by * inv, nz = w * inv;
         sincos_fast(om * dt, &sn, &cs);
-        const float ax = mx, ay = my, az = mz;
-        const float kk = (1.f - cs) * (nx * ax + ny * ay + nz * az);
-        mx = ax * cs - (ny * az - nz * ay) * sn + nx * kk;
-        my = ay * cs - (nz * ax - nx * az) * sn + ny * kk;
-        mz = az * cs - (nx * ay - ny * ax) * sn + nz * kk;
+        const float ax = mx, ay = my, az = md;
+        const float k1 = 1.f - cs;
+        const float kk = k1 * (nx * ax + ny * ay + nz * az);
+        mx = ax * cs - (ny * az - nz * ay) * sn + nx * kk + m0 * (nx * nz * k1 - ny * sn);
+        my = ay * cs - (nz * ax - nx * az) * sn + ny * kk + m0 * (ny * nz * k1 + nx * sn);
+        md = az * cs - (nx * ay - ny * ax) * sn + nz * kk - m0 * (k1 * (nx * nx + ny * ny));
     }
 }
 
@@ -122,14 +130,14 @@ __global__ __launch_bounds__(256) void exc_fwd_kernel(ExcParams p) {
     ExcVoxel k;
     exc_setup(p, vv, k);
     const float w0 = p.dB0[vv] * p.gyro;
-    float mx[SPT], my[SPT], mz[SPT], wb[SPT];
+    float mx[SPT], my[SPT], mz[SPT], wb[SPT];  // mz: deviation Mz - m0
     int64_t sidx[SPT];
 #pragma unroll
     for (int s = 0; s < SPT; s++) {
         sidx[s] = vv + nvox * (int64_t)(grp * SPT + s);
         mx[s] = p.M0x[sidx[s]];
         my[s] = p.M0y[sidx[s]];
-        mz[s] = p.M0z[sidx[s]];
+        mz[s] = p.M0z[sidx[s]] - k.m0s;
         wb[s] = w0 + p.dWRnd[sidx[s]];
     }
     int seg = 0, left = p.K;
@@ -153,10 +161,10 @@ __global__ __launch_bounds__(256) void exc_fwd_kernel(ExcParams p) {
 #pragma unroll
         for (int s = 0; s < SPT; s++) {
             float sn, cs;
-            exc_rotate(bx, by, wb[s] + gv, p.dt, mx[s], my[s], mz[s], sn, cs);
+            exc_rotate(bx, by, wb[s] + gv, p.dt, k.m0s, mx[s], my[s], mz[s], sn, cs);
             mx[s] = mx[s] * k.e2;
             my[s] = my[s] * k.e2;
-            mz[s] = relax_mz_ref(mz[s], k.e1, k.m0e);
+            mz[s] = mz[s] * k.e1;
         }
     }
     if (valid) {
@@ -250,7 +258,7 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
             if (c == 0) {
                 mx[s] = p.M0x[sidx[s]];
                 my[s] = p.M0y[sidx[s]];
-                mz[s] = p.M0z[sidx[s]];
+                mz[s] = p.M0z[sidx[s]] - k.m0s;
             } else {
                 const float *ck = p.ck + (int64_t)c * 3 * n_iso;
                 mx[s] = ck[sidx[s]];
@@ -265,7 +273,7 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
                 float sn, cs;
-                exc_rotate(bx, by, wb[s] + gv, dt, mx[s], my[s], mz[s], sn, cs);
+                exc_rotate(bx, by, wb[s] + gv, dt, k.m0s, mx[s], my[s], mz[s], sn, cs);
                 if (valid) {
                     sc[sidx[s]] = mx[s];
                     sc[n_iso + sidx[s]] = my[s];
@@ -275,7 +283,7 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                 }
                 mx[s] = mx[s] * k.e2;
                 my[s] = my[s] * k.e2;
-                mz[s] = relax_mz_ref(mz[s], k.e1, k.m0e);
+                mz[s] = mz[s] * k.e1;
             }
         }
         // adjoint through the segment
@@ -288,7 +296,8 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
             float gx = 0.f, gy = 0.f;
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
-                const float Mx = sc[sidx[s]], My = sc[n_iso + sidx[s]], Mz = sc[2 * n_iso + sidx[s]];
+                const float Mx = sc[sidx[s]], My = sc[n_iso + sidx[s]], Md = sc[2 * n_iso + sidx[s]];
+                const float Mz = Md + k.m0s;
                 const float sn = sc[3 * n_iso + sidx[s]], cs = sc[4 * n_iso + sidx[s]];
                 const float w = wb[s] + gv;
                 const float om2 = b2 + w * w;
@@ -303,7 +312,7 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                 }
                 const float px = lx[s] * k.e2, py = ly[s] * k.e2, pz = lz[s] * k.e1;  // lambda'
                 sT2 += lx[s] * Mx + ly[s] * My;
-                sT1 += lz[s] * (Mz - k.m0s);
+                sT1 += lz[s] * Md;
                 // u = lambda' x M'
                 const float ux = py * Mz - pz * My, uy = pz * Mx - px * Mz, uz = px * My - py * Mx;
                 // a = (1 - cos)/th^2, b = (th - sin)/th^3 (series for small th)

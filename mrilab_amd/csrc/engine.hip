@@ -354,7 +354,8 @@ struct mrs_plan {
     bool ran = false;
     float last_ms = 0.f;
     int timing = 0;
-    int model = 0, scheme = 0;        // 1: generalized multi-pool engine
+    int model = 0, scheme = 0;        // 0 single pool; 1 GM, 2 MT, 3 CEST, 4 ME multi-pool
+    int variant = 0;                  // model - 1 for the multi-pool engines
     DevBuf K;
     int flag[kGmMaxTypes] = {0};
     std::vector<hipEvent_t> tev;  // per launch: fwd begin, fwd end, reduce end
@@ -381,6 +382,42 @@ static mrs_status upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s
     HIP_TRY(hipMalloc(&b.p, bytes ? bytes : 4));
     if (bytes) HIP_TRY(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
     return MRS_OK;
+}
+
+// Rate matrix of the multi-pool kernels, [Ty][Ty][nvox] with plane q*Ty + t
+// the rate from pool t into pool q, from the K layout of each reference
+// engine: GM plane q*Ty + t (BlochKernelGM.cuh:417,423), ME plane t*Ty + q
+// (BlochKernelME.cuh:327,333, diagonal kept), MT planes 1 (water -> bound)
+// and 2 (bound -> water) (BlochKernelMT.cuh:151-154), CEST planes 0 (water
+// -> MT), Ty-1 (MT -> water), t-1 (water -> CEST pool t) and t+Ty-2 (CEST
+// pool t -> water) (BlochKernelCEST.cuh:157-160, :379-384).
+static void canonical_rates(const float *K, int variant, int Ty, int64_t nvox, std::vector<float> &R) {
+    R.assign((size_t)Ty * Ty * nvox, 0.f);
+    auto set = [&](int a, int b, int plane) {
+        std::copy(K + (size_t)plane * nvox, K + (size_t)(plane + 1) * nvox, R.begin() + (size_t)(b * Ty + a) * nvox);
+    };
+    switch (variant) {
+        case kVarME:
+            for (int a = 0; a < Ty; a++)
+                for (int b = 0; b < Ty; b++) set(a, b, a * Ty + b);
+            break;
+        case kVarMT:
+            set(0, 1, 1);
+            set(1, 0, 2);
+            break;
+        case kVarCEST:
+            set(0, 1, 0);
+            set(1, 0, Ty - 1);
+            for (int t = 2; t < Ty; t++) {
+                set(0, t, t - 1);
+                set(t, 0, t + Ty - 2);
+            }
+            break;
+        default:
+            for (int a = 0; a < Ty; a++)
+                for (int b = 0; b < Ty; b++)
+                    if (a != b) set(a, b, b * Ty + a);
+    }
 }
 
 mrs_status mrs::check_args(const mrs_vobj *vo, const mrs_vmag *vm, const mrs_vcoi *vc, const mrs_vctl *vt,
@@ -465,11 +502,25 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     P->timing = opt ? opt->timing : 0;
     P->model = opt ? opt->model : 0;
     P->scheme = opt ? opt->multipool_scheme : 0;
-    if (P->model == 1) {
-        if (!vo->K || !vo->TypeFlag) return fail(MRS_ERR_ARG, "multi-pool scan needs VObj.K and VObj.TypeFlag");
+    if (P->model < 0 || P->model > 4) return fail(MRS_ERR_ARG, "unknown engine model");
+    if (P->model >= 1) {
+        P->variant = P->model - 1;
+        if (!vo->K) return fail(MRS_ERR_ARG, "multi-pool scan needs VObj.K");
         if (vo->TypeNum < 2 || vo->TypeNum > kGmMaxTypes)
             return fail(MRS_ERR_UNSUPPORTED, "multi-pool engine supports TypeNum 2..4");
-        for (int t = 0; t < vo->TypeNum; t++) P->flag[t] = (int)vo->TypeFlag[t];
+        if (P->variant == kVarGM && !vo->TypeFlag) return fail(MRS_ERR_ARG, "the GM engine needs VObj.TypeFlag");
+        if (P->variant == kVarMT && vo->TypeNum != 2)
+            return fail(MRS_ERR_ARG, "the MT engine takes two pools (free water, bound)");
+        if (P->variant == kVarCEST && vo->TypeNum < 3)
+            return fail(MRS_ERR_ARG, "the CEST engine takes water, a bound pool and CEST pools (TypeNum >= 3)");
+        for (int t = 0; t < vo->TypeNum; t++) {
+            switch (P->variant) {
+                case kVarGM: P->flag[t] = (int)vo->TypeFlag[t]; break;
+                case kVarMT:
+                case kVarCEST: P->flag[t] = t == 1; break;  // pool 1 is the bound pool
+                default: P->flag[t] = 0;                    // ME: every pool free
+            }
+        }
     }
     P->run_mode = vt->RunMode;
     P->signal_num = vg->SignalNum;
@@ -482,22 +533,22 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     while (spt > 1 && (P->S % spt != 0 || spt > P->S)) spt >>= 1;
     if (spt != 1 && spt != 2 && spt != 4 && spt != 8) return fail(MRS_ERR_ARG, "spins_per_thread must be 1, 2, 4 or 8");
     if (P->S % spt) return fail(MRS_ERR_ARG, "spins_per_thread must divide SpinNum");
-    if (P->model == 1) spt = 1;
+    if (P->model >= 1) spt = 1;
     P->spt = spt;
     P->ngroups = P->S / spt;
     int block = opt && opt->block_size > 0 ? opt->block_size : (P->nvox >= (int64_t)256 * 512 ? 256 : 64);
     if (block % 64 || block > 256) return fail(MRS_ERR_ARG, "block_size must be 64, 128, 192 or 256");
     P->block = block;
     P->gx = (int)((P->nvox + block - 1) / block);
-    const int gy = P->model == 1 ? P->ngroups : P->Ty * P->ngroups;
+    const int gy = P->model >= 1 ? P->ngroups : P->Ty * P->ngroups;
     if (gy > 65535) return fail(MRS_ERR_UNSUPPORTED, "TypeNum*SpinNum/spins_per_thread too large");
     P->nblk = P->gx * gy;
     int nrx = 1;
     const int rxn = P->rx_default ? 1 : P->Rx;
     while (nrx < rxn) nrx <<= 1;
     P->nrx = nrx;
-    P->nv = P->model == 1 ? gm_values_per_isochromat(P->Ty, nrx) : 2 * nrx;
-    if (P->model == 1 && (nrx > 8 || !pick_gm(P->Ty, nrx, P->scheme)))
+    P->nv = P->model >= 1 ? gm_values_per_isochromat(P->Ty, nrx) : 2 * nrx;
+    if (P->model >= 1 && (nrx > 8 || !pick_gm(P->Ty, nrx, P->scheme, P->variant)))
         return fail(MRS_ERR_UNSUPPORTED, "multi-pool engine supports up to 8 receive coils");
     const size_t budget = (size_t)512 << 20;  // partial-buffer budget
     int64_t max_adc = (int64_t)(budget / ((size_t)P->nblk * P->nv * sizeof(float)));
@@ -515,7 +566,12 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
         HIP_TRY(hipMalloc(&P->M[k].p, P->ntot * sizeof(float)));
     }
     if ((rc = upload(P->dWRnd, vm->dWRnd, P->ntot * sizeof(float), st))) return rc;
-    if (P->model == 1 && (rc = upload(P->K, vo->K, vb * P->Ty * P->Ty, st))) return rc;
+    if (P->model >= 1) {
+        std::vector<float> rates;
+        canonical_rates(vo->K, P->variant, P->Ty, P->nvox, rates);
+        if ((rc = upload(P->K, rates.data(), rates.size() * sizeof(float), st))) return rc;
+        HIP_TRY(hipStreamSynchronize(st));  // rates is a host temporary
+    }
     if ((rc = upload(P->Rho, vo->Rho, vb * P->Ty, st))) return rc;
     if ((rc = upload(P->T1, vo->T1, vb * P->Ty, st))) return rc;
     if ((rc = upload(P->T2, vo->T2, vb * P->Ty, st))) return rc;
@@ -625,7 +681,9 @@ static mrs_status execute_gm(mrs_plan *P) {
         gp.cs[t] = P->cs[t];
         gp.flag[t] = P->flag[t];
     }
-    gm_launch_fn fn = pick_gm(P->Ty, P->nrx, P->scheme);
+    gp.variant = P->variant;
+    gp.self_x = P->variant == kVarME;
+    gm_launch_fn fn = pick_gm(P->Ty, P->nrx, P->scheme, P->variant);
     const dim3 grid(P->gx, P->ngroups), block(P->block);
     const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
     const uint32_t *rec = (const uint32_t *)P->rec.p;
@@ -691,7 +749,7 @@ static mrs_status execute(mrs_plan *P, int64_t adc_base) {
     fp.stride = P->dec.stride;
     fp.gyro = P->gyro;
     for (int t = 0; t < P->Ty; t++) fp.cs[t] = P->cs[t];
-    if (P->model == 1) return execute_gm(P);
+    if (P->model >= 1) return execute_gm(P);
     launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1, P->precision != 0);
     const dim3 grid(P->gx, P->Ty * P->ngroups), block(P->block);
     const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
@@ -770,7 +828,7 @@ mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs
         return rc;
     }
     P->dec.tx = P->Tx;
-    P->dec.coil_words = P->model == 1 ? 4 : 2;
+    P->dec.coil_words = P->model >= 1 ? 4 : 2;
     P->dec.stride = rec_stride(P->Tx, P->dec.coil_words);
     Decoder dec(vobj, vcoi, vctl, vseq, vvar, false);
     dec.run(P->dec);
@@ -921,7 +979,7 @@ static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, cons
     for (;;) {
         P->dec = Decoded();
         P->dec.tx = P->Tx;
-        P->dec.coil_words = P->model == 1 ? 4 : 2;
+        P->dec.coil_words = P->model >= 1 ? 4 : 2;
         P->dec.stride = rec_stride(P->Tx, P->dec.coil_words);
         if (!all.muts.empty()) P->dec.muts = all.muts;
         const DecodeStop stop = dec.run(P->dec);
@@ -1007,6 +1065,34 @@ mrs_status mrs_DoGMScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vco
     if (opt) o = *opt;
     o.model = 1;
     return mrs_DoScanAtGPU(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, &o);
+}
+
+static mrs_status multipool_entry(int model, mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
+                                  const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig,
+                                  const mrs_callbacks *cb, const mrs_options *opt) {
+    mrs_options o;
+    std::memset(&o, 0, sizeof o);
+    if (opt) o = *opt;
+    o.model = model;
+    return mrs_DoScanAtGPU(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, &o);
+}
+
+mrs_status mrs_DoMTScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                             const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
+                             const mrs_options *opt) {
+    return multipool_entry(2, vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt);
+}
+
+mrs_status mrs_DoCESTScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                               const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
+                               const mrs_options *opt) {
+    return multipool_entry(3, vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt);
+}
+
+mrs_status mrs_DoMEScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                             const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
+                             const mrs_options *opt) {
+    return multipool_entry(4, vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt);
 }
 
 mrs_status mrs_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,

@@ -14,6 +14,16 @@
 //             blocks, scalar bound-pool rows, diagonal exchange couplings).
 // One thread owns one isochromat of one voxel with all its pools; all pool
 // state stays in registers (fp64 for SCHEME 1) across the launch.
+//
+// The MT, CEST and ME engines (BlochKernelMT.cuh, BlochKernelCEST.cuh,
+// BlochKernelME.cuh) are variants of the same model, with K converted to one
+// rate matrix on the host:
+//   ME:   GM with every pool free and a nonzero diagonal of K applied;
+//   MT:   water (pool 0) and a bound pool (1) exchanging longitudinally only,
+//         water relaxing through the combined exponent exp(-dt/T1 - dt*k);
+//   CEST: MT plus free CEST pools t >= 2 exchanging with water only.
+// MT and CEST skip the voxel on the water pool's maps and sample water only
+// (WC: the water-centred split of those two kernels).
 #include "multipool.hpp"
 
 #include "decode.hpp"
@@ -50,7 +60,7 @@ struct GmState {
     T mx[TY], my[TY], mz[TY];
 };
 
-template <int TY, int NRX, int SCHEME>
+template <int TY, int NRX, int SCHEME, bool WC>
 __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_t *__restrict__ rec_g, int r0,
                                                        int r1) {
     using T = typename std::conditional<SCHEME == 0, float, double>::type;
@@ -94,7 +104,14 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
         skip[t] = t1[t] == 0.f || t2[t] == 0.f || rho[t] == 0.f;  // BlochKernelGM.cuh:176
         wb[t] = (w0 + p.dWRnd[sidx[t]]) + p.cs[t];
 #pragma unroll
-        for (int q = 0; q < TY; q++) kmat[t][q] = q == t ? 0.f : p.K[vv + nvox * ((int64_t)q * TY + t)];
+        for (int q = 0; q < TY; q++)
+            kmat[t][q] = (q == t && !p.self_x) ? 0.f : p.K[vv + nvox * ((int64_t)q * TY + t)];
+    }
+    // MT / CEST: the water pool's maps decide for the whole voxel
+    const bool wcv = p.variant == kVarMT || p.variant == kVarCEST;
+    if (WC || wcv) {
+#pragma unroll
+        for (int t = 1; t < TY; t++) skip[t] = skip[0];
     }
     float crx[NRX], cry[NRX];
 #pragma unroll
@@ -145,7 +162,11 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
             for (int j = 0; j < NVT; j++) val[j] = 0.f;
 #pragma unroll
             for (int t = 0; t < TY; t++) {
-                if (p.flag[t] != 0) continue;
+                if (WC || wcv) {
+                    if (t != 0) continue;  // water only (BlochKernelMT.cuh:230)
+                } else if (p.flag[t] != 0) {
+                    continue;
+                }
                 const float sx = (valid && !skip[t]) ? (float)st.mx[t] : 0.f;
                 const float sy = (valid && !skip[t]) ? (float)st.my[t] : 0.f;
 #pragma unroll
@@ -215,7 +236,35 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
             (void)wsum;
         }
         const int nrep = 1 + (int)(flags >> REC_REP_SHIFT);
-        if constexpr (SCHEME == 0) {
+        // rotation of free pool t over this step (BlochKernel.h:146-344 form)
+        auto rotate_free = [&](int t, float &ax, float &ay, float &az) {
+            const float w = rf ? (wb[t] + gv) + wrf : wb[t] + gv;
+            if (rf) {
+                const float om2 = (bx * bx + by * by) + w * w;
+                if (om2 > 0.f) {
+                    const float om = sqrtf(om2);
+                    const float inv = 1.0f / om;
+                    const float nx = bx * inv, ny = by * inv, nz = w * inv;
+                    float sn, cs;
+                    sincos_cr(om * dt, &sn, &cs);
+                    const float kk = (1.f - cs) * (nx * ax + ny * ay + nz * az);
+                    const float rx = ax * cs - (ny * az - nz * ay) * sn + nx * kk;
+                    const float ry = ay * cs - (nz * ax - nx * az) * sn + ny * kk;
+                    const float rz = az * cs - (nx * ay - ny * ax) * sn + nz * kk;
+                    ax = rx;
+                    ay = ry;
+                    az = rz;
+                }
+            } else {
+                float sn, cs;
+                sincos_cr(w * dt, &sn, &cs);
+                const float rx = ax * cs + ay * sn;
+                const float ry = ay * cs - ax * sn;
+                ax = rx;
+                ay = ry;
+            }
+        };
+        if constexpr (SCHEME == 0 && !WC) {
 #pragma clang fp contract(off)
             if ((flags & REC_NEWDT) || !have_e) {
                 have_e = true;
@@ -226,8 +275,7 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                     m0[t] = (rho[t] * (1.f - e1[t])) / (float)p.S;
                     float ko = 1.f;
 #pragma unroll
-                    for (int qq = 0; qq < TY; qq++) {
-                        if (qq == t) continue;
+                    for (int qq = 0; qq < TY; qq++) {  // the diagonal is zero except for ME
                         if (kmat[t][qq] != 0.f) ko = ko * expf_cr(-dt * kmat[t][qq]);
                         fin[t][qq] = kmat[qq][t] != 0.f ? 1.f - expf_cr(-dt * kmat[qq][t]) : 0.f;
                     }
@@ -247,31 +295,7 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                     if (skip[t] || (p.flag[t] != 0 && p.flag[t] != 1)) continue;
                     float ax = Bx[t], ay = By[t], az = Bz[t];
                     if (p.flag[t] == 0) {
-                        const float w = rf ? (wb[t] + gv) + wrf : wb[t] + gv;
-                        if (rf) {
-                            const float om2 = (bx * bx + by * by) + w * w;
-                            if (om2 > 0.f) {
-                                const float om = sqrtf(om2);
-                                const float inv = 1.0f / om;
-                                const float nx = bx * inv, ny = by * inv, nz = w * inv;
-                                float sn, cs;
-                                sincos_cr(om * dt, &sn, &cs);
-                                const float kk = (1.f - cs) * (nx * ax + ny * ay + nz * az);
-                                const float rx = ax * cs - (ny * az - nz * ay) * sn + nx * kk;
-                                const float ry = ay * cs - (nz * ax - nx * az) * sn + ny * kk;
-                                const float rz = az * cs - (nx * ay - ny * ax) * sn + nz * kk;
-                                ax = rx;
-                                ay = ry;
-                                az = rz;
-                            }
-                        } else {
-                            float sn, cs;
-                            sincos_cr(w * dt, &sn, &cs);
-                            const float rx = ax * cs + ay * sn;
-                            const float ry = ay * cs - ax * sn;
-                            ax = rx;
-                            ay = ry;
-                        }
+                        rotate_free(t, ax, ay, az);
                         ax = ax * e2[t];
                         ay = ay * e2[t];
                     } else if (rf) {
@@ -281,7 +305,7 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                     float kx = 0.f, ky = 0.f, kz = 0.f;
 #pragma unroll
                     for (int qq = 0; qq < TY; qq++) {
-                        if (qq == t || fin[t][qq] == 0.f) continue;
+                        if (fin[t][qq] == 0.f) continue;
                         if (p.flag[t] == 0) {
                             kx = kx + fin[t][qq] * Bx[qq];
                             ky = ky + fin[t][qq] * By[qq];
@@ -295,9 +319,85 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                     st.mz[t] = az * kout[t] + kz + m0[t];
                 }
             }
+        } else if constexpr (SCHEME == 0 && WC) {
+#pragma clang fp contract(off)
+            // MT (TY == 2) and CEST split: BlochKernelMT.cuh:322-414,
+            // BlochKernelCEST.cuh:361-610
+            if ((flags & REC_NEWDT) || !have_e) {
+                have_e = true;
+                const float k12 = kmat[0][1], k21 = kmat[1][0];
+                e2[0] = expf_cr(-dt / t2[0]);
+                e1[0] = expf_cr(-dt / t1[0] - dt * k12);  // water: one combined exponent
+                m0[0] = (rho[0] * (1.f - expf_cr(-dt / t1[0]))) / (float)p.S;
+                kout[0] = 1.f - expf_cr(-dt * k21);       // water gain from the bound pool
+                e1[1] = expf_cr(-dt / t1[1] - dt * k21);
+                m0[1] = (rho[1] * (1.f - expf_cr(-dt / t1[1]))) / (float)p.S;
+                kout[1] = 1.f - expf_cr(-dt * k12);       // bound gain from water
+#pragma unroll
+                for (int t = 2; t < TY; t++) {
+                    e2[t] = expf_cr(-dt / t2[t]);
+                    e1[t] = expf_cr(-dt / t1[t]);
+                    m0[t] = (rho[t] * (1.f - e1[t])) / (float)p.S;
+                    fin[0][t] = kmat[0][t] != 0.f ? expf_cr(-dt * kmat[0][t]) : 1.f;  // water -> t survival
+                    fin[t][0] = kmat[t][0] != 0.f ? expf_cr(-dt * kmat[t][0]) : 1.f;  // t -> water survival
+                }
+            }
+            for (int k = 0; k < nrep && !skip[0]; k++) {
+                float Bx[TY], By[TY], Bz[TY], ox[TY], oy[TY], oz[TY];
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    Bx[t] = ox[t] = st.mx[t];
+                    By[t] = oy[t] = st.my[t];
+                    Bz[t] = oz[t] = st.mz[t];
+                }
+#pragma unroll
+                for (int t = 0; t < TY; t++)
+                    if (t != 1) rotate_free(t, ox[t], oy[t], oz[t]);
+                // water
+                float ko = 1.f, kx = 0.f, ky = 0.f, kz = 0.f;
+#pragma unroll
+                for (int t = 2; t < TY; t++) {
+                    if (kmat[0][t] != 0.f) ko = ko * fin[0][t];
+                    if (kmat[t][0] != 0.f) {
+                        const float f = 1.f - fin[t][0];
+                        kx = kx + f * Bx[t];
+                        ky = ky + f * By[t];
+                        kz = kz + f * Bz[t];
+                    }
+                }
+                if constexpr (TY == 2) {
+                    st.mx[0] = ox[0] * e2[0];
+                    st.my[0] = oy[0] * e2[0];
+                    st.mz[0] = oz[0] * e1[0] + m0[0] + Bz[1] * kout[0];
+                } else {
+                    st.mx[0] = ox[0] * e2[0] * ko + kx;
+                    st.my[0] = oy[0] * e2[0] * ko + ky;
+                    st.mz[0] = oz[0] * e1[0] * ko + kz + Bz[1] * kout[0] + m0[0];
+                }
+                // CEST pools
+#pragma unroll
+                for (int t = 2; t < TY; t++) {
+                    const float bxc = ox[t] * e2[t], byc = oy[t] * e2[t], bzc = oz[t] * e1[t];
+                    const float kt = kmat[t][0] != 0.f ? fin[t][0] : 1.f;
+                    float ix = 0.f, iy = 0.f, iz = 0.f;
+                    if (kmat[0][t] != 0.f) {
+                        const float f = 1.f - fin[0][t];
+                        ix = f * Bx[0];
+                        iy = f * By[0];
+                        iz = f * Bz[0];
+                    }
+                    st.mx[t] = bxc * kt + ix;
+                    st.my[t] = byc * kt + iy;
+                    st.mz[t] = bzc * kt + iz + m0[t];
+                }
+                // bound pool
+                float bzb = Bz[1];
+                if (rf) bzb = Bz[1] * expf_cr(-wsat[1] * dt);
+                st.mz[1] = bzb * e1[1] + m0[1] + Bz[0] * kout[1];
+            }
         } else {
             // exact step: x <- exp(A*dt) x with the affine M0 term
-            double om[TY], r1d[TY], r2d[TY], kod[TY], aff[TY];
+            double om[TY], r1d[TY], r2d[TY], kod[TY], kodx[TY], aff[TY];
             double nrm = 0.0;
             const double bxd = bx, byd = by;
 #pragma unroll
@@ -306,15 +406,19 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                 r1d[t] = 0;
                 r2d[t] = 0;
                 kod[t] = 0;
+                kodx[t] = 0;
                 aff[t] = 0;
                 if (skip[t]) continue;
-                double ko = 0.0, kin = 0.0;
+                double ko = 0.0, kox = 0.0, kin = 0.0;
 #pragma unroll
                 for (int qq = 0; qq < TY; qq++) {
+                    if (qq == t) continue;  // self-exchange cancels exactly
                     ko += kmat[t][qq];
+                    if (!(wcv && p.flag[qq] == 1)) kox += kmat[t][qq];  // MT/CEST: no transverse loss to bound
                     kin += kmat[qq][t];
                 }
                 kod[t] = ko;
+                kodx[t] = kox;
                 r1d[t] = 1.0 / (double)t1[t];
                 r2d[t] = 1.0 / (double)t2[t];
                 aff[t] = r1d[t] * (double)rho[t] / p.S;
@@ -339,16 +443,18 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                     for (int qq = 0; qq < TY; qq++) {
                         if (qq == t) continue;
                         const double kk = kmat[qq][t];
-                        sx += kk * in[3 * qq];
-                        sy += kk * in[3 * qq + 1];
+                        if (!(wcv && p.flag[qq] == 1)) {
+                            sx += kk * in[3 * qq];
+                            sy += kk * in[3 * qq + 1];
+                        }
                         sz += kk * in[3 * qq + 2];
                     }
                     const double ix = in[3 * t], iy = in[3 * t + 1], iz = in[3 * t + 2];
                     if (skip[t]) {
                         out[3 * t] = out[3 * t + 1] = out[3 * t + 2] = 0.0;
                     } else if (p.flag[t] == 0) {
-                        out[3 * t] = om[t] * iy - byd * iz - (r2d[t] + kod[t]) * ix + sx;
-                        out[3 * t + 1] = bxd * iz - om[t] * ix - (r2d[t] + kod[t]) * iy + sy;
+                        out[3 * t] = om[t] * iy - byd * iz - (r2d[t] + kodx[t]) * ix + sx;
+                        out[3 * t + 1] = bxd * iz - om[t] * ix - (r2d[t] + kodx[t]) * iy + sy;
                         out[3 * t + 2] = byd * ix - bxd * iy - (r1d[t] + kod[t]) * iz + sz;
                     } else {
                         out[3 * t] = 0.0;
@@ -407,33 +513,37 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
     }
 }
 
-template <int TY, int NRX, int SCHEME>
+template <int TY, int NRX, int SCHEME, bool WC>
 static void launch_t(dim3 g, dim3 b, size_t lds, hipStream_t st, const GmParams &p, const uint32_t *rec, int r0,
                      int r1) {
-    bloch_gm_kernel<TY, NRX, SCHEME><<<g, b, lds, st>>>(p, rec, r0, r1);
+    bloch_gm_kernel<TY, NRX, SCHEME, WC><<<g, b, lds, st>>>(p, rec, r0, r1);
 }
 
 template <int TY, int NRX>
-static gm_launch_fn pick_scheme(int scheme) {
-    return scheme ? launch_t<TY, NRX, 1> : launch_t<TY, NRX, 0>;
+static gm_launch_fn pick_scheme(int scheme, int variant) {
+    if (scheme) return launch_t<TY, NRX, 1, false>;
+    if (variant == kVarMT || variant == kVarCEST) return launch_t<TY, NRX, 0, true>;
+    return launch_t<TY, NRX, 0, false>;
 }
 
 template <int TY>
-static gm_launch_fn pick_nrx(int nrx, int scheme) {
+static gm_launch_fn pick_nrx(int nrx, int scheme, int variant) {
     switch (nrx) {
-        case 1: return pick_scheme<TY, 1>(scheme);
-        case 2: return pick_scheme<TY, 2>(scheme);
-        case 4: return pick_scheme<TY, 4>(scheme);
-        case 8: return pick_scheme<TY, 8>(scheme);
+        case 1: return pick_scheme<TY, 1>(scheme, variant);
+        case 2: return pick_scheme<TY, 2>(scheme, variant);
+        case 4: return pick_scheme<TY, 4>(scheme, variant);
+        case 8: return pick_scheme<TY, 8>(scheme, variant);
     }
     return nullptr;
 }
 
-gm_launch_fn pick_gm(int ty, int nrx, int scheme) {
+gm_launch_fn pick_gm(int ty, int nrx, int scheme, int variant) {
+    if (variant == kVarMT && ty != 2) return nullptr;
+    if (variant == kVarCEST && ty < 3) return nullptr;
     switch (ty) {
-        case 2: return pick_nrx<2>(nrx, scheme);
-        case 3: return pick_nrx<3>(nrx, scheme);
-        case 4: return pick_nrx<4>(nrx, scheme);
+        case 2: return pick_nrx<2>(nrx, scheme, variant);
+        case 3: return pick_nrx<3>(nrx, scheme, variant);
+        case 4: return pick_nrx<4>(nrx, scheme, variant);
     }
     return nullptr;
 }

@@ -39,7 +39,8 @@ STATUS = {0: "MRS_OK", 1: "MRS_ERR_ARG", 2: "MRS_ERR_DEVICE", 3: "MRS_ERR_CALLBA
 
 # every symbol declared in include/mrilab_amd.h
 EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanAtCPU",
-           "mrs_DoScanAtGPU", "mrs_DoGMScanAtGPU", "mrs_plan_create", "mrs_plan_run", "mrs_plan_sync",
+           "mrs_DoScanAtGPU", "mrs_DoGMScanAtGPU", "mrs_DoMTScanAtGPU", "mrs_DoCESTScanAtGPU",
+           "mrs_DoMEScanAtGPU", "mrs_plan_create", "mrs_plan_run", "mrs_plan_sync",
            "mrs_plan_fetch", "mrs_plan_get_info", "mrs_plan_destroy", "mrs_batch_create", "mrs_batch_run",
            "mrs_batch_sync", "mrs_batch_fetch", "mrs_batch_get_info", "mrs_batch_destroy",
            "mrs_DoScanBatchAtGPU", "mrs_adjoint_create", "mrs_adjoint_forward", "mrs_adjoint_backward",
@@ -60,11 +61,20 @@ def load() -> C.CDLL:
             return _lib
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m mrilab_amd.build`")
+        # PyTorch for ROCm ships its own libamdhip64 with the same SONAME as
+        # ROCm's. Loaded first, it also serves this library (one HIP runtime
+        # per process, so device pointers pass between the two); loaded after
+        # ours, it would bring a second runtime that finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         P = C.POINTER
         scan_args = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC), P(abi.VSeqC),
                      P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC), P(abi.OptionsC)]
-        for fn in ("mrs_DoScanAtCPU", "mrs_DoScanAtGPU", "mrs_DoGMScanAtGPU"):
+        for fn in ("mrs_DoScanAtCPU", "mrs_DoScanAtGPU", "mrs_DoGMScanAtGPU", "mrs_DoMTScanAtGPU",
+                   "mrs_DoCESTScanAtGPU", "mrs_DoMEScanAtGPU"):
             getattr(lib, fn).argtypes = scan_args
             getattr(lib, fn).restype = C.c_int
         lib.mrs_last_error.restype = C.c_char_p
@@ -156,10 +166,19 @@ def _callbacks(ext: Optional[Callable], gpu_fetch: Optional[Callable]):
     return cb, (f_ext, f_fetch)
 
 
+# multi-pool engines: options.model and the planes of VObj.K in each layout
+MODELS = {"GM": 1, "MT": 2, "CEST": 3, "ME": 4}
+
+
+def k_planes(model: int, ty: int) -> int:
+    return 2 * (ty - 1) if model == MODELS["CEST"] else ty * ty
+
+
 def _scan(fn: str, VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=None,
-          options: Optional[abi.OptionsC] = None, multipool: bool = False) -> None:
+          options: Optional[abi.OptionsC] = None, multipool: bool = False, model: int = 1) -> None:
     lib = load()
-    P = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool=multipool)
+    kp = k_planes(model, int(VObj["TypeNum"])) if multipool else None
+    P = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool=multipool, k_planes=kp)
     cb, keep = _callbacks(ext, gpu_fetch)
     rc = getattr(lib, fn)(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi), C.byref(P.vctl),
                           C.byref(P.vseq), C.byref(P.vvar), C.byref(P.vsig),
@@ -182,6 +201,26 @@ def DoGMScanAtGPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=
           multipool=True)
 
 
+def DoMTScanAtGPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=None, options=None):
+    """Two-pool MT scan (DoMTScanAtGPU.cu contract): VObj.K [R,C,Z,4] with
+    K[...,1] water -> bound and K[...,2] bound -> water (0-based planes)."""
+    _scan("mrs_DoMTScanAtGPU", VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext, gpu_fetch, options,
+          multipool=True, model=MODELS["MT"])
+
+
+def DoCESTScanAtGPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=None, options=None):
+    """CEST scan (DoCESTScanAtGPU.cu contract): VObj.K [R,C,Z,2*(Ty-1)]."""
+    _scan("mrs_DoCESTScanAtGPU", VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext, gpu_fetch, options,
+          multipool=True, model=MODELS["CEST"])
+
+
+def DoMEScanAtGPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=None, options=None):
+    """Multi-exchange scan (DoMEScanAtGPU.cu contract): VObj.K [R,C,Z,Ty^2],
+    plane a*Ty + b the rate from pool a into pool b."""
+    _scan("mrs_DoMEScanAtGPU", VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext, gpu_fetch, options,
+          multipool=True, model=MODELS["ME"])
+
+
 def DoScanAtCPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch=None, options=None):
     """The DoScanAtCPU entry point (DoScanAtCPU.cpp:70 contract), executed on the GPU."""
     _scan("mrs_DoScanAtCPU", VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext, gpu_fetch, options)
@@ -192,8 +231,9 @@ class Plan:
 
     def __init__(self, VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, options=None):
         self._lib = load()
-        mp = options is not None and options.model == 1
-        self._packed = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool=mp)
+        mp = options is not None and options.model >= 1
+        kp = k_planes(options.model, int(VObj["TypeNum"])) if mp else None
+        self._packed = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool=mp, k_planes=kp)
         self._h = C.c_void_p()
         P = self._packed
         _check(self._lib.mrs_plan_create(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi),

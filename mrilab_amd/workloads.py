@@ -583,3 +583,59 @@ def excitation(shape=(128, 128), spins=32, n_steps=2000, dt=4e-6, peak_b1=15e-6,
     return Excitation(sc.VObj, sc.VMag, sc.VCoi, f32(grad), f32(target), f32(rf),
                       f32(T1.ravel(order="F")), f32(T2.ravel(order="F")), f32(dB0.ravel(order="F")), float(dt),
                       name=f"excitation_{R}x{C_}_S{spins}_n{n_steps}", aux=(sc.VCtl, sc.VSeq, sc.VVar, sc.VSig))
+
+
+def select_pools(sc: Scan, idx, cs_hz=None, t1_scale=None) -> Scan:
+    """Copy of a multi-pool scan keeping (or repeating) pools ``idx``; K
+    (GM layout, K[..., a, b] = rate a -> b), maps, TypeFlag, CS and the
+    signal arrays follow. ``cs_hz`` replaces VCtl.CS, ``t1_scale``
+    multiplies T1 per kept pool."""
+    idx = list(idx)
+    out = sc.copy()
+    o = out.VObj
+    for k in ("Mx", "My", "Mz", "Rho", "T1", "T2"):
+        o[k] = np.asfortranarray(o[k][..., idx])
+    if t1_scale is not None:
+        o["T1"] = np.asfortranarray((o["T1"] * np.asarray(t1_scale, np.float32)).astype(np.float32))
+    o["K"] = np.asfortranarray(o["K"][..., idx, :][..., idx])
+    o["TypeFlag"] = np.asarray(o["TypeFlag"])[idx].astype(np.float64)
+    o["TypeNum"] = np.int32(len(idx))
+    out.VMag["dWRnd"] = np.asfortranarray(out.VMag["dWRnd"][..., idx])
+    out.VCtl["CS"] = (np.asarray(out.VCtl["CS"])[idx] if cs_hz is None else np.asarray(cs_hz)).astype(np.float64)
+    n = int(out.VSig["SignalNum"]) * len(idx) * int(out.VCoi["RxCoilNum"])
+    out.VSig["Sx"], out.VSig["Sy"] = np.zeros(n), np.zeros(n)
+    return out
+
+
+def as_variant(sc: Scan, variant: str, self_rate: float = 0.0) -> Scan:
+    """VObj.K of a GM-layout multi-pool scan rewritten in the layout of the
+    MT, CEST or ME engine (BlochKernelMT.cuh:151-154,
+    BlochKernelCEST.cuh:157-160/:379-384, BlochKernelME.cuh:327-333).
+    MT keeps water <-> pool 1; CEST keeps water <-> every pool; ME keeps all
+    rates and adds ``self_rate`` on the diagonal."""
+    if variant == "GM":
+        return sc
+    out = sc.copy()
+    K = np.asarray(sc.VObj["K"], dtype=np.float64)
+    shp, Ty = K.shape[:3], K.shape[-1]
+    if variant == "ME":
+        Kn = np.zeros(shp + (Ty * Ty,))
+        for a in range(Ty):
+            for b in range(Ty):
+                Kn[..., a * Ty + b] = K[..., a, b] + (self_rate if a == b else 0.0)
+    elif variant == "MT":
+        if Ty != 2:
+            raise ValueError("the MT layout has two pools")
+        Kn = np.zeros(shp + (4,))
+        Kn[..., 1], Kn[..., 2] = K[..., 0, 1], K[..., 1, 0]
+    elif variant == "CEST":
+        if Ty < 3:
+            raise ValueError("the CEST layout has at least three pools")
+        Kn = np.zeros(shp + (2 * (Ty - 1),))
+        Kn[..., 0], Kn[..., Ty - 1] = K[..., 0, 1], K[..., 1, 0]
+        for t in range(2, Ty):
+            Kn[..., t - 1], Kn[..., t + Ty - 2] = K[..., 0, t], K[..., t, 0]
+    else:
+        raise ValueError(variant)
+    out.VObj["K"] = np.asfortranarray(Kn.astype(np.float32))
+    return out

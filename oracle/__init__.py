@@ -40,7 +40,8 @@ def _lib(precision: str) -> C.CDLL:
                                         P(abi.VSeqC), P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC)]
         lib.orc_DoScanAtCPU.restype = C.c_int
         lib.orc_DoGMScan.argtypes = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC),
-                                     P(abi.VSeqC), P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC), C.c_int]
+                                     P(abi.VSeqC), P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC), C.c_int,
+                                     C.c_int]
         lib.orc_DoGMScan.restype = C.c_int
         lib.orc_superlorentzian.argtypes = [C.c_double, C.c_double]
         lib.orc_superlorentzian.restype = C.c_double
@@ -74,14 +75,27 @@ def scan(sc, precision: str = "f64", callbacks=None, inplace: bool = False):
     return out
 
 
-def gm_scan(sc, precision: str = "f64", scheme: int = 0, inplace: bool = False):
-    """Multi-pool scan (MRiLab's GM model): scheme 0 = operator split as
-    BlochKernelGM.cuh, 1 = fp64 matrix exponential of the same generator."""
+VARIANTS = {"GM": 0, "MT": 1, "CEST": 2, "ME": 3}
+
+
+def k_planes(variant: str, ty: int) -> int:
+    """Planes of VObj.K for each multi-pool layout (CEST: 2*(Ty-1))."""
+    return 2 * (ty - 1) if variant == "CEST" else ty * ty
+
+
+def gm_scan(sc, precision: str = "f64", scheme: int = 0, inplace: bool = False, variant: str = "GM"):
+    """Multi-pool scan: variant "GM" (BlochKernelGM.cuh), "MT"
+    (BlochKernelMT.cuh), "CEST" (BlochKernelCEST.cuh) or "ME"
+    (BlochKernelME.cuh) with VObj.K in that kernel's layout; scheme 0 =
+    the kernel's operator split, 1 = fp64 matrix exponential of the same
+    generator."""
     out = sc if inplace else sc.copy()
     lib = _lib(precision)
-    P = abi.pack_scan(*out.structs, multipool=True)
+    ty = int(out.VObj["TypeNum"])
+    P = abi.pack_scan(*out.structs, multipool=True, k_planes=k_planes(variant, ty))
     rc = lib.orc_DoGMScan(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi), C.byref(P.vctl),
-                          C.byref(P.vseq), C.byref(P.vvar), C.byref(P.vsig), None, int(scheme))
+                          C.byref(P.vseq), C.byref(P.vvar), C.byref(P.vsig), None, int(scheme),
+                          VARIANTS[variant])
     if rc != 0:
         raise RuntimeError(f"oracle GM scan failed with status {rc}")
     abi.unpack_vvar(P, out.VVar)

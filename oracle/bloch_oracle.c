@@ -239,7 +239,10 @@ typedef struct orc_ctx {
     int64_t nxy;    /* R*C */
     int64_t nz;
     real *wMx, *wMy, *wMz; /* fp64 build keeps the state in double */
+    int variant;    /* multi-pool model: ORC_GM, ORC_MT, ORC_CEST, ORC_ME */
 } orc_ctx;
+
+enum { ORC_GM = 0, ORC_MT = 1, ORC_CEST = 2, ORC_ME = 3 };
 
 /* ======================================================================
  * Generalized multi-pool model (MRiLab's GM engine,
@@ -365,6 +368,59 @@ static int orc_gm_skip(const orc_ctx *C, int64_t v, int t)
     return C->vobj->T2[tv] == 0 || C->vobj->T1[tv] == 0 || C->vobj->Rho[tv] == 0;  /* :176 */
 }
 
+/* Pools frozen at a step: GM and ME skip each pool with a zero T1, T2 or
+ * Rho (BlochKernelGM.cuh:176, BlochKernelME.cuh:173); MT and CEST skip the
+ * whole voxel when the water pool's are zero (BlochKernelMT.cuh:121,
+ * BlochKernelCEST.cuh:133). */
+static int orc_vskip(const orc_ctx *C, int64_t v, int t)
+{
+    if (C->variant == ORC_MT || C->variant == ORC_CEST) return orc_gm_skip(C, v, 0);
+    return orc_gm_skip(C, v, t);
+}
+
+/* 0 free pool, 1 bound pool: GM reads TypeFlag; MT and CEST have the bound
+ * pool at index 1; ME pools are all free. */
+static int orc_pool_flag(const orc_ctx *C, int t)
+{
+    switch (C->variant) {
+    case ORC_MT: case ORC_CEST: return t == 1;
+    case ORC_ME: return 0;
+    default: return (int)C->vobj->TypeFlag[t];
+    }
+}
+
+/* Exchange rate from pool a into pool b, read from K as each reference
+ * kernel reads it:
+ *   GM   plane b*Ty + a (BlochKernelGM.cuh:417 go away, :423 come in)
+ *   ME   plane a*Ty + b (BlochKernelME.cuh:327, :333), diagonal included
+ *   MT   plane 1 = water -> bound, plane 2 = bound -> water
+ *        (BlochKernelMT.cuh:151-154)
+ *   CEST plane 0 = water -> MT, plane Ty-1 = MT -> water, plane t-1 =
+ *        water -> CEST pool t, plane t+Ty-2 = CEST pool t -> water
+ *        (BlochKernelCEST.cuh:157-160, :379-384)  */
+static real orc_rate(const orc_ctx *C, int64_t v, int a, int b)
+{
+    const int Ty = C->vobj->TypeNum;
+    const int64_t nvox = C->nvox;
+    const float *K = C->vobj->K;
+    switch (C->variant) {
+    case ORC_ME: return K[v + nvox * ((int64_t)a * Ty + b)];
+    case ORC_MT:
+        if (a == 0 && b == 1) return K[v + nvox * 1];
+        if (a == 1 && b == 0) return K[v + nvox * 2];
+        return 0;
+    case ORC_CEST:
+        if (a == 0 && b == 1) return K[v];
+        if (a == 1 && b == 0) return K[v + nvox * (int64_t)(Ty - 1)];
+        if (a == 0 && b >= 2) return K[v + nvox * (int64_t)(b - 1)];
+        if (a >= 2 && b == 0) return K[v + nvox * (int64_t)(a + Ty - 2)];
+        return 0;
+    default:
+        if (a == b) return 0;
+        return K[v + nvox * ((int64_t)b * Ty + a)];
+    }
+}
+
 /* One operator-split step of all pools of one (voxel, spin):
  * BlochKernelGM.cuh:159-476. */
 static void orc_gm_split_voxel(orc_ctx *C, int64_t v, int s, real dt, real gz, real gy, real gx,
@@ -382,12 +438,12 @@ static void orc_gm_split_voxel(orc_ctx *C, int64_t v, int s, real dt, real gz, r
         Bz[t] = C->wMz ? C->wMz[si[t]] : (real)C->vobj->Mz[si[t]];
     }
     for (int t = 0; t < Ty; t++) {
-        if (orc_gm_skip(C, v, t)) continue;
+        if (orc_vskip(C, v, t)) continue;
         const int64_t tv = v + (int64_t)t * nvox;
         const real Rho = C->vobj->Rho[tv], T1 = C->vobj->T1[tv], T2 = C->vobj->T2[tv];
         real mx = Bx[t], my = By[t], mz = Bz[t];
         real bx = mx, by = my, bz = mz;
-        const int flag = (int)C->vobj->TypeFlag[t];
+        const int flag = orc_pool_flag(C, t);
         if (flag == 0) {
             real dW = orc_gm_dW(C, v, si[t], t, gz, gy, gx);
             real b1, b2;
@@ -420,13 +476,14 @@ static void orc_gm_split_voxel(orc_ctx *C, int64_t v, int s, real dt, real gz, r
             by = by * e2;
         }
         bz = bz * e1;
-        /* exchange with start-of-step values of the other pools (:408-433) */
+        /* exchange with start-of-step values of the other pools (:408-433);
+         * ME also applies a nonzero diagonal of K (BlochKernelME.cuh:323-338) */
         real kout = 1, kx = 0, ky = 0, kz = 0;
         for (int p = 0; p < Ty; p++) {
-            if (p == t) continue;
-            real ko = C->vobj->K[v + nvox * ((int64_t)p * Ty + t)];
+            if (p == t && C->variant != ORC_ME) continue;
+            real ko = orc_rate(C, v, t, p);
             if (ko != 0) kout = kout * r_exp(-dt * ko);
-            real ki = C->vobj->K[v + nvox * ((int64_t)t * Ty + p)];
+            real ki = orc_rate(C, v, p, t);
             if (ki != 0) {
                 real f = 1 - r_exp(-dt * ki);
                 if (flag == 0) { kx = kx + f * Bx[p]; ky = ky + f * By[p]; }
@@ -444,6 +501,119 @@ static void orc_gm_split_voxel(orc_ctx *C, int64_t v, int s, real dt, real gz, r
         } else {
             if (flag == 0) { C->vobj->Mx[si[t]] = (float)mx; C->vobj->My[si[t]] = (float)my; }
             C->vobj->Mz[si[t]] = (float)mz;
+        }
+    }
+}
+
+/* One split step of the MT and CEST kernels for one (voxel, spin): water
+ * (pool 0) and the bound pool exchange longitudinally through a combined
+ * exponent; CEST pools (t >= 2) exchange all components with water.
+ * BlochKernelMT.cuh:238-414, BlochKernelCEST.cuh:271-610. */
+static void orc_wc_split_voxel(orc_ctx *C, int64_t v, int s, real dt, real gz, real gy, real gx,
+                               const double *rfAmp, const double *rfPhase, const double *rfFreq,
+                               gm_cache *gcs)
+{
+    const int Ty = C->vobj->TypeNum, S = C->vobj->SpinNum;
+    const int64_t nvox = C->nvox;
+    if (orc_gm_skip(C, v, 0)) return;
+    real Bx[GM_MAXT], By[GM_MAXT], Bz[GM_MAXT];
+    int64_t si[GM_MAXT];
+    for (int t = 0; t < Ty; t++) {
+        si[t] = v + nvox * ((int64_t)s + (int64_t)S * t);
+        Bx[t] = C->wMx ? C->wMx[si[t]] : (real)C->vobj->Mx[si[t]];
+        By[t] = C->wMy ? C->wMy[si[t]] : (real)C->vobj->My[si[t]];
+        Bz[t] = C->wMz ? C->wMz[si[t]] : (real)C->vobj->Mz[si[t]];
+    }
+    real ox[GM_MAXT], oy[GM_MAXT], oz[GM_MAXT];
+    const real k12 = orc_rate(C, v, 0, 1), k21 = orc_rate(C, v, 1, 0);
+    /* free pools: rotation */
+    for (int t = 0; t < Ty; t++) {
+        if (t == 1) continue;
+        real mx = Bx[t], my = By[t], mz = Bz[t], bx = mx, by = my, bz = mz;
+        real dW = orc_gm_dW(C, v, si[t], t, gz, gy, gx);
+        real b1, b2;
+        if (orc_rf_field(C, v, rfAmp, rfPhase, rfFreq, &b1, &b2, &dW)) {
+            orc_rf_rotate(dW, b1, b2, dt, mx, my, mz, &bx, &by, &bz);
+        } else {
+            real alpha = dW * dt, sA = r_sin(alpha), cA = r_cos(alpha);
+            bx = mx * cA + my * sA;
+            by = my * cA - mx * sA;
+            bz = mz;
+        }
+        ox[t] = bx; oy[t] = by; oz[t] = bz;
+    }
+    /* water */
+    {
+        const int64_t tv = v;
+        const real Rho = C->vobj->Rho[tv], T1 = C->vobj->T1[tv], T2 = C->vobj->T2[tv];
+        const real e2 = r_exp(-dt / T2);
+        const real e1k = r_exp(-dt / T1 - dt * k12);
+        const real m0 = (Rho * (1 - r_exp(-dt / T1))) / (real)S;
+        const real ek = 1 - r_exp(-dt * k21);
+        real kout = 1, kx = 0, ky = 0, kz = 0;
+        for (int t = 2; t < Ty; t++) {
+            const real ko = orc_rate(C, v, 0, t);
+            if (ko != 0) kout = kout * r_exp(-dt * ko);
+            const real ki = orc_rate(C, v, t, 0);
+            if (ki != 0) {
+                const real f = 1 - r_exp(-dt * ki);
+                kx = kx + f * Bx[t]; ky = ky + f * By[t]; kz = kz + f * Bz[t];
+            }
+        }
+        real mx, my, mz;
+        if (C->variant == ORC_MT) {
+            mx = ox[0] * e2;
+            my = oy[0] * e2;
+            mz = oz[0] * e1k + m0 + Bz[1] * ek;
+        } else {
+            mx = ox[0] * e2 * kout + kx;
+            my = oy[0] * e2 * kout + ky;
+            mz = oz[0] * e1k * kout + kz + Bz[1] * ek + m0;
+        }
+        ox[0] = mx; oy[0] = my; oz[0] = mz;
+    }
+    /* CEST pools */
+    for (int t = 2; t < Ty; t++) {
+        const int64_t tv = v + (int64_t)t * nvox;
+        const real Rho = C->vobj->Rho[tv], T1 = C->vobj->T1[tv], T2 = C->vobj->T2[tv];
+        const real e2 = r_exp(-dt / T2), e1 = r_exp(-dt / T1);
+        const real m0 = (Rho * (1 - e1)) / (real)S;
+        real bx = ox[t] * e2, by = oy[t] * e2, bz = oz[t] * e1;
+        real kout = 1, kx = 0, ky = 0, kz = 0;
+        const real ko = orc_rate(C, v, t, 0);
+        if (ko != 0) kout = r_exp(-dt * ko);
+        const real ki = orc_rate(C, v, 0, t);
+        if (ki != 0) {
+            const real f = 1 - r_exp(-dt * ki);
+            kx = f * Bx[0]; ky = f * By[0]; kz = f * Bz[0];
+        }
+        ox[t] = bx * kout + kx;
+        oy[t] = by * kout + ky;
+        oz[t] = bz * kout + kz + m0;
+    }
+    /* bound pool */
+    {
+        const int64_t tv = v + nvox;
+        const real Rho = C->vobj->Rho[tv], T1 = C->vobj->T1[tv], T2 = C->vobj->T2[tv];
+        real rfAmpSum = 0;
+        for (int i = 0; i < C->vcoi->TxCoilNum; i++) rfAmpSum += (real)fabs((double)(real)rfAmp[i]);
+        real bz = Bz[1];
+        if (rfAmpSum != 0) {
+            real W = (real)orc_gm_W(C, v, 1, T2, rfAmp, rfFreq, &gcs[s * Ty + 1]);
+            bz = Bz[1] * r_exp(-W * dt);
+        }
+        const real e1k = r_exp(-dt / T1 - dt * k21);
+        const real m0 = (Rho * (1 - r_exp(-dt / T1))) / (real)S;
+        const real ek = 1 - r_exp(-dt * k12);
+        oz[1] = bz * e1k + m0 + Bz[0] * ek;
+    }
+    for (int t = 0; t < Ty; t++) {
+        if (C->wMx) {
+            if (t != 1) { C->wMx[si[t]] = ox[t]; C->wMy[si[t]] = oy[t]; }
+            C->wMz[si[t]] = oz[t];
+        } else {
+            if (t != 1) { C->vobj->Mx[si[t]] = (float)ox[t]; C->vobj->My[si[t]] = (float)oy[t]; }
+            C->vobj->Mz[si[t]] = (float)oz[t];
         }
     }
 }
@@ -517,14 +687,20 @@ static void orc_gm_expm_voxel(orc_ctx *C, int64_t v, int s, double dt, double gz
             wrf += 2 * ORC_PI * rfFreq[i];
         }
     }
+    /* MT and CEST: water and the bound pool exchange longitudinally only */
+    const int wc = C->variant == ORC_MT || C->variant == ORC_CEST;
     for (int t = 0; t < Ty; t++) {
-        if (orc_gm_skip(C, v, t)) continue;
+        if (orc_vskip(C, v, t)) continue;
         const int64_t tv = v + (int64_t)t * nvox;
         const double Rho = C->vobj->Rho[tv], R1 = 1.0 / C->vobj->T1[tv], R2 = 1.0 / C->vobj->T2[tv];
-        const int flag = (int)C->vobj->TypeFlag[t];
-        double kout = 0;
-        for (int p = 0; p < Ty; p++)
-            if (p != t) kout += C->vobj->K[v + nvox * ((int64_t)p * Ty + t)];
+        const int flag = orc_pool_flag(C, t);
+        double kout = 0, koutx = 0;
+        for (int p = 0; p < Ty; p++) {
+            if (p == t) continue;
+            const double r = orc_rate(C, v, t, p);
+            kout += r;
+            if (!(wc && orc_pool_flag(C, p) == 1)) koutx += r;
+        }
         const int z = 3 * t + 2;
         if (flag == 0) {
             const int xi = 3 * t, yi = 3 * t + 1;
@@ -536,11 +712,11 @@ static void orc_gm_expm_voxel(orc_ctx *C, int64_t v, int s, double dt, double gz
             A[xi * n + yi] += w;   A[xi * n + z] -= byr;
             A[yi * n + z] += bxr;  A[yi * n + xi] -= w;
             A[z * n + xi] += byr;  A[z * n + yi] -= bxr;
-            A[xi * n + xi] -= R2 + kout;
-            A[yi * n + yi] -= R2 + kout;
+            A[xi * n + xi] -= R2 + koutx;
+            A[yi * n + yi] -= R2 + koutx;
             for (int p = 0; p < Ty; p++) {
-                if (p == t) continue;
-                double ki = C->vobj->K[v + nvox * ((int64_t)t * Ty + p)];
+                if (p == t || (wc && orc_pool_flag(C, p) == 1)) continue;
+                double ki = orc_rate(C, v, p, t);
                 A[xi * n + 3 * p] += ki;
                 A[yi * n + 3 * p + 1] += ki;
             }
@@ -555,7 +731,7 @@ static void orc_gm_expm_voxel(orc_ctx *C, int64_t v, int s, double dt, double gz
         A[z * n + (n - 1)] += R1 * Rho / S;
         for (int p = 0; p < Ty; p++) {
             if (p == t) continue;
-            A[z * n + 3 * p + 2] += C->vobj->K[v + nvox * ((int64_t)t * Ty + p)];
+            A[z * n + 3 * p + 2] += orc_rate(C, v, p, t);
         }
     }
     orc_expm_action(n, A, dt, x);
@@ -621,10 +797,11 @@ static void orc_capture(orc_ctx *c)
  */
 static int orc_scan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                     const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar,
-                    mrs_vsig *vsig, const mrs_callbacks *cb, int model)
+                    mrs_vsig *vsig, const mrs_callbacks *cb, int model, int variant)
 {
     orc_ctx C;
     memset(&C, 0, sizeof C);
+    C.variant = variant;
     C.vobj = vobj; C.vmag = vmag; C.vcoi = vcoi; C.vctl = vctl; C.vseq = vseq;
     C.vvar = vvar; C.vsig = vsig; C.cb = cb;
     const int64_t SpinMxNum = (int64_t)vobj->dims[0] * vobj->dims[1];
@@ -734,8 +911,12 @@ static int orc_scan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
         for (int Typei = 0; Typei < Ty; Typei++) {
             for (int Spini = 0; Spini < S; Spini++) {
                 const int64_t base = (int64_t)Typei * S * C.nvox + (int64_t)Spini * C.nvox;
-                /* the GM engine samples free pools only (BlochKernelGM.cuh:192) */
-                const int sample = vvar->ADC == 1 && (model == 0 || vobj->TypeFlag[Typei] == 0);
+                /* GM samples free pools only (BlochKernelGM.cuh:192), ME every pool
+                 * (BlochKernelME.cuh:186-220), MT and CEST the water pool
+                 * (BlochKernelMT.cuh:230, BlochKernelCEST.cuh:255) */
+                const int sample = vvar->ADC == 1 &&
+                    (model == 0 || (variant == ORC_MT || variant == ORC_CEST ? Typei == 0
+                                                                            : orc_pool_flag(&C, Typei) == 0));
                 if (sample) {                                        /* :445-491 */
                     for (int c = 0; c < Rx; c++) {
                         const float *Cx = vcoi->RxCoilx + (int64_t)c * C.nvox, *Cy = vcoi->RxCoily + (int64_t)c * C.nvox;
@@ -758,7 +939,7 @@ static int orc_scan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                             } else {
                                 b3 = b4;
                             }
-                            if (model && orc_gm_skip(&C, v, Typei)) { b1 = 0; b3 = 0; }
+                            if (model && orc_vskip(&C, v, Typei)) { b1 = 0; b3 = 0; }
                             buf1[v] = b1; buf3[v] = b3;
                         }
                         vsig->Sx[(int64_t)Typei * Rx * SignalNum + (int64_t)c * SignalNum + Signali] += orc_sum(buf1, C.nvox);
@@ -796,7 +977,9 @@ static int orc_scan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
             for (v = 0; v < C.nvox; v++) {
                 for (int sp = 0; sp < S; sp++) {
                     gm_cache *gc = gcs + (size_t)v * S * Ty;
-                    if (model == 1)
+                    if (model == 1 && (variant == ORC_MT || variant == ORC_CEST))
+                        orc_wc_split_voxel(&C, v, sp, rdt, gz, gy, gx, vvar->rfAmp, vvar->rfPhase, vvar->rfFreq, gc);
+                    else if (model == 1)
                         orc_gm_split_voxel(&C, v, sp, rdt, gz, gy, gx, vvar->rfAmp, vvar->rfPhase, vvar->rfFreq, gc);
                     else
                         orc_gm_expm_voxel(&C, v, sp, vvar->dt, vvar->GzAmp, vvar->GyAmp, vvar->GxAmp,
@@ -834,15 +1017,20 @@ int orc_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                     const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar,
                     mrs_vsig *vsig, const mrs_callbacks *cb)
 {
-    return orc_scan(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, 0);
+    return orc_scan(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, 0, 0);
 }
 
-/* The multi-pool scan of DoGMScanAtGPU.cu (same decoder): scheme 0 is the
- * GM kernel's operator split, scheme 1 the fp64 matrix exponential. */
+/* The multi-pool scans (same decoder): variant ORC_GM (DoGMScanAtGPU.cu),
+ * ORC_MT (DoMTScanAtGPU.cu), ORC_CEST (DoCESTScanAtGPU.cu) or ORC_ME
+ * (DoMEScanAtGPU.cu); scheme 0 is the reference kernel's operator split,
+ * scheme 1 the fp64 matrix exponential of the same generator. */
 int orc_DoGMScan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                  const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar,
-                 mrs_vsig *vsig, const mrs_callbacks *cb, int scheme)
+                 mrs_vsig *vsig, const mrs_callbacks *cb, int scheme, int variant)
 {
-    if (!vobj->K || !vobj->TypeFlag || vobj->TypeNum > GM_MAXT) return MRS_ERR_ARG;
-    return orc_scan(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, scheme ? 2 : 1);
+    if (!vobj->K || vobj->TypeNum > GM_MAXT || vobj->TypeNum < 2) return MRS_ERR_ARG;
+    if (variant == ORC_GM && !vobj->TypeFlag) return MRS_ERR_ARG;
+    if (variant == ORC_MT && vobj->TypeNum != 2) return MRS_ERR_ARG;
+    if (variant == ORC_CEST && vobj->TypeNum < 3) return MRS_ERR_ARG;
+    return orc_scan(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, scheme ? 2 : 1, variant);
 }

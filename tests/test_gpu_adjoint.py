@@ -1,7 +1,8 @@
 """GPU adjoint (mrs_adjoint_*) against the fp64 oracle gradient.
 
-The forward loss must match the fp32 oracle (same rounding of the
-relaxation, 1.5-ulp sincos) within 1e-5 and the fp64 one within 1e-4;
+The forward loss must match the fp32 oracle within 1e-5 (the kernels carry
+Mz - m0 instead of Mz and use a 1.5-ulp sincos, so the rounding is not the
+reference's) and the fp64 one within 1e-4;
 every gradient (rf, T1, T2, dB0) within 1e-3 rel-L2 of the fp64 reverse
 mode, which tests/test_oracle_adjoint.py pins to central differences.
 Checkpoint intervals only change what is recomputed, never the result.

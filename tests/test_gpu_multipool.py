@@ -112,3 +112,39 @@ def test_gm_z_spectrum_shape(gpu):
     s = np.abs(sig(g)).reshape(3, 5)
     assert np.all(s[2] == 0)
     assert np.argmin(s[0]) == 2 and s[0, 2] < 0.5 * min(s[0, 1], s[0, 3])
+
+
+def variant_case(name):
+    base = dict(n_offsets=3, sat_time=0.08)
+    if name == "MT":
+        sc = workloads.select_pools(workloads.mt_cest(shape=(6, 5, 1), **base), [0, 2])
+    elif name == "MT_s2_c3":
+        sc = workloads.select_pools(workloads.mt_cest(shape=(4, 4, 1), spins=2, n_coils=3, **base), [0, 2])
+    elif name == "CEST":
+        sc = workloads.select_pools(workloads.mt_cest(shape=(5, 4, 1), **base), [0, 2, 1], cs_hz=[0, 0, 447.0])
+    elif name == "CEST4_c2":
+        sc = workloads.select_pools(workloads.mt_cest(shape=(3, 3, 1), n_coils=2, **base), [0, 2, 1, 1],
+                                    cs_hz=[0, 0, 447.0, -380.0], t1_scale=[1, 1, 1, 1.5])
+    else:  # ME: every pool free, with a self-exchange rate on the diagonal
+        sc = workloads.select_pools(workloads.mt_cest(shape=(4, 4, 1), spins=2, **base), [0, 1, 2])
+    variant = name.split("_")[0].rstrip("4")
+    return workloads.as_variant(sc, variant, self_rate=20.0 if variant == "ME" else 0.0), variant
+
+
+ENTRY = {"MT": engine.DoMTScanAtGPU, "CEST": engine.DoCESTScanAtGPU, "ME": engine.DoMEScanAtGPU}
+
+
+@pytest.mark.parametrize("name", ["MT", "MT_s2_c3", "CEST", "CEST4_c2", "ME"])
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_variant_engines_match_oracle(gpu, name, scheme):
+    sc, variant = variant_case(name)
+    print(f"case {name} ({variant}) scheme {scheme}: {sc.name} pools={int(sc.VObj['TypeNum'])}", flush=True)
+    ref = oracle.gm_scan(sc, "f32" if scheme == 0 else "f64", scheme, variant=variant)
+    g = sc.copy()
+    ENTRY[variant](*g.structs, options=engine.make_options(multipool_scheme=scheme))
+    e_sig, e_mag = rel_l2(sig(g), sig(ref)), rel_l2(mag(g), mag(ref))
+    print(f"  vs oracle: signal {e_sig:.3e} magnetization {e_mag:.3e}", flush=True)
+    if scheme == 0:
+        assert e_sig < 1e-4 and e_mag < 1e-4
+    else:
+        assert e_sig < 1e-5 and e_mag < 1e-4
