@@ -134,27 +134,36 @@ def batched_section(n_sets: int, plain_sample: int, reps: int) -> dict:
 def adjoint_section(reps: int) -> dict:
     """Adjoint rf design: 128x128 slice x 32 isochromats, 2000 complex rf
     samples at 4 us under a spiral-in gradient, checkpoint every 64 steps;
-    device time of the forward pass and of the backward (gradient) pass."""
+    device time of the forward pass and of the backward (gradient) pass, the
+    latter with segments rebuilt backwards (default) and recomputed into
+    scratch memory."""
     import numpy as np
     from mrilab_amd import engine, workloads
     e = workloads.excitation()
-    adj = engine.Adjoint(e, checkpoint=64)
-    adj.forward(e.rf, e.T1, e.T2, e.dB0)
-    adj.backward()
-    fwd, bwd = [], []
-    for _ in range(reps):
+    out = {}
+    for mode in (0, 1):
+        adj = engine.Adjoint(e, checkpoint=64, segment_mode=mode)
         adj.forward(e.rf, e.T1, e.T2, e.dB0)
-        fwd.append(adj.info()["forward_ms"])
         adj.backward()
-        bwd.append(adj.info()["backward_ms"])
-    info = adj.info()
-    adj.close()
-    f, b = float(np.median(fwd)), float(np.median(bwd))
+        fwd, bwd = [], []
+        for _ in range(reps):
+            adj.forward(e.rf, e.T1, e.T2, e.dB0)
+            fwd.append(adj.info()["forward_ms"])
+            adj.backward()
+            bwd.append(adj.info()["backward_ms"])
+        info = adj.info()
+        adj.close()
+        out[mode] = (float(np.median(fwd)), float(np.median(bwd)), info)
+    f, b, info = out[0]
+    bs = out[1][1]
     iso_steps = info["n_iso"] * info["n_steps"]
-    log(f"adjoint: forward {f:.2f} ms, backward {b:.2f} ms, ratio {b / f:.2f}")
+    log(f"adjoint: forward {f:.2f} ms, backward {b:.2f} ms (rebuilt={info['segments_rebuilt']}), ratio {b / f:.2f}; "
+        f"backward through scratch {bs:.2f} ms")
     return {"workload": "rf design, 128x128 x 32 isochromats, 2000 complex rf samples at 4 us, spiral-in gradient, "
                         "L2 loss on the final Mxy, gradients w.r.t. rf, T1, T2, dB0, checkpoint K=64",
             "forward_ms": f, "backward_ms": b, "adjoint_over_forward": b / f,
+            "segments_rebuilt": info["segments_rebuilt"],
+            "backward_scratch_ms": bs, "adjoint_over_forward_scratch": bs / f,
             "forward_isochromat_steps_per_s": iso_steps / (f / 1e3), "checkpoints": info["n_checkpoints"],
             "spins_per_thread": info["spins_per_thread"]}
 

@@ -307,11 +307,20 @@ mrs_status mrs_DoScanBatchAtGPU(int32_t n_scans, mrs_vobj *const *vobj, const mr
  * Loss L = sum over voxels of |sum over isochromats (Mx + iMy) - target|^2.
  * mrs_adjoint_backward returns dL/drf, dL/dT1, dL/dT2 and dL/ddB0 by a
  * backward-in-time pass that keeps the forward state every `checkpoint`
- * steps and recomputes in between.
+ * steps. Between checkpoints it either rebuilds the states backwards from
+ * the checkpoint at the segment's end (inverse relaxation and rotation, no
+ * scratch memory) or recomputes the segment forwards into a scratch buffer
+ * of checkpoint * 5 * n_iso floats; rebuilding is used only where undoing a
+ * segment's relaxation amplifies rounding by at most 8 (K*dt <= 2.08*T2 and
+ * K*dt <= 2.08*T1 in every voxel).
  * ------------------------------------------------------------------------- */
 typedef struct mrs_excitation {
     int32_t n_steps;
     int32_t checkpoint;       /* K; 0 = 64 */
+    int32_t segment_mode;     /* 0 = rebuild segments backwards where that is
+                                 safe, else recompute them into scratch;
+                                 1 = always recompute into scratch */
+    int32_t reserved;
     double  dt;               /* s */
     const float *grad;        /* [n_steps][3] Gx, Gy, Gz (T/m), host */
     const float *target;      /* [nvox][2] target transverse magnetization, host */
@@ -330,6 +339,9 @@ mrs_status mrs_adjoint_backward(mrs_adjoint *adj, float *g_rf, float *g_T1, floa
 typedef struct mrs_adjoint_info {
     int64_t n_iso, n_steps;
     int32_t checkpoint, n_checkpoints, spins_per_thread, block_size;
+    int32_t segments_rebuilt; /* 1 if the last backward pass rebuilt the
+                                 segments backwards, 0 if it used scratch */
+    int32_t reserved;
     float   forward_ms;       /* device time of the last forward pass */
     float   backward_ms;      /* device time of the last backward pass */
 } mrs_adjoint_info;

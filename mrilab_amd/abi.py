@@ -103,13 +103,15 @@ class BatchInfoC(C.Structure):
 
 
 class ExcitationC(C.Structure):
-    _fields_ = [("n_steps", C.c_int32), ("checkpoint", C.c_int32), ("dt", C.c_double),
+    _fields_ = [("n_steps", C.c_int32), ("checkpoint", C.c_int32), ("segment_mode", C.c_int32),
+                ("reserved", C.c_int32), ("dt", C.c_double),
                 ("grad", C.POINTER(C.c_float)), ("target", C.POINTER(C.c_float))]
 
 
 class AdjointInfoC(C.Structure):
     _fields_ = [("n_iso", C.c_int64), ("n_steps", C.c_int64), ("checkpoint", C.c_int32),
                 ("n_checkpoints", C.c_int32), ("spins_per_thread", C.c_int32), ("block_size", C.c_int32),
+                ("segments_rebuilt", C.c_int32), ("reserved", C.c_int32),
                 ("forward_ms", C.c_float), ("backward_ms", C.c_float)]
 
 

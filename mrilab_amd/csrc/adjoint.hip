@@ -19,10 +19,17 @@
 //    every K steps the state goes to a checkpoint buffer in HBM; at the end
 //    each thread writes its share of the voxel's Mxy.
 //  * Backward kernel: the same threads walk the checkpoint segments from the
-//    last to the first. Per segment they recompute the forward from the
-//    checkpoint, storing M' = R M and (sin, cos) of each step in a scratch
-//    buffer (coalesced [step][component][isochromat]), then run the adjoint
-//    back through the segment. The rf gradient of each step is summed over
+//    last to the first. Two ways to get the states of a segment back:
+//    - rebuild (REBUILD = true): start from the checkpoint at the segment's
+//      end and undo each step, M' = E^-1 M_{j+1}, M_j = R^T M'. No memory
+//      traffic besides the checkpoint; the inverse relaxation amplifies
+//      rounding by E^-K at most, so the host picks this only where
+//      K*dt/T1 and K*dt/T2 stay below ln 8 in every voxel (the forward
+//      kernel flags the blocks that break this);
+//    - store: recompute the forward from the checkpoint at the segment's
+//      start, storing M' = R M and (sin, cos) of each step in a scratch
+//      buffer (coalesced [step][component][isochromat]).
+//    Then the adjoint runs back through the segment. The rf gradient of each step is summed over
 //    the wave (butterfly), the block (LDS) and the grid (block partials plus
 //    a fixed-order fp64 reduction), like the engine's ADC sums. Per-voxel
 //    gradients accumulate in fp32 within a segment and in fp64 across them.
@@ -48,11 +55,12 @@ struct ExcParams {
     const float *dWRnd, *M0x, *M0y, *M0z;    // [S][nvox]
     const float *q;                          // [n][3] gamma*Gz, gamma*Gy, gamma*Gx
     const float *rf;                         // [n][2] rf (T)
-    float *ck;                               // [C][3][n_iso], checkpoint c holds M_{cK} (c >= 1)
+    float *ck;                               // [C+1][3][n_iso], slot c holds M_{cK} (c >= 1), slot C M_n
     float *mxy_part;                         // [ngroups][nvox][2]
     float *scratch;                          // [K][5][n_iso]
     float *rf_part;                          // [n][nblk][2], step order reversed
     double *vox_part;                        // [ngroups][nvox][3]
+    int *amp;                                // [nblk] 1 if a voxel of the block cannot rebuild segments
     const float *resid;                      // [nvox][2] m_v - target_v
     int64_t nvox, n_iso;
     int S, n, K, C;
@@ -61,7 +69,12 @@ struct ExcParams {
 
 struct ExcVoxel {
     float x, y, z, wr, wi, m0s, e1, e2, t1, t2;
+    float a1, a2;  // -dt/T1, -dt/T2 (clamped at -200)
 };
+
+// Rebuilding a segment backwards multiplies rounding by at most
+// exp(K dt / T) <= 8.
+constexpr float kLnMaxAmp = 2.0794415f;
 
 __device__ __forceinline__ void exc_setup(const ExcParams &p, int64_t vv, ExcVoxel &k) {
     k.x = p.x[vv];
@@ -84,6 +97,8 @@ __device__ __forceinline__ void exc_setup(const ExcParams &p, int64_t vv, ExcVox
     if (a1 < -200.f) a1 = -200.f;
     k.e2 = (float)exp((double)a2);
     k.e1 = (float)exp((double)a1);
+    k.a1 = a1;
+    k.a2 = a2;
 }
 
 // B1 (gamma-scaled, rotated by the Tx map) and the gradient off-resonance of step j
@@ -101,6 +116,19 @@ __device__ __forceinline__ void exc_fields(const ExcParams &p, const ExcVoxel &k
 // deviation of Mz from m0; returns sin, cos of |Omega| dt (0, 1 without
 // rotation). R z - z = (-ny sn + nx nz k, nx sn + ny nz k, -k (nx^2 + ny^2)),
 // k = 1 - cos.
+// (Mx, My, d) <- R^T M (rotation by +|Omega| dt about n = Omega/|Omega|) in
+// the deviation form of exc_rotate: R^T z - z = (ny sn + nx nz k,
+// -nx sn + ny nz k, -k (nx^2 + ny^2)).
+__device__ __forceinline__ void exc_unrotate(float nx, float ny, float nz, float sn, float cs, float m0, float &mx,
+                                             float &my, float &md) {
+    const float ax = mx, ay = my, az = md;
+    const float k1 = 1.f - cs;
+    const float kk = k1 * (nx * ax + ny * ay + nz * az);
+    mx = ax * cs + (ny * az - nz * ay) * sn + nx * kk + m0 * (nx * nz * k1 + ny * sn);
+    my = ay * cs + (nz * ax - nx * az) * sn + ny * kk + m0 * (ny * nz * k1 - nx * sn);
+    md = az * cs + (nx * ay - ny * ax) * sn + nz * kk - m0 * (k1 * (nx * nx + ny * ny));
+}
+
 __device__ __forceinline__ void exc_rotate(float bx, float by, float w, float dt, float m0, float &mx, float &my,
                                            float &md, float &sn, float &cs) {
     const float om2 = (bx * bx + by * by) + w * w;
@@ -169,14 +197,22 @@ __global__ __launch_bounds__(256) void exc_fwd_kernel(ExcParams p) {
     }
     if (valid) {
         float sx = 0.f, sy = 0.f;
+        float *c = p.ck + (int64_t)p.C * 3 * p.n_iso;  // M_n, where the last segment is rebuilt from
 #pragma unroll
         for (int s = 0; s < SPT; s++) {
             sx += mx[s];
             sy += my[s];
+            c[sidx[s]] = mx[s];
+            c[p.n_iso + sidx[s]] = my[s];
+            c[2 * p.n_iso + sidx[s]] = mz[s];
         }
         p.mxy_part[(grp * nvox + v) * 2] = sx;
         p.mxy_part[(grp * nvox + v) * 2 + 1] = sy;
     }
+    const float kdt = (float)p.K;
+    const int bad = valid && (kdt * k.a1 < -kLnMaxAmp || kdt * k.a2 < -kLnMaxAmp);
+    const int any_bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) p.amp[blockIdx.y * gridDim.x + blockIdx.x] = any_bad;
 }
 
 // Per voxel: m_v = sum over spin groups (fixed order, fp64); resid = m - target.
@@ -215,7 +251,7 @@ __global__ __launch_bounds__(256) void exc_loss_kernel(const double *__restrict_
     if (threadIdx.x == 0) *loss = red[0];
 }
 
-template <int SPT>
+template <int SPT, bool REBUILD>
 __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
     extern __shared__ float red[];  // [kRedBatch][nwaves][2]
     const int64_t nvox = p.nvox;
@@ -248,42 +284,55 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
     }
     double gT1 = 0, gT2 = 0, gB0 = 0;
     int slot = 0, t0 = 0;  // t = n - 1 - j counts processed steps
+    // inverse relaxation factors of the rebuild
+    const float ie2 = (float)exp(-(double)k.a2), ie1 = (float)exp(-(double)k.a1);
     for (int c = p.C - 1; c >= 0; c--) {
         const int j0 = c * p.K;
         const int j1 = min(p.n, j0 + p.K);
-        // recompute the segment from its checkpoint; keep M' and (sin, cos)
         float mx[SPT], my[SPT], mz[SPT];
+        if constexpr (REBUILD) {
+            // M_{j1} from the checkpoint at the segment's end
+            const float *ck = p.ck + (int64_t)(c + 1) * 3 * n_iso;
 #pragma unroll
-        for (int s = 0; s < SPT; s++) {
-            if (c == 0) {
-                mx[s] = p.M0x[sidx[s]];
-                my[s] = p.M0y[sidx[s]];
-                mz[s] = p.M0z[sidx[s]] - k.m0s;
-            } else {
-                const float *ck = p.ck + (int64_t)c * 3 * n_iso;
+            for (int s = 0; s < SPT; s++) {
                 mx[s] = ck[sidx[s]];
                 my[s] = ck[n_iso + sidx[s]];
                 mz[s] = ck[2 * n_iso + sidx[s]];
             }
-        }
-        for (int j = j0; j < j1; j++) {
-            float bx, by, gv;
-            exc_fields(p, k, j, bx, by, gv);
-            float *sc = p.scratch + (int64_t)(j - j0) * 5 * n_iso;
+        } else {
+            // recompute the segment from its checkpoint; keep M' and (sin, cos)
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
-                float sn, cs;
-                exc_rotate(bx, by, wb[s] + gv, dt, k.m0s, mx[s], my[s], mz[s], sn, cs);
-                if (valid) {
-                    sc[sidx[s]] = mx[s];
-                    sc[n_iso + sidx[s]] = my[s];
-                    sc[2 * n_iso + sidx[s]] = mz[s];
-                    sc[3 * n_iso + sidx[s]] = sn;
-                    sc[4 * n_iso + sidx[s]] = cs;
+                if (c == 0) {
+                    mx[s] = p.M0x[sidx[s]];
+                    my[s] = p.M0y[sidx[s]];
+                    mz[s] = p.M0z[sidx[s]] - k.m0s;
+                } else {
+                    const float *ck = p.ck + (int64_t)c * 3 * n_iso;
+                    mx[s] = ck[sidx[s]];
+                    my[s] = ck[n_iso + sidx[s]];
+                    mz[s] = ck[2 * n_iso + sidx[s]];
                 }
-                mx[s] = mx[s] * k.e2;
-                my[s] = my[s] * k.e2;
-                mz[s] = mz[s] * k.e1;
+            }
+            for (int j = j0; j < j1; j++) {
+                float bx, by, gv;
+                exc_fields(p, k, j, bx, by, gv);
+                float *sc = p.scratch + (int64_t)(j - j0) * 5 * n_iso;
+#pragma unroll
+                for (int s = 0; s < SPT; s++) {
+                    float sn, cs;
+                    exc_rotate(bx, by, wb[s] + gv, dt, k.m0s, mx[s], my[s], mz[s], sn, cs);
+                    if (valid) {
+                        sc[sidx[s]] = mx[s];
+                        sc[n_iso + sidx[s]] = my[s];
+                        sc[2 * n_iso + sidx[s]] = mz[s];
+                        sc[3 * n_iso + sidx[s]] = sn;
+                        sc[4 * n_iso + sidx[s]] = cs;
+                    }
+                    mx[s] = mx[s] * k.e2;
+                    my[s] = my[s] * k.e2;
+                    mz[s] = mz[s] * k.e1;
+                }
             }
         }
         // adjoint through the segment
@@ -292,13 +341,10 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
             float bx, by, gv;
             exc_fields(p, k, j, bx, by, gv);
             const float b2 = bx * bx + by * by;
-            const float *sc = p.scratch + (int64_t)(j - j0) * 5 * n_iso;
+            const float *sc = REBUILD ? nullptr : p.scratch + (int64_t)(j - j0) * 5 * n_iso;
             float gx = 0.f, gy = 0.f;
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
-                const float Mx = sc[sidx[s]], My = sc[n_iso + sidx[s]], Md = sc[2 * n_iso + sidx[s]];
-                const float Mz = Md + k.m0s;
-                const float sn = sc[3 * n_iso + sidx[s]], cs = sc[4 * n_iso + sidx[s]];
                 const float w = wb[s] + gv;
                 const float om2 = b2 + w * w;
                 float nx = 0.f, ny = 0.f, nz = 0.f, th = 0.f;
@@ -310,6 +356,23 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                     nz = w * inv;
                     th = om * dt;
                 }
+                float Mx, My, Md, sn, cs;
+                if constexpr (REBUILD) {
+                    // M' = E^-1 M_{j+1}; (sin, cos) as the forward computed them
+                    Mx = mx[s] * ie2;
+                    My = my[s] * ie2;
+                    Md = mz[s] * ie1;
+                    sn = 0.f;
+                    cs = 1.f;
+                    if (om2 > 0.f) sincos_fast(th, &sn, &cs);
+                } else {
+                    Mx = sc[sidx[s]];
+                    My = sc[n_iso + sidx[s]];
+                    Md = sc[2 * n_iso + sidx[s]];
+                    sn = sc[3 * n_iso + sidx[s]];
+                    cs = sc[4 * n_iso + sidx[s]];
+                }
+                const float Mz = Md + k.m0s;
                 const float px = lx[s] * k.e2, py = ly[s] * k.e2, pz = lz[s] * k.e1;  // lambda'
                 sT2 += lx[s] * Mx + ly[s] * My;
                 sT1 += lz[s] * Md;
@@ -341,6 +404,13 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                 lx[s] = px * cs + (ny * pz - nz * py) * sn + nx * kk;
                 ly[s] = py * cs + (nz * px - nx * pz) * sn + ny * kk;
                 lz[s] = pz * cs + (nx * py - ny * px) * sn + nz * kk;
+                if constexpr (REBUILD) {
+                    // M_j = R^T M'
+                    mx[s] = Mx;
+                    my[s] = My;
+                    mz[s] = Md;
+                    if (om2 > 0.f) exc_unrotate(nx, ny, nz, sn, cs, k.m0s, mx[s], my[s], mz[s]);
+                }
             }
             // rf gradient of step j: (gx, gy) through B1 = gamma * w_tx * u
             // lanes past the last voxel read scratch they never wrote: select, never multiply
@@ -434,8 +504,12 @@ struct mrs_adjoint {
     float gyro = 0.f, dt = 0.f;
     float fwd_ms = 0.f, bwd_ms = 0.f;
     bool have_fwd = false;
+    int segment_mode = 0;
+    bool rebuild = false;       // the last forward pass allows rebuilding segments
+    bool last_rebuilt = false;  // what the last backward pass did
+    std::vector<int> amp_host;
     ADev x, y, z, mg, pe, rho, T1, T2, dB0, target, dWRnd, M0[3], q, rf, ck, mxy_part, scratch, rf_part, vox_part,
-        resid, mxy, loss, g_rf, g_T1, g_T2, g_dB0;
+        resid, mxy, loss, g_rf, g_T1, g_T2, g_dB0, amp;
     ~mrs_adjoint() {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -467,6 +541,7 @@ static ExcParams exc_params(mrs_adjoint *A) {
     p.scratch = (float *)A->scratch.p;
     p.rf_part = (float *)A->rf_part.p;
     p.vox_part = (double *)A->vox_part.p;
+    p.amp = (int *)A->amp.p;
     p.resid = (const float *)A->resid.p;
     p.nvox = A->nvox;
     p.n_iso = A->n_iso;
@@ -488,13 +563,21 @@ static void launch_exc_fwd(int spt, dim3 g, dim3 b, hipStream_t st, const ExcPar
     }
 }
 
-static void launch_exc_bwd(int spt, dim3 g, dim3 b, size_t lds, hipStream_t st, const ExcParams &p) {
+template <bool REBUILD>
+static void launch_exc_bwd_t(int spt, dim3 g, dim3 b, size_t lds, hipStream_t st, const ExcParams &p) {
     switch (spt) {
-        case 1: exc_bwd_kernel<1><<<g, b, lds, st>>>(p); break;
-        case 2: exc_bwd_kernel<2><<<g, b, lds, st>>>(p); break;
-        case 4: exc_bwd_kernel<4><<<g, b, lds, st>>>(p); break;
-        case 8: exc_bwd_kernel<8><<<g, b, lds, st>>>(p); break;
+        case 1: exc_bwd_kernel<1, REBUILD><<<g, b, lds, st>>>(p); break;
+        case 2: exc_bwd_kernel<2, REBUILD><<<g, b, lds, st>>>(p); break;
+        case 4: exc_bwd_kernel<4, REBUILD><<<g, b, lds, st>>>(p); break;
+        case 8: exc_bwd_kernel<8, REBUILD><<<g, b, lds, st>>>(p); break;
     }
+}
+
+static void launch_exc_bwd(bool rebuild, int spt, dim3 g, dim3 b, size_t lds, hipStream_t st, const ExcParams &p) {
+    if (rebuild)
+        launch_exc_bwd_t<true>(spt, g, b, lds, st, p);
+    else
+        launch_exc_bwd_t<false>(spt, g, b, lds, st, p);
 }
 
 // Copies n floats into dst (device buffer of the handle) from host or device memory.
@@ -541,6 +624,8 @@ mrs_status mrs_adjoint_create(const mrs_vobj *vo, const mrs_vmag *vm, const mrs_
     A->n = ex->n_steps;
     A->K = ex->checkpoint > 0 ? ex->checkpoint : 64;
     A->C = (A->n + A->K - 1) / A->K;
+    if (ex->segment_mode != 0 && ex->segment_mode != 1) return fail(MRS_ERR_ARG, "segment_mode must be 0 or 1");
+    A->segment_mode = ex->segment_mode;
     A->gyro = (float)vo->Gyro;
     A->dt = (float)ex->dt;
     // 2 spins per thread keeps the backward kernel at occupancy 4 (121 VGPRs)
@@ -571,15 +656,15 @@ mrs_status mrs_adjoint_create(const mrs_vobj *vo, const mrs_vmag *vm, const mrs_
         (rc = aupload(A->M0[0], vo->Mx, ib, st)) || (rc = aupload(A->M0[1], vo->My, ib, st)) ||
         (rc = aupload(A->M0[2], vo->Mz, ib, st)) || (rc = aupload(A->q, q.data(), q.size() * 4, st)) ||
         (rc = amalloc(A->rf, (size_t)A->n * 2 * 4)) || (rc = amalloc(A->T1, vb)) || (rc = amalloc(A->T2, vb)) ||
-        (rc = amalloc(A->dB0, vb)) || (rc = amalloc(A->ck, (size_t)A->C * 3 * ib)) ||
-        (rc = amalloc(A->mxy_part, (size_t)A->ngroups * 2 * vb)) ||
-        (rc = amalloc(A->scratch, (size_t)A->K * 5 * ib)) ||
+        (rc = amalloc(A->dB0, vb)) || (rc = amalloc(A->ck, (size_t)(A->C + 1) * 3 * ib)) ||
+        (rc = amalloc(A->mxy_part, (size_t)A->ngroups * 2 * vb)) || (rc = amalloc(A->amp, (size_t)nblk * 4)) ||
         (rc = amalloc(A->rf_part, (size_t)A->n * nblk * 2 * 4)) ||
         (rc = amalloc(A->vox_part, (size_t)A->ngroups * A->nvox * 3 * 8)) || (rc = amalloc(A->resid, 2 * vb)) ||
         (rc = amalloc(A->mxy, (size_t)A->nvox * 2 * 8)) || (rc = amalloc(A->loss, 8)) ||
         (rc = amalloc(A->g_rf, (size_t)A->n * 2 * 4)) || (rc = amalloc(A->g_T1, vb)) ||
         (rc = amalloc(A->g_T2, vb)) || (rc = amalloc(A->g_dB0, vb)))
         return rc;
+    A->amp_host.assign(nblk, 0);
     HIP_TRY(hipStreamSynchronize(st));
     guard.a = nullptr;
     *out = A;
@@ -608,8 +693,11 @@ mrs_status mrs_adjoint_forward(mrs_adjoint *A, const float *rf, const float *T1,
     HIP_TRY(hipEventRecord(A->ev[1], st));
     if (loss) HIP_TRY(hipMemcpyAsync(loss, A->loss.p, 8, hipMemcpyDeviceToHost, st));
     if (mxy) HIP_TRY(hipMemcpyAsync(mxy, A->mxy.p, (size_t)A->nvox * 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(A->amp_host.data(), A->amp.p, A->amp_host.size() * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipEventElapsedTime(&A->fwd_ms, A->ev[0], A->ev[1]));
+    A->rebuild = A->segment_mode == 0;
+    for (int f : A->amp_host) A->rebuild = A->rebuild && f == 0;
     A->have_fwd = true;
     return MRS_OK;
 }
@@ -621,10 +709,14 @@ mrs_status mrs_adjoint_backward(mrs_adjoint *A, float *g_rf, float *g_T1, float 
     if (!g_rf || !g_T1 || !g_T2 || !g_dB0) return fail(MRS_ERR_ARG, "NULL gradient pointer");
     HIP_TRY(hipSetDevice(A->device));
     hipStream_t st = A->stream;
+    if (!A->rebuild && !A->scratch.p) {
+        mrs_status rc = amalloc(A->scratch, (size_t)A->K * 5 * (size_t)A->n_iso * sizeof(float));
+        if (rc) return rc;
+    }
     const ExcParams p = exc_params(A);
     const size_t lds = (size_t)kRedBatch * (A->block / 64) * 2 * sizeof(float);
     HIP_TRY(hipEventRecord(A->ev[2], st));
-    launch_exc_bwd(A->spt, dim3(A->gx, A->ngroups), dim3(A->block), lds, st, p);
+    launch_exc_bwd(A->rebuild, A->spt, dim3(A->gx, A->ngroups), dim3(A->block), lds, st, p);
     HIP_TRY(hipGetLastError());
     const int nblk = A->gx * A->ngroups;
     exc_rf_reduce_kernel<<<(unsigned)((A->n + 255) / 256), 256, 0, st>>>((const float *)A->rf_part.p, nblk, A->n,
@@ -641,6 +733,7 @@ mrs_status mrs_adjoint_backward(mrs_adjoint *A, float *g_rf, float *g_T1, float 
     HIP_TRY(hipMemcpyAsync(g_dB0, A->g_dB0.p, (size_t)A->nvox * 4, kind, st));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipEventElapsedTime(&A->bwd_ms, A->ev[2], A->ev[3]));
+    A->last_rebuilt = A->rebuild;
     return MRS_OK;
 }
 
@@ -652,6 +745,8 @@ mrs_status mrs_adjoint_get_info(const mrs_adjoint *A, mrs_adjoint_info *info) {
     info->n_checkpoints = A->C;
     info->spins_per_thread = A->spt;
     info->block_size = A->block;
+    info->segments_rebuilt = A->last_rebuilt ? 1 : 0;
+    info->reserved = 0;
     info->forward_ms = A->fwd_ms;
     info->backward_ms = A->bwd_ms;
     return MRS_OK;

@@ -131,6 +131,10 @@ __global__ __launch_bounds__(256) void bloch_batch_kernel(BatchParams p, const u
                 cry[c] = 0.f;
             }
         }
+        if (!valid) {  // their state mirrors the last voxel's; keep it out of the sums
+#pragma unroll
+            for (int c = 0; c < NRX; c++) crx[c] = cry[c] = 0.f;
+        }
         const TxMaps txm{p.txmg, p.txpe, vg, nvox, 1};
         const crec_t rec = (crec_t)rec_g + d.rec_off;
         const int n_rec = d.n_rec;
@@ -145,15 +149,12 @@ __global__ __launch_bounds__(256) void bloch_batch_kernel(BatchParams p, const u
                 st.spoil();
             }
             if (flags & REC_ADC) {  // DoScanAtCPU.cpp:445-491, before the step
-                float sx = 0.f, sy = 0.f;
+                // lanes past the last voxel carry zero coil weights
+                float sx = st.mx[0], sy = st.my[0];
 #pragma unroll
-                for (int s = 0; s < SPT; s++) {
+                for (int s = 1; s < SPT; s++) {
                     sx += st.mx[s];
                     sy += st.my[s];
-                }
-                if (!valid) {
-                    sx = 0.f;
-                    sy = 0.f;
                 }
                 float val[NV];
 #pragma unroll

@@ -35,7 +35,7 @@ template <int SPT>
 struct SpinState {
     float mx[SPT], my[SPT], mz[SPT];
     float wb[SPT];            // off-resonance without gradients (rad/s)
-    float cc[SPT], ss[SPT];   // cached free-precession rotation
+    float cc[SPT], ss[SPT];   // cached free-precession rotation (times e2 in reference mode)
     float e1 = 1.f, e2 = 1.f, m0e = 0.f;  // reference relaxation factors
     float d1 = 0.f, d2 = 0.f;             // accurate relaxation factors
     bool have_rot = false, have_e = false;
@@ -49,8 +49,12 @@ struct SpinState {
     }
 };
 
-// One step of free precession + relaxation with a cached rotation (c, s)
-// (BlochKernel.h:314-368); PREC selects the relaxation form.
+// One step of free precession + relaxation with a cached rotation
+// (BlochKernel.h:314-368); PREC selects the relaxation form. In reference
+// mode the cached (c, s) already carry the T2 factor e2 (6 instead of 8
+// operations per isochromat-step; the transverse rounding differs from the
+// reference's rotate-then-scale by about an ulp per step, which damps out
+// with T2, whereas Mz keeps the reference's rounding exactly).
 template <int SPT, int PREC>
 __device__ __forceinline__ void free_step(SpinState<SPT> &st, float m0s) {
 #pragma unroll
@@ -59,8 +63,8 @@ __device__ __forceinline__ void free_step(SpinState<SPT> &st, float m0s) {
         const float rx = ax * st.cc[s] + ay * st.ss[s];
         const float ry = ay * st.cc[s] - ax * st.ss[s];
         if constexpr (PREC == 0) {
-            st.mx[s] = rx * st.e2;
-            st.my[s] = ry * st.e2;
+            st.mx[s] = rx;
+            st.my[s] = ry;
             st.mz[s] = relax_mz_ref(st.mz[s], st.e1, st.m0e);
         } else {
             st.mx[s] = rx - rx * st.d2;
@@ -156,11 +160,24 @@ __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst 
         if (flags & REC_ONCE) {
             // rotation used by this step only
 #pragma unroll
-            for (int s = 0; s < SPT; s++) sincos_fast((st.wb[s] + gv) * dt, &st.ss[s], &st.cc[s]);
+            for (int s = 0; s < SPT; s++) {
+                sincos_fast((st.wb[s] + gv) * dt, &st.ss[s], &st.cc[s]);
+                if constexpr (PREC == 0) {
+                    st.ss[s] *= st.e2;
+                    st.cc[s] *= st.e2;
+                }
+            }
             st.have_rot = false;
         } else if (!((flags & REC_SAME) && st.have_rot)) {
+            // a REC_SAME step has this step's dt, so e2 stays valid with the cache
 #pragma unroll
-            for (int s = 0; s < SPT; s++) sincos_cr((st.wb[s] + gv) * dt, &st.ss[s], &st.cc[s]);
+            for (int s = 0; s < SPT; s++) {
+                sincos_cr((st.wb[s] + gv) * dt, &st.ss[s], &st.cc[s]);
+                if constexpr (PREC == 0) {
+                    st.ss[s] *= st.e2;
+                    st.cc[s] *= st.e2;
+                }
+            }
             st.have_rot = true;
         }
         // this step, then the folded repeats of it

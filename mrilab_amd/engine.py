@@ -384,7 +384,9 @@ class Adjoint:
     fields); parameters may be numpy arrays (host) or float32 torch tensors on
     the GPU (device pointers)."""
 
-    def __init__(self, exc, checkpoint: int = 64, options=None):
+    def __init__(self, exc, checkpoint: int = 64, options=None, segment_mode: int = 0):
+        """segment_mode 0 rebuilds checkpoint segments backwards where that
+        is safe (no scratch memory), 1 always recomputes them into scratch."""
         self._lib = load()
         S = int(exc.VObj["SpinNum"])
         self._packed = abi.pack_scan(exc.VObj, exc.VMag, exc.VCoi, *exc.aux)
@@ -395,7 +397,8 @@ class Adjoint:
         self._target = np.ascontiguousarray(exc.target, dtype=np.float32)
         if self._grad.shape != (self.n, 3) or self._target.shape != (self.nvox, 2):
             raise ValueError("grad must be [n, 3] and target [nvox, 2]")
-        ex = abi.ExcitationC(n_steps=self.n, checkpoint=int(checkpoint), dt=float(exc.dt),
+        ex = abi.ExcitationC(n_steps=self.n, checkpoint=int(checkpoint), segment_mode=int(segment_mode),
+                             dt=float(exc.dt),
                              grad=self._grad.ctypes.data_as(C.POINTER(C.c_float)),
                              target=self._target.ctypes.data_as(C.POINTER(C.c_float)))
         self._h = C.c_void_p()
