@@ -36,6 +36,10 @@ struct SpinState {
     float mx[SPT], my[SPT], mz[SPT];
     float wb[SPT];            // off-resonance without gradients (rad/s)
     float cc[SPT], ss[SPT];   // cached free-precession rotation (times e2 in reference mode)
+    // rotation by (wb[s] - wb[0]) * dt for s >= 1 (times e2 in reference mode),
+    // refreshed with dt: single-use rotations of spin s are composed from
+    // spin 0's by angle addition
+    float cd[SPT], sd[SPT];
     float e1 = 1.f, e2 = 1.f, m0e = 0.f;  // reference relaxation factors
     float d1 = 0.f, d2 = 0.f;             // accurate relaxation factors
     bool have_rot = false, have_e = false;
@@ -99,6 +103,14 @@ __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst 
             st.d2 = (float)(-expm1(-(double)dt / (double)k.t2));
             st.d1 = (float)(-expm1(-(double)dt / (double)k.t1));
         }
+#pragma unroll
+        for (int s = 1; s < SPT; s++) {
+            sincos_cr((st.wb[s] - st.wb[0]) * dt, &st.sd[s], &st.cd[s]);
+            if constexpr (PREC == 0) {
+                st.sd[s] *= st.e2;
+                st.cd[s] *= st.e2;
+            }
+        }
     }
     const float gv = k.z * __uint_as_float(q[1]) + k.y * __uint_as_float(q[2]) + k.x * __uint_as_float(q[3]);
     if (flags & REC_RF) {
@@ -158,14 +170,22 @@ __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst 
         }
     } else {
         if (flags & REC_ONCE) {
-            // rotation used by this step only
+            // rotation used by this step only: spin 0's angle, then angle
+            // addition with the cached per-spin offsets (about 2 ulp, not
+            // accumulated since no later step reuses it)
+            float s0, c0;
+            sincos_fast((st.wb[0] + gv) * dt, &s0, &c0);
+            if constexpr (PREC == 0) {
+                st.ss[0] = s0 * st.e2;
+                st.cc[0] = c0 * st.e2;
+            } else {
+                st.ss[0] = s0;
+                st.cc[0] = c0;
+            }
 #pragma unroll
-            for (int s = 0; s < SPT; s++) {
-                sincos_fast((st.wb[s] + gv) * dt, &st.ss[s], &st.cc[s]);
-                if constexpr (PREC == 0) {
-                    st.ss[s] *= st.e2;
-                    st.cc[s] *= st.e2;
-                }
+            for (int s = 1; s < SPT; s++) {
+                st.cc[s] = c0 * st.cd[s] - s0 * st.sd[s];
+                st.ss[s] = s0 * st.cd[s] + c0 * st.sd[s];
             }
             st.have_rot = false;
         } else if (!((flags & REC_SAME) && st.have_rot)) {
@@ -180,9 +200,14 @@ __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst 
             }
             st.have_rot = true;
         }
-        // this step, then the folded repeats of it
+        // this step, then the folded repeats of it (in pairs: the compiler
+        // then alternates registers instead of copying the state each step)
         const int n = 1 + (int)(flags >> REC_REP_SHIFT);
-        for (int r = 0; r < n; r++) free_step<SPT, PREC>(st, k.m0s);
+        if (n & 1) free_step<SPT, PREC>(st, k.m0s);
+        for (int r = n & 1; r < n; r += 2) {
+            free_step<SPT, PREC>(st, k.m0s);
+            free_step<SPT, PREC>(st, k.m0s);
+        }
     }
 }
 
