@@ -81,9 +81,11 @@ def batched_section(n_sets: int, plain_sample: int, reps: int) -> dict:
     t0 = time.perf_counter()
     scans = workloads.fingerprint_batch(n_sets=n_sets)
     log(f"batched: {n_sets} fingerprint scans generated ({time.perf_counter() - t0:.1f} s)")
+    inputs = [sc.copy().structs for sc in scans]  # prepared outside the timers, as for the plain calls
     t0 = time.perf_counter()
-    b = engine.Batch([sc.copy().structs for sc in scans], options=engine.make_options(timing=1))
+    b = engine.Batch(inputs, options=engine.make_options(timing=1))
     t_create = time.perf_counter() - t0
+    del inputs
     info = b.info()
     log(f"batched: create {t_create:.2f} s (decode {info['decode_ms']:.0f} ms, {info['n_candidates']} shapes "
         f"tuned, best spt {info['spins_per_thread']} block {info['block_size']})")
@@ -168,6 +170,51 @@ def adjoint_section(reps: int) -> dict:
             "spins_per_thread": info["spins_per_thread"]}
 
 
+def multipool_section(sat_split: float, sat_exact: float, cpu_shape: int, threads: int) -> dict:
+    """Multi-pool exchange with MT saturation on the 256x256 3-pool phantom
+    (free water, myelin water, bound pool with a super-Lorentzian line):
+    the first of the 40 saturation offsets (-50 ppm), its Gaussian pulse
+    train cut to ``sat_split`` s (operator split, fp32) and ``sat_exact`` s
+    (fp64 matrix exponential), then readout. Device time per scan, and the
+    CPU oracle's split scheme on a ``cpu_shape``^2 crop for comparison."""
+    import oracle
+    from mrilab_amd import engine, workloads
+    out = {"workload": "256x256 voxels x 3 pools (Dirichlet(8,1,1) fractions, k~U[1,30]/s, T2b 10 us), first "
+                       "saturation offset (-50 ppm at 3 T) of 300 deg Gaussian pulses (20 ms on, 20 ms off), "
+                       "10 us steps"}
+    for scheme, sat in ((0, sat_split), (1, sat_exact)):
+        sc = workloads.mt_cest(shape=(256, 256, 1), n_offsets=1, sat_time=sat)
+        o = sc.copy()
+        opt = engine.make_options(multipool_scheme=scheme, timing=1, model=1)
+        with engine.Plan(*o.structs, options=opt) as p:
+            p.run()
+            p.sync()
+            t0 = time.perf_counter()
+            p.run()
+            p.sync()
+            el = time.perf_counter() - t0
+            info = p.info()
+        vs = 256 * 256 * info["n_steps"]
+        name = "split" if scheme == 0 else "exact"
+        out[name] = {"saturation_s": sat, "steps": info["n_steps"], "ms_per_scan": el * 1e3,
+                     "kernel_ms": info["fwd_kernel_ms"], "voxel_steps_per_s": vs / el,
+                     # 40 offsets x a 2 s train: time grows with the train length
+                     "full_config_s_estimated": 40 * (2.0 / sat) * el}
+        log(f"multipool {name}: {info['n_steps']} steps, {el * 1e3:.1f} ms, {vs / el:.3e} voxel-steps/s")
+    sc = workloads.mt_cest(shape=(cpu_shape, cpu_shape, 1), n_offsets=1, sat_time=sat_split)
+    oracle.set_threads(threads, "f32")
+    t0 = time.perf_counter()
+    oracle.gm_scan(sc, precision="f32", scheme=0)
+    el = time.perf_counter() - t0
+    oracle.set_threads(0, "f32")
+    vs = cpu_shape * cpu_shape * sc.n_steps()
+    out["cpu_split"] = {"value": vs / el, "unit": "voxel-steps/s", "cores": threads, "kind": "port",
+                        "sample": f"{cpu_shape}x{cpu_shape} crop, {sc.n_steps()} steps ({el:.1f} s), fp32 "
+                                  "restatement of BlochKernelGM.cuh's split, OpenMP over voxels"}
+    log(f"multipool cpu split: {vs / el:.3e} voxel-steps/s on {threads} threads")
+    return out
+
+
 def traffic_from_profile() -> float | None:
     """HBM bytes per forward launch measured by rocprofv3 (profiles/)."""
     path = os.path.join(ROOT, "profiles", "bloch_fwd_bssfp3d_traffic.json")
@@ -187,6 +234,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample-tr", type=int, default=2)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched fingerprinting section")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint section")
+    ap.add_argument("--no-multipool", action="store_true", help="skip the multi-pool section")
     ap.add_argument("--batch-sets", type=int, default=4096)
     ap.add_argument("--batch-plain-sample", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -280,6 +328,9 @@ def main() -> int:
         line["batched"] = batched_section(args.batch_sets, args.batch_plain_sample, 3)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_section(3)
+    if not args.no_multipool:
+        mp_threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        line["multipool"] = multipool_section(0.2, 0.04, 64, mp_threads)
     if not args.no_cpu_baseline:
         threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         log(f"cpu baseline: fp32 oracle on the first {args.cpu_sample_tr} TRs, {threads} threads")

@@ -529,9 +529,15 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     P->gyro = (float)vo->Gyro;
     for (int t = 0; t < P->Ty; t++)
         P->cs[t] = (float)((double)((float)vt->CS[t] * 2.0f) * kPi);  // BlochKernel.h:42
-    // spins per thread: largest of 8, 4, 2, 1 dividing SpinNum
+    // spins per thread: largest of 8, 4, 2, 1 dividing SpinNum; when not
+    // given, also small enough that the scan has a thread for every wave
+    // slot of the device (256 CUs x 4 SIMDs x 4 waves x 64 lanes) if it can
     int spt = opt && opt->spins_per_thread > 0 ? opt->spins_per_thread : 8;
     while (spt > 1 && (P->S % spt != 0 || spt > P->S)) spt >>= 1;
+    if (!(opt && opt->spins_per_thread > 0)) {
+        const int64_t iso = P->nvox * (int64_t)P->S * (P->model >= 1 ? 1 : P->Ty);
+        while (spt > 1 && iso / spt < (int64_t)256 * 4 * 4 * 64) spt >>= 1;
+    }
     if (spt != 1 && spt != 2 && spt != 4 && spt != 8) return fail(MRS_ERR_ARG, "spins_per_thread must be 1, 2, 4 or 8");
     if (P->S % spt) return fail(MRS_ERR_ARG, "spins_per_thread must divide SpinNum");
     if (P->model >= 1) spt = 1;

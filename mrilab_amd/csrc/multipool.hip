@@ -55,14 +55,20 @@ __device__ T superlorentzian(double dw, double t2b) {
     return G;
 }
 
+// longest Taylor series of one exact substep (h*|A| <= 4 needs about 35 terms)
+constexpr int kTaylorMax = 48;
+
 template <int TY, typename T>
 struct GmState {
     T mx[TY], my[TY], mz[TY];
 };
 
+// A multi-pool scan has one thread per voxel isochromat, usually about one
+// wave per SIMD, so the kernel may use the registers of 2 waves per SIMD
+// rather than spill to reach a higher occupancy.
 template <int TY, int NRX, int SCHEME, bool WC>
-__global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_t *__restrict__ rec_g, int r0,
-                                                       int r1) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void bloch_gm_kernel(
+    GmParams p, const uint32_t *__restrict__ rec_g, int r0, int r1) {
     using T = typename std::conditional<SCHEME == 0, float, double>::type;
     constexpr int NV = 2 * NRX;
     constexpr int NVT = (NV * TY <= 2) ? 2 : (NV * TY <= 4) ? 4 : (NV * TY <= 8) ? 8 : (NV * TY <= 16) ? 16
@@ -145,6 +151,15 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
     bool have_e = false;
     int slot = 0;
     int adc_batch0 = 0;
+    // first Tx coil of this voxel (magnitude, sin and cos of its phase):
+    // constant over the launch, so rf steps read no maps
+    const float tx_mg0 = p.txmg[vv];
+    float tx_sp0, tx_cp0;
+    sincos_cr(p.txpe[vv], &tx_sp0, &tx_cp0);
+    // free-precession rotation of each free pool, kept while the steps
+    // repeat with the same dt and fields (REC_SAME) and for folded repeats
+    float fsn[TY], fcs[TY];
+    bool have_frot = false;
 
     for (int r = r0; r < r1; ++r) {
         const crec_t q = rec + r * stride;
@@ -205,9 +220,11 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                 const float A = __uint_as_float(q[REC_HEAD + cw * c + 2]);
                 const float F = __uint_as_float(q[REC_HEAD + cw * c + 3]);
                 if (A == 0.f) continue;
-                float sp, cp;
-                sincos_cr(p.txpe[vv + (int64_t)c * nvox], &sp, &cp);
-                const float mg = p.txmg[vv + (int64_t)c * nvox];
+                float sp = tx_sp0, cp = tx_cp0, mg = tx_mg0;
+                if (c > 0) {
+                    sincos_cr(p.txpe[vv + (int64_t)c * nvox], &sp, &cp);
+                    mg = p.txmg[vv + (int64_t)c * nvox];
+                }
                 bx += mg * (cp * br - sp * bi);
                 by += mg * (cp * bi + sp * br);
                 // BlochKernelGM.cuh:323-380: W = pi*(gamma*B1)^2*G, averaged over active coils
@@ -236,9 +253,20 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
             (void)wsum;
         }
         const int nrep = 1 + (int)(flags >> REC_REP_SHIFT);
+        if constexpr (SCHEME == 0) {
+            if (rf) {
+                have_frot = false;
+            } else if (!((flags & REC_SAME) && have_frot)) {
+#pragma unroll
+                for (int t = 0; t < TY; t++) sincos_cr((wb[t] + gv) * dt, &fsn[t], &fcs[t]);
+                have_frot = true;
+            }
+        }
         // rotation of free pool t over this step (BlochKernel.h:146-344 form)
-        auto rotate_free = [&](int t, float &ax, float &ay, float &az) {
-            const float w = rf ? (wb[t] + gv) + wrf : wb[t] + gv;
+        // (pool values passed in, so the arrays stay in registers)
+        auto rotate_free = [&](float wbt, float fst, float fct, float &ax, float &ay,
+                               float &az) __attribute__((always_inline)) {
+            const float w = rf ? (wbt + gv) + wrf : wbt + gv;
             if (rf) {
                 const float om2 = (bx * bx + by * by) + w * w;
                 if (om2 > 0.f) {
@@ -256,10 +284,9 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                     az = rz;
                 }
             } else {
-                float sn, cs;
-                sincos_cr(w * dt, &sn, &cs);
-                const float rx = ax * cs + ay * sn;
-                const float ry = ay * cs - ax * sn;
+                // the cached sincos of (wb[t] + gv) * dt
+                const float rx = ax * fct + ay * fst;
+                const float ry = ay * fct - ax * fst;
                 ax = rx;
                 ay = ry;
             }
@@ -295,7 +322,7 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                     if (skip[t] || (p.flag[t] != 0 && p.flag[t] != 1)) continue;
                     float ax = Bx[t], ay = By[t], az = Bz[t];
                     if (p.flag[t] == 0) {
-                        rotate_free(t, ax, ay, az);
+                        rotate_free(wb[t], fsn[t], fcs[t], ax, ay, az);
                         ax = ax * e2[t];
                         ay = ay * e2[t];
                     } else if (rf) {
@@ -352,7 +379,7 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                 }
 #pragma unroll
                 for (int t = 0; t < TY; t++)
-                    if (t != 1) rotate_free(t, ox[t], oy[t], oz[t]);
+                    if (t != 1) rotate_free(wb[t], fsn[t], fcs[t], ox[t], oy[t], oz[t]);
                 // water
                 float ko = 1.f, kx = 0.f, ky = 0.f, kz = 0.f;
 #pragma unroll
@@ -432,9 +459,15 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                 }
                 nrm = fmax(nrm, rowmax);
             }
-            int nsub = (int)ceil(nrm * (double)dt * 2.0);
+            // substeps of h*|A| <= 4; the series runs until every active lane's
+            // term is below 1e-17 of its state (at most kTaylorMax terms)
+            int nsub = (int)ceil(nrm * (double)dt * 0.25);
             if (nsub < 1) nsub = 1;
             const double h = (double)dt / nsub;
+            const double hn = h * nrm;
+            double amax = 0.0;
+#pragma unroll
+            for (int t = 0; t < TY; t++) amax = fmax(amax, skip[t] ? 0.0 : fabs(aff[t]));
             auto apply = [&](const double *in, double *out) {
 #pragma unroll
                 for (int t = 0; t < TY; t++) {
@@ -463,15 +496,15 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                     }
                 }
             };
-            for (int k = 0; k < nrep; k++) {
+            // x <- exp(A*dt) x + (exp(A*dt) - I) A^-1 b, substep by substep:
+            // x += sum_k h^k/k! A^(k-1) (A x + b)
+            auto step_exact = [&](double *xv) __attribute__((always_inline)) {
                 for (int sub = 0; sub < nsub; sub++) {
-                    double xv[3 * TY], term[3 * TY], nt[3 * TY];
+                    double term[3 * TY], nt[3 * TY];
+                    double xmax = 0.0;
 #pragma unroll
-                    for (int t = 0; t < TY; t++) {
-                        xv[3 * t] = st.mx[t];
-                        xv[3 * t + 1] = st.my[t];
-                        xv[3 * t + 2] = st.mz[t];
-                    }
+                    for (int i = 0; i < 3 * TY; i++) xmax = fmax(xmax, fabs(xv[i]));
+                    const double tol = 1e-17 * (xmax + h * amax);
                     apply(xv, nt);
 #pragma unroll
                     for (int t = 0; t < TY; t++) nt[3 * t + 2] += skip[t] ? 0.0 : aff[t];
@@ -480,15 +513,90 @@ __global__ __launch_bounds__(256) void bloch_gm_kernel(GmParams p, const uint32_
                         term[i] = h * nt[i];
                         xv[i] += term[i];
                     }
-                    for (int kk = 2; kk <= 14; kk++) {
+                    for (int kk = 2; kk <= kTaylorMax; kk++) {
                         apply(term, nt);
                         const double f = h / kk;
+                        double tm = 0.0;
 #pragma unroll
                         for (int i = 0; i < 3 * TY; i++) {
                             term[i] = f * nt[i];
                             xv[i] += term[i];
+                            tm = fmax(tm, fabs(term[i]));
+                        }
+                        // terms shrink monotonically once kk exceeds h*|A|
+                        if (__all((double)kk > hn && tm <= tol)) break;
+                    }
+                }
+            };
+            if (!rf && nrep > 1) {
+                // held free step: the map of one step is affine and the same for
+                // every repeat; take its columns from basis states (transverse
+                // columns are complex: without rf, Mx + iMy of the free pools
+                // evolve complex-linearly) and apply it nrep times
+                double cz[TY];   // the affine constant (longitudinal only)
+                double tr[TY][TY], ti[TY][TY], lz[TY][TY];
+                // basis b = 0: zero state; 1 + t: unit Mx of pool t; 1 + TY + t:
+                // unit Mz of pool t. Every basis state is stepped (skipped and
+                // bound pools included), so the columns are those of the direct
+                // step; one copy of the step code, entries stored by
+                // compile-time index.
+#pragma unroll 1
+                for (int b = 0; b <= 2 * TY; b++) {
+                    const bool bz = b > TY;
+                    const int tb = bz ? b - 1 - TY : b - 1;
+                    const int ib = bz ? 3 * tb + 2 : 3 * tb;
+                    double e[3 * TY];
+#pragma unroll
+                    for (int i = 0; i < 3 * TY; i++) e[i] = (b > 0 && i == ib) ? 1.0 : 0.0;
+                    step_exact(e);
+#pragma unroll
+                    for (int q = 0; q < TY; q++) {
+                        if (b == 0) cz[q] = e[3 * q + 2];
+#pragma unroll
+                        for (int t = 0; t < TY; t++) {
+                            if (b > 0 && t == tb) {
+                                if (bz) {
+                                    lz[q][t] = e[3 * q + 2] - cz[q];
+                                } else {
+                                    tr[q][t] = e[3 * q];
+                                    ti[q][t] = e[3 * q + 1];
+                                }
+                            }
                         }
                     }
+                }
+                for (int k = 0; k < nrep; k++) {
+                    double ox[TY], oy[TY], oz[TY];
+#pragma unroll
+                    for (int t = 0; t < TY; t++) {
+                        ox[t] = st.mx[t];
+                        oy[t] = st.my[t];
+                        oz[t] = st.mz[t];
+                    }
+#pragma unroll
+                    for (int q = 0; q < TY; q++) {
+                        double ax = 0.0, ay = 0.0, az = cz[q];
+#pragma unroll
+                        for (int t = 0; t < TY; t++) {
+                            ax += tr[q][t] * ox[t] - ti[q][t] * oy[t];
+                            ay += tr[q][t] * oy[t] + ti[q][t] * ox[t];
+                            az += lz[q][t] * oz[t];
+                        }
+                        st.mx[q] = ax;
+                        st.my[q] = ay;
+                        st.mz[q] = az;
+                    }
+                }
+            } else {
+                for (int k = 0; k < nrep; k++) {
+                    double xv[3 * TY];
+#pragma unroll
+                    for (int t = 0; t < TY; t++) {
+                        xv[3 * t] = st.mx[t];
+                        xv[3 * t + 1] = st.my[t];
+                        xv[3 * t + 2] = st.mz[t];
+                    }
+                    step_exact(xv);
 #pragma unroll
                     for (int t = 0; t < TY; t++) {
                         st.mx[t] = xv[3 * t];
