@@ -215,6 +215,44 @@ def multipool_section(sat_split: float, sat_exact: float, cpu_shape: int, thread
     return out
 
 
+def recon_section(plan, sc, reps: int) -> dict:
+    """Signal to image on the scan's own k-space: the plan's device-resident
+    signal reconstructed per coil (fftshift(ifftn(fftshift)), fp64 hipFFT,
+    mrs_plan_recon), against the oracle (numpy) on the fetched signal."""
+    import numpy as np
+    import torch
+    import oracle.recon as orc
+    from mrilab_amd import engine
+    nro, npe, nsl, rx = 128, int(sc.VObj["Rho"].shape[0]), int(sc.VObj["Rho"].shape[2]), int(sc.VCoi["RxCoilNum"])
+    d = engine.recon_desc(nro, npe, nsl, 1, rx, 1)
+    img = plan.recon(d, device=True)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        img = plan.recon(d, device=True)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    gpu_s = float(np.median(ts))
+    vs = {k: np.zeros_like(v) for k, v in sc.VSig.items() if k in ("Sx", "Sy", "Kx", "Ky", "Kz")}
+    vs.update({k: v for k, v in sc.VSig.items() if k not in vs})
+    plan.fetch(VSig=vs)
+    t0 = time.perf_counter()
+    want = orc.cart_recon(vs["Sx"], vs["Sy"], nro, npe, nsl, 1, rx, 1)
+    cpu_s = time.perf_counter() - t0
+    got = img["Real"].cpu().numpy().transpose(range(4, -1, -1)) + 1j * img["Imag"].cpu().numpy().transpose(
+        range(4, -1, -1))
+    ref = want["Real"] + 1j * want["Imag"]
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    vox = int(np.prod(engine.recon_dims(d)))
+    log(f"recon: {gpu_s * 1e3:.2f} ms per call ({vox} image values), CPU oracle {cpu_s:.2f} s, rel-L2 {err:.1e}")
+    return {"workload": f"the scan's k-space: {nro} readout x {npe} x {nsl} phase encodes x {rx} coils -> "
+                        f"{nro - 1}x{npe}x{nsl} complex fp64 images per coil",
+            "ms_per_call": gpu_s * 1e3, "image_values_per_s": vox / gpu_s, "rel_l2_vs_oracle": err,
+            "cpu": {"seconds": cpu_s, "kind": "port", "cores": 1,
+                    "sample": "numpy restatement of DoCartRecon.m on the fetched signal (pocketfft)"}}
+
+
 def traffic_from_profile() -> float | None:
     """HBM bytes per forward launch measured by rocprofv3 (profiles/)."""
     path = os.path.join(ROOT, "profiles", "bloch_fwd_bssfp3d_traffic.json")
@@ -235,6 +273,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched fingerprinting section")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint section")
     ap.add_argument("--no-multipool", action="store_true", help="skip the multi-pool section")
+    ap.add_argument("--no-recon", action="store_true", help="skip the image reconstruction section")
     ap.add_argument("--batch-sets", type=int, default=4096)
     ap.add_argument("--batch-plain-sample", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -323,6 +362,8 @@ def main() -> int:
                               "flop_per_isochromat_step": flop}},
         "kernel_ms_per_step": {"bloch_fwd": fwd_ms / args.steps, "signal_reduce": red_ms / args.steps},
     }
+    if not args.no_recon and args.tr is None:
+        line["recon"] = recon_section(plan, sc, 5)
     plan.close()
     if not args.no_batched:
         line["batched"] = batched_section(args.batch_sets, args.batch_plain_sample, 3)

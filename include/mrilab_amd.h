@@ -348,6 +348,78 @@ typedef struct mrs_adjoint_info {
 mrs_status mrs_adjoint_get_info(const mrs_adjoint *adj, mrs_adjoint_info *info);
 void mrs_adjoint_destroy(mrs_adjoint *adj);
 
+/* ---------------------------------------------------------------------------
+ * Signal to image: Cartesian reconstruction and k-space noise.
+ * ------------------------------------------------------------------------- */
+typedef struct mrs_recon_desc {
+    int32_t res_freq;         /* VCtl.ResFreq: readout samples per echo */
+    int32_t res_phase;        /* VCtl.ResPhase */
+    int32_t slice_num;        /* VCtl.SliceNum */
+    int32_t te_per_tr;        /* VCtl.TEPerTR */
+    int32_t rx;               /* VCoi.RxCoilNum */
+    int32_t types;            /* VObj.TypeNum */
+    int32_t zf_kx, zf_ky;     /* VCtl.ZF_Kx, ZF_Ky; 0 = no zero filling */
+    int32_t zf_kz;            /* the factor of VCtl.ZF_Kz ('x2' -> 2) */
+    int32_t swap_xy;          /* 1 when VCtl.ScanPlane/FreqDir select the
+                                 second permute (DoCartRecon.m:45-73) */
+} mrs_recon_desc;
+/* VImg: images and the permuted k-space, each [d0][d1][d2][Rx][TEPerTR]
+ * column-major (dims from mrs_recon_dims), fp64. */
+typedef struct mrs_vimg {
+    double *Real, *Imag, *Mag, *Phase;
+    double *Sx, *Sy;
+} mrs_vimg;
+/* d0..d2 of the image volume, then Rx and TEPerTR. */
+mrs_status mrs_recon_dims(const mrs_recon_desc *desc, int64_t dims[5]);
+/* Reconstructs VSig.Sx/Sy (SignalNum = ResFreq*TEPerTR*ResPhase*SliceNum;
+ * index t*Rx*N + c*N + n) into VImg: per echo the types are summed, the
+ * last readout sample dropped, k-space zero-filled and permuted, then
+ * fftshift(ifftn(fftshift(.))) per coil, in fp64 on hipFFT. device = 1 when
+ * vsig->Sx/Sy and the VImg arrays are device memory of GPU 0.
+ * Replaces: DoCartRecon, Src/Recon/Default/DoCartRecon.m:3-91 */
+mrs_status mrs_DoCartRecon(const mrs_vsig *vsig, const mrs_recon_desc *desc, mrs_vimg *img, int32_t device);
+/* The same from the k-space a plan's last run left in device memory (no
+ * download of the signal). */
+mrs_status mrs_plan_recon(mrs_plan *plan, const mrs_recon_desc *desc, mrs_vimg *img, int32_t device);
+/* Adds scale * N(0, 1) noise to VSig.Sx and VSig.Sy (n values each), with
+ * a counter-based generator (Philox-4x32-10, Box-Muller in fp64) seeded by
+ * seed: the same seed gives the same noise on any device. scale is the
+ * DoAddNoise.m:8-16 NoiseScale; mrs_noise_scale computes it.
+ * Replaces: DoAddNoise, Src/Main/DoAddNoise.m:1-21 */
+mrs_status mrs_DoAddNoise(double *Sx, double *Sy, int64_t n, double scale, uint64_t seed, int32_t device);
+/* NoiseScale of DoAddNoise.m:8-16 from VCtl.BandWidth, NoiseLevel, NEX, B0,
+ * RFreq, RPhase, RSlice, ResFreq, ResPhase, SliceNum. */
+double mrs_noise_scale(double bandwidth, double noise_level, double nex, double b0, double rfreq, double rphase,
+                       double rslice, double res_freq, double res_phase, double slice_num);
+
+/* ---------------------------------------------------------------------------
+ * Local SAR (DoCalSARAtCPU.cpp, SARKernel.h). Per voxel and step with rf on,
+ * E = sum over Tx coils of rfAmp_c * TxE1_c (x, y, z components; the rf
+ * phase does not enter) and VVar.SAR += dt * E^2 per component; at each
+ * VSig.tSample time VSig.SAR(:, s) += sum_k VVar.SAR_k * VObj.ECon_k.
+ * ------------------------------------------------------------------------- */
+typedef struct mrs_sar_obj {
+    int32_t dims[3];          /* VObj.MassDen dims R, C, Z (Z = 1 for 2-D) */
+    const float *ECon;        /* VObj.ECon [R,C,Z,3] */
+    const float *TxE1x, *TxE1y, *TxE1z;  /* VCoi [R,C,Z,TxCoilNum] */
+    int32_t TxCoilNum;
+    int32_t TRNum;
+} mrs_sar_obj;
+typedef struct mrs_sar_sig {
+    float *SARBase;           /* VVar.SAR [R,C,Z,3], running, in/out */
+    int32_t SARi;             /* VVar.SARi, in/out */
+    int32_t reserved;
+    float *SAR;               /* VSig.SAR [R,C,Z,n_samples], accumulated with += */
+    double *Muts;             /* VSig.Muts [n_uts] */
+    const double *tSample;    /* VSig.tSample [n_samples] */
+    double *tRealSample;      /* VSig.tRealSample [n_samples] */
+    int64_t n_samples;
+} mrs_sar_sig;
+/* Runs the SAR event loop over VSeq from VVar's state, writes VVar and the
+ * sampled SAR back.
+ * Replaces: mexFunction, Lib/src/engine/CPUEngine/DoCalSARAtCPU.cpp:63 */
+mrs_status mrs_DoCalSARAtGPU(const mrs_sar_obj *obj, const mrs_vseq *vseq, mrs_vvar *vvar, mrs_sar_sig *sig);
+
 #ifdef __cplusplus
 }
 #endif

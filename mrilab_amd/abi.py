@@ -108,6 +108,30 @@ class ExcitationC(C.Structure):
                 ("grad", C.POINTER(C.c_float)), ("target", C.POINTER(C.c_float))]
 
 
+class ReconDescC(C.Structure):
+    _fields_ = [("res_freq", C.c_int32), ("res_phase", C.c_int32), ("slice_num", C.c_int32),
+                ("te_per_tr", C.c_int32), ("rx", C.c_int32), ("types", C.c_int32), ("zf_kx", C.c_int32),
+                ("zf_ky", C.c_int32), ("zf_kz", C.c_int32), ("swap_xy", C.c_int32)]
+
+
+class VImgC(C.Structure):
+    _fields_ = [("Real", C.c_void_p), ("Imag", C.c_void_p), ("Mag", C.c_void_p), ("Phase", C.c_void_p),
+                ("Sx", C.c_void_p), ("Sy", C.c_void_p)]
+
+
+class SarObjC(C.Structure):
+    _fields_ = [("dims", C.c_int32 * 3), ("ECon", C.POINTER(C.c_float)), ("TxE1x", C.POINTER(C.c_float)),
+                ("TxE1y", C.POINTER(C.c_float)), ("TxE1z", C.POINTER(C.c_float)), ("TxCoilNum", C.c_int32),
+                ("TRNum", C.c_int32)]
+
+
+class SarSigC(C.Structure):
+    _fields_ = [("SARBase", C.POINTER(C.c_float)), ("SARi", C.c_int32), ("reserved", C.c_int32),
+                ("SAR", C.POINTER(C.c_float)), ("Muts", C.POINTER(C.c_double)),
+                ("tSample", C.POINTER(C.c_double)), ("tRealSample", C.POINTER(C.c_double)),
+                ("n_samples", C.c_int64)]
+
+
 class AdjointInfoC(C.Structure):
     _fields_ = [("n_iso", C.c_int64), ("n_steps", C.c_int64), ("checkpoint", C.c_int32),
                 ("n_checkpoints", C.c_int32), ("spins_per_thread", C.c_int32), ("block_size", C.c_int32),
@@ -278,3 +302,28 @@ def unpack_vvar(P: Packed, VVar) -> None:
         VVar[f] = np.float64(getattr(P.vvar, f))
     for f in _VVAR_I32:
         VVar[f] = np.int32(getattr(P.vvar, f))
+
+
+def pack_sar(VObj, VCoi, VCtl, VVar, VSig, n_uts: int):
+    """The SAR engine's structs (DoCalSARAtCPU.cpp:115-166): VObj.MassDen
+    (dims) and ECon [R,C,Z,3]; VCoi.TxE1x/y/z [R,C,Z,Tx]; VCtl.TRNum;
+    VVar.SAR [R,C,Z,3] and SARi; VSig.SAR [R,C,Z,n_samples], Muts [n_uts],
+    tSample and tRealSample [n_samples]. Returns (obj, sig, keep-alive)."""
+    md = np.asarray(VObj["MassDen"])
+    shp = md.shape + (1,) * (3 - md.ndim)
+    nvox = int(np.prod(shp[:3]))
+    Tx = int(_scalar(VCoi, "VCoi", "TxCoilNum", np.int32))
+    ECon = _req(VObj, "VObj", "ECon", np.float32, 3 * nvox)
+    E = [_req(VCoi, "VCoi", f"TxE1{k}", np.float32, nvox * Tx) for k in "xyz"]
+    base = _req(VVar, "VVar", "SAR", np.float32, 3 * nvox, writable=True)
+    tS = _req(VSig, "VSig", "tSample", np.float64)
+    ns = tS.size
+    SAR = _req(VSig, "VSig", "SAR", np.float32, nvox * ns, writable=True)
+    Muts = _req(VSig, "VSig", "Muts", np.float64, n_uts, writable=True)
+    tR = _req(VSig, "VSig", "tRealSample", np.float64, ns, writable=True)
+    obj = SarObjC(ECon=_p(ECon, F32P), TxE1x=_p(E[0], F32P), TxE1y=_p(E[1], F32P), TxE1z=_p(E[2], F32P),
+                  TxCoilNum=Tx, TRNum=int(_scalar(VCtl, "VCtl", "TRNum", np.int32)))
+    obj.dims[0], obj.dims[1], obj.dims[2] = (int(x) for x in shp[:3])
+    sig = SarSigC(SARBase=_p(base, F32P), SARi=int(VVar.get("SARi", 0)), SAR=_p(SAR, F32P),
+                  Muts=_p(Muts, F64P), tSample=_p(tS, F64P), tRealSample=_p(tR, F64P), n_samples=ns)
+    return obj, sig, (ECon, E, base, tS, SAR, Muts, tR)

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmrilab_amd.so")
 ARCH = os.environ.get("MRS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["engine.hip", "multipool.hip", "batch.hip", "adjoint.hip"]
+HIP_SOURCES = ["engine.hip", "multipool.hip", "batch.hip", "adjoint.hip", "recon.hip", "sar.hip"]
 CXX_SOURCES = ["decode.cpp"]
 
 
@@ -44,7 +44,9 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc()))), "lib")
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, f"-L{rocm_lib}", "-lhipfft",
+           f"-Wl,-rpath,{rocm_lib}", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -942,6 +942,23 @@ mrs_status mrs_plan_fetch(mrs_plan *P, mrs_vobj *vobj, mrs_vvar *vvar, mrs_vsig 
     return MRS_OK;
 }
 
+}  // extern "C"
+
+mrs_status mrs::plan_signal(mrs_plan *P, PlanSignal *out) {
+    mrs_status rc = mrs_plan_sync(P);
+    if (rc) return rc;
+    out->sx = (const double *)P->Sx.p;
+    out->sy = (const double *)P->Sy.p;
+    out->n = P->signal_num;
+    out->types = P->Ty;
+    out->rx = P->Rx;
+    out->device = P->device;
+    out->stream = P->stream;
+    return MRS_OK;
+}
+
+extern "C" {
+
 mrs_status mrs_plan_get_info(const mrs_plan *P, mrs_plan_info *info) {
     if (!P || !info) return fail(MRS_ERR_ARG, "NULL argument");
     info->n_steps = P->dec.n_steps;

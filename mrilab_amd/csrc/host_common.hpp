@@ -21,6 +21,16 @@ mrs_status check_args(const mrs_vobj *vo, const mrs_vmag *vm, const mrs_vcoi *vc
 // caller's rfAmp/rfPhase/rfFreq arrays and filling them with rf[0..2].
 void copy_vvar_out(const mrs_vvar &src, const std::vector<double> *rf, int tx, mrs_vvar *dst);
 
+// The k-space a plan's last run left in device memory ([Ty][Rx][n] fp64,
+// as VSig.Sx/Sy), after the plan's stream has drained.
+struct PlanSignal {
+    const double *sx = nullptr, *sy = nullptr;
+    int64_t n = 0;
+    int types = 0, rx = 0, device = 0;
+    hipStream_t stream = nullptr;
+};
+mrs_status plan_signal(mrs_plan *plan, PlanSignal *out);
+
 }  // namespace mrs
 
 #define HIP_TRY(expr)                                                                           \

@@ -44,7 +44,8 @@ EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanA
            "mrs_plan_fetch", "mrs_plan_get_info", "mrs_plan_destroy", "mrs_batch_create", "mrs_batch_run",
            "mrs_batch_sync", "mrs_batch_fetch", "mrs_batch_get_info", "mrs_batch_destroy",
            "mrs_DoScanBatchAtGPU", "mrs_adjoint_create", "mrs_adjoint_forward", "mrs_adjoint_backward",
-           "mrs_adjoint_get_info", "mrs_adjoint_destroy")
+           "mrs_adjoint_get_info", "mrs_adjoint_destroy", "mrs_recon_dims", "mrs_DoCartRecon",
+           "mrs_plan_recon", "mrs_DoAddNoise", "mrs_noise_scale", "mrs_DoCalSARAtGPU")
 
 
 class MRSError(RuntimeError):
@@ -122,6 +123,18 @@ def load() -> C.CDLL:
         lib.mrs_adjoint_get_info.restype = C.c_int
         lib.mrs_adjoint_destroy.argtypes = [C.c_void_p]
         lib.mrs_adjoint_destroy.restype = None
+        lib.mrs_recon_dims.argtypes = [P(abi.ReconDescC), P(C.c_int64)]
+        lib.mrs_recon_dims.restype = C.c_int
+        lib.mrs_DoCartRecon.argtypes = [P(abi.VSigC), P(abi.ReconDescC), P(abi.VImgC), C.c_int32]
+        lib.mrs_DoCartRecon.restype = C.c_int
+        lib.mrs_plan_recon.argtypes = [C.c_void_p, P(abi.ReconDescC), P(abi.VImgC), C.c_int32]
+        lib.mrs_plan_recon.restype = C.c_int
+        lib.mrs_DoAddNoise.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_uint64, C.c_int32]
+        lib.mrs_DoAddNoise.restype = C.c_int
+        lib.mrs_noise_scale.argtypes = [C.c_double] * 10
+        lib.mrs_noise_scale.restype = C.c_double
+        lib.mrs_DoCalSARAtGPU.argtypes = [P(abi.SarObjC), P(abi.VSeqC), P(abi.VVarC), P(abi.SarSigC)]
+        lib.mrs_DoCalSARAtGPU.restype = C.c_int
         _lib = lib
         return lib
 
@@ -262,6 +275,13 @@ class Plan:
                                         C.byref(Q.vsig) if VSig is not None else None))
         if VVar is not None:
             abi.unpack_vvar(Q, VVar)
+
+    def recon(self, desc, device: bool = False) -> dict:
+        """Cartesian reconstruction of the k-space this plan's last run left
+        on the GPU (mrs_plan_recon): numpy arrays, or float64 CUDA tensors
+        when ``device``."""
+        return _recon_call(lambda img, dev: self._lib.mrs_plan_recon(self._h, C.byref(desc), C.byref(img), dev),
+                           desc, device)
 
     def info(self) -> dict:
         inf = abi.PlanInfoC()
@@ -461,3 +481,121 @@ class Adjoint:
 
 def device_count() -> int:
     return int(load().mrs_device_count())
+
+
+# ---------------------------------------------------------------------------
+# Signal to image: Cartesian reconstruction and k-space noise
+# ---------------------------------------------------------------------------
+def swap_xy(scan_plane: str, freq_dir: str) -> bool:
+    """Whether the scan plane and frequency direction select the second
+    permute of the reconstruction (DoCartRecon.m:45-73)."""
+    table = {"Axial": freq_dir == "A/P", "Sagittal": freq_dir == "S/I", "Coronal": freq_dir != "L/R"}
+    if scan_plane not in table:
+        raise ValueError(f"unknown ScanPlane {scan_plane!r}")
+    return table[scan_plane]
+
+
+def recon_desc(res_freq: int, res_phase: int, slice_num: int = 1, te_per_tr: int = 1, rx: int = 1, types: int = 1,
+               zf=None, scan_plane: str = "Axial", freq_dir: str = "R/L") -> abi.ReconDescC:
+    """mrs_recon_desc from VCtl-style values; ``zf`` = (ZF_Kx, ZF_Ky, ZF_Kz
+    factor) or None."""
+    zx, zy, zz = zf if zf is not None else (0, 0, 0)
+    return abi.ReconDescC(res_freq=res_freq, res_phase=res_phase, slice_num=slice_num, te_per_tr=te_per_tr, rx=rx,
+                          types=types, zf_kx=zx, zf_ky=zy, zf_kz=zz,
+                          swap_xy=1 if swap_xy(scan_plane, freq_dir) else 0)
+
+
+def recon_dims(desc) -> tuple:
+    dims = (C.c_int64 * 5)()
+    _check(load().mrs_recon_dims(C.byref(desc), dims))
+    return tuple(int(d) for d in dims)
+
+
+_IMG_FIELDS = ("Real", "Imag", "Mag", "Phase", "Sx", "Sy")
+
+
+def _recon_call(call, desc, device: bool) -> dict:
+    dims = recon_dims(desc)
+    n = int(np.prod(dims))
+    if device:
+        import torch
+        out = {k: torch.empty(n, dtype=torch.float64, device="cuda") for k in _IMG_FIELDS}
+        img = abi.VImgC(**{k: C.c_void_p(v.data_ptr()) for k, v in out.items()})
+        torch.cuda.synchronize()
+        _check(call(img, 1))
+        # column-major [d0, d1, d2, Rx, TEPerTR]: reverse the dims for torch
+        return {k: v.view(*dims[::-1]) for k, v in out.items()}
+    out = {k: np.empty(n, dtype=np.float64) for k in _IMG_FIELDS}
+    img = abi.VImgC(**{k: C.c_void_p(v.ctypes.data) for k, v in out.items()})
+    _check(call(img, 0))
+    return {k: v.reshape(dims, order="F") for k, v in out.items()}
+
+
+def DoCartRecon(Sx, Sy, desc, device: bool = False) -> dict:
+    """VImg from VSig.Sx/Sy (numpy float64 arrays, or float64 CUDA tensors
+    with ``device``); arrays [d0, d1, d2, Rx, TEPerTR] in MATLAB order
+    (numpy order "F"; torch tensors with the dims reversed)."""
+    lib = load()
+    if device:
+        import torch
+        sx, sy = Sx.contiguous(), Sy.contiguous()
+        if sx.dtype != torch.float64 or not sx.is_cuda:
+            raise TypeError("device recon takes float64 CUDA tensors")
+        n = sx.numel() // (desc.rx * desc.types)
+        vs = abi.VSigC(Sx=C.cast(C.c_void_p(sx.data_ptr()), C.POINTER(C.c_double)),
+                       Sy=C.cast(C.c_void_p(sy.data_ptr()), C.POINTER(C.c_double)), SignalNum=n)
+        keep = (sx, sy)
+    else:
+        sx = np.ascontiguousarray(np.asarray(Sx, dtype=np.float64).ravel(order="F"))
+        sy = np.ascontiguousarray(np.asarray(Sy, dtype=np.float64).ravel(order="F"))
+        n = sx.size // (desc.rx * desc.types)
+        vs = abi.VSigC(Sx=sx.ctypes.data_as(C.POINTER(C.c_double)), Sy=sy.ctypes.data_as(C.POINTER(C.c_double)),
+                       SignalNum=n)
+        keep = (sx, sy)
+    out = _recon_call(lambda img, dev: lib.mrs_DoCartRecon(C.byref(vs), C.byref(desc), C.byref(img), dev),
+                      desc, device)
+    del keep
+    return out
+
+
+def DoAddNoise(Sx, Sy, scale: float, seed: int) -> None:
+    """Adds scale * N(0, 1) to Sx and Sy in place (float64 numpy arrays or
+    CUDA tensors of equal size)."""
+    lib = load()
+    if hasattr(Sx, "data_ptr"):
+        if not (Sx.is_contiguous() and Sy.is_contiguous()) or Sx.numel() != Sy.numel():
+            raise ValueError("Sx and Sy must be contiguous tensors of equal size")
+        _check(lib.mrs_DoAddNoise(C.c_void_p(Sx.data_ptr()), C.c_void_p(Sy.data_ptr()), Sx.numel(), float(scale),
+                                  int(seed), 1 if Sx.is_cuda else 0))
+        return
+    for a in (Sx, Sy):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and (a.flags.c_contiguous or a.flags.f_contiguous)):
+            raise TypeError("Sx and Sy must be contiguous float64 numpy arrays")
+    if Sx.size != Sy.size:
+        raise ValueError("Sx and Sy must have equal sizes")
+    _check(lib.mrs_DoAddNoise(C.c_void_p(Sx.ctypes.data), C.c_void_p(Sy.ctypes.data), Sx.size, float(scale),
+                              int(seed), 0))
+
+
+def noise_scale(bandwidth, noise_level, nex, b0, rfreq, rphase, rslice, res_freq, res_phase, slice_num) -> float:
+    """NoiseScale of DoAddNoise.m:8-16 (VCtl fields)."""
+    return float(load().mrs_noise_scale(*[float(x) for x in (bandwidth, noise_level, nex, b0, rfreq, rphase,
+                                                              rslice, res_freq, res_phase, slice_num)]))
+
+
+# ---------------------------------------------------------------------------
+# Local SAR
+# ---------------------------------------------------------------------------
+def DoCalSARAtGPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig) -> None:
+    """Local SAR of the sequence (DoCalSARAtCPU.cpp semantics) on the GPU, in
+    place: VVar.SAR/SARi, the decoder fields of VVar and VSig.SAR, Muts and
+    tRealSample are written. The structs carry the SAR fields
+    (VObj.MassDen/ECon, VCoi.TxE1x/y/z, VVar.SAR/SARi, VSig.SAR/tSample/
+    tRealSample/Muts; see workloads.sar_scan)."""
+    lib = load()
+    P = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig)
+    obj, sig, keep = abi.pack_sar(VObj, VCoi, VCtl, VVar, VSig, int(P.vseq.n_uts))
+    _check(lib.mrs_DoCalSARAtGPU(C.byref(obj), C.byref(P.vseq), C.byref(P.vvar), C.byref(sig)))
+    abi.unpack_vvar(P, VVar)
+    VVar["SARi"] = np.int32(sig.SARi)
+    del keep

@@ -639,3 +639,46 @@ def as_variant(sc: Scan, variant: str, self_rate: float = 0.0) -> Scan:
         raise ValueError(variant)
     out.VObj["K"] = np.asfortranarray(Kn.astype(np.float32))
     return out
+
+
+def sar_scan(base: Scan | None = None, n_tx: int = 2, n_samples: int = 8, seed: int = 5) -> Scan:
+    """A scan with the SAR engine's fields (DoCalSARAtCPU.cpp:115-166) on
+    top of ``base`` (default: the 64x64 GRE): per Tx coil an E field per
+    unit rf amplitude (V/m/T, Gaussian blobs of random sign per component),
+    ECon = sigma/(2*density) ~ U[1e-4, 5e-4], VVar.SAR zero, and
+    ``n_samples`` sample times spread over the sequence (the first at 0)."""
+    sc = (gre2d() if base is None else base).copy()
+    rng = np.random.default_rng(seed)
+    shape = sc.VObj["Mz"].shape[:3] if sc.VObj["Mz"].ndim >= 3 else sc.VObj["Mz"].shape[:2] + (1,)
+    R, C_, Z = (tuple(shape) + (1, 1, 1))[:3]
+    nvox = R * C_ * Z
+    yy, xx, zz = np.meshgrid(np.linspace(-1, 1, R), np.linspace(-1, 1, C_), np.linspace(-1, 1, Z), indexing="ij")
+    E = np.empty((3, R, C_, Z, n_tx))
+    for c in range(n_tx):
+        cy, cx = rng.uniform(-0.7, 0.7, 2)
+        blob = np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2 + 0.3 * zz ** 2) / (2 * 0.4 ** 2))
+        for k in range(3):
+            E[k, ..., c] = rng.choice([-1.0, 1.0]) * rng.uniform(2e5, 1e6) * blob
+    rf = sc.VSeq["rfCoilLine"]
+    if n_tx > 1:  # spread the rf lines over the coils: coil = 1 + (pulse index mod n_tx)
+        sc.VSeq["rfCoilLine"] = (1.0 + (np.arange(rf.size) // 2) % n_tx).astype(np.float64)
+    sc.VCoi["TxCoilNum"] = np.int32(n_tx)
+    for k, name in enumerate("xyz"):
+        sc.VCoi[f"TxE1{name}"] = np.asfortranarray(E[k].astype(np.float32))
+    sc.VCoi["TxCoilmg"] = np.ones((R, C_, Z, n_tx), dtype=np.float32, order=F)
+    sc.VCoi["TxCoilpe"] = np.zeros((R, C_, Z, n_tx), dtype=np.float32, order=F)
+    sc.VObj["MassDen"] = np.full((R, C_, Z), 1000.0, dtype=np.float32, order=F)
+    sc.VObj["ECon"] = np.asfortranarray(rng.uniform(1e-4, 5e-4, (R, C_, Z, 3)).astype(np.float32))
+    for f in ("rfAmp", "rfPhase", "rfFreq"):
+        sc.VVar[f] = np.zeros(n_tx)
+    sc.VVar["SAR"] = np.zeros((R, C_, Z, 3), dtype=np.float32, order=F)
+    sc.VVar["SARi"] = np.int32(0)
+    uts = sc.VSeq["utsLine"]
+    total = float(np.sum(np.diff(uts)[np.diff(uts) > 0]))
+    # the last sample just inside the end: Muts sums dt step by step
+    sc.VSig["tSample"] = np.linspace(0.0, total * (1 - 1e-9), n_samples)
+    sc.VSig["tRealSample"] = np.zeros(n_samples)
+    sc.VSig["SAR"] = np.zeros((R, C_, Z, n_samples), dtype=np.float32, order=F)
+    sc.VSig["Muts"] = np.zeros(uts.size)
+    sc.name = f"sar_{sc.name}_tx{n_tx}"
+    return sc

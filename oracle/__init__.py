@@ -15,6 +15,8 @@ import ctypes as C
 import os
 import subprocess
 
+import numpy as np
+
 from mrilab_amd import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -45,6 +47,8 @@ def _lib(precision: str) -> C.CDLL:
         lib.orc_DoGMScan.restype = C.c_int
         lib.orc_superlorentzian.argtypes = [C.c_double, C.c_double]
         lib.orc_superlorentzian.restype = C.c_double
+        lib.orc_DoCalSAR.argtypes = [P(abi.SarObjC), P(abi.VSeqC), P(abi.VVarC), P(abi.SarSigC)]
+        lib.orc_DoCalSAR.restype = C.c_int
         lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_set_relax_mode.argtypes = [C.c_int]
         _libs[precision] = lib
@@ -161,3 +165,20 @@ def exc_grad(exc, rf, T1, T2, dB0, precision: str = "f64"):
     L = lib.orc_exc_grad(C.byref(s), *[p[1] for p in ps], *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in g])
     del keep, ps
     return L, g[0].reshape(s.n, 2), g[1], g[2], g[3]
+
+
+def sar(sc, precision: str = "f32", inplace: bool = False):
+    """Local SAR (orc_DoCalSAR, DoCalSARAtCPU.cpp restated) of a Scan whose
+    structs carry the SAR fields (workloads.sar_scan); returns the Scan with
+    VVar.SAR/SARi and VSig.SAR/Muts/tRealSample written."""
+    out = sc if inplace else sc.copy()
+    lib = _lib(precision)
+    P = abi.pack_scan(*out.structs)
+    obj, sig, keep = abi.pack_sar(out.VObj, out.VCoi, out.VCtl, out.VVar, out.VSig, int(P.vseq.n_uts))
+    rc = lib.orc_DoCalSAR(C.byref(obj), C.byref(P.vseq), C.byref(P.vvar), C.byref(sig))
+    if rc != 0:
+        raise RuntimeError(f"oracle SAR failed with status {rc}")
+    abi.unpack_vvar(P, out.VVar)
+    out.VVar["SARi"] = np.int32(sig.SARi)
+    del keep
+    return out

@@ -95,3 +95,37 @@ def noise_scale(bandwidth, noise_level, nex, b0, rfreq, rphase, rslice, res_freq
     return ((np.sqrt(bandwidth / bw_ref) * (noise_level / noise_ref))
             / (np.sqrt(nex / nex_ref) * (b0 / b0_ref) * (rfreq * rphase * rslice / vol_ref)
                * np.sqrt(res_freq * res_phase * slice_num / adc_ref)))
+
+
+# Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), vectorised over
+# counters: the generator mrs_DoAddNoise uses, restated for the tests.
+_M0, _M1, _W0, _W1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
+
+
+def philox4x32_10(ctr, key):
+    """ctr: uint32 array [..., 4]; key: (k0, k1). Returns uint32 [..., 4]."""
+    c = [np.asarray(ctr, dtype=np.uint64)[..., i] for i in range(4)]
+    k0, k1 = np.uint64(key[0]), np.uint64(key[1])
+    mask = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = np.uint64(_M0) * c[0]
+        p1 = np.uint64(_M1) * c[2]
+        hi0, lo0 = p0 >> np.uint64(32), p0 & mask
+        hi1, lo1 = p1 >> np.uint64(32), p1 & mask
+        c = [hi1 ^ c[1] ^ k0, lo1, hi0 ^ c[3] ^ k1, lo0]
+        k0 = (k0 + np.uint64(_W0)) & mask
+        k1 = (k1 + np.uint64(_W1)) & mask
+    return np.stack(c, axis=-1).astype(np.uint32)
+
+
+def noise_normals(n: int, seed: int):
+    """The pair of N(0, 1) values mrs_DoAddNoise adds (times scale) to
+    Sx[i] and Sy[i]: Philox of counter (i lo, i hi, 0, 0) and key (seed lo,
+    seed hi), two 53-bit uniforms, Box-Muller in fp64."""
+    i = np.arange(n, dtype=np.uint64)
+    ctr = np.stack([i & np.uint64(0xFFFFFFFF), i >> np.uint64(32), np.zeros_like(i), np.zeros_like(i)], axis=-1)
+    w = philox4x32_10(ctr, (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)).astype(np.float64)
+    u1 = (np.floor(w[:, 0] / 32) * 67108864.0 + np.floor(w[:, 1] / 64) + 0.5) / 9007199254740992.0
+    u2 = (np.floor(w[:, 2] / 32) * 67108864.0 + np.floor(w[:, 3] / 64) + 0.5) / 9007199254740992.0
+    r = np.sqrt(-2.0 * np.log(u1))
+    return r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)
