@@ -253,6 +253,45 @@ def recon_section(plan, sc, reps: int) -> dict:
                     "sample": "numpy restatement of DoCartRecon.m on the fetched signal (pocketfft)"}}
 
 
+def sar_section(threads: int, cpu_tr: int = 64) -> dict:
+    """Local SAR (mrs_DoCalSARAtGPU) of the 3-D bSSFP scan's sequence on its
+    128x128x64 phantom, transmitted on 2 coils in parallel (E fields per
+    coil, ECon), 16 sample times; per call: host event loop, uploads, the
+    voxel kernel, downloads. The oracle runs the same on the first
+    ``cpu_tr`` TRs."""
+    import numpy as np
+    import oracle
+    from mrilab_amd import engine, workloads
+    sc = workloads.sar_scan(workloads.bssfp3d(), n_tx=2, n_samples=16)
+    a = sc.copy()
+    engine.DoCalSARAtGPU(*a.structs)  # warm-up (module load)
+    ts = []
+    for _ in range(3):
+        a = sc.copy()
+        t0 = time.perf_counter()
+        engine.DoCalSARAtGPU(*a.structs)
+        ts.append(time.perf_counter() - t0)
+    gpu_s = float(np.median(ts))
+    steps = int(a.VSig["SAR_rf_steps"])
+    n_tr = int(sc.VCtl["TRNum"])
+    small = workloads.sar_scan(workloads.bssfp3d(n_tr=cpu_tr), n_tx=2, n_samples=4)
+    oracle.set_threads(threads, "f32")
+    t0 = time.perf_counter()
+    o = oracle.sar(small)
+    cpu_s = time.perf_counter() - t0
+    oracle.set_threads(0, "f32")
+    cpu_steps = int(o.VSig["SAR_rf_steps"])
+    nvox = int(np.prod(sc.VObj["MassDen"].shape))
+    log(f"sar: {gpu_s * 1e3:.1f} ms per call ({nvox} voxels, {steps} rf steps of {n_tr} TRs); "
+        f"CPU oracle {cpu_s:.2f} s for {cpu_steps} rf steps")
+    return {"workload": f"128x128x64 voxels, 2 Tx coils in parallel, the bSSFP sequence ({n_tr} TRs, {steps} "
+                        "steps with rf on), 16 SAR samples",
+            "ms_per_call": gpu_s * 1e3, "voxel_rf_steps_per_s": nvox * steps / gpu_s,
+            "cpu": {"seconds": cpu_s, "rf_steps": cpu_steps, "voxel_rf_steps_per_s": nvox * cpu_steps / cpu_s,
+                    "cores": threads, "kind": "port",
+                    "sample": f"first {cpu_tr} TRs, C restatement of DoCalSARAtCPU, OpenMP over voxels"}}
+
+
 def traffic_from_profile() -> float | None:
     """HBM bytes per forward launch measured by rocprofv3 (profiles/)."""
     path = os.path.join(ROOT, "profiles", "bloch_fwd_bssfp3d_traffic.json")
@@ -274,6 +313,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint section")
     ap.add_argument("--no-multipool", action="store_true", help="skip the multi-pool section")
     ap.add_argument("--no-recon", action="store_true", help="skip the image reconstruction section")
+    ap.add_argument("--no-sar", action="store_true", help="skip the local SAR section")
     ap.add_argument("--batch-sets", type=int, default=4096)
     ap.add_argument("--batch-plain-sample", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -372,6 +412,9 @@ def main() -> int:
     if not args.no_multipool:
         mp_threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         line["multipool"] = multipool_section(0.2, 0.04, 64, mp_threads)
+    if not args.no_sar:
+        sar_threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        line["sar"] = sar_section(sar_threads)
     if not args.no_cpu_baseline:
         threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         log(f"cpu baseline: fp32 oracle on the first {args.cpu_sample_tr} TRs, {threads} threads")

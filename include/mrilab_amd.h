@@ -414,6 +414,7 @@ typedef struct mrs_sar_sig {
     const double *tSample;    /* VSig.tSample [n_samples] */
     double *tRealSample;      /* VSig.tRealSample [n_samples] */
     int64_t n_samples;
+    int64_t n_rf_steps;       /* out: steps with rf on that the call summed */
 } mrs_sar_sig;
 /* Runs the SAR event loop over VSeq from VVar's state, writes VVar and the
  * sampled SAR back.

@@ -129,7 +129,7 @@ class SarSigC(C.Structure):
     _fields_ = [("SARBase", C.POINTER(C.c_float)), ("SARi", C.c_int32), ("reserved", C.c_int32),
                 ("SAR", C.POINTER(C.c_float)), ("Muts", C.POINTER(C.c_double)),
                 ("tSample", C.POINTER(C.c_double)), ("tRealSample", C.POINTER(C.c_double)),
-                ("n_samples", C.c_int64)]
+                ("n_samples", C.c_int64), ("n_rf_steps", C.c_int64)]
 
 
 class AdjointInfoC(C.Structure):

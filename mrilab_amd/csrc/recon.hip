@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "host_common.hpp"
@@ -122,6 +123,55 @@ __global__ __launch_bounds__(256) void recon_gather_kernel(GatherParams p) {
     }
 }
 
+// The same gather when the volume is the transposed k-space (no second
+// permute): the source runs along the readout, the output along the phase
+// direction, so a 32x32 tile goes through LDS and both sides stay
+// coalesced. Block 32x8; blockIdx.x walks the (phase, readout) tiles and
+// blockIdx.y the (slice, coil) planes.
+__global__ __launch_bounds__(256) void recon_gather_t_kernel(GatherParams p, int tiles0, int64_t planes) {
+    __shared__ double tx_[32][33], ty_[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int64_t tile = blockIdx.x;
+    const int64_t x0 = (tile % tiles0) * 32;  // j0: phase (output fastest)
+    const int64_t y0 = (tile / tiles0) * 32;  // j1: readout (source fastest)
+    for (int64_t pl = blockIdx.y; pl < planes; pl += gridDim.y) {
+        const int64_t c = pl / p.g.f2, j2 = pl - c * p.g.f2;
+        const int64_t sl = j2 - p.g.o2;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int64_t j0 = x0 + ty + 8 * k, j1 = y0 + tx;
+            const int64_t ro = j1 - p.g.o0, pe = j0 - p.g.o1;
+            double vx = 0.0, vy = 0.0;
+            if (j0 < p.g.f0 && j1 < p.g.f1 && ro >= 0 && ro < p.g.rf && pe >= 0 && pe < p.g.rp && sl >= 0 &&
+                sl < p.g.sl) {
+                const int64_t n = (int64_t)p.e * p.rf_full + ro + (int64_t)p.rf_full * p.te * (pe + p.g.rp * sl);
+                for (int t = 0; t < p.types; t++) {
+                    const int64_t kk = ((int64_t)t * p.rx + c) * p.N + n;
+                    vx += p.sx[kk];
+                    vy += p.sy[kk];
+                }
+            }
+            tx_[ty + 8 * k][tx] = vx;
+            ty_[ty + 8 * k][tx] = vy;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int64_t j0 = x0 + tx, j1 = y0 + ty + 8 * k;
+            if (j0 < p.g.f0 && j1 < p.g.f1) {
+                const double vx = tx_[tx][ty + 8 * k], vy = ty_[tx][ty + 8 * k];
+                const int64_t i = c * p.vol + j0 + p.g.f0 * (j1 + p.g.f1 * j2);
+                p.ksx[i] = vx;
+                p.ksy[i] = vy;
+                const int64_t s0 = (j0 + p.g.f0 / 2) % p.g.f0, s1 = (j1 + p.g.f1 / 2) % p.g.f1,
+                              s2 = (j2 + p.g.f2 / 2) % p.g.f2;
+                p.X[c * p.vol + s0 + p.g.f0 * (s1 + p.g.f1 * s2)] = make_double2(vx, vy);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 struct OutParams {
     const double2 *X;  // ifftn result (unscaled) [Rx][f2][f1][f0]
     double *re, *im, *mag, *ph;
@@ -214,13 +264,48 @@ static unsigned grid_for(int64_t n) {
                                                                     " in " #expr);               \
     } while (0)
 
-struct FftPlan {
-    hipfftHandle h = 0;
+// The last shape's hipFFT plan and FFT buffer, kept for the next call
+// (creating a plan costs tens of ms; a transform of config-1 size well
+// under one). Calls hold the lock for their whole run, so concurrent
+// recons serialise instead of sharing the buffer. Kept until the process
+// ends or the shape changes.
+struct FftCache {
+    std::mutex mu;
     bool made = false;
-    ~FftPlan() {
-        if (made) (void)hipfftDestroy(h);
-    }
+    int device = -1, rx = 0;
+    int64_t f0 = 0, f1 = 0, f2 = 0;
+    hipfftHandle h = 0;
+    void *X = nullptr;
 };
+static FftCache g_fft;
+
+static mrs_status fft_cache_get(int device, const ReconDims &g, int rx) {
+    if (g_fft.made && g_fft.device == device && g_fft.rx == rx && g_fft.f0 == g.f0 && g_fft.f1 == g.f1 &&
+        g_fft.f2 == g.f2)
+        return MRS_OK;
+    if (g_fft.made) {
+        (void)hipfftDestroy(g_fft.h);
+        if (g_fft.X) (void)hipFree(g_fft.X);
+        g_fft.made = false;
+        g_fft.X = nullptr;
+    }
+    const int64_t vol = g.f0 * g.f1 * g.f2;
+    HIP_TRY(hipMalloc(&g_fft.X, (size_t)(vol * rx) * sizeof(double2)));
+    int n[3] = {(int)g.f2, (int)g.f1, (int)g.f0};  // row-major dims: f0 varies fastest
+    const hipfftResult r = hipfftPlanMany(&g_fft.h, 3, n, nullptr, 1, (int)vol, nullptr, 1, (int)vol, HIPFFT_Z2Z, rx);
+    if (r != HIPFFT_SUCCESS) {
+        (void)hipFree(g_fft.X);
+        g_fft.X = nullptr;
+        return fail(MRS_ERR_DEVICE, "hipfftPlanMany failed with " + std::to_string((int)r));
+    }
+    g_fft.made = true;
+    g_fft.device = device;
+    g_fft.rx = rx;
+    g_fft.f0 = g.f0;
+    g_fft.f1 = g.f1;
+    g_fft.f2 = g.f2;
+    return MRS_OK;
+}
 
 // Reconstruction from device-resident k-space sx/sy ([Ty][Rx][N]) into the
 // VImg arrays (device pointers when dev_out, else host).
@@ -236,8 +321,7 @@ static mrs_status recon_device(const double *sx, const double *sy, int64_t N, co
     const int64_t vol = g.f0 * g.f1 * g.f2, per_echo = vol * d->rx;
     if (g.f0 > INT32_MAX || g.f1 > INT32_MAX || g.f2 > INT32_MAX || per_echo > ((int64_t)1 << 40))
         return fail(MRS_ERR_UNSUPPORTED, "image volume too large");
-    RDev X, o[6];
-    if ((rc = rmalloc(X, (size_t)per_echo * sizeof(double2)))) return rc;
+    RDev o[6];
     double *out[6];
     double *const dst[6] = {img->Real, img->Imag, img->Mag, img->Phase, img->Sx, img->Sy};
     for (int k = 0; k < 6; k++) {
@@ -248,11 +332,12 @@ static mrs_status recon_device(const double *sx, const double *sy, int64_t N, co
             out[k] = (double *)o[k].p;
         }
     }
-    FftPlan plan;
-    int n[3] = {(int)g.f2, (int)g.f1, (int)g.f0};  // row-major dims: f0 varies fastest
-    FFT_TRY(hipfftPlanMany(&plan.h, 3, n, nullptr, 1, (int)vol, nullptr, 1, (int)vol, HIPFFT_Z2Z, d->rx));
-    plan.made = true;
-    FFT_TRY(hipfftSetStream(plan.h, st));
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_fft.mu);
+    if ((rc = fft_cache_get(dev, g, d->rx))) return rc;
+    FFT_TRY(hipfftSetStream(g_fft.h, st));
+    struct { void *p; } X{g_fft.X};
     for (int e = 0; e < d->te_per_tr; e++) {
         const int64_t off = (dev_out ? (int64_t)e * per_echo : 0);
         GatherParams gp;
@@ -270,9 +355,16 @@ static mrs_status recon_device(const double *sx, const double *sy, int64_t N, co
         gp.rx = d->rx;
         gp.types = d->types;
         gp.swap = d->swap_xy ? 1 : 0;
-        recon_gather_kernel<<<grid_for(per_echo), 256, 0, st>>>(gp);
+        if (d->swap_xy) {
+            recon_gather_kernel<<<grid_for(per_echo), 256, 0, st>>>(gp);
+        } else {
+            const int64_t t0 = (g.f0 + 31) / 32, t1 = (g.f1 + 31) / 32, planes = g.f2 * d->rx;
+            if (t0 * t1 > INT32_MAX) return fail(MRS_ERR_UNSUPPORTED, "image plane too large");
+            const dim3 grid((unsigned)(t0 * t1), (unsigned)std::min<int64_t>(planes, 65535));
+            recon_gather_t_kernel<<<grid, 256, 0, st>>>(gp, (int)t0, planes);
+        }
         HIP_TRY(hipGetLastError());
-        FFT_TRY(hipfftExecZ2Z(plan.h, (hipfftDoubleComplex *)X.p, (hipfftDoubleComplex *)X.p, HIPFFT_BACKWARD));
+        FFT_TRY(hipfftExecZ2Z(g_fft.h, (hipfftDoubleComplex *)X.p, (hipfftDoubleComplex *)X.p, HIPFFT_BACKWARD));
         OutParams op;
         op.X = (const double2 *)X.p;
         op.re = out[0] + off;

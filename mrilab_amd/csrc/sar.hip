@@ -233,8 +233,10 @@ mrs_status mrs_DoCalSARAtGPU(const mrs_sar_obj *o, const mrs_vseq *vs, mrs_vvar 
         return fail(MRS_ERR_ARG, "VVar.utsi/rfi or SARi out of range");
     const int64_t nvox = (int64_t)o->dims[0] * o->dims[1] * o->dims[2];
     SarDecoded d;
+    g->n_rf_steps = 0;
     mrs_status rc = sar_decode(o, vs, vv, g, d);
     if (rc) return rc;
+    g->n_rf_steps = d.n_steps;
     if (d.rec.empty()) return MRS_OK;  // nothing reaches the voxels
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));

@@ -598,4 +598,5 @@ def DoCalSARAtGPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig) -> None:
     _check(lib.mrs_DoCalSARAtGPU(C.byref(obj), C.byref(P.vseq), C.byref(P.vvar), C.byref(sig)))
     abi.unpack_vvar(P, VVar)
     VVar["SARi"] = np.int32(sig.SARi)
+    VSig["SAR_rf_steps"] = int(sig.n_rf_steps)  # steps with rf on in this call
     del keep

@@ -659,9 +659,22 @@ def sar_scan(base: Scan | None = None, n_tx: int = 2, n_samples: int = 8, seed: 
         blob = np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2 + 0.3 * zz ** 2) / (2 * 0.4 ** 2))
         for k in range(3):
             E[k, ..., c] = rng.choice([-1.0, 1.0]) * rng.uniform(2e5, 1e6) * blob
-    rf = sc.VSeq["rfCoilLine"]
-    if n_tx > 1:  # spread the rf lines over the coils: coil = 1 + (pulse index mod n_tx)
-        sc.VSeq["rfCoilLine"] = (1.0 + (np.arange(rf.size) // 2) % n_tx).astype(np.float64)
+    if n_tx > 1:
+        # parallel transmit: every rf event is sent on all coils at once
+        # (multi-line rf: n_tx coincident rf events, one per coil, coil c
+        # with amplitude weight w_c and phase offset p_c)
+        q = sc.VSeq
+        fl = np.asarray(q["flagsLine"]).reshape(6, -1, order="F")
+        rep = np.where(fl[0] > 0, n_tx, 1)
+        q["tsLine"] = np.repeat(q["tsLine"], rep)
+        q["flagsLine"] = np.asfortranarray(np.repeat(fl, rep, axis=1))
+        w = rng.uniform(0.5, 1.5, n_tx)
+        ph = rng.uniform(-np.pi, np.pi, n_tx)
+        n_rf = q["rfAmpLine"].size
+        q["rfAmpLine"] = (np.repeat(q["rfAmpLine"], n_tx).reshape(n_rf, n_tx) * w).ravel()
+        q["rfPhaseLine"] = (np.repeat(q["rfPhaseLine"], n_tx).reshape(n_rf, n_tx) + ph).ravel()
+        q["rfFreqLine"] = np.repeat(q["rfFreqLine"], n_tx)
+        q["rfCoilLine"] = np.tile(np.arange(1.0, n_tx + 1.0), n_rf)
     sc.VCoi["TxCoilNum"] = np.int32(n_tx)
     for k, name in enumerate("xyz"):
         sc.VCoi[f"TxE1{name}"] = np.asfortranarray(E[k].astype(np.float32))

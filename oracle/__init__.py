@@ -180,5 +180,6 @@ def sar(sc, precision: str = "f32", inplace: bool = False):
         raise RuntimeError(f"oracle SAR failed with status {rc}")
     abi.unpack_vvar(P, out.VVar)
     out.VVar["SARi"] = np.int32(sig.SARi)
+    out.VSig["SAR_rf_steps"] = int(sig.n_rf_steps)  # steps with rf on in this call
     del keep
     return out

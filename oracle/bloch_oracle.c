@@ -38,6 +38,7 @@ static int g_threads = 0; /* 0: min(slices, MaxThreadNum) like the reference */
 static int g_relax = 0;   /* 0: reference order; 1: expm1 form (error study only) */
 
 void orc_set_threads(int n) { g_threads = n; }
+int orc_get_threads(void) { return g_threads; }
 void orc_set_relax_mode(int m) { g_relax = m; }
 
 int orc_real_size(void) { return (int)sizeof(real); }

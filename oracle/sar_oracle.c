@@ -24,15 +24,18 @@
 #endif
 typedef ORC_REAL real;
 
+int orc_get_threads(void); /* bloch_oracle.c: threads set by orc_set_threads, 0 = one */
+
 /* SARKernel.h:8-63 over all voxels (the reference splits them by slice
  * across OpenMP threads; each voxel is independent). */
-static void sar_kernel(const mrs_sar_obj *o, float *base, int64_t nvox, real dt, const double *rfAmp) {
+static int sar_kernel(const mrs_sar_obj *o, float *base, int64_t nvox, real dt, const double *rfAmp) {
     const int Tx = o->TxCoilNum;
     float amp_sum = 0.f; /* SARKernel.h:13-19 */
     for (int i = 0; i < Tx; i++) amp_sum += fabsf((float)rfAmp[i]);
-    if (amp_sum == 0.f) return;
+    if (amp_sum == 0.f) return 0;
     int64_t v;
-#pragma omp parallel for schedule(static)
+    const int nth = orc_get_threads() > 0 ? orc_get_threads() : 1;
+#pragma omp parallel for num_threads(nth) schedule(static)
     for (v = 0; v < nvox; v++) {
         real bx = 0, by = 0, bz = 0; /* ippsZero_32f :30-32 */
         for (int i = 0; i < Tx; i++) { /* :34-45 */
@@ -53,6 +56,7 @@ static void sar_kernel(const mrs_sar_obj *o, float *base, int64_t nvox, real dt,
         base[nvox + v] = (float)((real)base[nvox + v] + by);
         base[2 * nvox + v] = (float)((real)base[2 * nvox + v] + bz);
     }
+    return 1;
 }
 
 /* DoCalSARAtCPU.cpp:223-238 / :329-353: store one SAR sample. */
@@ -79,6 +83,7 @@ int orc_DoCalSAR(const mrs_sar_obj *o, const mrs_vseq *vs, mrs_vvar *vv, mrs_sar
     const int Tx = o->TxCoilNum;
     int64_t i = 0;
     double flag[6];
+    g->n_rf_steps = 0;
     while (i < MaxStep - 1) { /* :191 */
         for (int f = 0; f < 6; f++) flag[f] = 0;
         if (vs->tsLine[i] != vs->tsLine[i + 1]) { /* :200-236: sum the flags of coincident events */
@@ -137,7 +142,7 @@ int orc_DoCalSAR(const mrs_sar_obj *o, const mrs_vseq *vs, mrs_vvar *vv, mrs_sar
             if (vv->utsi > 0) g->Muts[vv->utsi] = g->Muts[vv->utsi - 1];
             continue;
         }
-        sar_kernel(o, g->SARBase, nvox, (real)(float)vv->dt, vv->rfAmp); /* :315-340 */
+        g->n_rf_steps += sar_kernel(o, g->SARBase, nvox, (real)(float)vv->dt, vv->rfAmp); /* :315-340 */
         if (vv->utsi > 0) g->Muts[vv->utsi] = g->Muts[vv->utsi - 1] + vv->dt; /* :342 */
         /* :344-354 sample after the step, no miss for the end point */
         if (g->SARi < MaxSARStep) {

@@ -29,6 +29,7 @@ def _compare(a, b, tag):
     assert np.array_equal(x, y), f"{tag} VSig.SAR max diff {np.abs(x - y).max():.3e}"
     for k in ("Muts", "tRealSample"):
         assert np.array_equal(a.VSig[k], b.VSig[k]), f"{tag} {k}"
+    assert a.VSig["SAR_rf_steps"] == b.VSig["SAR_rf_steps"] > 0, tag
     for k in ("SARi", "utsi", "rfi", "TRCount"):
         assert int(a.VVar[k]) == int(b.VVar[k]), f"{tag} {k}"
     for k in ("rfAmp", "rfPhase", "rfFreq"):
