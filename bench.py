@@ -87,8 +87,9 @@ def batched_section(n_sets: int, plain_sample: int, reps: int) -> dict:
     t_create = time.perf_counter() - t0
     del inputs
     info = b.info()
-    log(f"batched: create {t_create:.2f} s (decode {info['decode_ms']:.0f} ms, {info['n_candidates']} shapes "
-        f"tuned, best spt {info['spins_per_thread']} block {info['block_size']})")
+    log(f"batched: create {t_create:.2f} s (library {info['create_ms']:.0f} ms: decode {info['decode_ms']:.0f} ms, "
+        f"tuning {info['tune_ms']:.0f} ms over {info['n_candidates']} shapes; best spt {info['spins_per_thread']} "
+        f"block {info['block_size']})")
     b.run()
     b.sync()
     run_ms = []
@@ -123,6 +124,7 @@ def batched_section(n_sets: int, plain_sample: int, reps: int) -> dict:
             "isochromat_steps": iso_steps, "value": iso_steps / wall_run, "unit": "isochromat-steps/s",
             "ms_per_batch_run": wall_run * 1e3, "device_ms_per_run": float(np.median(run_ms)),
             "end_to_end_s": batched_e2e, "create_s": t_create, "decode_ms": info["decode_ms"],
+            "tune_ms": info["tune_ms"], "library_create_ms": info["create_ms"], "fetch_s": t_fetch,
             "plain_per_scan_ms": float(np.mean(plain)) * 1e3, "plain_sample": int(plain_sample),
             "plain_total_s_estimated": plain_total,
             "speedup": plain_total / wall_run, "speedup_end_to_end": plain_total / batched_e2e,

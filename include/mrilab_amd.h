@@ -287,6 +287,8 @@ typedef struct mrs_batch_info {
     float   last_run_ms;      /* timing = 1: device time of the last run */
     float   tune_best_ms, tune_worst_ms;
     double  decode_ms;        /* host time spent decoding the sequences at create */
+    double  tune_ms;          /* host time spent in the autotuner at create */
+    double  create_ms;        /* host time of mrs_batch_create as a whole */
 } mrs_batch_info;
 mrs_status mrs_batch_get_info(const mrs_batch *batch, mrs_batch_info *info);
 void mrs_batch_destroy(mrs_batch *batch);

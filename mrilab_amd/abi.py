@@ -99,7 +99,8 @@ class BatchInfoC(C.Structure):
                 ("n_iso_steps", C.c_int64), ("n_adc", C.c_int64),
                 ("spins_per_thread", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32),
                 ("tuned", C.c_int32), ("n_candidates", C.c_int32), ("last_run_ms", C.c_float),
-                ("tune_best_ms", C.c_float), ("tune_worst_ms", C.c_float), ("decode_ms", C.c_double)]
+                ("tune_best_ms", C.c_float), ("tune_worst_ms", C.c_float), ("decode_ms", C.c_double),
+                ("tune_ms", C.c_double), ("create_ms", C.c_double)]
 
 
 class ExcitationC(C.Structure):

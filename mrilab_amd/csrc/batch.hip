@@ -295,7 +295,7 @@ struct mrs_batch {
     bool tuned = false;
     int n_candidates = 0;
     float tune_best = 0.f, tune_worst = 0.f;
-    double decode_ms = 0;
+    double decode_ms = 0, tune_ms = 0, create_ms = 0;
     float last_ms = 0.f;
     bool ran = false;
     std::vector<Decoded> dec;
@@ -453,6 +453,7 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
                             mrs_vvar *const *vvar, mrs_vsig *const *vsig, const mrs_options *opt, mrs_batch **out) {
     if (!out) return fail(MRS_ERR_ARG, "out is NULL");
     *out = nullptr;
+    const auto t_create = std::chrono::steady_clock::now();
     if (n <= 0) return fail(MRS_ERR_ARG, "n_scans must be positive");
     if (!vobj || !vmag || !vcoi || !vctl || !vseq || !vvar || !vsig) return fail(MRS_ERR_ARG, "NULL array argument");
     mrs_status rc;
@@ -635,11 +636,15 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
     if (spts.empty()) return fail(MRS_ERR_ARG, "spins_per_thread must be 1, 2, 4 or 8 and divide every SpinNum");
     if (blocks.empty()) return fail(MRS_ERR_ARG, "block_size must be 64, 128 or 256");
     int best_spt = spts[0], best_block = blocks[0];
+    const auto t_tune = std::chrono::steady_clock::now();
     if (spts.size() * blocks.size() > 1) {
         float best = 1e30f, worst = 0.f;
         for (int s : spts) {
             B->spt = s;
             if ((rc = batch_layout(B, s))) return rc;
+            // a shape with fewer tiles than the device has SIMDs leaves SIMDs
+            // idle that a smaller spins-per-thread would use: not timed
+            if (s != spts[0] && B->n_tiles < (int64_t)B->cus * 4) continue;
             for (int b : blocks) {
                 float ms = 0.f;
                 if ((rc = batch_time(B, b, 2, &ms))) return rc;
@@ -654,12 +659,14 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
         }
         B->tuned = true;
         B->tune_best = best;
+        B->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_tune).count();
         B->tune_worst = worst;
     }
     B->spt = best_spt;
     B->block = best_block;
     if ((rc = batch_layout(B, best_spt))) return rc;
     if ((rc = batch_capture(B))) return rc;
+    B->create_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_create).count();
     guard.b = nullptr;
     *out = B;
     return MRS_OK;
@@ -749,6 +756,8 @@ mrs_status mrs_batch_get_info(const mrs_batch *B, mrs_batch_info *info) {
     info->tune_best_ms = B->tune_best;
     info->tune_worst_ms = B->tune_worst;
     info->decode_ms = B->decode_ms;
+    info->tune_ms = B->tune_ms;
+    info->create_ms = B->create_ms;
     return MRS_OK;
 }
 
