@@ -16,35 +16,40 @@ import numpy as np
 
 F32P = C.POINTER(C.c_float)
 F64P = C.POINTER(C.c_double)
+# Array fields of the input/output structs are declared as raw addresses
+# (the same size as the C pointers): filling them from numpy is a plain
+# integer store instead of a ctypes cast per field. VVarC keeps typed
+# pointers, since Ext callbacks receive it.
+AP = C.c_void_p
 
 
 class VObjC(C.Structure):
     _fields_ = [("Gyro", C.c_double), ("SpinNum", C.c_int32), ("TypeNum", C.c_int32),
                 ("dims", C.c_int32 * 3),
-                ("Mx", F32P), ("My", F32P), ("Mz", F32P),
-                ("Rho", F32P), ("T1", F32P), ("T2", F32P),
-                ("K", F32P), ("TypeFlag", F64P)]
+                ("Mx", AP), ("My", AP), ("Mz", AP),
+                ("Rho", AP), ("T1", AP), ("T2", AP),
+                ("K", AP), ("TypeFlag", AP)]
 
 
 class VMagC(C.Structure):
-    _fields_ = [("dB0", F32P), ("dWRnd", F32P), ("Gzgrid", F32P), ("Gygrid", F32P),
-                ("Gxgrid", F32P)]
+    _fields_ = [("dB0", AP), ("dWRnd", AP), ("Gzgrid", AP), ("Gygrid", AP),
+                ("Gxgrid", AP)]
 
 
 class VCoiC(C.Structure):
-    _fields_ = [("TxCoilmg", F32P), ("TxCoilpe", F32P), ("RxCoilx", F32P), ("RxCoily", F32P),
+    _fields_ = [("TxCoilmg", AP), ("TxCoilpe", AP), ("RxCoilx", AP), ("RxCoily", AP),
                 ("TxCoilNum", C.c_int32), ("RxCoilNum", C.c_int32),
                 ("TxCoilDefault", C.c_double), ("RxCoilDefault", C.c_double)]
 
 
 class VCtlC(C.Structure):
-    _fields_ = [("CS", F64P), ("TRNum", C.c_int32), ("RunMode", C.c_int32),
+    _fields_ = [("CS", AP), ("TRNum", C.c_int32), ("RunMode", C.c_int32),
                 ("MaxThreadNum", C.c_int32), ("GPUIndex", C.c_int32),
                 ("MultiTransmit", C.c_int32), ("MasterTxCoil", C.c_int32)]
 
 
 class VSeqC(C.Structure):
-    _fields_ = [(n, F64P) for n in ("utsLine", "tsLine", "rfAmpLine", "rfPhaseLine",
+    _fields_ = [(n, AP) for n in ("utsLine", "tsLine", "rfAmpLine", "rfPhaseLine",
                                     "rfFreqLine", "rfCoilLine", "GzAmpLine", "GyAmpLine",
                                     "GxAmpLine", "ADCLine", "ExtLine", "flagsLine")] + \
               [(n, C.c_int64) for n in ("n_ts", "n_uts", "n_rf", "n_Gz", "n_Gy", "n_Gx",
@@ -65,9 +70,9 @@ class VVarC(C.Structure):
 
 
 class VSigC(C.Structure):
-    _fields_ = [("Sx", F64P), ("Sy", F64P), ("Kz", F64P), ("Ky", F64P), ("Kx", F64P),
+    _fields_ = [("Sx", AP), ("Sy", AP), ("Kz", AP), ("Ky", AP), ("Kx", AP),
                 ("SignalNum", C.c_int32),
-                ("Mx", F32P), ("My", F32P), ("Mz", F32P), ("Muts", F64P)]
+                ("Mx", AP), ("My", AP), ("Mz", AP), ("Muts", AP)]
 
 
 GPU_FETCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_int))
@@ -121,15 +126,15 @@ class VImgC(C.Structure):
 
 
 class SarObjC(C.Structure):
-    _fields_ = [("dims", C.c_int32 * 3), ("ECon", C.POINTER(C.c_float)), ("TxE1x", C.POINTER(C.c_float)),
-                ("TxE1y", C.POINTER(C.c_float)), ("TxE1z", C.POINTER(C.c_float)), ("TxCoilNum", C.c_int32),
+    _fields_ = [("dims", C.c_int32 * 3), ("ECon", AP), ("TxE1x", AP),
+                ("TxE1y", AP), ("TxE1z", AP), ("TxCoilNum", C.c_int32),
                 ("TRNum", C.c_int32)]
 
 
 class SarSigC(C.Structure):
-    _fields_ = [("SARBase", C.POINTER(C.c_float)), ("SARi", C.c_int32), ("reserved", C.c_int32),
-                ("SAR", C.POINTER(C.c_float)), ("Muts", C.POINTER(C.c_double)),
-                ("tSample", C.POINTER(C.c_double)), ("tRealSample", C.POINTER(C.c_double)),
+    _fields_ = [("SARBase", AP), ("SARi", C.c_int32), ("reserved", C.c_int32),
+                ("SAR", AP), ("Muts", AP),
+                ("tSample", AP), ("tRealSample", AP),
                 ("n_samples", C.c_int64), ("n_rf_steps", C.c_int64)]
 
 
@@ -179,6 +184,11 @@ def _scalar(struct, sname, field, dtype):
     return a.reshape(-1)[0].item()
 
 
+def _a(a: np.ndarray | None):
+    """Address of a numpy buffer (None for NULL) for an AP field."""
+    return None if a is None else a.ctypes.data
+
+
 def _p(a: np.ndarray | None, typ):
     if a is None:
         return C.cast(None, typ)
@@ -219,28 +229,28 @@ def pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool: bool = False,
             TF = _req(VObj, "VObj", "TypeFlag", np.float64, Ty)
     vobj = VObjC(Gyro=float(_scalar(VObj, "VObj", "Gyro", np.float64)), SpinNum=S, TypeNum=Ty)
     vobj.dims[0], vobj.dims[1], vobj.dims[2] = R, Cc, Z
-    vobj.Mx, vobj.My, vobj.Mz = _p(Mx, F32P), _p(My, F32P), _p(Mz, F32P)
-    vobj.Rho, vobj.T1, vobj.T2 = _p(Rho, F32P), _p(T1, F32P), _p(T2, F32P)
-    vobj.K, vobj.TypeFlag = _p(K, F32P), _p(TF, F64P)
+    vobj.Mx, vobj.My, vobj.Mz = _a(Mx), _a(My), _a(Mz)
+    vobj.Rho, vobj.T1, vobj.T2 = _a(Rho), _a(T1), _a(T2)
+    vobj.K, vobj.TypeFlag = _a(K), _a(TF)
 
-    vmag = VMagC(dB0=_p(_req(VMag, "VMag", "dB0", np.float32, nvox), F32P),
-                 dWRnd=_p(_req(VMag, "VMag", "dWRnd", np.float32, nvox * S * Ty), F32P),
-                 Gzgrid=_p(_req(VMag, "VMag", "Gzgrid", np.float32, nvox), F32P),
-                 Gygrid=_p(_req(VMag, "VMag", "Gygrid", np.float32, nvox), F32P),
-                 Gxgrid=_p(_req(VMag, "VMag", "Gxgrid", np.float32, nvox), F32P))
+    vmag = VMagC(dB0=_a(_req(VMag, "VMag", "dB0", np.float32, nvox)),
+                 dWRnd=_a(_req(VMag, "VMag", "dWRnd", np.float32, nvox * S * Ty)),
+                 Gzgrid=_a(_req(VMag, "VMag", "Gzgrid", np.float32, nvox)),
+                 Gygrid=_a(_req(VMag, "VMag", "Gygrid", np.float32, nvox)),
+                 Gxgrid=_a(_req(VMag, "VMag", "Gxgrid", np.float32, nvox)))
 
     Tx = int(_scalar(VCoi, "VCoi", "TxCoilNum", np.int32))
     Rx = int(_scalar(VCoi, "VCoi", "RxCoilNum", np.int32))
-    vcoi = VCoiC(TxCoilmg=_p(_req(VCoi, "VCoi", "TxCoilmg", np.float32, nvox * Tx), F32P),
-                 TxCoilpe=_p(_req(VCoi, "VCoi", "TxCoilpe", np.float32, nvox * Tx), F32P),
-                 RxCoilx=_p(_req(VCoi, "VCoi", "RxCoilx", np.float32, nvox * Rx), F32P),
-                 RxCoily=_p(_req(VCoi, "VCoi", "RxCoily", np.float32, nvox * Rx), F32P),
+    vcoi = VCoiC(TxCoilmg=_a(_req(VCoi, "VCoi", "TxCoilmg", np.float32, nvox * Tx)),
+                 TxCoilpe=_a(_req(VCoi, "VCoi", "TxCoilpe", np.float32, nvox * Tx)),
+                 RxCoilx=_a(_req(VCoi, "VCoi", "RxCoilx", np.float32, nvox * Rx)),
+                 RxCoily=_a(_req(VCoi, "VCoi", "RxCoily", np.float32, nvox * Rx)),
                  TxCoilNum=Tx, RxCoilNum=Rx,
                  TxCoilDefault=float(_scalar(VCoi, "VCoi", "TxCoilDefault", np.float64)),
                  RxCoilDefault=float(_scalar(VCoi, "VCoi", "RxCoilDefault", np.float64)))
 
     mt = VCtl.get("MultiTransmit", "off")
-    vctl = VCtlC(CS=_p(_req(VCtl, "VCtl", "CS", np.float64, Ty), F64P),
+    vctl = VCtlC(CS=_a(_req(VCtl, "VCtl", "CS", np.float64, Ty)),
                  TRNum=int(_scalar(VCtl, "VCtl", "TRNum", np.int32)),
                  RunMode=int(_scalar(VCtl, "VCtl", "RunMode", np.int32)),
                  MaxThreadNum=int(_scalar(VCtl, "VCtl", "MaxThreadNum", np.int32)),
@@ -257,7 +267,7 @@ def pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool: bool = False,
         raise ValueError("VSeq.flagsLine must be 6 x numel(tsLine)")
     if seq_fields["rfCoilLine"].size != seq_fields["rfAmpLine"].size:
         raise ValueError("VSeq.rfCoilLine and rfAmpLine differ in length")
-    vseq = VSeqC(**{f: _p(a, F64P) for f, a in seq_fields.items()},
+    vseq = VSeqC(**{f: _a(a) for f, a in seq_fields.items()},
                  n_ts=n_ts, n_uts=seq_fields["utsLine"].size, n_rf=seq_fields["rfAmpLine"].size,
                  n_Gz=seq_fields["GzAmpLine"].size, n_Gy=seq_fields["GyAmpLine"].size,
                  n_Gx=seq_fields["GxAmpLine"].size, n_ADC=seq_fields["ADCLine"].size,
@@ -280,17 +290,17 @@ def pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool: bool = False,
     N = int(_scalar(VSig, "VSig", "SignalNum", np.int32))
     Sx = _req(VSig, "VSig", "Sx", np.float64, N * Ty * Rx, writable=True)
     Sy = _req(VSig, "VSig", "Sy", np.float64, N * Ty * Rx, writable=True)
-    vsig = VSigC(Sx=_p(Sx, F64P), Sy=_p(Sy, F64P),
-                 Kz=_p(_req(VSig, "VSig", "Kz", np.float64, N, writable=True), F64P),
-                 Ky=_p(_req(VSig, "VSig", "Ky", np.float64, N, writable=True), F64P),
-                 Kx=_p(_req(VSig, "VSig", "Kx", np.float64, N, writable=True), F64P),
+    vsig = VSigC(Sx=_a(Sx), Sy=_a(Sy),
+                 Kz=_a(_req(VSig, "VSig", "Kz", np.float64, N, writable=True)),
+                 Ky=_a(_req(VSig, "VSig", "Ky", np.float64, N, writable=True)),
+                 Kx=_a(_req(VSig, "VSig", "Kx", np.float64, N, writable=True)),
                  SignalNum=N)
     if vctl.RunMode == 1:
         nuts = seq_fields["utsLine"].size
-        vsig.Mx = _p(_req(VSig, "VSig", "Mx", np.float32, Mz.size * nuts, writable=True), F32P)
-        vsig.My = _p(_req(VSig, "VSig", "My", np.float32, Mz.size * nuts, writable=True), F32P)
-        vsig.Mz = _p(_req(VSig, "VSig", "Mz", np.float32, Mz.size * nuts, writable=True), F32P)
-        vsig.Muts = _p(_req(VSig, "VSig", "Muts", np.float64, nuts, writable=True), F64P)
+        vsig.Mx = _a(_req(VSig, "VSig", "Mx", np.float32, Mz.size * nuts, writable=True))
+        vsig.My = _a(_req(VSig, "VSig", "My", np.float32, Mz.size * nuts, writable=True))
+        vsig.Mz = _a(_req(VSig, "VSig", "Mz", np.float32, Mz.size * nuts, writable=True))
+        vsig.Muts = _a(_req(VSig, "VSig", "Muts", np.float64, nuts, writable=True))
     P.vobj, P.vmag, P.vcoi, P.vctl, P.vseq, P.vvar, P.vsig = vobj, vmag, vcoi, vctl, vseq, vvar, vsig
     P.keep += [VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, seq_fields, K, TF]
     return P
@@ -322,9 +332,9 @@ def pack_sar(VObj, VCoi, VCtl, VVar, VSig, n_uts: int):
     SAR = _req(VSig, "VSig", "SAR", np.float32, nvox * ns, writable=True)
     Muts = _req(VSig, "VSig", "Muts", np.float64, n_uts, writable=True)
     tR = _req(VSig, "VSig", "tRealSample", np.float64, ns, writable=True)
-    obj = SarObjC(ECon=_p(ECon, F32P), TxE1x=_p(E[0], F32P), TxE1y=_p(E[1], F32P), TxE1z=_p(E[2], F32P),
+    obj = SarObjC(ECon=_a(ECon), TxE1x=_a(E[0]), TxE1y=_a(E[1]), TxE1z=_a(E[2]),
                   TxCoilNum=Tx, TRNum=int(_scalar(VCtl, "VCtl", "TRNum", np.int32)))
     obj.dims[0], obj.dims[1], obj.dims[2] = (int(x) for x in shp[:3])
-    sig = SarSigC(SARBase=_p(base, F32P), SARi=int(VVar.get("SARi", 0)), SAR=_p(SAR, F32P),
-                  Muts=_p(Muts, F64P), tSample=_p(tS, F64P), tRealSample=_p(tR, F64P), n_samples=ns)
+    sig = SarSigC(SARBase=_a(base), SARi=int(VVar.get("SARi", 0)), SAR=_a(SAR),
+                  Muts=_a(Muts), tSample=_a(tS), tRealSample=_a(tR), n_samples=ns)
     return obj, sig, (ECon, E, base, tS, SAR, Muts, tR)

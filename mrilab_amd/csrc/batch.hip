@@ -642,9 +642,10 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
         for (int s : spts) {
             B->spt = s;
             if ((rc = batch_layout(B, s))) return rc;
-            // a shape with fewer tiles than the device has SIMDs leaves SIMDs
-            // idle that a smaller spins-per-thread would use: not timed
-            if (s != spts[0] && B->n_tiles < (int64_t)B->cus * 4) continue;
+            // a shape with fewer active threads than one wave per SIMD
+            // leaves SIMDs idle that a smaller spins-per-thread would use:
+            // not timed
+            if (s != spts[0] && B->total_iso / s < (int64_t)B->cus * 4 * 64) continue;
             for (int b : blocks) {
                 float ms = 0.f;
                 if ((rc = batch_time(B, b, 2, &ms))) return rc;

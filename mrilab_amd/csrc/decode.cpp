@@ -3,6 +3,8 @@
 // producer of step records. The Bloch arithmetic itself runs on the GPU.
 #include "decode.hpp"
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstring>
 
@@ -104,26 +106,38 @@ void Decoder::emit(Decoded &out, bool step, bool adc, bool capture) {
     w[5] = flags;
 }
 
-void compress_runs(Decoded &d) {
+void compress_runs(Decoded &d, int64_t max_adc_run) {
     const int st = d.stride;
     if (d.cap_slot.empty()) {
         const int64_t n = d.n_rec();
         const uint32_t plain = REC_ADC | REC_RF | REC_SPOIL | REC_CAPTURE | REC_NOSTEP | REC_NEWDT;
-        int64_t out = -1;  // index of the last kept record
-        bool open = false; // the last kept record accepts repeats
+        const uint32_t low = (1u << REC_REP_SHIFT) - 1;
+        const int64_t run_cap = std::min<int64_t>(max_adc_run, (int64_t)REC_REP_MAX + 1);
+        int64_t out = -1;      // index of the last kept record
+        bool open = false;     // the last kept record accepts plain repeats
+        bool open_adc = false; // the last kept record accepts ADC-run repeats
         for (int64_t r = 0; r < n; r++) {
             const uint32_t *w = d.rec.data() + r * st;
             const uint32_t f = w[5];
+            if (open_adc && (f & low) == (REC_ADC | REC_SAME)) {
+                uint32_t *k = d.rec.data() + out * st;
+                const uint32_t rep = ((k[5] & REC_ADCRUN) ? (k[5] >> REC_REP_SHIFT) : 0) + 1;
+                k[5] = (k[5] & low) | REC_ADCRUN | (rep << REC_REP_SHIFT);
+                if ((int64_t)rep + 1 >= run_cap) open_adc = false;
+                continue;
+            }
             if (open && (f & REC_SAME) && !(f & plain)) {
                 uint32_t *k = d.rec.data() + out * st;
                 const uint32_t rep = (k[5] >> REC_REP_SHIFT) + 1;
-                k[5] = (k[5] & ((1u << REC_REP_SHIFT) - 1)) | (rep << REC_REP_SHIFT);
+                k[5] = (k[5] & low) | (rep << REC_REP_SHIFT);
                 if (rep == REC_REP_MAX) open = false;
                 continue;
             }
             out++;
             if (out != r) std::memmove(d.rec.data() + out * st, w, st * sizeof(uint32_t));
-            open = !(f & (REC_RF | REC_NOSTEP | REC_CAPTURE));
+            const bool free_step = !(f & (REC_RF | REC_NOSTEP | REC_CAPTURE));
+            open_adc = run_cap > 1 && free_step && (f & REC_ADC);
+            open = free_step && !open_adc;
         }
         d.rec.resize((size_t)(out + 1) * st);
     }

@@ -19,12 +19,15 @@ enum : uint32_t {
     REC_NOSTEP = 1u << 5,   // no precession or relaxation (dt <= 0 event carrier)
     REC_NEWDT = 1u << 6,    // dt differs from the previous step's
     REC_ONCE = 1u << 7,     // free step whose rotation no later step reuses
+    REC_ADCRUN = 1u << 8,   // ADC run: the repeats are (ADC sample, free step) pairs
 };
-// Bits 8..31 of the flags word: number of further plain free steps with
-// this record's dt and fields (no ADC, rf, spoiler or capture) that follow
-// it; compress_runs() folds such steps into the record before them.
-constexpr int REC_REP_SHIFT = 8;
-constexpr uint32_t REC_REP_MAX = (1u << 24) - 1;
+// Bits 9..31 of the flags word: the repeat count. For a plain record, the
+// number of further plain free steps with this record's dt and fields (no
+// ADC, rf, spoiler or capture) that follow it; for an REC_ADCRUN record,
+// the number of further steps that each take an ADC sample first and then
+// step with the same dt and fields. compress_runs() folds them.
+constexpr int REC_REP_SHIFT = 9;
+constexpr uint32_t REC_REP_MAX = (1u << 23) - 1;
 
 // Record layout, 32-bit words:
 //   0 dt (s)            1 gamma*Gz (rad/s/m)  2 gamma*Gy   3 gamma*Gx
@@ -53,7 +56,16 @@ struct Decoded {
 // (repeat count in the flags word; not when records carry RunMode-1
 // capture slots), then marks free-step records whose rotation is used by
 // that step only (REC_ONCE: no repeats and the next step is not REC_SAME).
-void compress_runs(Decoded &d);
+// With max_adc_run > 0, a free step with an ADC sample also absorbs the
+// ADC steps with the same dt and fields that follow it (REC_ADCRUN, at
+// most max_adc_run samples per record) instead of plain steps; only
+// kernels that execute REC_ADCRUN may be given such a stream.
+void compress_runs(Decoded &d, int64_t max_adc_run = 0);
+// ADC samples a record takes.
+inline int64_t rec_adc_samples(uint32_t flags) {
+    if (!(flags & REC_ADC)) return 0;
+    return (flags & REC_ADCRUN) ? 1 + (int64_t)(flags >> REC_REP_SHIFT) : 1;
+}
 
 // Stop reasons of Decoder::run.
 enum DecodeStop { DECODE_END = 0, DECODE_CALLBACK = 1 };

@@ -139,7 +139,8 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
             __asm__ volatile("");
             st.spoil();
         }
-        if (flags & REC_ADC) {  // DoScanAtCPU.cpp:445-491, before the step
+        // DoScanAtCPU.cpp:445-491, before the step
+        auto adc_sample = [&]() __attribute__((always_inline)) {
             // lanes past the last voxel carry zero coil weights
             float sx = st.mx[0], sy = st.my[0];
 #pragma unroll
@@ -163,8 +164,18 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
                 adc_batch0 += kRedBatch;
                 slot = 0;
             }
+        };
+        // an ADC run is this step and then (sample, step) pairs with the
+        // cached rotation; every other record is one pass of this loop
+        const bool run = (flags & REC_ADCRUN) != 0;
+        const int npass = run ? 1 + (int)(flags >> REC_REP_SHIFT) : 1;
+        for (int i = 0; i < npass; i++) {
+            if (flags & REC_ADC) adc_sample();
+            if (i > 0)
+                free_step<SPT, PREC>(st, k.m0s);
+            else if (!(flags & REC_NOSTEP))
+                bloch_step<SPT, GEN, PREC>(st, k, q, run ? flags & ((1u << REC_REP_SHIFT) - 1) : flags, txm);
         }
-        if (!(flags & REC_NOSTEP)) bloch_step<SPT, GEN, PREC>(st, k, q, flags, txm);
         if (GEN && (flags & REC_CAPTURE) && valid) {  // DoScanAtCPU.cpp:544-552
             const int64_t base = (int64_t)p.cap_slot[r] * nvox * p.S * p.Ty;
 #pragma unroll
@@ -458,8 +469,8 @@ static void partition(mrs_plan *P) {
     const int stride = P->dec.stride;
     int64_t r0 = 0, nadc = 0, adc0 = 0;
     for (int64_t r = 0; r < n; r++) {
-        const bool adc = (P->dec.rec[r * stride + 5] & REC_ADC) != 0;
-        if (adc && nadc == P->max_adc) {
+        const int64_t na = rec_adc_samples(P->dec.rec[r * stride + 5]);
+        if (na && nadc + na > P->max_adc) {
             P->launch_r0.push_back(r0);
             P->launch_r1.push_back(r);
             P->launch_adc0.push_back(adc0);
@@ -468,7 +479,7 @@ static void partition(mrs_plan *P) {
             r0 = r;
             nadc = 0;
         }
-        if (adc) nadc++;
+        nadc += na;
     }
     if (n > r0) {
         P->launch_r0.push_back(r0);
@@ -840,7 +851,7 @@ mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs
     Decoder dec(vobj, vcoi, vctl, vseq, vvar, false);
     dec.run(P->dec);
     dec.flush_tail(P->dec);
-    compress_runs(P->dec);
+    compress_runs(P->dec, P->model == 0 ? P->max_adc : 0);
     if ((int64_t)P->dec.adc_ref.size() > P->signal_num) {
         delete P;
         return fail(MRS_ERR_ARG, "VSig.SignalNum is smaller than the number of ADC samples");
@@ -1008,7 +1019,7 @@ static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, cons
         if (!all.muts.empty()) P->dec.muts = all.muts;
         const DecodeStop stop = dec.run(P->dec);
         if (stop == DECODE_END) dec.flush_tail(P->dec);
-        compress_runs(P->dec);
+        compress_runs(P->dec, P->model == 0 ? P->max_adc : 0);
         if (adc_base + (int64_t)P->dec.adc_ref.size() > P->signal_num)
             return fail(MRS_ERR_ARG, "VSig.SignalNum is smaller than the number of ADC samples");
         if ((rc = upload_program(P))) return rc;

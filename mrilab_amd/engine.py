@@ -542,15 +542,13 @@ def DoCartRecon(Sx, Sy, desc, device: bool = False) -> dict:
         if sx.dtype != torch.float64 or not sx.is_cuda:
             raise TypeError("device recon takes float64 CUDA tensors")
         n = sx.numel() // (desc.rx * desc.types)
-        vs = abi.VSigC(Sx=C.cast(C.c_void_p(sx.data_ptr()), C.POINTER(C.c_double)),
-                       Sy=C.cast(C.c_void_p(sy.data_ptr()), C.POINTER(C.c_double)), SignalNum=n)
+        vs = abi.VSigC(Sx=sx.data_ptr(), Sy=sy.data_ptr(), SignalNum=n)
         keep = (sx, sy)
     else:
         sx = np.ascontiguousarray(np.asarray(Sx, dtype=np.float64).ravel(order="F"))
         sy = np.ascontiguousarray(np.asarray(Sy, dtype=np.float64).ravel(order="F"))
         n = sx.size // (desc.rx * desc.types)
-        vs = abi.VSigC(Sx=sx.ctypes.data_as(C.POINTER(C.c_double)), Sy=sy.ctypes.data_as(C.POINTER(C.c_double)),
-                       SignalNum=n)
+        vs = abi.VSigC(Sx=sx.ctypes.data, Sy=sy.ctypes.data, SignalNum=n)
         keep = (sx, sy)
     out = _recon_call(lambda img, dev: lib.mrs_DoCartRecon(C.byref(vs), C.byref(desc), C.byref(img), dev),
                       desc, device)
