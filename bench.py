@@ -255,14 +255,15 @@ def recon_section(plan, sc, reps: int) -> dict:
 
 def sar_section(threads: int, cpu_tr: int = 64) -> dict:
     """Local SAR (mrs_DoCalSARAtGPU) of the 3-D bSSFP scan's sequence on its
-    128x128x64 phantom, transmitted on 2 coils in parallel (E fields per
-    coil, ECon), 16 sample times; per call: host event loop, uploads, the
-    voxel kernel, downloads. The oracle runs the same on the first
-    ``cpu_tr`` TRs."""
+    128x128x64 phantom (a tissue ellipsoid in air), transmitted on 2 coils
+    in parallel (E fields per coil, ECon), 16 sample times; per call: host
+    event loop, uploads, the voxel kernel, downloads. The oracle runs the
+    same on the first ``cpu_tr`` TRs. Then the 10-gram average
+    (mrs_DoSARAverageAtGPU) of 4 windows of that SAR."""
     import numpy as np
     import oracle
     from mrilab_amd import engine, workloads
-    sc = workloads.sar_scan(workloads.bssfp3d(), n_tx=2, n_samples=16)
+    sc = workloads.sar_scan(workloads.bssfp3d(), n_tx=2, n_samples=16, body=True)
     a = sc.copy()
     engine.DoCalSARAtGPU(*a.structs)  # warm-up (module load)
     ts = []
@@ -289,7 +290,63 @@ def sar_section(threads: int, cpu_tr: int = 64) -> dict:
             "ms_per_call": gpu_s * 1e3, "voxel_rf_steps_per_s": nvox * steps / gpu_s,
             "cpu": {"seconds": cpu_s, "rf_steps": cpu_steps, "voxel_rf_steps_per_s": nvox * cpu_steps / cpu_s,
                     "cores": threads, "kind": "port",
-                    "sample": f"first {cpu_tr} TRs, C restatement of DoCalSARAtCPU, OpenMP over voxels"}}
+                    "sample": f"first {cpu_tr} TRs, C restatement of DoCalSARAtCPU, OpenMP over voxels"},
+            "average": sar_average_section(a, threads)}
+
+
+def sar_average_section(sc, threads: int, grams: float = 10.0, cpu_slices=(28, 36)) -> dict:
+    """N-gram average of the SAR engine's output: 4 windows of the 16
+    samples (sar_window: DoSARWatchExec.m's rate / (2 MassDen) and power),
+    then mrs_DoSARAverageAtGPU on device-resident fp64 inputs; the oracle
+    (DoSARAverageAtCPU restated) on ``cpu_slices`` of the same volume,
+    compared bit for bit there."""
+    import numpy as np
+    import torch
+    import oracle
+    from mrilab_amd import engine
+    ts = sc.VSig["tSample"]
+    idx = [0, 4, 8, 12, ts.size - 1]
+    SAR, Power = engine.sar_window(sc.VObj, sc.VSig, ts[idx[:-1]], ts[idx[1:]])
+    M = np.asarray(sc.VObj["MassDen"], dtype=np.float64)
+    d = tuple(float(sc.VObj[k]) for k in ("XDimRes", "YDimRes", "ZDimRes"))
+
+    def dev(a):  # column-major device tensor
+        t = torch.from_numpy(np.ascontiguousarray(a.transpose(*reversed(range(a.ndim))))).cuda()
+        return t.permute(*reversed(range(a.ndim)))
+
+    dS, dP, dM = dev(SAR), dev(Power), dev(M)
+    engine.DoSARAverageAtGPU(dS, dP, dM, *d, grams)  # warm-up
+    torch.cuda.synchronize()
+    tt = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        a, b = engine.DoSARAverageAtGPU(dS, dP, dM, *d, grams)
+        torch.cuda.synchronize()
+        tt.append(time.perf_counter() - t0)
+    gpu_s = float(np.median(tt))
+    t0 = time.perf_counter()
+    engine.DoSARAverageAtGPU(SAR, Power, M, *d, grams)
+    host_s = time.perf_counter() - t0
+    oracle.set_threads(threads, "f64")
+    t0 = time.perf_counter()
+    oa, ob = oracle.sar_average(SAR, Power, M, *d, grams, slices=cpu_slices)
+    cpu_s = time.perf_counter() - t0
+    oracle.set_threads(0, "f64")
+    s0, s1 = cpu_slices
+    ga, gb = a.cpu().numpy(), b.cpu().numpy()
+    exact = bool(np.array_equal(ga[:, :, s0:s1], oa[:, :, s0:s1]) and np.array_equal(gb[:, :, s0:s1], ob[:, :, s0:s1]))
+    T = SAR.shape[3]
+    tissue = int(np.count_nonzero(M))
+    cpu_tissue = int(np.count_nonzero(M[:, :, s0:s1]))
+    log(f"sar average: {gpu_s * 1e3:.2f} ms per call ({tissue} tissue voxels x {T} windows, {grams} g; host arrays "
+        f"{host_s * 1e3:.1f} ms); CPU oracle {cpu_s:.2f} s for slices {s0}-{s1 - 1}; bit-exact there: {exact}")
+    return {"workload": f"{grams:g}-gram average, 128x128x64 voxels ({tissue} with tissue, "
+                        f"{d[0] * 1e3:g} mm voxels), {T} SAR windows",
+            "ms_per_call": gpu_s * 1e3, "ms_per_call_host_arrays": host_s * 1e3,
+            "tissue_voxel_windows_per_s": tissue * T / gpu_s, "bit_exact_vs_oracle": exact,
+            "cpu": {"seconds": cpu_s, "tissue_voxel_windows_per_s": cpu_tissue * T / cpu_s, "cores": threads,
+                    "kind": "port",
+                    "sample": f"slices {s0}-{s1 - 1}, C restatement of DoSARAverageAtCPU, OpenMP over rows"}}
 
 
 def traffic_from_profile() -> float | None:

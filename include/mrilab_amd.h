@@ -423,6 +423,22 @@ typedef struct mrs_sar_sig {
  * Replaces: mexFunction, Lib/src/engine/CPUEngine/DoCalSARAtCPU.cpp:63 */
 mrs_status mrs_DoCalSARAtGPU(const mrs_sar_obj *obj, const mrs_vseq *vseq, mrs_vvar *vvar, mrs_sar_sig *sig);
 
+/* N-gram local SAR average ([aveSAR, avePower] = DoSARAverageAtCPU(SAR,
+ * Power, MassDen, dx, dy, dz, grams)). SAR, Power, aveSAR and avePower are
+ * fp64 [rows, cols, slices, times] (dims[0..3]), MassDen fp64 [rows, cols,
+ * slices] in kg/m^3, dx/dy/dz the voxel size in m. Per voxel with
+ * MassDen != 0 and SAR > 0 at the first time point, a voxel-index sphere
+ * grows one voxel of radius at a time until it holds more than grams*1e-3
+ * kg, and the sums of the last two spheres are interpolated to exactly that
+ * mass; other voxels get 0. Results equal the reference's arithmetic bit
+ * for bit. Returns MRS_ERR_ARG when the volume's mass does not exceed
+ * grams*1e-3 kg (the reference loops forever there). device != 0: all
+ * arrays are device memory of GPU 0.
+ * Replaces: mexFunction, Lib/src/utilities/DoSARAverageAtCPU/DoSARAverageAtCPU.cpp:19 */
+mrs_status mrs_DoSARAverageAtGPU(const double *SAR, const double *Power, const double *MassDen, const int64_t dims[4],
+                                 double dx, double dy, double dz, double grams, double *aveSAR, double *avePower,
+                                 int32_t device);
+
 #ifdef __cplusplus
 }
 #endif

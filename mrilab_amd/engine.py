@@ -45,7 +45,8 @@ EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanA
            "mrs_batch_sync", "mrs_batch_fetch", "mrs_batch_get_info", "mrs_batch_destroy",
            "mrs_DoScanBatchAtGPU", "mrs_adjoint_create", "mrs_adjoint_forward", "mrs_adjoint_backward",
            "mrs_adjoint_get_info", "mrs_adjoint_destroy", "mrs_recon_dims", "mrs_DoCartRecon",
-           "mrs_plan_recon", "mrs_DoAddNoise", "mrs_noise_scale", "mrs_DoCalSARAtGPU")
+           "mrs_plan_recon", "mrs_DoAddNoise", "mrs_noise_scale", "mrs_DoCalSARAtGPU",
+           "mrs_DoSARAverageAtGPU")
 
 
 class MRSError(RuntimeError):
@@ -135,6 +136,10 @@ def load() -> C.CDLL:
         lib.mrs_noise_scale.restype = C.c_double
         lib.mrs_DoCalSARAtGPU.argtypes = [P(abi.SarObjC), P(abi.VSeqC), P(abi.VVarC), P(abi.SarSigC)]
         lib.mrs_DoCalSARAtGPU.restype = C.c_int
+        lib.mrs_DoSARAverageAtGPU.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(C.c_int64), C.c_double,
+                                              C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                              C.c_int32]
+        lib.mrs_DoSARAverageAtGPU.restype = C.c_int
         _lib = lib
         return lib
 
@@ -598,3 +603,81 @@ def DoCalSARAtGPU(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig) -> None:
     VVar["SARi"] = np.int32(sig.SARi)
     VSig["SAR_rf_steps"] = int(sig.n_rf_steps)  # steps with rf on in this call
     del keep
+
+
+def sar_window(VObj, VSig, t_start, t_end):
+    """Unaveraged local SAR and power of the windows [t_start[i], t_end[i]]
+    from the sampled VSig.SAR (DoSARWatchExec.m:109-126): per window
+    (VSig.SAR(:,:,:,idx2) - VSig.SAR(:,:,:,idx1)) / (tReal(idx2) - tReal(idx1))
+    divided by 2*MassDen for SAR (W/kg) and scaled by dx*dy*dz/2 for power
+    (W), with NaN set to 0; idx1/idx2 are the samples at t_start/t_end. The
+    arithmetic is single precision like the reference's single VSig.SAR and
+    MassDen; returns float64 [rows, cols, slices, windows] arrays."""
+    ts = np.asarray(VSig["tSample"], dtype=np.float64).ravel()
+    tr = np.asarray(VSig["tRealSample"], dtype=np.float64).ravel()
+    S = np.asarray(VSig["SAR"], dtype=np.float32)
+    S = S.reshape(S.shape[:3] + (-1,), order="F") if S.ndim >= 4 else S.reshape(S.shape + (1,) * (4 - S.ndim),
+                                                                                      order="F")
+    m = np.asarray(VObj["MassDen"], dtype=np.float32).reshape(S.shape[:3], order="F")
+    vox = np.float32(float(VObj["XDimRes"]) * float(VObj["YDimRes"]) * float(VObj["ZDimRes"]) / 2)
+    t_start = np.atleast_1d(np.asarray(t_start, dtype=np.float64))
+    t_end = np.atleast_1d(np.asarray(t_end, dtype=np.float64))
+    sar = np.zeros(S.shape[:3] + (t_end.size,), order="F")
+    power = np.zeros_like(sar)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for i, (a, b) in enumerate(zip(t_start, t_end)):
+            i1, i2 = np.flatnonzero(ts == a), np.flatnonzero(ts == b)
+            if i1.size != 1 or i2.size != 1:
+                raise ValueError(f"window {i}: t_start/t_end must each match one VSig.tSample")
+            rate = (S[..., i2[0]] - S[..., i1[0]]) / np.float32(tr[i2[0]] - tr[i1[0]])
+            sar[..., i] = rate / (np.float32(2) * m)
+            power[..., i] = rate * vox
+    sar[np.isnan(sar)] = 0
+    power[np.isnan(power)] = 0
+    return sar, power
+
+
+def DoSARAverageAtGPU(SAR, Power, MassDen, dx: float, dy: float, dz: float, grams: float):
+    """N-gram local SAR and power averages ([aveSAR, avePower] of
+    DoSARAverageAtCPU.cpp), bit for bit the reference's fp64 arithmetic.
+    SAR and Power are [rows, cols, slices(, times)], MassDen [rows, cols,
+    slices] in kg/m^3, dx/dy/dz in m. Numpy inputs (any float dtype, read
+    in column-major order) give numpy float64 outputs; CUDA float64
+    tensors, contiguous with the rows axis fastest (a permuted view of a
+    [times, slices, cols, rows] tensor, or flat), give CUDA tensors shaped
+    like SAR with the same strides."""
+    lib = load()
+    if hasattr(SAR, "data_ptr"):
+        import torch
+
+        ts = (SAR, Power, MassDen)
+        if not all(t.is_cuda and t.dtype == torch.float64 for t in ts):
+            raise TypeError("SAR, Power and MassDen must all be CUDA float64 tensors")
+        if SAR.shape != Power.shape or SAR.stride() != Power.stride():
+            raise ValueError("SAR and Power differ in shape or strides")
+        shp = (tuple(SAR.shape) + (1, 1, 1, 1))[:4]
+        if MassDen.numel() != shp[0] * shp[1] * shp[2]:
+            raise ValueError("MassDen does not match SAR's first three dimensions")
+        for t in ts:
+            if t.dim() > 1 and not t.permute(*reversed(range(t.dim()))).is_contiguous():
+                raise ValueError("arrays must be column-major contiguous (rows fastest)")
+        a = torch.empty_like(SAR)
+        b = torch.empty_like(SAR)
+        dims = (C.c_int64 * 4)(*shp)
+        _check(lib.mrs_DoSARAverageAtGPU(C.c_void_p(SAR.data_ptr()), C.c_void_p(Power.data_ptr()),
+                                         C.c_void_p(MassDen.data_ptr()), dims, float(dx), float(dy), float(dz),
+                                         float(grams), C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), 1))
+        return a, b
+    sar = np.asfortranarray(SAR, dtype=np.float64)
+    pw = np.asfortranarray(Power, dtype=np.float64)
+    m = np.asfortranarray(MassDen, dtype=np.float64)
+    shp = (tuple(sar.shape) + (1, 1, 1, 1))[:4]
+    if len(sar.shape) > 4 or pw.shape != sar.shape or m.size != shp[0] * shp[1] * shp[2]:
+        raise ValueError("SAR, Power and MassDen shapes differ")
+    a = np.zeros(sar.shape, order="F")
+    b = np.zeros(sar.shape, order="F")
+    dims = (C.c_int64 * 4)(*shp)
+    _check(lib.mrs_DoSARAverageAtGPU(C.c_void_p(sar.ctypes.data), C.c_void_p(pw.ctypes.data),
+                                     C.c_void_p(m.ctypes.data), dims, float(dx), float(dy), float(dz), float(grams),
+                                     C.c_void_p(a.ctypes.data), C.c_void_p(b.ctypes.data), 0))
+    return a, b

@@ -94,7 +94,8 @@ def make_structs(Rho, T1, T2, dB0, res, spin_num=1, dWRnd=None, cs=None,
     VObj = {"Gyro": np.float64(GYRO), "SpinNum": np.int32(S), "TypeNum": np.int32(Ty),
             "Mx": f32(np.zeros_like(Mz)), "My": f32(np.zeros_like(Mz)), "Mz": f32(Mz),
             "Rho": f32(Rho), "T1": f32(T1), "T2": f32(T2),
-            "ChemShift": np.float64(0.0)}
+            "ChemShift": np.float64(0.0), "XDimRes": np.float64(res[0]), "YDimRes": np.float64(res[1]),
+            "ZDimRes": np.float64(res[2])}
     if dWRnd is None:
         dWRnd = np.zeros((R, C_, Z, S, Ty))
     VMag = {"dB0": f32(dB0), "dWRnd": f32(dWRnd), "Gzgrid": f32(Gz), "Gygrid": f32(Gy),
@@ -641,12 +642,16 @@ def as_variant(sc: Scan, variant: str, self_rate: float = 0.0) -> Scan:
     return out
 
 
-def sar_scan(base: Scan | None = None, n_tx: int = 2, n_samples: int = 8, seed: int = 5) -> Scan:
+def sar_scan(base: Scan | None = None, n_tx: int = 2, n_samples: int = 8, seed: int = 5,
+             body: bool = False) -> Scan:
     """A scan with the SAR engine's fields (DoCalSARAtCPU.cpp:115-166) on
     top of ``base`` (default: the 64x64 GRE): per Tx coil an E field per
     unit rf amplitude (V/m/T, Gaussian blobs of random sign per component),
     ECon = sigma/(2*density) ~ U[1e-4, 5e-4], VVar.SAR zero, and
-    ``n_samples`` sample times spread over the sequence (the first at 0)."""
+    ``n_samples`` sample times spread over the sequence (the first at 0).
+    MassDen is 1000 kg/m^3 everywhere, or with ``body`` a tissue ellipsoid
+    (semi-axes 0.8, 0.65, 0.9 of the half field of view) of 1000 +- 100
+    kg/m^3 in smooth variation inside and air (MassDen = ECon = 0) outside."""
     sc = (gre2d() if base is None else base).copy()
     rng = np.random.default_rng(seed)
     shape = sc.VObj["Mz"].shape[:3] if sc.VObj["Mz"].ndim >= 3 else sc.VObj["Mz"].shape[:2] + (1,)
@@ -680,8 +685,15 @@ def sar_scan(base: Scan | None = None, n_tx: int = 2, n_samples: int = 8, seed: 
         sc.VCoi[f"TxE1{name}"] = np.asfortranarray(E[k].astype(np.float32))
     sc.VCoi["TxCoilmg"] = np.ones((R, C_, Z, n_tx), dtype=np.float32, order=F)
     sc.VCoi["TxCoilpe"] = np.zeros((R, C_, Z, n_tx), dtype=np.float32, order=F)
-    sc.VObj["MassDen"] = np.full((R, C_, Z), 1000.0, dtype=np.float32, order=F)
-    sc.VObj["ECon"] = np.asfortranarray(rng.uniform(1e-4, 5e-4, (R, C_, Z, 3)).astype(np.float32))
+    econ = rng.uniform(1e-4, 5e-4, (R, C_, Z, 3))
+    if body:
+        inside = (yy / 0.65) ** 2 + (xx / 0.8) ** 2 + (zz / 0.9) ** 2 <= 1.0
+        dens = 1000.0 + 100.0 * np.sin(3.0 * xx + 1.0) * np.cos(2.0 * yy - 0.5) * np.cos(1.5 * zz)
+        sc.VObj["MassDen"] = np.asfortranarray(np.where(inside, dens, 0.0).astype(np.float32))
+        econ[~inside] = 0.0
+    else:
+        sc.VObj["MassDen"] = np.full((R, C_, Z), 1000.0, dtype=np.float32, order=F)
+    sc.VObj["ECon"] = np.asfortranarray(econ.astype(np.float32))
     for f in ("rfAmp", "rfPhase", "rfFreq"):
         sc.VVar[f] = np.zeros(n_tx)
     sc.VVar["SAR"] = np.zeros((R, C_, Z, 3), dtype=np.float32, order=F)

@@ -49,6 +49,10 @@ def _lib(precision: str) -> C.CDLL:
         lib.orc_superlorentzian.restype = C.c_double
         lib.orc_DoCalSAR.argtypes = [P(abi.SarObjC), P(abi.VSeqC), P(abi.VVarC), P(abi.SarSigC)]
         lib.orc_DoCalSAR.restype = C.c_int
+        D = C.POINTER(C.c_double)
+        lib.orc_DoSARAverage.argtypes = [D, D, D, C.POINTER(C.c_int64), C.c_double, C.c_double, C.c_double,
+                                         C.c_double, D, D, C.c_int64, C.c_int64]
+        lib.orc_DoSARAverage.restype = C.c_int
         lib.orc_set_threads.argtypes = [C.c_int]
         lib.orc_set_relax_mode.argtypes = [C.c_int]
         _libs[precision] = lib
@@ -183,3 +187,30 @@ def sar(sc, precision: str = "f32", inplace: bool = False):
     out.VSig["SAR_rf_steps"] = int(sig.n_rf_steps)  # steps with rf on in this call
     del keep
     return out
+
+
+def sar_average(SAR, Power, MassDen, dx: float, dy: float, dz: float, grams: float, slices=None):
+    """N-gram local SAR average (orc_DoSARAverage, DoSARAverageAtCPU.cpp
+    restated): SAR and Power [rows, cols, slices(, times)] and MassDen
+    [rows, cols, slices], all read as fp64 in column-major order. Returns
+    (aveSAR, avePower) shaped as SAR. ``slices = (s0, s1)`` computes only
+    those slices (the others stay 0)."""
+    sar = np.asfortranarray(SAR, dtype=np.float64)
+    pw = np.asfortranarray(Power, dtype=np.float64)
+    m = np.asfortranarray(MassDen, dtype=np.float64)
+    shp = (tuple(sar.shape) + (1, 1, 1, 1))[:4]
+    if pw.shape != sar.shape or (tuple(m.shape) + (1, 1, 1))[:3] != shp[:3]:
+        raise ValueError("SAR, Power and MassDen shapes differ")
+    s0, s1 = (0, shp[2]) if slices is None else slices
+    a = np.zeros(sar.shape, order="F")
+    b = np.zeros(sar.shape, order="F")
+    D = C.POINTER(C.c_double)
+    dims = (C.c_int64 * 4)(*shp)
+    rc = _lib("f64").orc_DoSARAverage(sar.ctypes.data_as(D), pw.ctypes.data_as(D), m.ctypes.data_as(D), dims,
+                                       float(dx), float(dy), float(dz), float(grams), a.ctypes.data_as(D),
+                                       b.ctypes.data_as(D), int(s0), int(s1))
+    if rc == -1:
+        raise ValueError("the volume's mass does not exceed grams*1e-3 kg")
+    if rc != 0:
+        raise RuntimeError(f"oracle SAR average failed with status {rc}")
+    return a, b
