@@ -349,6 +349,36 @@ def sar_average_section(sc, threads: int, grams: float = 10.0, cpu_slices=(28, 3
                     "sample": f"slices {s0}-{s1 - 1}, C restatement of DoSARAverageAtCPU, OpenMP over rows"}}
 
 
+def coil_sum_section(n_tr: int) -> dict:
+    """Forward kernel time per launch on the first ``n_tr`` TRs of the same
+    scan with each coil sum: the default MFMA projection and the wavefront
+    butterfly (mrs_options.coil_sum = 1), with the signals' rel-L2."""
+    import numpy as np
+    from mrilab_amd import engine, workloads
+    sc = workloads.bssfp3d(n_tr=n_tr)
+    out, sig = {}, {}
+    for name, cs in (("mfma", 0), ("butterfly", 1)):
+        o = sc.copy()
+        with engine.Plan(*o.structs, options=engine.make_options(timing=1, coil_sum=cs)) as p:
+            p.run()
+            p.sync()
+            ms = []
+            for _ in range(3):
+                p.run()
+                p.sync()
+                inf = p.info()
+                ms.append(inf["fwd_kernel_ms"] / inf["n_launches"])
+            p.fetch(o.VObj, o.VVar, o.VSig)
+        out[f"{name}_ms_per_launch"] = float(np.median(ms))
+        sig[name] = np.asarray(o.VSig["Sx"]) + 1j * np.asarray(o.VSig["Sy"])
+    d = sig["mfma"] - sig["butterfly"]
+    out["rel_l2_between"] = float(np.linalg.norm(d) / np.linalg.norm(sig["butterfly"]))
+    out["tr"] = n_tr
+    log(f"coil sum: MFMA {out['mfma_ms_per_launch']:.2f} ms, butterfly {out['butterfly_ms_per_launch']:.2f} ms "
+        f"per launch; rel-L2 between {out['rel_l2_between']:.1e}")
+    return out
+
+
 def traffic_from_profile() -> float | None:
     """HBM bytes per forward launch measured by rocprofv3 (profiles/)."""
     path = os.path.join(ROOT, "profiles", "bloch_fwd_bssfp3d_traffic.json")
@@ -462,6 +492,8 @@ def main() -> int:
     if not args.no_recon and args.tr is None:
         line["recon"] = recon_section(plan, sc, 5)
     plan.close()
+    if args.tr is None:
+        line["coil_sum"] = coil_sum_section(64)
     if not args.no_batched:
         line["batched"] = batched_section(args.batch_sets, args.batch_plain_sample, 3)
     if not args.no_adjoint:

@@ -143,7 +143,11 @@ typedef struct mrs_options {
                                  1 = GM, 2 = MT, 3 = CEST, 4 = ME (VObj.K in
                                  that engine's layout); set by the entry
                                  point, or by the caller of mrs_plan_create */
-    int32_t reserved[1];
+    int32_t coil_sum;         /* single-pool forward kernel with 4 or 8 receive
+                                 coils: 0 = MFMA projection of LDS-staged
+                                 samples onto the coil maps, 1 = per-lane
+                                 coil products and a wavefront butterfly
+                                 (other coil counts always take the latter) */
 } mrs_options;
 
 /* Thread-local message of the last failed call on this thread. */

@@ -87,7 +87,7 @@ class OptionsC(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("spins_per_thread", C.c_int32),
                 ("max_adc_per_launch", C.c_int32), ("multipool_scheme", C.c_int32),
                 ("precision_mode", C.c_int32), ("timing", C.c_int32), ("model", C.c_int32),
-                ("reserved", C.c_int32 * 1)]
+                ("coil_sum", C.c_int32)]
 
 
 class PlanInfoC(C.Structure):

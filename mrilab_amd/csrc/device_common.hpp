@@ -137,6 +137,24 @@ __device__ __forceinline__ float relax_mz_ref(float mz, float e1, float m0e) {
 
 // Block-level flush of buffered per-wave ADC values: sums the waves of the
 // block in LDS order and writes one partial per (sample, value).
+// MFMA coil projection (forward kernel, 4 or 8 receive coils). Each wave
+// stages its lanes' per-sample spin sums (sum Mx, sum My) in LDS, kMfmaRows
+// samples at a time, stored voxel-major with a padded row of kMfmaPad
+// float2 (conflict-free writes across lanes and reads along samples), next
+// to its 64 voxels' coil maps (kMfmaCoils float2 (Cx, Cy) per voxel). Every
+// kMfmaRows samples one 16x16x4 f32 MFMA chain forms
+//   D[sample][2c]   = sum_v (sx Cx_vc + sy Cy_vc)
+//   D[sample][2c+1] = sum_v (sy Cx_vc - sx Cy_vc)
+// with K running over (voxel, component): 32 MFMAs reduce the wave's 64
+// voxels for 8 samples and 8 coils (rows 8..15 of the 16-row tile repeat
+// rows 0..7 and are dropped).
+constexpr int kMfmaRows = 8;
+constexpr int kMfmaPad = kMfmaRows + 1;
+constexpr int kMfmaCoils = 8;
+constexpr int kMfmaWaveFloat2 = 64 * kMfmaPad + 64 * kMfmaCoils;  // per wave: samples, then coil maps
+
+typedef float mfma_v4f __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ void flush_partials(const float *red, int nslot, int nwaves, int nv, float *partial,
                                                int64_t adc0, int nblk, int blk, int vstride) {
     for (int i = threadIdx.x; i < nslot * nv; i += blockDim.x) {

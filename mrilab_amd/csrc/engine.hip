@@ -12,13 +12,17 @@
 //  * Free precession with the same dt and fields as the previous step reuses
 //    the per-spin rotation (cos, sin); relaxation factors are recomputed only
 //    when dt changes.
-//  * At an ADC step each thread forms sum_spins(Mx + iMy) * conj(C_c) for its
-//    coils; a transposed butterfly inside the wavefront leaves one coil value
-//    per lane group, the waves of a block add theirs in LDS, and each block
-//    writes one partial per sample. A second kernel sums the block partials
-//    of each sample in a fixed order (compensated fp32 per lane, fp64 across
-//    lanes), demodulates by rfRef and adds into Sx/Sy. No atomics: results
-//    are bitwise reproducible.
+//  * At an ADC step each thread forms sum_spins(Mx + iMy); the signal per
+//    coil is that times conj(C_c) summed over the voxels. With 4 or 8 coils
+//    each wave stages 8 samples of its 64 voxels' sums in LDS and one MFMA
+//    chain (16x16x4 f32, K over voxel and component) projects them onto the
+//    coil maps held beside them; with other coil counts each thread forms
+//    its coil products and a transposed butterfly inside the wavefront
+//    leaves one coil value per lane group. The waves of a block add theirs
+//    in LDS, and each block writes one partial per sample. A second kernel
+//    sums the block partials of each sample in a fixed order (compensated
+//    fp32 per lane, fp64 across lanes), demodulates by rfRef and adds into
+//    Sx/Sy. No atomics: results are bitwise reproducible.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,12 +61,18 @@ struct FwdParams {
 
 // GEN = false: single Tx coil and no RunMode-1 capture (the common case);
 // GEN = true: any Tx coil count and capture. PREC: relaxation form
-// (0 reference rounding, 1 accurate).
-template <int SPT, int NRX, bool GEN, int PREC>
-__global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g,
-                                                        int r0, int r1) {
+// (0 reference rounding, 1 accurate). Up to 4 spins per thread the kernel
+// asks for 4 waves per SIMD (at most 128 registers): measured on the 3-D
+// bSSFP scan with 8 coils, 3 waves per SIMD ran 14 % slower, and the few
+// values the limit moves to scratch are spilled before the step loop and
+// reloaded after it.
+// MFC: the MFMA coil projection (4 or 8 coils), else the butterfly.
+template <int SPT, int NRX, bool GEN, int PREC, bool MFC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPT <= 4 ? 4 : 1)))
+void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, int r1) {
     constexpr int NV = 2 * NRX;
-    extern __shared__ float red[];  // [kRedBatch][nwaves][NV]
+    constexpr bool MF = MFC && (NRX == 4 || NRX == 8);
+    extern __shared__ float red[];  // [kRedBatch][nwaves][NV], then (MF) kMfmaWaveFloat2 float2 per wave
     const crec_t rec = (crec_t)rec_g;
     const int stride = GEN ? p.stride : 8;
     const int lane = threadIdx.x & 63;
@@ -118,6 +128,13 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
 #pragma unroll
         for (int c = 0; c < NRX; c++) crx[c] = cry[c] = 0.f;
     }
+    float2 *const msum = MF ? (float2 *)(red + kRedBatch * nwaves * NV) + wave * kMfmaWaveFloat2 : nullptr;
+    float2 *const mcoil = MF ? msum + 64 * kMfmaPad : nullptr;
+    if constexpr (MF) {
+#pragma unroll
+        for (int c = 0; c < kMfmaCoils; c++)
+            mcoil[lane * kMfmaCoils + c] = c < NRX ? make_float2(crx[c], cry[c]) : make_float2(0.f, 0.f);
+    }
     k.wtx_r = k.wtx_i = 0.f;
     if (!GEN || p.Tx == 1) {
         float sp, cp;
@@ -130,6 +147,33 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
 
     int slot = 0;
     int adc_batch0 = 0;
+    // MF: project the staged samples [row0, row0 + nrows) of the red batch
+    // onto the coils and leave this wave's sums in red
+    auto mfma_project = [&](int row0, int nrows) __attribute__((always_inline)) {
+        if constexpr (MF) {
+            // this wave's LDS writes precede its reads (LDS is in order per wave)
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const int g = lane >> 4, j = lane & 15, srow = lane & (kMfmaRows - 1), c = j >> 1;
+            const bool im = (j & 1) != 0;
+            mfma_v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+            for (int kk = 0; kk < 16; kk++) {
+                const int vox = 4 * kk + g;
+                const float2 a = msum[vox * kMfmaPad + srow];
+                const float2 w = mcoil[vox * kMfmaCoils + c];
+                const float b0 = im ? -w.y : w.x, b1 = im ? w.x : w.y;
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b0, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b1, acc, 0, 0, 0);
+            }
+            // lane holds D[4g + i][j], i = 0..3
+            if (g < kMfmaRows / 4 && j < NV) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (4 * g + i < nrows) red[((row0 + 4 * g + i) * nwaves + wave) * NV + j] = acc[i];
+            }
+        }
+    };
     for (int r = r0; r < r1; ++r) {
         const crec_t q = rec + r * stride;
         const uint32_t flags = q[5];
@@ -147,6 +191,18 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
             for (int s = 1; s < SPT; s++) {
                 sx += st.mx[s];
                 sy += st.my[s];
+            }
+            if constexpr (MF) {
+                msum[lane * kMfmaPad + (slot & (kMfmaRows - 1))] = make_float2(sx, sy);
+                if ((++slot & (kMfmaRows - 1)) == 0) mfma_project(slot - kMfmaRows, kMfmaRows);
+                if (slot == kRedBatch) {
+                    __syncthreads();
+                    flush_partials(red, kRedBatch, nwaves, NV, p.partial, adc_batch0, nblk, blk, NV);
+                    __syncthreads();
+                    adc_batch0 += kRedBatch;
+                    slot = 0;
+                }
+                return;
             }
             float val[NV];
 #pragma unroll
@@ -187,6 +243,8 @@ __global__ __launch_bounds__(256) void bloch_fwd_kernel(FwdParams p, const uint3
         }
     }
     if (slot > 0) {
+        if (MF && (slot & (kMfmaRows - 1)) != 0)
+            mfma_project(slot & ~(kMfmaRows - 1), slot & (kMfmaRows - 1));
         __syncthreads();
         flush_partials(red, slot, nwaves, NV, p.partial, adc_batch0, nblk, blk, NV);
     }
@@ -277,44 +335,49 @@ mrs_status fail(mrs_status st, const std::string &msg) {
 
 typedef void (*fwd_fn)(FwdParams, const uint32_t *, int64_t, int64_t);
 
-template <int SPT, int NRX, bool GEN, int PREC>
+template <int SPT, int NRX, bool GEN, int PREC, bool MFC>
 static void launch_fwd(dim3 grid, dim3 block, size_t lds, hipStream_t st, const FwdParams &p,
                        const uint32_t *rec, int r0, int r1) {
-    bloch_fwd_kernel<SPT, NRX, GEN, PREC><<<grid, block, lds, st>>>(p, rec, r0, r1);
+    bloch_fwd_kernel<SPT, NRX, GEN, PREC, MFC><<<grid, block, lds, st>>>(p, rec, r0, r1);
 }
 
 typedef void (*launch_fn)(dim3, dim3, size_t, hipStream_t, const FwdParams &, const uint32_t *, int, int);
 
 template <int SPT, bool GEN, int PREC>
-static launch_fn pick_nrx(int nrx) {
+static launch_fn pick_nrx(int nrx, bool mfma) {
     switch (nrx) {
-        case 1: return launch_fwd<SPT, 1, GEN, PREC>;
-        case 2: return launch_fwd<SPT, 2, GEN, PREC>;
-        case 4: return launch_fwd<SPT, 4, GEN, PREC>;
-        case 8: return launch_fwd<SPT, 8, GEN, PREC>;
-        case 16: return launch_fwd<SPT, 16, GEN, PREC>;
-        case 32: return launch_fwd<SPT, 32, GEN, PREC>;
+        case 1: return launch_fwd<SPT, 1, GEN, PREC, false>;
+        case 2: return launch_fwd<SPT, 2, GEN, PREC, false>;
+        case 4: return mfma ? launch_fwd<SPT, 4, GEN, PREC, true> : launch_fwd<SPT, 4, GEN, PREC, false>;
+        case 8: return mfma ? launch_fwd<SPT, 8, GEN, PREC, true> : launch_fwd<SPT, 8, GEN, PREC, false>;
+        case 16: return launch_fwd<SPT, 16, GEN, PREC, false>;
+        case 32: return launch_fwd<SPT, 32, GEN, PREC, false>;
     }
     return nullptr;
 }
 
 template <bool GEN, int PREC>
-static launch_fn pick_spt(int spt, int nrx) {
+static launch_fn pick_spt(int spt, int nrx, bool mfma) {
     switch (spt) {
-        case 1: return pick_nrx<1, GEN, PREC>(nrx);
-        case 2: return pick_nrx<2, GEN, PREC>(nrx);
-        case 4: return pick_nrx<4, GEN, PREC>(nrx);
-        case 8: return pick_nrx<8, GEN, PREC>(nrx);
+        case 1: return pick_nrx<1, GEN, PREC>(nrx, mfma);
+        case 2: return pick_nrx<2, GEN, PREC>(nrx, mfma);
+        case 4: return pick_nrx<4, GEN, PREC>(nrx, mfma);
+        case 8: return pick_nrx<8, GEN, PREC>(nrx, mfma);
     }
     return nullptr;
 }
 
+// Whether the forward kernel for nrx (padded) receive coils sums the coil
+// signal with the MFMA projection (coil_sum option 0) or the butterfly (1).
+static bool use_mfma(int nrx, int coil_sum) { return coil_sum == 0 && (nrx == 4 || nrx == 8); }
+
 // Forward kernel for spt spins per thread, nrx (padded) receive coils,
-// the general variant (multi-Tx, capture) or not, and relaxation form prec.
-static launch_fn pick_fwd(int spt, int nrx, bool gen, int prec) {
+// the general variant (multi-Tx, capture) or not, relaxation form prec and
+// the coil sum of use_mfma.
+static launch_fn pick_fwd(int spt, int nrx, bool gen, int prec, bool mfma) {
     if (prec)
-        return gen ? pick_spt<true, 1>(spt, nrx) : pick_spt<false, 1>(spt, nrx);
-    return gen ? pick_spt<true, 0>(spt, nrx) : pick_spt<false, 0>(spt, nrx);
+        return gen ? pick_spt<true, 1>(spt, nrx, mfma) : pick_spt<false, 1>(spt, nrx, mfma);
+    return gen ? pick_spt<true, 0>(spt, nrx, mfma) : pick_spt<false, 0>(spt, nrx, mfma);
 }
 
 void launch_signal_reduce(int nv, dim3 grid, hipStream_t st, const float *partial, int nblk, int bstep,
@@ -367,6 +430,7 @@ struct mrs_plan {
     float last_ms = 0.f;
     int timing = 0;
     int model = 0, scheme = 0;        // 0 single pool; 1 GM, 2 MT, 3 CEST, 4 ME multi-pool
+    int coil_sum = 0;                 // mrs_options.coil_sum
     int variant = 0;                  // model - 1 for the multi-pool engines
     DevBuf K;
     int flag[kGmMaxTypes] = {0};
@@ -514,6 +578,7 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     P->timing = opt ? opt->timing : 0;
     P->model = opt ? opt->model : 0;
     P->scheme = opt ? opt->multipool_scheme : 0;
+    P->coil_sum = opt ? opt->coil_sum : 0;
     if (P->model < 0 || P->model > 4) return fail(MRS_ERR_ARG, "unknown engine model");
     if (P->model >= 1) {
         P->variant = P->model - 1;
@@ -573,7 +638,7 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     max_adc = std::max<int64_t>(kRedBatch, std::min<int64_t>(max_adc, 65536));
     if (opt && opt->max_adc_per_launch > 0) max_adc = opt->max_adc_per_launch;
     P->max_adc = max_adc;
-    if (!pick_fwd(P->spt, P->nrx, true, 0)) return fail(MRS_ERR_UNSUPPORTED, "no kernel for this coil count");
+    if (!pick_fwd(P->spt, P->nrx, true, 0, false)) return fail(MRS_ERR_UNSUPPORTED, "no kernel for this coil count");
 
     hipStream_t st = P->stream;
     const size_t vb = (size_t)P->nvox * sizeof(float);
@@ -768,9 +833,11 @@ static mrs_status execute(mrs_plan *P, int64_t adc_base) {
     fp.gyro = P->gyro;
     for (int t = 0; t < P->Ty; t++) fp.cs[t] = P->cs[t];
     if (P->model >= 1) return execute_gm(P);
-    launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1, P->precision != 0);
+    const bool mfma = use_mfma(P->nrx, P->coil_sum);
+    launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1, P->precision != 0, mfma);
     const dim3 grid(P->gx, P->Ty * P->ngroups), block(P->block);
-    const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
+    const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float) +
+                       (mfma ? (size_t)(P->block / 64) * kMfmaWaveFloat2 * sizeof(float2) : 0);
     const uint32_t *rec = (const uint32_t *)P->rec.p;
     const size_t nl = P->launch_r0.size();
     if (P->timing) {
