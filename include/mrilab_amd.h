@@ -443,6 +443,48 @@ mrs_status mrs_DoSARAverageAtGPU(const double *SAR, const double *Power, const d
                                  double dx, double dy, double dz, double grams, double *aveSAR, double *avePower,
                                  int32_t device);
 
+/* ---------------------------------------------------------------------------
+ * Sequence compiler: a PSD XML file (MRiLab's MRiLabSeq format: CVs, then
+ * Pulses sections of rf / GzSS / GyPE / GxR / ADC / Ext elements whose
+ * attributes are MATLAB expressions of CVn, VCtl.x, VVar.x) and the VCtl
+ * scan parameters to the VSeq lines, as DoPulseGen runs the PSD function
+ * that DoWriteXML2m writes. Elements: rfSinc, rfRect, rfGaussian, rfFermi,
+ * GzCartesian, GzSelective, GyCartesian, GxCartesian, ADCCartesian, ExtBit;
+ * any other element is an MRS_ERR_ARG naming it.
+ * ------------------------------------------------------------------------- */
+typedef struct mrs_seq_param {
+    const char *name;         /* VCtl field, e.g. "TR", "FlipAng", "ME_TEs" */
+    const double *value;      /* n numbers (VCtl.x(i) indexes them from 1) */
+    int32_t n;
+    int32_t reserved;
+    const char *text;         /* text fields (TEAnchor, MultiTransmit, DP_Flag);
+                                 NULL for numbers */
+} mrs_seq_param;
+typedef struct mrs_seq mrs_seq;  /* a compiled sequence, owned by the library */
+typedef struct mrs_seq_lines {   /* caller buffers of the sizes mrs_seq_counts gives */
+    double *utsLine, *tsLine;
+    double *rfAmpLine, *rfPhaseLine, *rfFreqLine, *rfCoilLine;
+    double *GzAmpLine, *GyAmpLine, *GxAmpLine, *ADCLine, *ExtLine;
+    double *flagsLine;        /* [6, n_ts] column-major */
+} mrs_seq_lines;
+/* Compiles psd_xml (the file's text) for the VCtl parameters (TR, TE,
+ * FlipAng, FOVFreq/FOVPhase/FOVSlice, ResFreq/ResPhase, SliceNum,
+ * FirstPhNum/SecondPhNum, BandWidth, RFreq, MinUpdRate, MasterTxCoil
+ * (default 1), TEAnchor (default 'Middle'), TEAnchorTime (default 0),
+ * MultiTransmit (default 'off'), DP_Flag/DP_TR/DP_FlipAng/DP_Num and
+ * whatever the PSD's expressions name) and gyro = VObj.Gyro (rad/s/T).
+ * Replaces: DoPulseGen(Simuh), Src/Main/DoPulseGen.m:2-117 with
+ * Src/Main/DoWriteXML2m.m:11-337 and the Macro/SeqElem elements above
+ * (the gradient and update-rate checks of DoPulseGen.m:119-134 are not
+ * made). */
+mrs_status mrs_seq_compile(const char *psd_xml, const mrs_seq_param *vctl, int32_t n_params, double gyro,
+                           mrs_seq **out);
+/* counts = {n_ts, n_uts, n_rf, n_Gz, n_Gy, n_Gx, n_ADC, n_Ext, TRs}. */
+mrs_status mrs_seq_counts(const mrs_seq *seq, int64_t counts[9]);
+/* Copies the lines into the caller's buffers. */
+mrs_status mrs_seq_copy(const mrs_seq *seq, const mrs_seq_lines *dst);
+void mrs_seq_destroy(mrs_seq *seq);
+
 #ifdef __cplusplus
 }
 #endif

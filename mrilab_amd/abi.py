@@ -338,3 +338,14 @@ def pack_sar(VObj, VCoi, VCtl, VVar, VSig, n_uts: int):
     sig = SarSigC(SARBase=_a(base), SARi=int(VVar.get("SARi", 0)), SAR=_a(SAR),
                   Muts=_a(Muts), tSample=_a(tS), tRealSample=_a(tR), n_samples=ns)
     return obj, sig, (ECon, E, base, tS, SAR, Muts, tR)
+
+
+class SeqParamC(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("value", C.POINTER(C.c_double)), ("n", C.c_int32),
+                ("reserved", C.c_int32), ("text", C.c_char_p)]
+
+
+class SeqLinesC(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in
+                ("utsLine", "tsLine", "rfAmpLine", "rfPhaseLine", "rfFreqLine", "rfCoilLine", "GzAmpLine",
+                 "GyAmpLine", "GxAmpLine", "ADCLine", "ExtLine", "flagsLine")]

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libmrilab_amd.so")
 ARCH = os.environ.get("MRS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["engine.hip", "multipool.hip", "batch.hip", "adjoint.hip", "recon.hip", "sar.hip", "sar_average.hip"]
-CXX_SOURCES = ["decode.cpp"]
+CXX_SOURCES = ["decode.cpp", "seqgen.cpp"]
 
 
 def hipcc() -> str:

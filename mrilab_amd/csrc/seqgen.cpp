@@ -1,0 +1,996 @@
+// Sequence compiler: a PSD XML (MRiLab's MRiLabSeq format) and the VCtl
+// scan parameters to the VSeq lines the scan engines take, without MATLAB.
+//
+// MRiLab turns a PSD XML into a MATLAB function (DoWriteXML2m.m:11-337),
+// which DoPulseGen.m:18-117 calls once per TR and concatenates; each
+// sequence element is a function of Macro/SeqElem. This file restates that
+// pipeline in C++:
+//  - XmlReader: elements, attributes and nesting of the PSD file;
+//  - Expr: the attribute expressions (numbers, CVn, VCtl.x, VCtl.x(i),
+//    VVar.x, VObj.Gyro, pi, Inf, + - * / ^, unary signs, parentheses, abs,
+//    rem, mod, floor, ceil, round, sqrt, exp, max, min), evaluated in fp64
+//    left to right as MATLAB does; "$k'a','b',..." picks option k;
+//  - the elements rfSinc, rfRect, rfGaussian, rfFermi, GzSelective,
+//    GzCartesian, GyCartesian, GxCartesian, ADCCartesian and ExtBit, with
+//    StdTrap/StdRect/StdRamp/StdBit, DoB1Scaling and MATLAB's linspace;
+//  - psd_tr: one TR of the generated PSD function (section gating by
+//    TRStart/TREnd/Freq/Switch, rf TE anchoring, master-coil selection,
+//    clipping to the section, |rf| < eps to 0, time offset, then the
+//    stable sort of the merged time points, unique, and zeroed first and
+//    last amplitudes);
+//  - compile: DoPulseGen's slice x phase loop with dummy pulses and line
+//    concatenation.
+// Times are seconds, gradients T/m, rf amplitudes T, as in MRiLab.
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mrilab_amd.h"
+
+namespace mrs {
+// engine.hip: records msg as mrs_last_error() and returns st
+mrs_status fail(mrs_status st, const std::string &msg);
+}  // namespace mrs
+
+struct mrs_seq {
+    std::vector<double> uts, ts, rfAmp, rfPhase, rfFreq, rfCoil, Gz, Gy, Gx, ADC, Ext, flags;
+    int32_t n_tr = 0;
+};
+
+namespace mrs {
+namespace seqgen {
+
+using Vec = std::vector<double>;
+constexpr double kPi = 3.141592653589793;  // MATLAB's pi
+constexpr double kEps = 2.220446049250313e-16;
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// ---------------------------------------------------------------- XML
+struct Node {
+    std::string name;
+    std::vector<std::pair<std::string, std::string>> attrs;
+    std::vector<Node> kids;
+    const std::string *attr(const std::string &n) const {
+        for (auto &a : attrs)
+            if (a.first == n) return &a.second;
+        return nullptr;
+    }
+};
+
+class XmlReader {
+   public:
+    explicit XmlReader(const std::string &s) : s_(s) {}
+    Node document() {
+        skip_misc();
+        if (i_ >= s_.size() || s_[i_] != '<') throw Error("PSD XML: no root element");
+        Node root = element();
+        skip_misc();
+        if (i_ != s_.size()) throw Error("PSD XML: text after the root element");
+        return root;
+    }
+
+   private:
+    const std::string &s_;
+    size_t i_ = 0;
+    bool starts(const char *p) const { return s_.compare(i_, std::strlen(p), p) == 0; }
+    void skip_ws() {
+        while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) i_++;
+    }
+    void skip_until(const char *end) {
+        const size_t k = s_.find(end, i_);
+        if (k == std::string::npos) throw Error(std::string("PSD XML: missing ") + end);
+        i_ = k + std::strlen(end);
+    }
+    void skip_misc() {  // whitespace, <?...?>, <!--...-->, <!DOCTYPE ...>
+        for (;;) {
+            skip_ws();
+            if (starts("<?"))
+                skip_until("?>");
+            else if (starts("<!--"))
+                skip_until("-->");
+            else if (starts("<!"))
+                skip_until(">");
+            else
+                return;
+        }
+    }
+    std::string name() {
+        const size_t b = i_;
+        while (i_ < s_.size() && (std::isalnum((unsigned char)s_[i_]) || s_[i_] == '_' || s_[i_] == ':' ||
+                                  s_[i_] == '-' || s_[i_] == '.'))
+            i_++;
+        if (b == i_) throw Error("PSD XML: expected a name at offset " + std::to_string(b));
+        return s_.substr(b, i_ - b);
+    }
+    static std::string unescape(const std::string &v) {
+        std::string o;
+        for (size_t k = 0; k < v.size(); k++) {
+            if (v[k] != '&') {
+                o += v[k];
+                continue;
+            }
+            const size_t e = v.find(';', k);
+            if (e == std::string::npos) throw Error("PSD XML: bad entity");
+            const std::string ent = v.substr(k + 1, e - k - 1);
+            if (ent == "lt") o += '<';
+            else if (ent == "gt") o += '>';
+            else if (ent == "amp") o += '&';
+            else if (ent == "quot") o += '"';
+            else if (ent == "apos") o += '\'';
+            else throw Error("PSD XML: unknown entity &" + ent + ";");
+            k = e;
+        }
+        return o;
+    }
+    Node element() {
+        i_++;  // '<'
+        Node n;
+        n.name = name();
+        for (;;) {
+            skip_ws();
+            if (i_ >= s_.size()) throw Error("PSD XML: unterminated tag <" + n.name);
+            if (starts("/>")) {
+                i_ += 2;
+                return n;
+            }
+            if (s_[i_] == '>') {
+                i_++;
+                break;
+            }
+            std::string an = name();
+            skip_ws();
+            if (i_ >= s_.size() || s_[i_] != '=') throw Error("PSD XML: attribute " + an + " has no value");
+            i_++;
+            skip_ws();
+            const char q = i_ < s_.size() ? s_[i_] : 0;
+            if (q != '"' && q != '\'') throw Error("PSD XML: unquoted attribute " + an);
+            const size_t e = s_.find(q, i_ + 1);
+            if (e == std::string::npos) throw Error("PSD XML: unterminated attribute " + an);
+            n.attrs.emplace_back(an, unescape(s_.substr(i_ + 1, e - i_ - 1)));
+            i_ = e + 1;
+        }
+        for (;;) {  // content: child elements; text is ignored
+            skip_misc();
+            if (i_ >= s_.size()) throw Error("PSD XML: missing </" + n.name + ">");
+            if (starts("</")) {
+                i_ += 2;
+                if (name() != n.name) throw Error("PSD XML: mismatched </...> for <" + n.name + ">");
+                skip_ws();
+                if (i_ >= s_.size() || s_[i_] != '>') throw Error("PSD XML: bad closing tag");
+                i_++;
+                return n;
+            }
+            if (s_[i_] == '<') {
+                n.kids.push_back(element());
+            } else {
+                while (i_ < s_.size() && s_[i_] != '<') i_++;
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------- values
+struct Ctx {
+    std::map<std::string, Vec> num;          // VCtl numbers (arrays allowed)
+    std::map<std::string, std::string> txt;  // VCtl text fields
+    std::map<std::string, double> cv;        // CVs of the PSD
+    double tr_count = 0, slice_count = 0, phase_count = 0, gyro = 0;
+
+    double vctl(const std::string &f, int idx = 1) const {
+        auto it = num.find(f);
+        if (it == num.end()) throw Error("VCtl." + f + " is not set");
+        if (idx < 1 || idx > (int)it->second.size())
+            throw Error("VCtl." + f + "(" + std::to_string(idx) + ") is out of range");
+        return it->second[idx - 1];
+    }
+    std::string vtext(const std::string &f, const std::string &dflt) const {
+        auto it = txt.find(f);
+        return it == txt.end() ? dflt : it->second;
+    }
+    bool has(const std::string &f) const { return num.count(f) > 0; }
+};
+
+// Recursive-descent evaluator of a MATLAB scalar expression, left to right
+// within a precedence level (MATLAB: ^ binds tighter than unary minus).
+class Expr {
+   public:
+    Expr(const std::string &s, const Ctx &c) : s_(s), c_(c) {}
+    double run() {
+        const double v = sum();
+        ws();
+        if (i_ != s_.size()) throw Error("cannot evaluate '" + s_ + "'");
+        return v;
+    }
+
+   private:
+    const std::string &s_;
+    const Ctx &c_;
+    size_t i_ = 0;
+    void ws() {
+        while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) i_++;
+    }
+    bool eat(char ch) {
+        ws();
+        if (i_ < s_.size() && s_[i_] == ch) {
+            i_++;
+            return true;
+        }
+        return false;
+    }
+    double sum() {
+        double v = prod();
+        for (;;) {
+            if (eat('+'))
+                v = v + prod();
+            else if (eat('-'))
+                v = v - prod();
+            else
+                return v;
+        }
+    }
+    double prod() {
+        double v = unary();
+        for (;;) {
+            if (eat('*'))
+                v = v * unary();
+            else if (eat('/'))
+                v = v / unary();
+            else
+                return v;
+        }
+    }
+    double unary() {
+        if (eat('-')) return -unary();
+        if (eat('+')) return unary();
+        return power();
+    }
+    double power() {
+        double v = primary();
+        while (eat('^')) {
+            double e;
+            if (eat('-'))
+                e = -primary();
+            else {
+                eat('+');
+                e = primary();
+            }
+            v = std::pow(v, e);
+        }
+        return v;
+    }
+    std::string ident() {
+        ws();
+        const size_t b = i_;
+        while (i_ < s_.size() && (std::isalnum((unsigned char)s_[i_]) || s_[i_] == '_')) i_++;
+        return s_.substr(b, i_ - b);
+    }
+    std::vector<double> args() {
+        std::vector<double> a;
+        if (eat(')')) return a;
+        for (;;) {
+            a.push_back(sum());
+            if (eat(')')) return a;
+            if (!eat(',')) throw Error("expected ',' or ')' in '" + s_ + "'");
+        }
+    }
+    double primary() {
+        ws();
+        if (i_ >= s_.size()) throw Error("unexpected end of '" + s_ + "'");
+        const char ch = s_[i_];
+        if (ch == '(') {
+            i_++;
+            const double v = sum();
+            if (!eat(')')) throw Error("missing ')' in '" + s_ + "'");
+            return v;
+        }
+        if (std::isdigit((unsigned char)ch) || ch == '.') {
+            const char *p = s_.c_str() + i_;
+            char *end = nullptr;
+            const double v = std::strtod(p, &end);
+            if (end == p) throw Error("bad number in '" + s_ + "'");
+            i_ += (size_t)(end - p);
+            return v;
+        }
+        std::string id = ident();
+        if (id.empty()) throw Error("cannot evaluate '" + s_ + "'");
+        if (id == "VCtl" || id == "VVar" || id == "VObj") {
+            if (!eat('.')) throw Error("expected a field after " + id);
+            const std::string f = ident();
+            if (id == "VVar") {
+                if (f == "TRCount") return c_.tr_count;
+                if (f == "SliceCount") return c_.slice_count;
+                if (f == "PhaseCount") return c_.phase_count;
+                throw Error("VVar." + f + " is not available to a PSD");
+            }
+            if (id == "VObj") {
+                if (f == "Gyro") return c_.gyro;
+                throw Error("VObj." + f + " is not available to a PSD");
+            }
+            if (eat('(')) {
+                const auto a = args();
+                if (a.size() != 1 || a[0] != std::floor(a[0])) throw Error("bad index of VCtl." + f);
+                return c_.vctl(f, (int)a[0]);
+            }
+            return c_.vctl(f);
+        }
+        if (id == "pi") return kPi;
+        if (id == "Inf" || id == "inf") return INFINITY;
+        if (id == "NaN" || id == "nan") return NAN;
+        if (id == "eps") return kEps;
+        auto cv = c_.cv.find(id);
+        if (cv != c_.cv.end()) return cv->second;
+        if (eat('(')) {
+            const auto a = args();
+            auto one = [&](void) {
+                if (a.size() != 1) throw Error(id + " takes one argument");
+                return a[0];
+            };
+            auto two = [&](void) {
+                if (a.size() != 2) throw Error(id + " takes two arguments");
+            };
+            if (id == "abs") return std::fabs(one());
+            if (id == "floor") return std::floor(one());
+            if (id == "ceil") return std::ceil(one());
+            if (id == "round") return std::round(one());  // half away from zero, as MATLAB
+            if (id == "sqrt") return std::sqrt(one());
+            if (id == "exp") return std::exp(one());
+            if (id == "rem") {
+                two();
+                return a[1] == 0 ? NAN : std::fmod(a[0], a[1]);
+            }
+            if (id == "mod") {
+                two();
+                if (a[1] == 0) return a[0];
+                const double r = std::fmod(a[0], a[1]);
+                return (r != 0 && ((r < 0) != (a[1] < 0))) ? r + a[1] : r;
+            }
+            if (id == "max" || id == "min") {
+                two();
+                return id == "max" ? std::max(a[0], a[1]) : std::min(a[0], a[1]);
+            }
+            throw Error("unknown function " + id + " in '" + s_ + "'");
+        }
+        throw Error("unknown name " + id + " in '" + s_ + "'");
+    }
+};
+
+// An attribute as DoWriteXML2m.m:219-232 turns it into MATLAB: "$k'a','b'"
+// selects option k, Notes is text, anything else an expression.
+struct Param {
+    bool text = false, empty = false;
+    double v = 0;
+    std::string s;
+};
+
+static std::vector<std::string> option_list(const std::string &v) {
+    std::vector<std::string> out;
+    size_t k = 0;
+    while (k < v.size()) {
+        while (k < v.size() && (std::isspace((unsigned char)v[k]) || v[k] == ',')) k++;
+        if (k >= v.size()) break;
+        if (v[k] != '\'') throw Error("bad option list '" + v + "'");
+        const size_t e = v.find('\'', k + 1);
+        if (e == std::string::npos) throw Error("bad option list '" + v + "'");
+        out.push_back(v.substr(k + 1, e - k - 1));
+        k = e + 1;
+    }
+    return out;
+}
+
+static Param param_of(const std::string &name, const std::string &v, const Ctx &c) {
+    Param p;
+    if (v.empty()) {
+        p.empty = true;
+        return p;
+    }
+    if (v[0] == '$') {
+        if (v.size() < 2 || !std::isdigit((unsigned char)v[1])) throw Error("bad option attribute " + name);
+        const auto opts = option_list(v.substr(2));
+        const int k = v[1] - '0';
+        if (k < 1 || k > (int)opts.size()) throw Error("option index out of range in " + name);
+        p.text = true;
+        p.s = opts[k - 1];
+        return p;
+    }
+    if (name == "Notes") {
+        p.text = true;
+        p.s = v;
+        return p;
+    }
+    p.v = Expr(v, c).run();
+    return p;
+}
+
+struct Params {
+    std::map<std::string, Param> m;
+    std::string elem;
+    double num(const std::string &n) const {
+        auto it = m.find(n);
+        if (it == m.end() || it->second.empty) throw Error(elem + "." + n + " is not set");
+        if (it->second.text) throw Error(elem + "." + n + " must be numeric");
+        return it->second.v;
+    }
+    std::string text(const std::string &n) const {
+        auto it = m.find(n);
+        if (it == m.end() || it->second.empty) throw Error(elem + "." + n + " is not set");
+        if (!it->second.text) throw Error(elem + "." + n + " must be an option");
+        return it->second.s;
+    }
+};
+
+// ---------------------------------------------------------------- helpers
+// MATLAB linspace: d1 + (k*(d2 - d1))/(n - 1), with the ends set exactly.
+static Vec linspace(double d1, double d2, double nd) {
+    const int64_t n = (int64_t)std::floor(nd);
+    if (n < 1) return Vec();
+    if (n == 1) return Vec{d2};
+    Vec y((size_t)n);
+    const double n1 = (double)(n - 1), d = d2 - d1;
+    for (int64_t k = 0; k < n; k++) y[(size_t)k] = d1 + ((double)k * d) / n1;
+    y[0] = d1;
+    y[(size_t)n - 1] = d2;
+    return y;
+}
+
+// [t, m] = unique(t) (sorted, first occurrence) applied to (amp, t).
+static void unique_pair(Vec &amp, Vec &t) {
+    std::vector<size_t> ix(t.size());
+    std::iota(ix.begin(), ix.end(), 0);
+    std::stable_sort(ix.begin(), ix.end(), [&](size_t a, size_t b) { return t[a] < t[b]; });
+    Vec a2, t2;
+    for (size_t k = 0; k < ix.size(); k++) {
+        if (k > 0 && t[ix[k]] == t2.back()) continue;
+        t2.push_back(t[ix[k]]);
+        a2.push_back(amp[ix[k]]);
+    }
+    amp.swap(a2);
+    t.swap(t2);
+}
+
+static void append(Vec &a, const Vec &b) { a.insert(a.end(), b.begin(), b.end()); }
+
+// StdTrap.m: ramp up, flat top, ramp down; then unique on time.
+static void std_trap(double t0, double t1, double tu, double td, double amp, double su, double sf, double sd,
+                     Vec &g, Vec &t) {
+    g.clear();
+    t.clear();
+    append(t, linspace(t0, tu, su));
+    append(g, linspace(0.0, amp, su));
+    const Vec tf = linspace(tu, td, sf);
+    append(t, tf);
+    g.insert(g.end(), tf.size(), amp);
+    append(t, linspace(td, t1, sd));
+    append(g, linspace(amp, 0.0, sd));
+    unique_pair(g, t);
+}
+
+// DoB1Scaling.m: gain giving flip angle FA with the decoder's min-hold rule.
+static double b1_gain(const Vec &p, double dt, double fa, double gyro) {
+    const size_t n = p.size();
+    std::vector<char> flag(n, 0);
+    for (size_t k = 0; k + 1 < n; k++) {
+        const double d = std::fabs(p[k + 1]) - std::fabs(p[k]);
+        if (d < 0) flag[k + 1] = 1;
+        if (d >= 0) flag[k] = 1;
+    }
+    double s = 0;
+    for (size_t k = 0; k < n; k++) s = s + (flag[k] ? p[k] : 0.0);
+    return ((fa / 180) * kPi) / (gyro * s * dt);
+}
+
+// Duplicates/DupSpacing of every element: repeat the samples, shifting the
+// copies by DupSpacing.
+static void duplicate(std::vector<Vec *> vals, Vec &t, const Params &p) {
+    const double dups = std::max(1.0, p.num("Duplicates")), sp = std::max(0.0, p.num("DupSpacing"));
+    if (dups == 1 || sp == 0) return;
+    const int64_t nd = (int64_t)dups;
+    for (Vec *v : vals) {
+        Vec o;
+        for (int64_t d = 0; d < nd; d++) append(o, *v);
+        v->swap(o);
+    }
+    Vec o;
+    for (int64_t d = 0; d < nd; d++)
+        for (double x : t) o.push_back(x + (double)d * sp);
+    t.swap(o);
+}
+
+// ---------------------------------------------------------------- elements
+struct RfOut {
+    Vec amp, phase, freq, coil, t;
+};
+
+static void rf_finish(RfOut &r, const Params &p, const Ctx &c) {
+    const size_t n = r.t.size();
+    const double dt = p.num("dt");
+    r.amp.front() = 0;
+    r.amp.back() = 0;
+    const double g = b1_gain(r.amp, dt, p.num("FA"), c.gyro);
+    for (double &a : r.amp) a = g * a;
+    r.phase.assign(n, p.num("rfPhase"));
+    r.freq.assign(n, p.num("rfFreq"));
+    r.coil.assign(n, p.num("CoilID"));
+    r.phase.front() = r.phase.back() = 0;
+    r.freq.front() = r.freq.back() = 0;
+    duplicate({&r.amp, &r.freq, &r.phase, &r.coil}, r.t, p);
+}
+
+static RfOut rf_element(const std::string &name, const Params &p, const Ctx &c) {
+    RfOut r;
+    const double ts = p.num("tStart"), te = p.num("tEnd"), dt = p.num("dt");
+    r.t = linspace(ts, te, std::ceil((te - ts) / dt) + 1);
+    const size_t n = r.t.size();
+    if (n < 2) throw Error(name + ": fewer than 2 samples");
+    r.amp.resize(n);
+    if (name == "rfSinc") {  // rfSinc.m:24-42
+        const std::string apod = p.text("Apod");
+        const double a = apod == "Hamming" ? 0.46 : apod == "Hanning" ? 0.5 : 0.0;
+        if (apod != "Non" && apod != "Hamming" && apod != "Hanning") throw Error("rfSinc: unknown Apod " + apod);
+        const double tbp = p.num("TBP"), t0 = (te - ts) / tbp;
+        double mx = -INFINITY;
+        std::vector<char> nan(n, 0);
+        for (size_t k = 0; k < n; k++) {
+            const double t = r.t[k] - (te - ts) / 2 - ts;  // centred, then shifted back below
+            const double v = ((t0 * std::sin(kPi * t / t0)) / (kPi * t)) * ((1 - a) + a * std::cos((kPi * t) / (tbp * t0)));
+            r.amp[k] = v;
+            r.t[k] = t + (te - ts) / 2 + ts;
+            if (std::isnan(v))
+                nan[k] = 1;
+            else
+                mx = std::max(mx, v);
+        }
+        for (size_t k = 0; k < n; k++)
+            if (nan[k]) r.amp[k] = mx;
+    } else if (name == "rfRect") {  // rfRect.m
+        std::fill(r.amp.begin(), r.amp.end(), 1.0);
+    } else if (name == "rfGaussian") {  // rfGaussian.m:19-23
+        const double sigma = (te - ts) / 7.434;
+        for (size_t k = 0; k < n; k++) {
+            const double d = r.t[k] - (te - ts) / 2 - ts;
+            r.amp[k] = std::exp(-(d * d) / (2 * (sigma * sigma)));
+        }
+    } else if (name == "rfFermi") {  // rfFermi.m:20-23
+        const double pw = p.num("PW"), tw = ((te - ts) - 2 * pw) / 13.81;
+        for (size_t k = 0; k < n; k++)
+            r.amp[k] = 1 / (1 + std::exp((std::fabs(r.t[k] - (te - ts) / 2 - ts) - pw) / tw));
+    } else {
+        throw Error("rf element " + name + " is not supported");
+    }
+    rf_finish(r, p, c);
+    return r;
+}
+
+struct GOut {
+    Vec amp, t;
+};
+
+// GzCartesian.m / GyCartesian.m: phase-encoding trapezoid and optional rephaser.
+static GOut g_cartesian_pe(const Params &p, const Ctx &c, bool z) {
+    const double t1s = p.num("t1Start"), t1e = p.num("t1End"), t2s = p.num("t2Start"), t2e = p.num("t2End");
+    const double tr = p.num("tRamp");
+    const double s1 = p.num(z ? "Gz1Sign" : "Gy1Sign"), s2 = p.num(z ? "Gz2Sign" : "Gy2Sign");
+    const double fov = c.vctl(z ? "FOVSlice" : "FOVPhase");
+    const double step = z ? (c.slice_count - std::floor(c.vctl("SliceNum") / 2) - 1)
+                          : (c.phase_count - c.vctl("ResPhase") / 2 - 1);
+    const double gbar = c.gyro / (2 * kPi);
+    GOut o;
+    Vec g2, t2;
+    double dG = 1 / (gbar * (t1e - t1s) * fov);
+    std_trap(t1s - tr, t1e + tr, t1s, t1e, step * dG * s1, 2, 2, 2, o.amp, o.t);
+    dG = 1 / (gbar * (t2e - t2s) * fov);
+    std_trap(t2s - tr, t2e + tr, t2s, t2e, step * dG * s2, 2, 2, 2, g2, t2);
+    if (s2 != 0) {
+        append(o.amp, g2);
+        append(o.t, t2);
+    }
+    unique_pair(o.amp, o.t);
+    duplicate({&o.amp}, o.t, p);
+    return o;
+}
+
+// GxCartesian.m: prephaser, readout centred at t2Middle + TEAnchorTime, rewinder.
+static GOut gx_cartesian(const Params &p, const Ctx &c) {
+    const double t1s = p.num("t1Start"), t2m = p.num("t2Middle"), t3s = p.num("t3Start"), tr = p.num("tRamp");
+    const double s1 = p.num("Gx1Sign"), s2 = p.num("Gx2Sign"), s3 = p.num("Gx3Sign");
+    const double gbar = c.gyro / (2 * kPi);
+    const double gx = (1 / c.vctl("FOVFreq")) / (gbar * (1 / c.vctl("BandWidth")));
+    const double th = 1 / (2 * gbar * gx * c.vctl("RFreq"));
+    const double ta = c.vctl("TEAnchorTime");
+    Vec g1, tt1, g2, tt2, g3, tt3;
+    std_trap(t1s - tr, t1s + th + tr, t1s, t1s + th, gx * s1, 2, 2, 2, g1, tt1);
+    std_trap(t2m + ta - th - tr, t2m + ta + th + tr, t2m + ta - th, t2m + ta + th, gx * s2, 2, 2, 2, g2, tt2);
+    std_trap(t3s - tr, t3s + th + tr, t3s, t3s + th, gx * s3, 2, 2, 2, g3, tt3);
+    GOut o;
+    if (s1 != 0) {
+        o.amp = g1;
+        o.t = tt1;
+    }
+    append(o.amp, g2);
+    append(o.t, tt2);
+    if (s3 != 0) {
+        append(o.amp, g3);
+        append(o.t, tt3);
+    }
+    unique_pair(o.amp, o.t);
+    duplicate({&o.amp}, o.t, p);
+    return o;
+}
+
+// GzSelective.m: slice-select trapezoid with optional pre- and rephasers.
+static GOut gz_selective(const Params &p) {
+    const double t2s = p.num("t2Start"), t2e = p.num("t2End"), tr = p.num("tRamp"), ga = p.num("GzAmp");
+    const double s1 = p.num("Gz1Sign"), s2 = p.num("Gz2Sign"), s3 = p.num("Gz3Sign");
+    Vec g1, tt1, g2, tt2, g3, tt3;
+    const double half = (t2e - t2s) / 2;
+    std_trap(0, half + 2 * tr, tr, half + tr, s1 * ga, 2, 2, 2, g1, tt1);
+    std_trap(t2s - tr, t2e + tr, t2s, t2e, s2 * ga, 2, 2, 2, g2, tt2);
+    std_trap(0, half + 2 * tr, tr, half + tr, s3 * ga, 2, 2, 2, g3, tt3);
+    GOut o;
+    if (s1 != 0) {
+        o.amp = g1;
+        const double last = tt1.back();
+        for (double t : tt1) o.t.push_back(t2s - tr - last + t);
+    }
+    append(o.amp, g2);
+    append(o.t, tt2);
+    if (s3 != 0) {
+        append(o.amp, g3);
+        for (double t : tt3) o.t.push_back(t2e + tr + t);
+    }
+    unique_pair(o.amp, o.t);
+    duplicate({&o.amp}, o.t, p);
+    return o;
+}
+
+// ADCCartesian.m: ResFreq samples over the readout flat top.
+static GOut adc_cartesian(const Params &p, const Ctx &c) {
+    const double tm = p.num("tMiddle");
+    const double gbar = c.gyro / (2 * kPi);
+    const double gx = (1 / c.vctl("FOVFreq")) / (gbar * (1 / c.vctl("BandWidth")));
+    const double th = 1 / (2 * gbar * gx * c.vctl("RFreq"));
+    const double ta = c.vctl("TEAnchorTime"), mu = c.vctl("MinUpdRate");
+    GOut o;
+    std_trap(tm + ta - th - mu, tm + ta + th + mu, tm + ta - th, tm + ta + th, 1, 2, c.vctl("ResFreq"), 2, o.amp, o.t);
+    unique_pair(o.amp, o.t);
+    duplicate({&o.amp}, o.t, p);
+    return o;
+}
+
+// ExtBit.m / StdBit.m: one flagged point between two zeros.
+static GOut ext_bit(const Params &p, const Ctx &c) {
+    const double ts = p.num("tStart");
+    GOut o;
+    o.t = linspace(ts, ts + 2 * c.vctl("MinUpdRate"), 3);
+    o.amp.assign(o.t.size(), p.num("Ext"));
+    o.amp.front() = 0;
+    o.amp.back() = 0;
+    duplicate({&o.amp}, o.t, p);
+    return o;
+}
+
+// ---------------------------------------------------------------- one TR
+struct Lines {
+    Vec rfAmp, rfPhase, rfFreq, rfCoil, rfT, Gz, GzT, Gy, GyT, Gx, GxT, ADC, ADCT, Ext, ExtT;
+};
+
+struct TrOut {
+    Vec uts, ts, flags, rfAmp, rfPhase, rfFreq, rfCoil, Gz, Gy, Gx, ADC, Ext;
+};
+
+static Params params_of(const Node &e, const Ctx &c) {
+    Params p;
+    p.elem = e.name;
+    for (auto &a : e.attrs) p.m[a.first] = param_of(a.first, a.second, c);
+    return p;
+}
+
+static bool section_on(const Params &s, const Ctx &c) {
+    const double start = s.num("TRStart"), end = s.num("TREnd"), freq = s.num("Freq");
+    const double r = std::fmod(c.tr_count - start, freq);
+    return !(c.tr_count < start || c.tr_count > end || r != 0 || s.text("Switch") == "off");
+}
+
+// Stable sort of a line by time, keeping the paired amplitudes.
+static std::vector<size_t> order_of(const Vec &t) {
+    std::vector<size_t> ix(t.size());
+    std::iota(ix.begin(), ix.end(), 0);
+    std::stable_sort(ix.begin(), ix.end(), [&](size_t a, size_t b) { return t[a] < t[b]; });
+    return ix;
+}
+
+static Vec take(const Vec &v, const std::vector<size_t> &ix) {
+    Vec o(ix.size());
+    for (size_t k = 0; k < ix.size(); k++) o[k] = v[ix[k]];
+    return o;
+}
+
+// The generated PSD function for the current VVar counters
+// (DoWriteXML2m.m:148-337).
+static TrOut psd_tr(const Node &root, Ctx &c) {
+    Lines all;
+    Vec se;
+    int sec = 0;
+    for (auto &pulses : root.kids) {
+        if (pulses.name != "Pulses") continue;
+        sec++;
+        const Params s = params_of(pulses, c);
+        const double tS = s.num("tS"), tE = s.num("tE");
+        if (!(tS < tE)) throw Error("SE setting is incorrect for Pulses " + std::to_string(sec) + "!");
+        if (!section_on(s, c)) continue;
+        Lines L;
+        for (auto &grp : pulses.kids) {
+            for (auto &el : grp.kids) {
+                const Params p = params_of(el, c);
+                if (p.text("Switch") != "on") continue;
+                if (grp.name == "rf") {
+                    if (p.text("AnchorTE") == "on") {
+                        const std::string a = c.vtext("TEAnchor", "Middle");
+                        if (a == "Start") c.num["TEAnchorTime"] = {p.num("tStart")};
+                        else if (a == "Middle") c.num["TEAnchorTime"] = {(p.num("tStart") + p.num("tEnd")) / 2};
+                        else if (a == "End") c.num["TEAnchorTime"] = {p.num("tEnd")};
+                    }
+                    RfOut r = rf_element(el.name, p, c);
+                    const bool multi = c.vtext("MultiTransmit", "off") != "off";
+                    if (multi || c.vctl("MasterTxCoil") == r.coil.front()) {
+                        append(L.rfAmp, r.amp);
+                        append(L.rfPhase, r.phase);
+                        append(L.rfFreq, r.freq);
+                        append(L.rfCoil, r.coil);
+                        append(L.rfT, r.t);
+                    }
+                } else if (grp.name == "GzSS") {
+                    GOut g;
+                    if (el.name == "GzCartesian") g = g_cartesian_pe(p, c, true);
+                    else if (el.name == "GzSelective") g = gz_selective(p);
+                    else throw Error("GzSS element " + el.name + " is not supported");
+                    append(L.Gz, g.amp);
+                    append(L.GzT, g.t);
+                } else if (grp.name == "GyPE") {
+                    if (el.name != "GyCartesian") throw Error("GyPE element " + el.name + " is not supported");
+                    GOut g = g_cartesian_pe(p, c, false);
+                    append(L.Gy, g.amp);
+                    append(L.GyT, g.t);
+                } else if (grp.name == "GxR") {
+                    if (el.name != "GxCartesian") throw Error("GxR element " + el.name + " is not supported");
+                    GOut g = gx_cartesian(p, c);
+                    append(L.Gx, g.amp);
+                    append(L.GxT, g.t);
+                } else if (grp.name == "ADC") {
+                    if (el.name != "ADCCartesian") throw Error("ADC element " + el.name + " is not supported");
+                    GOut g = adc_cartesian(p, c);
+                    append(L.ADC, g.amp);
+                    append(L.ADCT, g.t);
+                } else if (grp.name == "Ext") {
+                    if (el.name != "ExtBit") throw Error("Ext element " + el.name + " is not supported");
+                    GOut g = ext_bit(p, c);
+                    append(L.Ext, g.amp);
+                    append(L.ExtT, g.t);
+                } else {
+                    throw Error("unknown sequence line " + grp.name);
+                }
+            }
+        }
+        // clip to the section, rf below eps to 0, offset by tS
+        const double span = tE - tS;
+        auto clip = [&](std::vector<Vec *> vals, Vec &t) {
+            std::vector<size_t> keep;
+            for (size_t k = 0; k < t.size(); k++)
+                if (!(t[k] < 0 || t[k] > span)) keep.push_back(k);
+            for (Vec *v : vals) *v = take(*v, keep);
+            t = take(t, keep);
+            for (double &x : t) x = x + tS;
+        };
+        clip({&L.rfAmp, &L.rfPhase, &L.rfFreq, &L.rfCoil}, L.rfT);
+        for (double &a : L.rfAmp)
+            if (std::fabs(a) < kEps) a = 0;
+        clip({&L.Gz}, L.GzT);
+        clip({&L.Gy}, L.GyT);
+        clip({&L.Gx}, L.GxT);
+        clip({&L.ADC}, L.ADCT);
+        clip({&L.Ext}, L.ExtT);
+        append(all.rfAmp, L.rfAmp);
+        append(all.rfPhase, L.rfPhase);
+        append(all.rfFreq, L.rfFreq);
+        append(all.rfCoil, L.rfCoil);
+        append(all.rfT, L.rfT);
+        append(all.Gz, L.Gz);
+        append(all.GzT, L.GzT);
+        append(all.Gy, L.Gy);
+        append(all.GyT, L.GyT);
+        append(all.Gx, L.Gx);
+        append(all.GxT, L.GxT);
+        append(all.ADC, L.ADC);
+        append(all.ADCT, L.ADCT);
+        append(all.Ext, L.Ext);
+        append(all.ExtT, L.ExtT);
+        se.push_back(tS);
+        se.push_back(tE);
+    }
+    if (all.rfT.empty()) throw Error("rf sequence line can not be empty! Master Tx coil element must be used.");
+    if (all.GzT.empty()) throw Error("GzSS sequence line can not be empty!");
+    if (all.GyT.empty()) throw Error("GyPE sequence line can not be empty!");
+    if (all.GxT.empty()) throw Error("GxR sequence line can not be empty!");
+    if (all.ADCT.empty()) throw Error("ADC sequence line can not be empty!");
+    if (all.ExtT.empty()) throw Error("Ext sequence line can not be empty!");
+    const double t0 = *std::min_element(se.begin(), se.end()), t1 = *std::max_element(se.begin(), se.end());
+    TrOut o;
+    Vec ts{t0, t1};
+    std::vector<int> fl{-1, -1};
+    const Vec *lines[6] = {&all.rfT, &all.GzT, &all.GyT, &all.GxT, &all.ADCT, &all.ExtT};
+    for (int l = 0; l < 6; l++)
+        for (double t : *lines[l]) {
+            ts.push_back(t);
+            fl.push_back(l);
+        }
+    for (double &t : ts) t = t - t0;
+    const auto ix = order_of(ts);
+    o.ts = take(ts, ix);
+    o.flags.assign(6 * ix.size(), 0.0);
+    for (size_t k = 0; k < ix.size(); k++)
+        if (fl[ix[k]] >= 0) o.flags[6 * k + fl[ix[k]]] = 1;
+    for (size_t k = 0; k < o.ts.size(); k++)
+        if (k == 0 || o.ts[k] != o.ts[k - 1]) o.uts.push_back(o.ts[k]);
+    auto sorted = [&](const Vec &t, std::vector<Vec *> vals, std::vector<Vec *> outs) {
+        Vec tt = t;
+        for (double &x : tt) x = x - t0;
+        const auto jx = order_of(tt);
+        for (size_t v = 0; v < vals.size(); v++) *outs[v] = take(*vals[v], jx);
+    };
+    sorted(all.rfT, {&all.rfAmp, &all.rfPhase, &all.rfFreq, &all.rfCoil}, {&o.rfAmp, &o.rfPhase, &o.rfFreq, &o.rfCoil});
+    sorted(all.GzT, {&all.Gz}, {&o.Gz});
+    sorted(all.GyT, {&all.Gy}, {&o.Gy});
+    sorted(all.GxT, {&all.Gx}, {&o.Gx});
+    sorted(all.ADCT, {&all.ADC}, {&o.ADC});
+    sorted(all.ExtT, {&all.Ext}, {&o.Ext});
+    for (Vec *v : {&o.rfAmp, &o.rfPhase, &o.rfFreq, &o.Gz, &o.Gy, &o.Gx, &o.ADC, &o.Ext}) {
+        v->front() = 0;
+        v->back() = 0;
+    }
+    return o;
+}
+
+// DoPulseGen.m:18-117: the slice x phase loop, dummy pulses, concatenation.
+static void compile(const Node &root, Ctx &c, mrs_seq &out) {
+    if (root.name != "MRiLabSeq") throw Error("not a PSD: the root element is <" + root.name + ">");
+    for (auto &k : root.kids)
+        if (k.name == "CVs")
+            for (auto &a : k.attrs) c.cv[a.first] = a.second.empty() ? NAN : Expr(a.second, c).run();
+    const double tr0 = c.vctl("TR"), fa0 = c.vctl("FlipAng");
+    const bool dp_on = c.vtext("DP_Flag", "off") == "on";
+    const double n2 = c.vctl("SecondPhNum"), n1 = c.vctl("FirstPhNum");
+    if (!c.has("TEAnchorTime")) c.num["TEAnchorTime"] = {0.0};
+    if (!c.has("MasterTxCoil")) c.num["MasterTxCoil"] = {1.0};
+    int dp = 0;
+    double s = 1, j = 1;
+    c.tr_count = 0;
+    while (s <= n2) {
+        c.slice_count = s;
+        bool restarted = false;
+        while (j <= n1) {
+            c.phase_count = j;
+            c.tr_count = c.tr_count + 1;
+            if (c.txt.count("DP_Flag")) {
+                if (dp_on && dp == 0) {
+                    c.num["TR"] = {c.vctl("DP_TR")};
+                    c.num["FlipAng"] = {c.vctl("DP_FlipAng")};
+                } else {
+                    c.num["TR"] = {tr0};
+                    c.num["FlipAng"] = {fa0};
+                }
+            }
+            TrOut t = psd_tr(root, c);
+            append(out.rfAmp, t.rfAmp);
+            append(out.rfPhase, t.rfPhase);
+            append(out.rfFreq, t.rfFreq);
+            append(out.rfCoil, t.rfCoil);
+            append(out.Gz, t.Gz);
+            append(out.Gy, t.Gy);
+            append(out.Gx, t.Gx);
+            append(out.ADC, t.ADC);
+            append(out.Ext, t.Ext);
+            append(out.uts, t.uts);
+            append(out.ts, t.ts);
+            append(out.flags, t.flags);
+            out.n_tr++;
+            if (c.txt.count("DP_Flag") && dp_on && dp == 0) {
+                for (Vec *v : {&out.Gz, &out.Gy, &out.Gx, &out.ADC}) std::fill(v->begin(), v->end(), 0.0);
+                if (c.tr_count == c.vctl("DP_Num")) {
+                    s = 1;
+                    j = 1;
+                    dp = 1;
+                    restarted = true;
+                    break;
+                }
+            }
+            j = j + 1;
+        }
+        if (restarted) continue;
+        j = 1;
+        s = s + 1;
+    }
+    if (c.vtext("MultiTransmit", "off") == "off") std::fill(out.rfCoil.begin(), out.rfCoil.end(), 1.0);
+}
+
+}  // namespace seqgen
+}  // namespace mrs
+
+using namespace mrs;
+
+extern "C" {
+
+mrs_status mrs_seq_compile(const char *psd_xml, const mrs_seq_param *vctl, int32_t n_params, double gyro,
+                           mrs_seq **out) {
+    if (!out) return fail(MRS_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!psd_xml) return fail(MRS_ERR_ARG, "psd_xml is NULL");
+    if (n_params < 0 || (n_params > 0 && !vctl)) return fail(MRS_ERR_ARG, "bad VCtl parameter list");
+    if (!(gyro > 0) || !std::isfinite(gyro)) return fail(MRS_ERR_ARG, "gyro must be positive");
+    try {
+        seqgen::Ctx c;
+        c.gyro = gyro;
+        for (int32_t i = 0; i < n_params; i++) {
+            const mrs_seq_param &p = vctl[i];
+            if (!p.name || !*p.name) return fail(MRS_ERR_ARG, "VCtl parameter without a name");
+            if (p.text) {
+                c.txt[p.name] = p.text;
+            } else {
+                if (p.n < 1 || !p.value) return fail(MRS_ERR_ARG, std::string("VCtl.") + p.name + " has no value");
+                c.num[p.name] = seqgen::Vec(p.value, p.value + p.n);
+            }
+        }
+        const std::string text(psd_xml);
+        const seqgen::Node root = seqgen::XmlReader(text).document();
+        std::unique_ptr<mrs_seq> s(new mrs_seq());
+        seqgen::compile(root, c, *s);
+        *out = s.release();
+        return MRS_OK;
+    } catch (const seqgen::Error &e) {
+        return fail(MRS_ERR_ARG, std::string("PSD: ") + e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(MRS_ERR_NOMEM, "PSD: out of host memory");
+    } catch (const std::exception &e) {
+        return fail(MRS_ERR_ARG, std::string("PSD: ") + e.what());
+    }
+}
+
+mrs_status mrs_seq_counts(const mrs_seq *s, int64_t counts[9]) {
+    if (!s || !counts) return fail(MRS_ERR_ARG, "NULL argument");
+    counts[0] = (int64_t)s->ts.size();
+    counts[1] = (int64_t)s->uts.size();
+    counts[2] = (int64_t)s->rfAmp.size();
+    counts[3] = (int64_t)s->Gz.size();
+    counts[4] = (int64_t)s->Gy.size();
+    counts[5] = (int64_t)s->Gx.size();
+    counts[6] = (int64_t)s->ADC.size();
+    counts[7] = (int64_t)s->Ext.size();
+    counts[8] = s->n_tr;
+    return MRS_OK;
+}
+
+mrs_status mrs_seq_copy(const mrs_seq *s, const mrs_seq_lines *dst) {
+    if (!s || !dst) return fail(MRS_ERR_ARG, "NULL argument");
+    const std::pair<const std::vector<double> *, double *> pairs[] = {
+        {&s->uts, dst->utsLine},     {&s->ts, dst->tsLine},         {&s->rfAmp, dst->rfAmpLine},
+        {&s->rfPhase, dst->rfPhaseLine}, {&s->rfFreq, dst->rfFreqLine}, {&s->rfCoil, dst->rfCoilLine},
+        {&s->Gz, dst->GzAmpLine},    {&s->Gy, dst->GyAmpLine},      {&s->Gx, dst->GxAmpLine},
+        {&s->ADC, dst->ADCLine},     {&s->Ext, dst->ExtLine},       {&s->flags, dst->flagsLine}};
+    for (auto &p : pairs) {
+        if (!p.second && !p.first->empty()) return fail(MRS_ERR_ARG, "a destination line is NULL");
+        if (!p.first->empty()) std::memcpy(p.second, p.first->data(), p.first->size() * sizeof(double));
+    }
+    return MRS_OK;
+}
+
+void mrs_seq_destroy(mrs_seq *s) { delete s; }
+
+}  // extern "C"

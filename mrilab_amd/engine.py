@@ -46,7 +46,7 @@ EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanA
            "mrs_DoScanBatchAtGPU", "mrs_adjoint_create", "mrs_adjoint_forward", "mrs_adjoint_backward",
            "mrs_adjoint_get_info", "mrs_adjoint_destroy", "mrs_recon_dims", "mrs_DoCartRecon",
            "mrs_plan_recon", "mrs_DoAddNoise", "mrs_noise_scale", "mrs_DoCalSARAtGPU",
-           "mrs_DoSARAverageAtGPU")
+           "mrs_DoSARAverageAtGPU", "mrs_seq_compile", "mrs_seq_counts", "mrs_seq_copy", "mrs_seq_destroy")
 
 
 class MRSError(RuntimeError):
@@ -140,6 +140,14 @@ def load() -> C.CDLL:
                                               C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                               C.c_int32]
         lib.mrs_DoSARAverageAtGPU.restype = C.c_int
+        lib.mrs_seq_compile.argtypes = [C.c_char_p, P(abi.SeqParamC), C.c_int32, C.c_double, P(C.c_void_p)]
+        lib.mrs_seq_compile.restype = C.c_int
+        lib.mrs_seq_counts.argtypes = [C.c_void_p, P(C.c_int64)]
+        lib.mrs_seq_counts.restype = C.c_int
+        lib.mrs_seq_copy.argtypes = [C.c_void_p, P(abi.SeqLinesC)]
+        lib.mrs_seq_copy.restype = C.c_int
+        lib.mrs_seq_destroy.argtypes = [C.c_void_p]
+        lib.mrs_seq_destroy.restype = None
         _lib = lib
         return lib
 
@@ -681,3 +689,51 @@ def DoSARAverageAtGPU(SAR, Power, MassDen, dx: float, dy: float, dz: float, gram
                                      C.c_void_p(m.ctypes.data), dims, float(dx), float(dy), float(dz), float(grams),
                                      C.c_void_p(a.ctypes.data), C.c_void_p(b.ctypes.data), 0))
     return a, b
+
+
+# ---------------------------------------------------------------------------
+# Sequence compiler
+# ---------------------------------------------------------------------------
+SEQ_LINES = ("utsLine", "tsLine", "rfAmpLine", "rfPhaseLine", "rfFreqLine", "rfCoilLine", "GzAmpLine",
+             "GyAmpLine", "GxAmpLine", "ADCLine", "ExtLine", "flagsLine")
+
+
+def compile_psd(psd_xml: str, VCtl: dict, gyro: float = 267538030.3797068) -> dict:
+    """VSeq lines of a PSD XML (its text) for the scan parameters ``VCtl``
+    (numbers, numeric arrays, or strings for TEAnchor / MultiTransmit /
+    DP_Flag), as DoPulseGen builds them (mrs_seq_compile). Returns a dict
+    with the VSeq line names (flagsLine [6, n_ts] column-major) and
+    ``TRNum``, the number of TRs."""
+    lib = load()
+    keep, params = [], []
+    for name, v in VCtl.items():
+        p = abi.SeqParamC()
+        nb = name.encode()
+        keep.append(nb)
+        p.name = nb
+        if isinstance(v, str):
+            tb = v.encode()
+            keep.append(tb)
+            p.text = tb
+        else:
+            a = np.ascontiguousarray(np.atleast_1d(np.asarray(v, dtype=np.float64)).ravel())
+            keep.append(a)
+            p.value = a.ctypes.data_as(C.POINTER(C.c_double))
+            p.n = a.size
+        params.append(p)
+    arr = (abi.SeqParamC * max(1, len(params)))(*params)
+    h = C.c_void_p()
+    _check(lib.mrs_seq_compile(psd_xml.encode(), arr, len(params), float(gyro), C.byref(h)))
+    try:
+        cnt = (C.c_int64 * 9)()
+        _check(lib.mrs_seq_counts(h, cnt))
+        n_ts, n_uts, n_rf, n_gz, n_gy, n_gx, n_adc, n_ext, n_tr = list(cnt)
+        sizes = (n_uts, n_ts, n_rf, n_rf, n_rf, n_rf, n_gz, n_gy, n_gx, n_adc, n_ext, 6 * n_ts)
+        out = {k: np.zeros(n) for k, n in zip(SEQ_LINES, sizes)}
+        dst = abi.SeqLinesC(*[out[k].ctypes.data_as(C.POINTER(C.c_double)) for k in SEQ_LINES])
+        _check(lib.mrs_seq_copy(h, C.byref(dst)))
+    finally:
+        lib.mrs_seq_destroy(h)
+    out["flagsLine"] = np.asfortranarray(out["flagsLine"].reshape(6, n_ts, order="F"))
+    out["TRNum"] = int(n_tr)
+    return out

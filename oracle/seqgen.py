@@ -1,0 +1,394 @@
+"""Sequence compiler oracle -- TEST INFRASTRUCTURE ONLY.
+
+A Python restatement of MRiLab's PSD pipeline, the checker of
+mrs_seq_compile: DoWriteXML2m.m:148-337 (the generated PSD function for one
+TR), DoPulseGen.m:18-117 (the slice x phase loop, dummy pulses,
+concatenation) and the Macro/SeqElem elements rfSinc.m, rfRect.m,
+rfGaussian.m, rfFermi.m, GzCartesian.m, GyCartesian.m, GzSelective.m,
+GxCartesian.m, ADCCartesian.m, ExtBit.m with StdTrap.m, StdBit.m and
+Src/Main/DoB1Scaling.m. It parses the XML with xml.etree, evaluates
+attribute expressions with Python's own arithmetic (MATLAB's precedence and
+left-to-right order coincide with Python's for + - * / and unary minus; ^
+becomes **), and builds arrays with numpy. MATLAB semantics it keeps:
+linspace as d1 + (k*(d2-d1))/(n-1) with exact ends, unique() keeping the
+first occurrence, stable sorts, round half away from zero.
+"""
+from __future__ import annotations
+
+import math
+import re
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+EPS = 2.220446049250313e-16
+
+
+def linspace(d1, d2, n):
+    """MATLAB linspace."""
+    n = int(math.floor(n))
+    if n < 1:
+        return np.zeros(0)
+    if n == 1:
+        return np.array([float(d2)])
+    k = np.arange(n, dtype=np.float64)
+    y = d1 + (k * (d2 - d1)) / (n - 1)
+    y[0] = d1
+    y[-1] = d2
+    return y
+
+
+def unique_first(t, *vals):
+    """[t, m] = unique(t); vals(m)."""
+    t = np.asarray(t, dtype=np.float64)
+    order = np.argsort(t, kind="stable")
+    ts = t[order]
+    keep = np.ones(ts.size, dtype=bool)
+    keep[1:] = ts[1:] != ts[:-1]
+    idx = order[keep]
+    return (t[idx],) + tuple(np.asarray(v)[idx] for v in vals)
+
+
+def std_trap(t0, t1, tu, td, amp, su, sf, sd):
+    t = np.concatenate([linspace(t0, tu, su), linspace(tu, td, sf), linspace(td, t1, sd)])
+    g = np.concatenate([linspace(0.0, amp, su), np.full(int(math.floor(sf)), float(amp)), linspace(amp, 0.0, sd)])
+    t, g = unique_first(t, g)
+    return g, t
+
+
+def b1_gain(p, dt, fa, gyro):
+    """DoB1Scaling.m."""
+    p = np.asarray(p, dtype=np.float64)
+    flag = np.zeros(p.size)
+    d = np.diff(np.abs(p))
+    flag[np.nonzero(d < 0)[0] + 1] = 1
+    flag[:-1][d >= 0] = 1
+    s = 0.0
+    for x in (p * flag):
+        s = s + x
+    return ((fa / 180) * math.pi) / (gyro * s * dt)
+
+
+def duplicate(vals, t, P):
+    dups = max(1.0, P["Duplicates"])
+    sp = max(0.0, P["DupSpacing"])
+    if dups == 1 or sp == 0:
+        return vals, t
+    n = int(dups)
+    vals = [np.tile(v, n) for v in vals]
+    t = np.concatenate([t + d * sp for d in range(n)])
+    return vals, t
+
+
+class Ctx:
+    def __init__(self, vctl, gyro):
+        self.vctl = dict(vctl)
+        self.vctl.setdefault("TEAnchorTime", 0.0)
+        self.vctl.setdefault("MasterTxCoil", 1.0)
+        self.gyro = gyro
+        self.cv = {}
+        self.tr = self.slice = self.phase = 0.0
+
+    def num(self, k):
+        v = self.vctl[k]
+        if isinstance(v, str):
+            raise ValueError(f"VCtl.{k} is text")
+        return float(np.atleast_1d(v)[0])
+
+    def text(self, k, default):
+        v = self.vctl.get(k, default)
+        return v if isinstance(v, str) else default
+
+
+def _mround(x):
+    return math.floor(x + 0.5) if x >= 0 else -math.floor(-x + 0.5)
+
+
+def evaluate(expr, c: Ctx):
+    """A MATLAB scalar expression of the PSD attributes."""
+    src = expr.replace("^", "**")
+
+    def vctl_ref(m):
+        name, idx = m.group(1), m.group(2)
+        return f"_V({name!r}, {idx})" if idx else f"_V({name!r}, 1)"
+
+    src = re.sub(r"VCtl\.(\w+)(?:\(([^()]+)\))?", vctl_ref, src)
+    src = re.sub(r"VVar\.(\w+)", lambda m: f"_VV({m.group(1)!r})", src)
+    src = src.replace("VObj.Gyro", "_gyro")
+
+    def _V(name, idx):
+        v = np.atleast_1d(c.vctl[name])
+        return float(v[int(idx) - 1])
+
+    def _VV(name):
+        return {"TRCount": c.tr, "SliceCount": c.slice, "PhaseCount": c.phase}[name]
+
+    def _rem(a, b):
+        return math.fmod(a, b) if b != 0 else math.nan
+
+    def _mod(a, b):
+        if b == 0:
+            return a
+        r = math.fmod(a, b)
+        return r + b if r != 0 and ((r < 0) != (b < 0)) else r
+
+    ns = {"_V": _V, "_VV": _VV, "_gyro": c.gyro, "pi": math.pi, "Inf": math.inf, "inf": math.inf,
+          "NaN": math.nan, "eps": EPS, "abs": abs, "floor": math.floor, "ceil": math.ceil, "round": _mround,
+          "sqrt": math.sqrt, "exp": math.exp, "rem": _rem, "mod": _mod, "max": max, "min": min}
+    ns.update(c.cv)
+    return float(eval(src, {"__builtins__": {}}, ns))  # noqa: S307 - fixture expressions only
+
+
+def attr_value(name, v, c):
+    if v == "":
+        return None
+    if v[0] == "$":
+        opts = re.findall(r"'([^']*)'", v[2:])
+        return opts[int(v[1]) - 1]
+    if name == "Notes":
+        return v
+    return evaluate(v, c)
+
+
+def params(el, c):
+    return {k: attr_value(k, v, c) for k, v in el.attrib.items()}
+
+
+# ------------------------------------------------------------- elements
+def rf_element(name, P, c):
+    ts, te, dt = P["tStart"], P["tEnd"], P["dt"]
+    t = linspace(ts, te, math.ceil((te - ts) / dt) + 1)
+    if name == "rfSinc":
+        a = {"Non": 0.0, "Hamming": 0.46, "Hanning": 0.5}[P["Apod"]]
+        t = t - (te - ts) / 2 - ts
+        t0 = (te - ts) / P["TBP"]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            amp = ((t0 * np.sin(math.pi * t / t0)) / (math.pi * t)) * ((1 - a) + a * np.cos((math.pi * t) / (P["TBP"] * t0)))
+        amp[np.isnan(amp)] = np.nanmax(amp)
+        t = t + (te - ts) / 2 + ts
+    elif name == "rfRect":
+        amp = np.ones(t.size)
+    elif name == "rfGaussian":
+        sigma = (te - ts) / 7.434
+        d = t - (te - ts) / 2 - ts
+        amp = np.exp(-(d * d) / (2 * (sigma * sigma)))
+    elif name == "rfFermi":
+        tw = ((te - ts) - 2 * P["PW"]) / 13.81
+        amp = 1 / (1 + np.exp((np.abs(t - (te - ts) / 2 - ts) - P["PW"]) / tw))
+    else:
+        raise ValueError(f"rf element {name} is not supported")
+    amp[0] = 0
+    amp[-1] = 0
+    amp = b1_gain(amp, dt, P["FA"], c.gyro) * amp
+    n = t.size
+    phase = np.full(n, float(P["rfPhase"]))
+    freq = np.full(n, float(P["rfFreq"]))
+    coil = np.full(n, float(P["CoilID"]))
+    phase[[0, -1]] = 0
+    freq[[0, -1]] = 0
+    (amp, freq, phase, coil), t = duplicate([amp, freq, phase, coil], t, P)
+    return amp, phase, freq, coil, t
+
+
+def g_pe(P, c, z):
+    gbar = c.gyro / (2 * math.pi)
+    if z:
+        step = c.slice - math.floor(c.num("SliceNum") / 2) - 1
+        fov, s1, s2 = c.num("FOVSlice"), P["Gz1Sign"], P["Gz2Sign"]
+    else:
+        step = c.phase - c.num("ResPhase") / 2 - 1
+        fov, s1, s2 = c.num("FOVPhase"), P["Gy1Sign"], P["Gy2Sign"]
+    t1s, t1e, t2s, t2e, tr = P["t1Start"], P["t1End"], P["t2Start"], P["t2End"], P["tRamp"]
+    g1, tt1 = std_trap(t1s - tr, t1e + tr, t1s, t1e, step * (1 / (gbar * (t1e - t1s) * fov)) * s1, 2, 2, 2)
+    g2, tt2 = std_trap(t2s - tr, t2e + tr, t2s, t2e, step * (1 / (gbar * (t2e - t2s) * fov)) * s2, 2, 2, 2)
+    if s2 != 0:
+        g1, tt1 = np.concatenate([g1, g2]), np.concatenate([tt1, tt2])
+    t, g = unique_first(tt1, g1)
+    (g,), t = duplicate([g], t, P)
+    return g, t
+
+
+def readout_geometry(c):
+    gbar = c.gyro / (2 * math.pi)
+    gx = (1 / c.num("FOVFreq")) / (gbar * (1 / c.num("BandWidth")))
+    return gx, 1 / (2 * gbar * gx * c.num("RFreq"))
+
+
+def gx_cartesian(P, c):
+    gx, th = readout_geometry(c)
+    ta, tr = c.num("TEAnchorTime"), P["tRamp"]
+    t1s, t2m, t3s = P["t1Start"], P["t2Middle"], P["t3Start"]
+    parts = []
+    if P["Gx1Sign"] != 0:
+        parts.append(std_trap(t1s - tr, t1s + th + tr, t1s, t1s + th, gx * P["Gx1Sign"], 2, 2, 2))
+    parts.append(std_trap(t2m + ta - th - tr, t2m + ta + th + tr, t2m + ta - th, t2m + ta + th, gx * P["Gx2Sign"],
+                          2, 2, 2))
+    if P["Gx3Sign"] != 0:
+        parts.append(std_trap(t3s - tr, t3s + th + tr, t3s, t3s + th, gx * P["Gx3Sign"], 2, 2, 2))
+    t, g = unique_first(np.concatenate([p[1] for p in parts]), np.concatenate([p[0] for p in parts]))
+    (g,), t = duplicate([g], t, P)
+    return g, t
+
+
+def gz_selective(P):
+    t2s, t2e, tr, ga = P["t2Start"], P["t2End"], P["tRamp"], P["GzAmp"]
+    half = (t2e - t2s) / 2
+    g1, tt1 = std_trap(0, half + 2 * tr, tr, half + tr, P["Gz1Sign"] * ga, 2, 2, 2)
+    g2, tt2 = std_trap(t2s - tr, t2e + tr, t2s, t2e, P["Gz2Sign"] * ga, 2, 2, 2)
+    g3, tt3 = std_trap(0, half + 2 * tr, tr, half + tr, P["Gz3Sign"] * ga, 2, 2, 2)
+    g, t = g2, tt2
+    if P["Gz1Sign"] != 0:
+        g, t = np.concatenate([g1, g]), np.concatenate([t2s - tr - tt1[-1] + tt1, t])
+    if P["Gz3Sign"] != 0:
+        g, t = np.concatenate([g, g3]), np.concatenate([t, t2e + tr + tt3])
+    t, g = unique_first(t, g)
+    (g,), t = duplicate([g], t, P)
+    return g, t
+
+
+def adc_cartesian(P, c):
+    _, th = readout_geometry(c)
+    tm, ta, mu = P["tMiddle"], c.num("TEAnchorTime"), c.num("MinUpdRate")
+    g, t = std_trap(tm + ta - th - mu, tm + ta + th + mu, tm + ta - th, tm + ta + th, 1, 2, c.num("ResFreq"), 2)
+    (g,), t = duplicate([g], t, P)
+    return g, t
+
+
+def ext_bit(P, c):
+    t = linspace(P["tStart"], P["tStart"] + 2 * c.num("MinUpdRate"), 3)
+    g = np.full(3, float(P["Ext"]))
+    g[[0, -1]] = 0
+    (g,), t = duplicate([g], t, P)
+    return g, t
+
+
+# ------------------------------------------------------------- one TR
+def psd_tr(root, c):
+    L = {k: [] for k in ("rfA", "rfP", "rfF", "rfC", "rfT", "Gz", "GzT", "Gy", "GyT", "Gx", "GxT", "ADC", "ADCT",
+                         "Ext", "ExtT")}
+    se = []
+    for sec, pulses in enumerate([k for k in root if k.tag == "Pulses"], start=1):
+        S = params(pulses, c)
+        tS, tE = S["tS"], S["tE"]
+        if not tS < tE:
+            raise ValueError(f"SE setting is incorrect for Pulses {sec}!")
+        if (c.tr < S["TRStart"] or c.tr > S["TREnd"] or math.fmod(c.tr - S["TRStart"], S["Freq"]) != 0
+                or S["Switch"] == "off"):
+            continue
+        sl = {k: [] for k in L}
+        for grp in pulses:
+            for el in grp:
+                P = params(el, c)
+                if P["Switch"] != "on":
+                    continue
+                if grp.tag == "rf":
+                    if P["AnchorTE"] == "on":
+                        a = c.text("TEAnchor", "Middle")
+                        c.vctl["TEAnchorTime"] = {"Start": P["tStart"], "Middle": (P["tStart"] + P["tEnd"]) / 2,
+                                                  "End": P["tEnd"]}.get(a, c.vctl["TEAnchorTime"])
+                    amp, ph, fr, co, t = rf_element(el.tag, P, c)
+                    if c.text("MultiTransmit", "off") != "off" or c.num("MasterTxCoil") == co[0]:
+                        for k, v in zip(("rfA", "rfP", "rfF", "rfC", "rfT"), (amp, ph, fr, co, t)):
+                            sl[k].append(v)
+                    continue
+                if grp.tag == "GzSS":
+                    g, t = g_pe(P, c, True) if el.tag == "GzCartesian" else gz_selective(P)
+                    key = "Gz"
+                elif grp.tag == "GyPE":
+                    g, t = g_pe(P, c, False)
+                    key = "Gy"
+                elif grp.tag == "GxR":
+                    g, t = gx_cartesian(P, c)
+                    key = "Gx"
+                elif grp.tag == "ADC":
+                    g, t = adc_cartesian(P, c)
+                    key = "ADC"
+                else:
+                    g, t = ext_bit(P, c)
+                    key = "Ext"
+                sl[key].append(g)
+                sl[key + "T"].append(t)
+        cat = {k: (np.concatenate(v) if v else np.zeros(0)) for k, v in sl.items()}
+        span = tE - tS
+        for tkey, vkeys in (("rfT", ("rfA", "rfP", "rfF", "rfC")), ("GzT", ("Gz",)), ("GyT", ("Gy",)),
+                            ("GxT", ("Gx",)), ("ADCT", ("ADC",)), ("ExtT", ("Ext",))):
+            keep = ~((cat[tkey] < 0) | (cat[tkey] > span))
+            for vk in vkeys:
+                cat[vk] = cat[vk][keep]
+            cat[tkey] = cat[tkey][keep] + tS
+        cat["rfA"][np.abs(cat["rfA"]) < EPS] = 0
+        for k in L:
+            L[k].append(cat[k])
+        se += [tS, tE]
+    A = {k: (np.concatenate(v) if v else np.zeros(0)) for k, v in L.items()}
+    t0, t1 = min(se), max(se)
+    ts = np.concatenate([[t0, t1], A["rfT"], A["GzT"], A["GyT"], A["GxT"], A["ADCT"], A["ExtT"]]) - t0
+    line = np.concatenate([[-1, -1]] + [np.full(A[k].size, i) for i, k in
+                                         enumerate(("rfT", "GzT", "GyT", "GxT", "ADCT", "ExtT"))])
+    order = np.argsort(ts, kind="stable")
+    ts = ts[order]
+    line = line[order]
+    flags = np.zeros((6, ts.size))
+    for i in range(6):
+        flags[i, line == i] = 1
+    uts = np.unique(ts)
+    out = {"uts": uts, "ts": ts, "flags": flags}
+    for tkey, vkeys in (("rfT", ("rfA", "rfP", "rfF", "rfC")), ("GzT", ("Gz",)), ("GyT", ("Gy",)),
+                        ("GxT", ("Gx",)), ("ADCT", ("ADC",)), ("ExtT", ("Ext",))):
+        o = np.argsort(A[tkey] - t0, kind="stable")
+        for vk in vkeys:
+            out[vk] = A[vk][o]
+    for vk in ("rfA", "rfP", "rfF", "Gz", "Gy", "Gx", "ADC", "Ext"):
+        out[vk][[0, -1]] = 0
+    return out
+
+
+def compile_psd(xml_text: str, vctl: dict, gyro: float = 267538030.3797068) -> dict:
+    """VSeq lines (same keys as mrilab_amd.engine.compile_psd)."""
+    root = ET.fromstring(xml_text)
+    c = Ctx(vctl, gyro)
+    for k in root:
+        if k.tag == "CVs":
+            for n, v in k.attrib.items():
+                c.cv[n] = evaluate(v, c) if v else math.nan
+    tr0, fa0 = c.num("TR"), c.num("FlipAng")
+    has_dp = "DP_Flag" in c.vctl
+    dp_on = c.text("DP_Flag", "off") == "on"
+    keys = ("rfA", "rfP", "rfF", "rfC", "Gz", "Gy", "Gx", "ADC", "Ext", "uts", "ts")
+    acc = {k: [] for k in keys}
+    flags = []
+    dp, s, j = 0, 1, 1
+    n2, n1 = c.num("SecondPhNum"), c.num("FirstPhNum")
+    n_tr = 0
+    while s <= n2:
+        c.slice = s
+        restarted = False
+        while j <= n1:
+            c.phase = j
+            c.tr += 1
+            if has_dp:
+                c.vctl["TR"], c.vctl["FlipAng"] = ((c.num("DP_TR"), c.num("DP_FlipAng")) if dp_on and dp == 0
+                                                   else (tr0, fa0))
+            o = psd_tr(root, c)
+            for k in keys:
+                acc[k].append(o[k])
+            flags.append(o["flags"])
+            n_tr += 1
+            if has_dp and dp_on and dp == 0:
+                for k in ("Gz", "Gy", "Gx", "ADC"):
+                    acc[k] = [a * 0 for a in acc[k]]
+                if c.tr == c.num("DP_Num"):
+                    s, j, dp, restarted = 1, 1, 1, True
+                    break
+            j += 1
+        if restarted:
+            continue
+        j = 1
+        s += 1
+    cat = {k: np.concatenate(v) for k, v in acc.items()}
+    if c.text("MultiTransmit", "off") == "off":
+        cat["rfC"] = np.ones_like(cat["rfC"])
+    return {"utsLine": cat["uts"], "tsLine": cat["ts"], "rfAmpLine": cat["rfA"], "rfPhaseLine": cat["rfP"],
+            "rfFreqLine": cat["rfF"], "rfCoilLine": cat["rfC"], "GzAmpLine": cat["Gz"], "GyAmpLine": cat["Gy"],
+            "GxAmpLine": cat["Gx"], "ADCLine": cat["ADC"], "ExtLine": cat["Ext"],
+            "flagsLine": np.asfortranarray(np.concatenate(flags, axis=1)), "TRNum": n_tr}

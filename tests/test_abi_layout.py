@@ -22,7 +22,8 @@ PAIRS = [
     ("mrs_callbacks", abi.CallbacksC), ("mrs_options", abi.OptionsC), ("mrs_plan_info", abi.PlanInfoC),
     ("mrs_batch_info", abi.BatchInfoC), ("mrs_excitation", abi.ExcitationC),
     ("mrs_adjoint_info", abi.AdjointInfoC), ("mrs_recon_desc", abi.ReconDescC), ("mrs_vimg", abi.VImgC),
-    ("mrs_sar_obj", abi.SarObjC), ("mrs_sar_sig", abi.SarSigC),
+    ("mrs_sar_obj", abi.SarObjC), ("mrs_sar_sig", abi.SarSigC), ("mrs_seq_param", abi.SeqParamC),
+    ("mrs_seq_lines", abi.SeqLinesC),
 ]
 
 

@@ -1,0 +1,33 @@
+"""A scan whose sequence comes from the sequence compiler (mrs_seq_compile)
+run on the GPU engine against the oracle scan of the same lines."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from mrilab_amd import engine, workloads
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PSD = open(os.path.join(HERE, "golden", "psd_gre3d_small.xml")).read()
+VCTL = dict(TR=8e-3, TE=4e-3, FlipAng=20.0, FOVFreq=0.064, FOVPhase=0.064, FOVSlice=0.032, ResFreq=16,
+            ResPhase=8, SliceNum=4, FirstPhNum=8, SecondPhNum=4, BandWidth=20e3, RFreq=0.064 / 16,
+            MinUpdRate=1e-6, TEAnchor="Middle", MultiTransmit="off")
+
+
+@pytest.mark.parametrize("spins", [1, 4])
+def test_compiled_psd_scan_matches_oracle(gpu, spins):
+    sc = workloads.psd_scan(PSD, VCTL, shape=(16, 16, 4), spins=spins)
+    print(f"case {sc.name} spins {spins}", flush=True)
+    g = sc.copy()
+    engine.DoScanAtGPU(*g.structs)
+    o = oracle.scan(sc, "f32")
+    sig = lambda s: s.VSig["Sx"] + 1j * s.VSig["Sy"]
+    e = rel_l2(sig(g), sig(o))
+    print(f"  signal rel-L2 {e:.3e}", flush=True)
+    assert e < 1e-4
+    for k in ("Kx", "Ky", "Kz"):
+        np.testing.assert_array_equal(g.VSig[k], o.VSig[k])

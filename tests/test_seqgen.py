@@ -1,0 +1,123 @@
+"""The sequence compiler of the engine library (mrs_seq_compile, through
+engine.compile_psd) against its oracle (oracle/seqgen.py, pinned by
+tests/test_oracle_seqgen.py): the same lines on the GRE fixture and on
+variants that exercise every element, option and expression form the
+compiler takes; errors for what it does not. Runs on the CPU: the compiler
+is host code in the library."""
+import os
+
+import numpy as np
+import pytest
+
+from mrilab_amd import engine
+from oracle import seqgen
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PSD = open(os.path.join(HERE, "golden", "psd_gre3d_small.xml")).read()
+VCTL = dict(TR=8e-3, TE=4e-3, FlipAng=20.0, FOVFreq=0.064, FOVPhase=0.064, FOVSlice=0.032, ResFreq=16,
+            ResPhase=8, SliceNum=4, FirstPhNum=8, SecondPhNum=4, BandWidth=20e3, RFreq=0.064 / 16,
+            MinUpdRate=1e-6, TEAnchor="Middle", MultiTransmit="off")
+ON = "$1'on','off'"
+
+
+def same(psd, vctl):
+    a = engine.compile_psd(psd, vctl)
+    b = seqgen.compile_psd(psd, vctl)
+    assert a["TRNum"] == b["TRNum"]
+    for k in engine.SEQ_LINES:
+        x, y = np.asarray(a[k]), np.asarray(b[k])
+        assert x.shape == y.shape, k
+        if k in ("utsLine", "tsLine", "flagsLine", "ADCLine", "ExtLine", "rfCoilLine"):
+            np.testing.assert_array_equal(x, y, err_msg=k)
+        else:  # libm and numpy transcendental functions may differ in the last bit
+            np.testing.assert_allclose(x, y, rtol=1e-14, atol=1e-30, err_msg=k)
+    return a
+
+
+def variant(rf=None, gz=None, ext_extra="", sections=""):
+    psd = PSD
+    if rf is not None:
+        i = psd.index("<rfSinc")
+        psd = psd[:i] + rf + psd[psd.index("/>", i) + 2:]
+    if gz is not None:
+        i = psd.index("<GzCartesian")
+        psd = psd[:i] + gz + psd[psd.index("/>", i) + 2:]
+    if ext_extra:
+        psd = psd.replace("</Ext>", ext_extra + "</Ext>", 1)
+    if sections:
+        psd = psd.replace("</MRiLabSeq>", sections + "</MRiLabSeq>")
+    return psd
+
+
+def test_fixture_matches_oracle():
+    q = same(PSD, VCTL)
+    assert q["TRNum"] == 32 and q["flagsLine"].shape == (6, q["tsLine"].size)
+
+
+@pytest.mark.parametrize("apod", [1, 2, 3])
+@pytest.mark.parametrize("anchor", ["Start", "Middle", "End"])
+def test_sinc_apodisation_and_te_anchor(apod, anchor):
+    rf = (f'<rfSinc AnchorTE="{ON}" Apod="${apod}\'Non\',\'Hamming\',\'Hanning\'" CoilID="1" DupSpacing="0" '
+          f'Duplicates="1" FA="VCtl.FlipAng*2" Notes="x" Switch="{ON}" TBP="6" dt="8e-6" rfFreq="100" '
+          f'rfPhase="pi/2" tEnd="0.6e-3" tStart="0.05e-3"/>')
+    same(variant(rf=rf), dict(VCTL, TEAnchor=anchor))
+
+
+@pytest.mark.parametrize("shape", ["rfGaussian", "rfRect", "rfFermi"])
+def test_other_rf_shapes(shape):
+    extra = ' PW="0.15e-3"' if shape == "rfFermi" else ""
+    rf = (f'<{shape} AnchorTE="{ON}" CoilID="1" DupSpacing="0" Duplicates="1" FA="VCtl.FlipAng" Notes="x"{extra} '
+          f'Switch="{ON}" dt="5e-6" rfFreq="-250" rfPhase="0.3" tEnd="0.5e-3" tStart="0"/>')
+    same(variant(rf=rf), VCTL)
+
+
+def test_slice_selective_gradient_and_rf_duplicates():
+    rf = (f'<rfSinc AnchorTE="{ON}" Apod="$1\'Non\',\'Hamming\',\'Hanning\'" CoilID="1" DupSpacing="0.7e-3" '
+          f'Duplicates="2" FA="VCtl.FlipAng" Notes="x" Switch="{ON}" TBP="4" dt="10e-6" rfFreq="0" rfPhase="0" '
+          f'tEnd="1.4e-3" tStart="1e-3"/>')
+    gz = (f'<GzSelective DupSpacing="0" Duplicates="1" GzAmp="8e-3" Gz1Sign="-1" Gz2Sign="1" Gz3Sign="-1" '
+          f'Notes="ss" Switch="{ON}" t2End="1.4e-3" t2Start="1e-3" tRamp="80e-6"/>')
+    same(variant(rf=rf, gz=gz), VCTL)
+
+
+def test_expressions_and_gated_sections():
+    ext = (f'<ExtBit DupSpacing="0.1e-3" Duplicates="abs(rem(VVar.TRCount,2)*2-1)+1" Ext="7" Notes="rfRef" '
+           f'Switch="{ON}" tStart="CV2*(9/10)+VCtl.ME_TEs(2)-2^-10*0"/>')
+    sec = (f'<Pulses Freq="3" Notes="gated" Switch="{ON}" TREnd="7" TRStart="2" tE="VCtl.TR+0.5e-3" '
+           f'tS="VCtl.TR"><Ext><ExtBit DupSpacing="0" Duplicates="1" Ext="5" Notes="t" Switch="{ON}" '
+           f'tStart="mod(-1,3)*1e-4"/></Ext></Pulses>')
+    q = same(variant(ext_extra=ext, sections=sec), dict(VCTL, ME_TEs=[1e-3, 2e-3], FirstPhNum=4, SecondPhNum=2))
+    assert np.count_nonzero(q["ExtLine"] == 5) == 2  # TRs 2 and 5
+
+
+def test_dummy_pulses_and_multitransmit():
+    rf2 = (f'<rfRect AnchorTE="$2\'on\',\'off\'" CoilID="2" DupSpacing="0" Duplicates="1" FA="10" Notes="x" '
+           f'Switch="{ON}" dt="10e-6" rfFreq="0" rfPhase="0" tEnd="0.3e-3" tStart="0.1e-3"/>')
+    psd = PSD.replace("</rf>", rf2 + "</rf>", 1)
+    v = dict(VCTL, DP_Flag="on", DP_Num=2.0, DP_TR=6e-3, DP_FlipAng=45.0, FirstPhNum=3, SecondPhNum=2)
+    a = same(psd, v)                                    # single Tx: coil 2's pulse dropped
+    b = same(psd, dict(v, MultiTransmit="on"))          # multi Tx: both coils, rfCoilLine kept
+    assert set(np.unique(a["rfCoilLine"])) == {1.0}
+    assert set(np.unique(b["rfCoilLine"])) == {1.0, 2.0}
+    c = same(psd, dict(v, MasterTxCoil=2.0))            # master coil 2: only its pulse
+    assert a["rfAmpLine"].size == 41 * a["TRNum"] and c["rfAmpLine"].size == 21 * c["TRNum"]
+    assert b["rfAmpLine"].size == (41 + 21) * b["TRNum"]
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ("<GxEPI ", "GxEPI"),
+    ('tE="VCtl.TR"', "SE setting"),
+    ('FA="VCtl.FlipAng"', "VCtl.Nope"),
+])
+def test_errors_name_the_cause(bad, msg):
+    psd = PSD
+    if bad == "<GxEPI ":
+        psd = psd.replace("<GxCartesian ", "<GxEPI ")
+    elif bad.startswith("tE"):
+        psd = psd.replace('tE="VCtl.TR"', 'tE="0"')
+    else:
+        psd = psd.replace('FA="VCtl.FlipAng"', 'FA="VCtl.Nope"')
+    with pytest.raises(engine.MRSError, match=msg):
+        engine.compile_psd(psd, VCTL)
+    with pytest.raises(engine.MRSError, match="XML"):
+        engine.compile_psd(PSD.replace("</MRiLabSeq>", ""), VCTL)
