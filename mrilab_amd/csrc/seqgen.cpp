@@ -717,6 +717,11 @@ static Vec take(const Vec &v, const std::vector<size_t> &ix) {
 // The generated PSD function for the current VVar counters
 // (DoWriteXML2m.m:148-337).
 static TrOut psd_tr(const Node &root, Ctx &c) {
+    // the generated function assigns its CVs on every call, so a CV may
+    // depend on the VVar counters of this TR (DoWriteXML2m.m:33-39)
+    for (auto &k : root.kids)
+        if (k.name == "CVs")
+            for (auto &a : k.attrs) c.cv[a.first] = a.second.empty() ? NAN : Expr(a.second, c).run();
     Lines all;
     Vec se;
     int sec = 0;
@@ -862,9 +867,6 @@ static TrOut psd_tr(const Node &root, Ctx &c) {
 // DoPulseGen.m:18-117: the slice x phase loop, dummy pulses, concatenation.
 static void compile(const Node &root, Ctx &c, mrs_seq &out) {
     if (root.name != "MRiLabSeq") throw Error("not a PSD: the root element is <" + root.name + ">");
-    for (auto &k : root.kids)
-        if (k.name == "CVs")
-            for (auto &a : k.attrs) c.cv[a.first] = a.second.empty() ? NAN : Expr(a.second, c).run();
     const double tr0 = c.vctl("TR"), fa0 = c.vctl("FlipAng");
     const bool dp_on = c.vtext("DP_Flag", "off") == "on";
     const double n2 = c.vctl("SecondPhNum"), n1 = c.vctl("FirstPhNum");

@@ -264,6 +264,10 @@ def ext_bit(P, c):
 
 # ------------------------------------------------------------- one TR
 def psd_tr(root, c):
+    for k in root:  # the CVs are assigned on every call of the PSD function
+        if k.tag == "CVs":
+            for n, v in k.attrib.items():
+                c.cv[n] = evaluate(v, c) if v else math.nan
     L = {k: [] for k in ("rfA", "rfP", "rfF", "rfC", "rfT", "Gz", "GzT", "Gy", "GyT", "Gx", "GxT", "ADC", "ADCT",
                          "Ext", "ExtT")}
     se = []
@@ -347,10 +351,6 @@ def compile_psd(xml_text: str, vctl: dict, gyro: float = 267538030.3797068) -> d
     """VSeq lines (same keys as mrilab_amd.engine.compile_psd)."""
     root = ET.fromstring(xml_text)
     c = Ctx(vctl, gyro)
-    for k in root:
-        if k.tag == "CVs":
-            for n, v in k.attrib.items():
-                c.cv[n] = evaluate(v, c) if v else math.nan
     tr0, fa0 = c.num("TR"), c.num("FlipAng")
     has_dp = "DP_Flag" in c.vctl
     dp_on = c.text("DP_Flag", "off") == "on"

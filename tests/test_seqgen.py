@@ -90,6 +90,17 @@ def test_expressions_and_gated_sections():
     assert np.count_nonzero(q["ExtLine"] == 5) == 2  # TRs 2 and 5
 
 
+def test_cvs_are_evaluated_every_tr():
+    """The PSD function assigns its CVs on every call: a CV of VVar.TRCount
+    alternates the readout polarity here (Gx2Sign = CV3 = +-1)."""
+    psd = PSD.replace('CV3="0"', 'CV3="abs(rem(VVar.TRCount,2)*2-1)*(1-2*rem(VVar.TRCount,2))"')
+    psd = psd.replace('Gx2Sign="1"', 'Gx2Sign="CV3"')
+    q = same(psd, dict(VCTL, FirstPhNum=4, SecondPhNum=1))
+    per_tr = np.split(q["GxAmpLine"], 4)
+    readout = [p[-2] for p in per_tr]  # the readout's flat top ends the line of each TR
+    assert [float(np.sign(x)) for x in readout] == [-1.0, 1.0, -1.0, 1.0]
+
+
 def test_dummy_pulses_and_multitransmit():
     rf2 = (f'<rfRect AnchorTE="$2\'on\',\'off\'" CoilID="2" DupSpacing="0" Duplicates="1" FA="10" Notes="x" '
            f'Switch="{ON}" dt="10e-6" rfFreq="0" rfPhase="0" tEnd="0.3e-3" tStart="0.1e-3"/>')
