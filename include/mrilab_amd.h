@@ -449,8 +449,10 @@ mrs_status mrs_DoSARAverageAtGPU(const double *SAR, const double *Power, const d
  * attributes are MATLAB expressions of CVn, VCtl.x, VVar.x) and the VCtl
  * scan parameters to the VSeq lines, as DoPulseGen runs the PSD function
  * that DoWriteXML2m writes. Elements: rfSinc, rfRect, rfGaussian, rfFermi,
- * GzCartesian, GzSelective, GyCartesian, GxCartesian, ADCCartesian, ExtBit;
- * any other element is an MRS_ERR_ARG naming it.
+ * GzCartesian, GzSelective, GyCartesian, GxCartesian, ADCCartesian, the EPI
+ * (GyEPI, GxEPI, ADCEPI), FSE (GyFSE, GxFSE, ADCFSE), radial (GyRadial,
+ * GxRadial, ADCRadial) and spiral (GySpiral, GxSpiral, ADCSpiral) elements,
+ * ExtBit; any other element is an MRS_ERR_ARG naming it.
  * ------------------------------------------------------------------------- */
 typedef struct mrs_seq_param {
     const char *name;         /* VCtl field, e.g. "TR", "FlipAng", "ME_TEs" */
@@ -471,8 +473,12 @@ typedef struct mrs_seq_lines {   /* caller buffers of the sizes mrs_seq_counts g
  * FlipAng, FOVFreq/FOVPhase/FOVSlice, ResFreq/ResPhase, SliceNum,
  * FirstPhNum/SecondPhNum, BandWidth, RFreq, MinUpdRate, MasterTxCoil
  * (default 1), TEAnchor (default 'Middle'), TEAnchorTime (default 0),
- * MultiTransmit (default 'off'), DP_Flag/DP_TR/DP_FlipAng/DP_Num and
- * whatever the PSD's expressions name) and gyro = VObj.Gyro (rad/s/T).
+ * MultiTransmit (default 'off'), DP_Flag/DP_TR/DP_FlipAng/DP_Num, the
+ * technique fields the elements read (MaxGrad, MaxSlewRate, RPhase;
+ * EPI_ETL/EPI_ESP/EPI_ShotNum/EPI_EchoShifting; FSE_ETL/FSE_ESP/
+ * FSE_ShotNum; R_AngPattern/R_AngRange (text)/R_SpokeNum/R_SampPerSpoke;
+ * S_Lamda/S_SlewRate/S_SlewRate0/S_Gradient/S_ShotNum) and whatever the
+ * PSD's expressions name) and gyro = VObj.Gyro (rad/s/T).
  * Replaces: DoPulseGen(Simuh), Src/Main/DoPulseGen.m:2-117 with
  * Src/Main/DoWriteXML2m.m:11-337 and the Macro/SeqElem elements above
  * (the gradient and update-rate checks of DoPulseGen.m:119-134 are not

@@ -666,6 +666,306 @@ static GOut adc_cartesian(const Params &p, const Ctx &c) {
     return o;
 }
 
+// ------------------------------------------------- EPI, FSE, radial, spiral
+static double sgn(double x) { return x > 0 ? 1.0 : x < 0 ? -1.0 : 0.0; }
+
+static bool echo_shift(const Ctx &c) { return c.vtext("EPI_EchoShifting", "off") == "on"; }
+
+// GxAreaTrapezoid2.m / GyAreaTrapezoid2.m: area (1/m) at the largest
+// gradient MaxGrad, ramps at MaxSlewRate, at least MinUpdRate long.
+static GOut area_trap2(double ts, double area, const Ctx &c) {
+    const double a = std::fabs(area) / (c.gyro / (2 * kPi)), mu = c.vctl("MinUpdRate");
+    double amp = c.vctl("MaxGrad"), te;
+    if (a / amp < mu) {
+        te = mu + ts;
+        amp = a / mu;
+    } else {
+        te = a / amp + ts;
+    }
+    const double tr = std::max(mu, amp / c.vctl("MaxSlewRate"));
+    GOut o;
+    std_trap(ts - tr, te + tr, ts, te, amp * sgn(area), 2, 2, 2, o.amp, o.t);
+    return o;
+}
+
+// GyAreaTrapezoid.m: area (1/m) over a fixed flat top [ts, te].
+static GOut area_trap(double ts, double te, double area, const Ctx &c) {
+    const double a = std::fabs(area) / (c.gyro / (2 * kPi));
+    const double amp = a / (te - ts), tr = std::max(c.vctl("MinUpdRate"), amp / c.vctl("MaxSlewRate"));
+    GOut o;
+    std_trap(ts - tr, te + tr, ts, te, amp * sgn(area), 2, 2, 2, o.amp, o.t);
+    return o;
+}
+
+// ADCBlock.m: sSample samples over [ts, te] (one flagged point if 1).
+static GOut adc_block(double ts, double te, double n, const Ctx &c) {
+    GOut o;
+    if (n == 1) {
+        o.t = linspace(ts, te, 3);
+        o.amp = {0.0, 1.0, 0.0};
+    } else {
+        const double mu = c.vctl("MinUpdRate");
+        std_trap(ts - mu, te + mu, ts, te, 1, 2, n, 2, o.amp, o.t);
+    }
+    return o;
+}
+
+static void cat(GOut &o, const GOut &g, double scale = 1.0) {
+    for (double a : g.amp) o.amp.push_back(a * scale);
+    append(o.t, g.t);
+}
+
+// GxEPI.m: area prephaser, then EPI_ETL readout lobes of alternating sign.
+static GOut gx_epi(const Params &p, const Ctx &c) {
+    const double t1s = p.num("t1Start"), t2m = p.num("t2Middle"), s1 = p.num("Gx1Sign"), s2 = p.num("Gx2Sign");
+    const double gbar = c.gyro / (2 * kPi);
+    const GOut pre = area_trap2(t1s, (1 / c.vctl("RFreq")) / 2, c);
+    const double gx = (1 / c.vctl("FOVFreq")) / (gbar * (1 / c.vctl("BandWidth")));
+    const double th = 1 / (2 * gbar * gx * c.vctl("RFreq"));
+    const double esp = c.vctl("EPI_ESP"), etl = c.vctl("EPI_ETL");
+    double off = (t2m + c.vctl("TEAnchorTime")) - (std::floor(etl / 2) + 0.5) * esp;
+    if (echo_shift(c)) off = off + (c.phase_count - 1) * (esp / c.vctl("EPI_ShotNum"));
+    const double tr = esp / 2 - th;
+    GOut o;
+    cat(o, pre, s1);
+    for (int i = 1; i <= (int)etl; i++) {
+        const double ts = off + (i - 1) * esp + tr, te = ts + 2 * th;
+        GOut g;
+        std_trap(ts - tr, te + tr, ts, te, gx, 2, 2, 2, g.amp, g.t);
+        cat(o, g, ((i - 1) % 2 ? -1.0 : 1.0) * s2);
+    }
+    unique_pair(o.amp, o.t);
+    return o;
+}
+
+// GyEPI.m: area prephaser to the shot's first line, then EPI_ETL - 1 blips.
+static GOut gy_epi(const Params &p, const Ctx &c) {
+    const double t1s = p.num("t1Start"), t2m = p.num("t2Middle"), s1 = p.num("Gy1Sign"), s2 = p.num("Gy2Sign");
+    const double kmax = (1 / c.vctl("RPhase")) / 2, etl = c.vctl("EPI_ETL"), esp = c.vctl("EPI_ESP");
+    const double dk = (2 * kmax) / etl;
+    GOut o;
+    cat(o, area_trap2(t1s, kmax - (c.phase_count - 1) * (dk / c.vctl("EPI_ShotNum")), c), s1);
+    double off = (t2m + c.vctl("TEAnchorTime")) - (std::floor(etl / 2) - 0.5) * esp;
+    if (echo_shift(c)) off = off + (c.phase_count - 1) * (esp / c.vctl("EPI_ShotNum"));
+    for (int i = 1; i <= (int)etl - 1; i++) cat(o, area_trap2(off + (i - 1) * esp, dk, c), s2);
+    unique_pair(o.amp, o.t);
+    return o;
+}
+
+// ADCEPI.m: ResFreq samples on the flat top of every readout lobe.
+static GOut adc_epi(const Params &p, const Ctx &c) {
+    const double tm = p.num("tMiddle"), gbar = c.gyro / (2 * kPi);
+    const double gx = (1 / c.vctl("FOVFreq")) / (gbar * (1 / c.vctl("BandWidth")));
+    const double th = 1 / (2 * gbar * gx * c.vctl("RFreq"));
+    const double esp = c.vctl("EPI_ESP"), etl = c.vctl("EPI_ETL");
+    double off = (tm + c.vctl("TEAnchorTime")) - (std::floor(etl / 2) + 0.5) * esp;
+    if (echo_shift(c)) off = off + (c.phase_count - 1) * (esp / c.vctl("EPI_ShotNum"));
+    const double tr = esp / 2 - th;
+    GOut o;
+    for (int i = 1; i <= (int)etl; i++) {
+        const double ts = off + (i - 1) * esp + tr;
+        cat(o, adc_block(ts, ts + 2 * th, c.vctl("ResFreq"), c));
+    }
+    unique_pair(o.amp, o.t);
+    return o;
+}
+
+// GxFSE.m: prephaser, then FSE_ETL readouts FSE_ESP apart, ramps at MaxSlewRate.
+static GOut gx_fse(const Params &p, const Ctx &c) {
+    const double t1s = p.num("t1Start"), t2m = p.num("t2Middle"), s1 = p.num("Gx1Sign"), s2 = p.num("Gx2Sign");
+    const double gbar = c.gyro / (2 * kPi);
+    const double gx = (1 / c.vctl("FOVFreq")) / (gbar * (1 / c.vctl("BandWidth")));
+    const double th = 1 / (2 * gbar * gx * c.vctl("RFreq")), tr = gx / c.vctl("MaxSlewRate");
+    GOut o;
+    std_trap(t1s - tr, t1s + th + tr, t1s, t1s + th, gx * s1, 2, 2, 2, o.amp, o.t);
+    const double esp = c.vctl("FSE_ESP"), etl = c.vctl("FSE_ETL");
+    const double off = (t2m + c.vctl("TEAnchorTime")) - std::floor(etl / 2) * esp;
+    for (int i = 1; i <= (int)etl; i++) {
+        const double m = off + (i - 1) * esp;
+        GOut g;
+        std_trap(m - th - tr, m + th + tr, m - th, m + th, gx * s2, 2, 2, 2, g.amp, g.t);
+        cat(o, g);
+    }
+    unique_pair(o.amp, o.t);
+    return o;
+}
+
+// GyFSE.m: per echo a phase encode before it and a rewinder after it.
+static GOut gy_fse(const Params &p, const Ctx &c) {
+    const double tm = p.num("tMiddle"), g1 = p.num("tGy1"), g2 = p.num("tGy2"), toff = p.num("tOffset");
+    const double s1 = p.num("Gy1Sign"), s2 = p.num("Gy2Sign");
+    const double kmax = (1 / c.vctl("RPhase")) / 2, etl = c.vctl("FSE_ETL"), esp = c.vctl("FSE_ESP");
+    const double dk = (2 * kmax) / etl;
+    const double off = (tm + c.vctl("TEAnchorTime")) - std::floor(etl / 2) * esp;
+    GOut o;
+    for (int i = 1; i <= (int)etl; i++) {
+        const double k = -(kmax - (c.phase_count - 1) * (dk / c.vctl("FSE_ShotNum"))) + (i - 1) * dk;
+        double ts = off + (i - 1) * esp - toff;
+        cat(o, area_trap(ts, ts + g1, k * s1, c));
+        ts = off + (i - 1) * esp + toff - g2;
+        cat(o, area_trap(ts, ts + g2, k * s2, c));
+    }
+    unique_pair(o.amp, o.t);
+    return o;
+}
+
+// ADCFSE.m: ResFreq samples per echo.
+static GOut adc_fse(const Params &p, const Ctx &c) {
+    const double tm = p.num("tMiddle"), gbar = c.gyro / (2 * kPi);
+    const double gx = (1 / c.vctl("FOVFreq")) / (gbar * (1 / c.vctl("BandWidth")));
+    const double th = 1 / (2 * gbar * gx * c.vctl("RFreq")), tr = c.vctl("MinUpdRate");
+    const double esp = c.vctl("FSE_ESP"), etl = c.vctl("FSE_ETL");
+    const double off = (tm + c.vctl("TEAnchorTime")) - std::floor(etl / 2) * esp;
+    GOut o;
+    for (int i = 1; i <= (int)etl; i++) {
+        const double m = off + (i - 1) * esp;
+        GOut g;
+        std_trap(m - th - tr, m + th + tr, m - th, m + th, 1, 2, c.vctl("ResFreq"), 2, g.amp, g.t);
+        cat(o, g);
+    }
+    unique_pair(o.amp, o.t);
+    return o;
+}
+
+// GxRadial.m / GyRadial.m: the Cartesian readout rotated to spoke
+// PhaseCount - 1 (cos for x, sin for y).
+static GOut g_radial(const Params &p, const Ctx &c, bool y) {
+    const double t1s = p.num("t1Start"), t2m = p.num("t2Middle"), t3s = p.num("t3Start"), tr = p.num("tRamp");
+    const double s1 = p.num(y ? "Gy1Sign" : "Gx1Sign"), s2 = p.num(y ? "Gy2Sign" : "Gx2Sign"),
+                 s3 = p.num(y ? "Gy3Sign" : "Gx3Sign");
+    const double fov = c.vctl("FOVFreq"), res = c.vctl("ResFreq"), gbar = c.gyro / (2 * kPi);
+    double inc;
+    const std::string pat = c.vtext("R_AngPattern", "Linear");
+    if (pat == "Linear") {
+        const std::string range = c.vtext("R_AngRange", "");
+        if (range.empty()) throw Error("VCtl.R_AngRange is not set");
+        inc = Expr(range, c).run() / c.vctl("R_SpokeNum");
+    } else if (pat == "Golden") {
+        inc = 111.246 * (kPi / 180);
+    } else {
+        throw Error("VCtl.R_AngPattern must be Linear or Golden");
+    }
+    const double gx = (1 / fov) / (gbar * (1 / c.vctl("BandWidth")));
+    const double th = 1 / (2 * gbar * gx * (fov / res));
+    const double ang = inc * (c.phase_count - 1);
+    const double amp = gx * (y ? std::sin(ang) : std::cos(ang)), ta = c.vctl("TEAnchorTime");
+    Vec g1, tt1, g2, tt2, g3, tt3;
+    std_trap(t1s - tr, t1s + th + tr, t1s, t1s + th, amp * s1, 2, 2, 2, g1, tt1);
+    std_trap(t2m + ta - th - tr, t2m + ta + th + tr, t2m + ta - th, t2m + ta + th, amp * s2, 2, 2, 2, g2, tt2);
+    std_trap(t3s - tr, t3s + th + tr, t3s, t3s + th, amp * s3, 2, 2, 2, g3, tt3);
+    GOut o;
+    if (s1 != 0) {
+        o.amp = g1;
+        o.t = tt1;
+    }
+    append(o.amp, g2);
+    append(o.t, tt2);
+    if (s3 != 0) {
+        append(o.amp, g3);
+        append(o.t, tt3);
+    }
+    unique_pair(o.amp, o.t);
+    return o;
+}
+
+// ADCRadial.m: R_SampPerSpoke samples on the spoke's flat top.
+static GOut adc_radial(const Params &p, const Ctx &c) {
+    const double tm = p.num("tMiddle"), fov = c.vctl("FOVFreq"), res = c.vctl("ResFreq");
+    const double gbar = c.gyro / (2 * kPi), gx = (1 / fov) / (gbar * (1 / c.vctl("BandWidth")));
+    const double th = 1 / (2 * gbar * gx * (fov / res)), ta = c.vctl("TEAnchorTime"), mu = c.vctl("MinUpdRate");
+    GOut o;
+    std_trap(tm + ta - th - mu, tm + ta + th + mu, tm + ta - th, tm + ta + th, 1, 2, c.vctl("R_SampPerSpoke"), 2,
+             o.amp, o.t);
+    return o;
+}
+
+// a:d:b, with a relative tolerance on the end point (MATLAB's colon).
+static Vec colon(double a, double d, double b) {
+    const double q = (b - a) / d;
+    if (!(q >= 0) || !std::isfinite(q)) return q == 0 ? Vec{a} : Vec();
+    const int64_t n = (int64_t)std::floor(q + 1e-10) + 1;
+    Vec y((size_t)n);
+    for (int64_t k = 0; k < n; k++) y[(size_t)k] = a + (double)k * d;
+    return y;
+}
+
+struct SpiralShape {
+    Vec t, theta;
+    double tacq;
+};
+
+// The variable-density spiral of GxSpiral.m / ADCSpiral.m: angle Theta(t)
+// under the slew-rate limited start and the gradient-limited tail.
+static SpiralShape spiral_shape(const Ctx &c, double dt, bool need_samples) {
+    const double fov = c.vctl("FOVFreq"), res = c.vctl("ResFreq"), g = c.gyro;
+    const double lam = c.vctl("S_Lamda"), sr = c.vctl("S_SlewRate"), grad = c.vctl("S_Gradient");
+    const double kmax = res / (2 * fov), thmax = kmax / lam;
+    const double beta = (g / (2 * kPi)) * (sr / lam);
+    const double a2 = std::pow((9 * beta / 4), 1.0 / 3);
+    const double cl = sr / c.vctl("S_SlewRate0");
+    const double ts = std::pow(((3 * g * grad) / (4 * kPi * lam * (a2 * a2))), 3);
+    auto th1 = [&](double t) { return (0.5 * beta * (t * t)) / (cl + (beta / (2 * a2)) * std::pow(t, 4.0 / 3)); };
+    const double thts = th1(ts);
+    SpiralShape s;
+    if (thts < thmax) {
+        Vec t1 = need_samples ? colon(0, dt, ts) : Vec{0.0, ts};
+        const double t1e = need_samples ? t1.back() : ts;
+        const double th1e = th1(t1e);
+        s.tacq = t1e + ((kPi * lam) / (g * grad)) * ((thmax * thmax) - (th1e * th1e));
+        if (need_samples) {
+            for (size_t k = 0; k + 1 < t1.size(); k++) {
+                s.t.push_back(t1[k]);
+                s.theta.push_back(th1(t1[k]));
+            }
+            for (double t : colon(t1e, dt, s.tacq)) {
+                s.t.push_back(t);
+                s.theta.push_back(std::sqrt((th1e * th1e) + (g / (kPi * lam)) * grad * (t - t1e)));
+            }
+        }
+    } else {
+        s.tacq = ((2 * kPi * fov) / (3 * c.vctl("S_ShotNum"))) *
+                 std::sqrt(kPi / (g * sr * std::pow(c.vctl("RFreq"), 3)));
+        if (need_samples)
+            for (double t : colon(0, dt, s.tacq)) {
+                s.t.push_back(t);
+                s.theta.push_back(th1(t));
+            }
+    }
+    return s;
+}
+
+// GxSpiral.m / GySpiral.m: the spiral gradient of interleave PhaseCount.
+static GOut g_spiral(const Params &p, const Ctx &c, bool y) {
+    const double ts = p.num("tStart"), dt = p.num("dt");
+    const SpiralShape s = spiral_shape(c, dt, true);
+    const size_t n = s.t.size();
+    if (n < 2) throw Error("spiral: fewer than 2 samples");
+    const double rot = (c.phase_count - 1) * (2 * kPi / c.vctl("S_ShotNum"));
+    const double lam = c.vctl("S_Lamda"), g = c.gyro;
+    GOut o;
+    for (size_t k = 0; k < n; k++) {
+        const double dth = k == 0 ? 0.0 : (s.theta[k] - s.theta[k - 1]) / (s.t[k] - s.t[k - 1]);
+        const double a = s.theta[k] + rot;
+        const double f = y ? (std::sin(a) + s.theta[k] * std::cos(a)) : (std::cos(a) - s.theta[k] * std::sin(a));
+        o.amp.push_back(((2 * kPi) / g) * lam * dth * f);
+        o.t.push_back(ts + c.vctl("TEAnchorTime") + s.t[k]);
+    }
+    o.t.push_back(o.t.back() + std::max(c.vctl("MinUpdRate"), std::fabs(o.amp.back() / c.vctl("S_SlewRate"))));
+    o.amp.push_back(0.0);
+    unique_pair(o.amp, o.t);
+    return o;
+}
+
+// ADCSpiral.m: floor(Tacq*BandWidth) samples over the spiral readout.
+static GOut adc_spiral(const Params &p, const Ctx &c) {
+    const double ts = p.num("tStart"), ta = c.vctl("TEAnchorTime"), mu = c.vctl("MinUpdRate");
+    const double t = spiral_shape(c, 0, false).tacq;
+    GOut o;
+    std_trap(ts + ta - mu, ts + ta + t + mu, ts + ta, ts + ta + t, 1, 2, std::floor(t * c.vctl("BandWidth")), 2,
+             o.amp, o.t);
+    return o;
+}
+
 // ExtBit.m / StdBit.m: one flagged point between two zeros.
 static GOut ext_bit(const Params &p, const Ctx &c) {
     const double ts = p.num("tStart");
@@ -761,18 +1061,33 @@ static TrOut psd_tr(const Node &root, Ctx &c) {
                     append(L.Gz, g.amp);
                     append(L.GzT, g.t);
                 } else if (grp.name == "GyPE") {
-                    if (el.name != "GyCartesian") throw Error("GyPE element " + el.name + " is not supported");
-                    GOut g = g_cartesian_pe(p, c, false);
+                    GOut g;
+                    if (el.name == "GyCartesian") g = g_cartesian_pe(p, c, false);
+                    else if (el.name == "GyEPI") g = gy_epi(p, c);
+                    else if (el.name == "GyFSE") g = gy_fse(p, c);
+                    else if (el.name == "GyRadial") g = g_radial(p, c, true);
+                    else if (el.name == "GySpiral") g = g_spiral(p, c, true);
+                    else throw Error("GyPE element " + el.name + " is not supported");
                     append(L.Gy, g.amp);
                     append(L.GyT, g.t);
                 } else if (grp.name == "GxR") {
-                    if (el.name != "GxCartesian") throw Error("GxR element " + el.name + " is not supported");
-                    GOut g = gx_cartesian(p, c);
+                    GOut g;
+                    if (el.name == "GxCartesian") g = gx_cartesian(p, c);
+                    else if (el.name == "GxEPI") g = gx_epi(p, c);
+                    else if (el.name == "GxFSE") g = gx_fse(p, c);
+                    else if (el.name == "GxRadial") g = g_radial(p, c, false);
+                    else if (el.name == "GxSpiral") g = g_spiral(p, c, false);
+                    else throw Error("GxR element " + el.name + " is not supported");
                     append(L.Gx, g.amp);
                     append(L.GxT, g.t);
                 } else if (grp.name == "ADC") {
-                    if (el.name != "ADCCartesian") throw Error("ADC element " + el.name + " is not supported");
-                    GOut g = adc_cartesian(p, c);
+                    GOut g;
+                    if (el.name == "ADCCartesian") g = adc_cartesian(p, c);
+                    else if (el.name == "ADCEPI") g = adc_epi(p, c);
+                    else if (el.name == "ADCFSE") g = adc_fse(p, c);
+                    else if (el.name == "ADCRadial") g = adc_radial(p, c);
+                    else if (el.name == "ADCSpiral") g = adc_spiral(p, c);
+                    else throw Error("ADC element " + el.name + " is not supported");
                     append(L.ADC, g.amp);
                     append(L.ADCT, g.t);
                 } else if (grp.name == "Ext") {

@@ -5,7 +5,10 @@ mrs_seq_compile: DoWriteXML2m.m:148-337 (the generated PSD function for one
 TR), DoPulseGen.m:18-117 (the slice x phase loop, dummy pulses,
 concatenation) and the Macro/SeqElem elements rfSinc.m, rfRect.m,
 rfGaussian.m, rfFermi.m, GzCartesian.m, GyCartesian.m, GzSelective.m,
-GxCartesian.m, ADCCartesian.m, ExtBit.m with StdTrap.m, StdBit.m and
+GxCartesian.m, ADCCartesian.m, the EPI (GxEPI.m, GyEPI.m, ADCEPI.m), FSE
+(GxFSE.m, GyFSE.m, ADCFSE.m), radial (GxRadial.m, GyRadial.m, ADCRadial.m)
+and spiral (GxSpiral.m, GySpiral.m, ADCSpiral.m) elements, ExtBit.m, with
+StdTrap.m, StdBit.m, ADCBlock.m, the area trapezoids and
 Src/Main/DoB1Scaling.m. It parses the XML with xml.etree, evaluates
 attribute expressions with Python's own arithmetic (MATLAB's precedence and
 left-to-right order coincide with Python's for + - * / and unary minus; ^
@@ -262,6 +265,242 @@ def ext_bit(P, c):
     return g, t
 
 
+# ------------------------------------------------- EPI, FSE, radial, spiral
+def area_trap2(ts, area, c):
+    """GxAreaTrapezoid2.m / GyAreaTrapezoid2.m."""
+    a = abs(area) / (c.gyro / (2 * math.pi))
+    mu = c.num("MinUpdRate")
+    amp = c.num("MaxGrad")
+    if a / amp < mu:
+        te = mu + ts
+        amp = a / mu
+    else:
+        te = a / amp + ts
+    tr = max(mu, amp / c.num("MaxSlewRate"))
+    return std_trap(ts - tr, te + tr, ts, te, amp * np.sign(area), 2, 2, 2)
+
+
+def area_trap(ts, te, area, c):
+    """GyAreaTrapezoid.m."""
+    amp = (abs(area) / (c.gyro / (2 * math.pi))) / (te - ts)
+    tr = max(c.num("MinUpdRate"), amp / c.num("MaxSlewRate"))
+    return std_trap(ts - tr, te + tr, ts, te, amp * np.sign(area), 2, 2, 2)
+
+
+def adc_block(ts, te, n, c):
+    """ADCBlock.m."""
+    if n == 1:
+        return np.array([0.0, 1.0, 0.0]), linspace(ts, te, 3)
+    mu = c.num("MinUpdRate")
+    return std_trap(ts - mu, te + mu, ts, te, 1, 2, n, 2)
+
+
+def _join(parts):
+    g = np.concatenate([p[0] for p in parts])
+    t = np.concatenate([p[1] for p in parts])
+    t, g = unique_first(t, g)
+    return g, t
+
+
+def _epi_offset(tm, c, half):
+    esp, etl = c.num("EPI_ESP"), c.num("EPI_ETL")
+    off = (tm + c.num("TEAnchorTime")) - (math.floor(etl / 2) + half) * esp
+    if c.text("EPI_EchoShifting", "off") == "on":
+        off = off + (c.phase - 1) * (esp / c.num("EPI_ShotNum"))
+    return off
+
+
+def gx_epi(P, c):
+    gx, th = readout_geometry(c)
+    esp, etl = c.num("EPI_ESP"), c.num("EPI_ETL")
+    pre = area_trap2(P["t1Start"], (1 / c.num("RFreq")) / 2, c)
+    off = _epi_offset(P["t2Middle"], c, 0.5)
+    tr = esp / 2 - th
+    parts = [(pre[0] * P["Gx1Sign"], pre[1])]
+    for i in range(1, int(etl) + 1):
+        ts = off + (i - 1) * esp + tr
+        g, t = std_trap(ts - tr, ts + 2 * th + tr, ts, ts + 2 * th, gx, 2, 2, 2)
+        parts.append((g * ((-1.0) ** (i - 1) * P["Gx2Sign"]), t))
+    return _join(parts)
+
+
+def gy_epi(P, c):
+    kmax = (1 / c.num("RPhase")) / 2
+    etl, esp = c.num("EPI_ETL"), c.num("EPI_ESP")
+    dk = (2 * kmax) / etl
+    pre = area_trap2(P["t1Start"], kmax - (c.phase - 1) * (dk / c.num("EPI_ShotNum")), c)
+    parts = [(pre[0] * P["Gy1Sign"], pre[1])]
+    off = _epi_offset(P["t2Middle"], c, -0.5)
+    for i in range(1, int(etl)):
+        g, t = area_trap2(off + (i - 1) * esp, dk, c)
+        parts.append((g * P["Gy2Sign"], t))
+    return _join(parts)
+
+
+def adc_epi(P, c):
+    _, th = readout_geometry(c)
+    esp, etl = c.num("EPI_ESP"), c.num("EPI_ETL")
+    off = _epi_offset(P["tMiddle"], c, 0.5)
+    tr = esp / 2 - th
+    parts = []
+    for i in range(1, int(etl) + 1):
+        ts = off + (i - 1) * esp + tr
+        parts.append(adc_block(ts, ts + 2 * th, c.num("ResFreq"), c))
+    return _join(parts)
+
+
+def _fse_offset(tm, c):
+    return (tm + c.num("TEAnchorTime")) - math.floor(c.num("FSE_ETL") / 2) * c.num("FSE_ESP")
+
+
+def gx_fse(P, c):
+    gx, th = readout_geometry(c)
+    tr = gx / c.num("MaxSlewRate")
+    t1s = P["t1Start"]
+    parts = [std_trap(t1s - tr, t1s + th + tr, t1s, t1s + th, gx * P["Gx1Sign"], 2, 2, 2)]
+    off = _fse_offset(P["t2Middle"], c)
+    for i in range(1, int(c.num("FSE_ETL")) + 1):
+        m = off + (i - 1) * c.num("FSE_ESP")
+        parts.append(std_trap(m - th - tr, m + th + tr, m - th, m + th, gx * P["Gx2Sign"], 2, 2, 2))
+    return _join(parts)
+
+
+def gy_fse(P, c):
+    kmax = (1 / c.num("RPhase")) / 2
+    etl, esp = c.num("FSE_ETL"), c.num("FSE_ESP")
+    dk = (2 * kmax) / etl
+    off = _fse_offset(P["tMiddle"], c)
+    parts = []
+    for i in range(1, int(etl) + 1):
+        k = -(kmax - (c.phase - 1) * (dk / c.num("FSE_ShotNum"))) + (i - 1) * dk
+        ts = off + (i - 1) * esp - P["tOffset"]
+        parts.append(area_trap(ts, ts + P["tGy1"], k * P["Gy1Sign"], c))
+        ts = off + (i - 1) * esp + P["tOffset"] - P["tGy2"]
+        parts.append(area_trap(ts, ts + P["tGy2"], k * P["Gy2Sign"], c))
+    return _join(parts)
+
+
+def adc_fse(P, c):
+    _, th = readout_geometry(c)
+    tr = c.num("MinUpdRate")
+    off = _fse_offset(P["tMiddle"], c)
+    parts = []
+    for i in range(1, int(c.num("FSE_ETL")) + 1):
+        m = off + (i - 1) * c.num("FSE_ESP")
+        parts.append(std_trap(m - th - tr, m + th + tr, m - th, m + th, 1, 2, c.num("ResFreq"), 2))
+    return _join(parts)
+
+
+def _radial_geometry(c):
+    fov, res = c.num("FOVFreq"), c.num("ResFreq")
+    gbar = c.gyro / (2 * math.pi)
+    gx = (1 / fov) / (gbar * (1 / c.num("BandWidth")))
+    return gx, 1 / (2 * gbar * gx * (fov / res))
+
+
+def g_radial(P, c, y):
+    pat = c.text("R_AngPattern", "Linear")
+    if pat == "Linear":
+        inc = evaluate(c.text("R_AngRange", ""), c) / c.num("R_SpokeNum")
+    elif pat == "Golden":
+        inc = 111.246 * (math.pi / 180)
+    else:
+        raise ValueError("VCtl.R_AngPattern must be Linear or Golden")
+    gx, th = _radial_geometry(c)
+    ang = inc * (c.phase - 1)
+    amp = gx * (math.sin(ang) if y else math.cos(ang))
+    ax = "Gy" if y else "Gx"
+    t1s, t2m, t3s, tr = P["t1Start"], P["t2Middle"], P["t3Start"], P["tRamp"]
+    ta = c.num("TEAnchorTime")
+    parts = []
+    if P[ax + "1Sign"] != 0:
+        parts.append(std_trap(t1s - tr, t1s + th + tr, t1s, t1s + th, amp * P[ax + "1Sign"], 2, 2, 2))
+    parts.append(std_trap(t2m + ta - th - tr, t2m + ta + th + tr, t2m + ta - th, t2m + ta + th,
+                          amp * P[ax + "2Sign"], 2, 2, 2))
+    if P[ax + "3Sign"] != 0:
+        parts.append(std_trap(t3s - tr, t3s + th + tr, t3s, t3s + th, amp * P[ax + "3Sign"], 2, 2, 2))
+    return _join(parts)
+
+
+def adc_radial(P, c):
+    _, th = _radial_geometry(c)
+    tm, ta, mu = P["tMiddle"], c.num("TEAnchorTime"), c.num("MinUpdRate")
+    return std_trap(tm + ta - th - mu, tm + ta + th + mu, tm + ta - th, tm + ta + th, 1, 2,
+                    c.num("R_SampPerSpoke"), 2)
+
+
+def colon(a, d, b):
+    """a:d:b with a relative tolerance on the end point."""
+    q = (b - a) / d
+    if not q >= 0 or not math.isfinite(q):
+        return np.array([a]) if q == 0 else np.zeros(0)
+    n = int(math.floor(q + 1e-10)) + 1
+    return a + np.arange(n, dtype=np.float64) * d
+
+
+def spiral_shape(c, dt, need_samples):
+    fov, res, g = c.num("FOVFreq"), c.num("ResFreq"), c.gyro
+    lam, sr, grad = c.num("S_Lamda"), c.num("S_SlewRate"), c.num("S_Gradient")
+    thmax = (res / (2 * fov)) / lam
+    beta = (g / (2 * math.pi)) * (sr / lam)
+    a2 = (9 * beta / 4) ** (1.0 / 3)
+    cl = sr / c.num("S_SlewRate0")
+    ts = ((3 * g * grad) / (4 * math.pi * lam * (a2 * a2))) ** 3
+
+    def th1(t):  # libm's pow, element by element, as the compiled library calls it
+        if np.ndim(t):
+            return np.array([th1(x) for x in t])
+        t = float(t)
+        return (0.5 * beta * (t * t)) / (cl + (beta / (2 * a2)) * math.pow(t, 4.0 / 3))
+
+    if th1(ts) < thmax:
+        t1 = colon(0.0, dt, ts) if need_samples else np.array([0.0, ts])
+        t1e = t1[-1] if need_samples else ts
+        th1e = th1(t1e)
+        tacq = t1e + ((math.pi * lam) / (g * grad)) * ((thmax * thmax) - (th1e * th1e))
+        if not need_samples:
+            return None, None, tacq
+        t2 = colon(t1e, dt, tacq)
+        th2 = np.array([math.sqrt((th1e * th1e) + (g / (math.pi * lam)) * grad * (x - t1e)) for x in t2])
+        return np.r_[t1[:-1], t2], np.r_[th1(t1[:-1]), th2], tacq
+    tacq = ((2 * math.pi * fov) / (3 * c.num("S_ShotNum"))) * math.sqrt(math.pi / (g * sr * c.num("RFreq") ** 3))
+    if not need_samples:
+        return None, None, tacq
+    t = colon(0.0, dt, tacq)
+    return t, th1(t), tacq
+
+
+def g_spiral(P, c, y):
+    t, th, _ = spiral_shape(c, P["dt"], True)
+    rot = (c.phase - 1) * (2 * math.pi / c.num("S_ShotNum"))
+    dth = np.r_[0.0, np.diff(th) / np.diff(t)]
+    a = th + rot
+    sa = np.array([math.sin(x) for x in a])
+    ca = np.array([math.cos(x) for x in a])
+    f = (sa + th * ca) if y else (ca - th * sa)
+    g = ((2 * math.pi) / c.gyro) * c.num("S_Lamda") * dth * f
+    tt = P["tStart"] + c.num("TEAnchorTime") + t
+    tt = np.r_[tt, tt[-1] + max(c.num("MinUpdRate"), abs(g[-1] / c.num("S_SlewRate")))]
+    g = np.r_[g, 0.0]
+    tt, g = unique_first(tt, g)
+    return g, tt
+
+
+def adc_spiral(P, c):
+    _, _, t = spiral_shape(c, 0.0, False)
+    ts, ta, mu = P["tStart"], c.num("TEAnchorTime"), c.num("MinUpdRate")
+    return std_trap(ts + ta - mu, ts + ta + t + mu, ts + ta, ts + ta + t, 1, 2,
+                    math.floor(t * c.num("BandWidth")), 2)
+
+
+GY = {"GyCartesian": lambda P, c: g_pe(P, c, False), "GyEPI": gy_epi, "GyFSE": gy_fse,
+      "GyRadial": lambda P, c: g_radial(P, c, True), "GySpiral": lambda P, c: g_spiral(P, c, True)}
+GX = {"GxCartesian": lambda P, c: gx_cartesian(P, c), "GxEPI": gx_epi, "GxFSE": gx_fse,
+      "GxRadial": lambda P, c: g_radial(P, c, False), "GxSpiral": lambda P, c: g_spiral(P, c, False)}
+ADCS = {"ADCCartesian": lambda P, c: adc_cartesian(P, c), "ADCEPI": adc_epi, "ADCFSE": adc_fse,
+        "ADCRadial": adc_radial, "ADCSpiral": adc_spiral}
+
+
 # ------------------------------------------------------------- one TR
 def psd_tr(root, c):
     for k in root:  # the CVs are assigned on every call of the PSD function
@@ -299,13 +538,13 @@ def psd_tr(root, c):
                     g, t = g_pe(P, c, True) if el.tag == "GzCartesian" else gz_selective(P)
                     key = "Gz"
                 elif grp.tag == "GyPE":
-                    g, t = g_pe(P, c, False)
+                    g, t = GY[el.tag](P, c)
                     key = "Gy"
                 elif grp.tag == "GxR":
-                    g, t = gx_cartesian(P, c)
+                    g, t = GX[el.tag](P, c)
                     key = "Gx"
                 elif grp.tag == "ADC":
-                    g, t = adc_cartesian(P, c)
+                    g, t = ADCS[el.tag](P, c)
                     key = "ADC"
                 else:
                     g, t = ext_bit(P, c)

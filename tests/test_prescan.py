@@ -1,0 +1,43 @@
+"""mrilab_amd.prescan: VCtl derived from attribute files as DoPreScan.m:33-111
+derives it before the sequence is compiled."""
+import pytest
+
+from mrilab_amd import prescan
+
+ATTRS = ('<?xml version="1.0"?><SimuAttr Description="x"><Imaging BandWidth="200e3" FOVFreq="20e-2" '
+         'FOVPhase="16e-2" FlipAng="90" FreqDir="$1\'A/P\',\'L/R\',\'S/I\'" ResFreq="80" ResPhase="41" '
+         'SliceNum="3" SliceThick="6e-3" TE="90e-3" TR="10000e-3"/><Advanced MultiTransmit="$1\'off\',\'on\'" '
+         'TEAnchor="$2\'Start\', \'Middle\', \'End\'"/></SimuAttr>')
+
+
+def test_read_attrs_options_and_numbers():
+    a = prescan.read_attrs(ATTRS)
+    assert a["FreqDir"] == "A/P" and a["MultiTransmit"] == "off" and a["TEAnchor"] == "Middle"
+    assert a["BandWidth"] == 200e3 and a["ResPhase"] == 41.0 and "Description" not in a
+
+
+def test_cartesian():
+    v = prescan.vctl_from_files(ATTRS)
+    assert v["ResFreq"] == 81 and v["RFreq"] == pytest.approx(0.2 / 80)  # odd readout: a sample on kx = 0
+    assert v["ResPhase"] == 40 and v["RPhase"] == pytest.approx(0.16 / 40)  # even phase lines
+    assert v["SliceNum"] == 2 and v["FOVSlice"] == pytest.approx(2 * 6e-3) and v["RSlice"] == 6e-3
+    assert (v["FirstPhNum"], v["SecondPhNum"], v["TEAnchorTime"]) == (40, 2, 0.0)
+
+
+def test_epi_and_fse():
+    e = prescan.vctl_from_files(ATTRS, '<EPI EPI_ESP="1e-3" EPI_ETL="11" EPI_EchoShifting="$1\'on\',\'off\'" '
+                                       'EPI_ShotNum="4"/>')
+    assert e["EPI_ETL"] == 10 and e["ResPhase"] == 40 and e["FirstPhNum"] == 4 and e["EPI_EchoShifting"] == "on"
+    f = prescan.vctl_from_files(ATTRS, '<FSE FSE_ESP="10e-3" FSE_ETL="16" FSE_ShotNum="3"/>')
+    assert f["ResPhase"] == 48 and f["FirstPhNum"] == 3 and f["ResFreq"] == 81
+
+
+def test_radial_and_spiral_square_pixels():
+    r = prescan.vctl_from_files(ATTRS, '<Radial R_AngPattern="$2\'Linear\',\'Golden\'" '
+                                       'R_AngRange="$1\'pi\',\'2*pi\'" R_SampPerSpoke="100" R_SpokeNum="80"/>')
+    assert r["R_AngPattern"] == "Golden" and r["R_AngRange"] == "pi"
+    assert r["ResFreq"] == 80 and r["ResPhase"] == 80 and r["FOVPhase"] == 0.2 and r["FirstPhNum"] == 80
+    assert r["RFreq"] == r["RPhase"] == pytest.approx(0.2 / 80)
+    s = prescan.vctl_from_files(ATTRS, '<Spiral S_Gradient="500e-3" S_Lamda="2.1" S_ShotNum="8" '
+                                       'S_SlewRate="150" S_SlewRate0="100"/>', SliceNum=1.0)
+    assert s["FirstPhNum"] == 8 and s["SecondPhNum"] == 1 and s["FOVSlice"] == 6e-3

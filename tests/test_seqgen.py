@@ -115,15 +115,29 @@ def test_dummy_pulses_and_multitransmit():
     assert b["rfAmpLine"].size == (41 + 21) * b["TRNum"]
 
 
+@pytest.mark.parametrize("name,args", [
+    ("epi", ()), ("epi", ("on", 6.0, 3.0)), ("fse", ()), ("fse", (6.0, 1.0)),
+    ("radial", ()), ("radial", ("Golden",)), ("radial", ("Linear", "2*pi", 5.0, 1.0)),
+    ("spiral", ()), ("spiral", (0.5,)), ("spiral", (0.02, 2.0, 7e-6)),
+])
+def test_echo_train_and_non_cartesian_elements(name, args):
+    """EPI, FSE, radial and spiral variants of the fixture (tests/seq_techniques.py)."""
+    import seq_techniques
+
+    psd, v = getattr(seq_techniques, name)(*args)
+    q = same(psd, v)
+    assert q["TRNum"] == v["FirstPhNum"] * v["SecondPhNum"]
+
+
 @pytest.mark.parametrize("bad,msg", [
-    ("<GxEPI ", "GxEPI"),
+    ("<GxUser ", "GxUser"),
     ('tE="VCtl.TR"', "SE setting"),
     ('FA="VCtl.FlipAng"', "VCtl.Nope"),
 ])
 def test_errors_name_the_cause(bad, msg):
     psd = PSD
-    if bad == "<GxEPI ":
-        psd = psd.replace("<GxCartesian ", "<GxEPI ")
+    if bad == "<GxUser ":
+        psd = psd.replace("<GxCartesian ", "<GxUser ")
     elif bad.startswith("tE"):
         psd = psd.replace('tE="VCtl.TR"', 'tE="0"')
     else:
