@@ -31,3 +31,24 @@ def test_compiled_psd_scan_matches_oracle(gpu, spins):
     assert e < 1e-4
     for k in ("Kx", "Ky", "Kz"):
         np.testing.assert_array_equal(g.VSig[k], o.VSig[k])
+
+
+@pytest.mark.parametrize("name,args", [("epi", ()), ("fse", ()), ("radial", ("Golden",)), ("spiral", ()),
+                                       ("spiral", (0.5,))])
+def test_compiled_technique_scan_matches_oracle(gpu, name, args):
+    """EPI (k zigzag and blips), FSE (refocusing train with k reversal),
+    radial and spiral PSDs from the compiler, scanned on the GPU."""
+    from tests import seq_techniques
+
+    psd, v = getattr(seq_techniques, name)(*args)
+    sc = workloads.psd_scan(psd, v, shape=(8, 8, 2), spins=2)
+    print(f"case {name}{args} {sc.name}", flush=True)
+    g = sc.copy()
+    engine.DoScanAtGPU(*g.structs)
+    o = oracle.scan(sc, "f32")
+    sig = lambda s: s.VSig["Sx"] + 1j * s.VSig["Sy"]
+    e = rel_l2(sig(g), sig(o))
+    print(f"  signal rel-L2 {e:.3e}", flush=True)
+    assert e < 1e-4
+    for k in ("Kx", "Ky", "Kz"):
+        np.testing.assert_array_equal(g.VSig[k], o.VSig[k])
