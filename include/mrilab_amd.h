@@ -449,7 +449,9 @@ mrs_status mrs_DoSARAverageAtGPU(const double *SAR, const double *Power, const d
  * attributes are MATLAB expressions of CVn, VCtl.x, VVar.x) and the VCtl
  * scan parameters to the VSeq lines, as DoPulseGen runs the PSD function
  * that DoWriteXML2m writes. Elements: rfSinc, rfRect, rfGaussian, rfFermi,
- * GzCartesian, GzSelective, GyCartesian, GxCartesian, ADCCartesian, the EPI
+ * the adiabatic rfHyperbolicSecant, rfTanhTan, rfBIR and rfBIREF,
+ * GzCartesian, GzSelective, GzSelective2, {Gx,Gy,Gz}Trapezoid,
+ * {Gx,Gy,Gz}AreaTrapezoid(2), GyCartesian, GxCartesian, ADCCartesian, the EPI
  * (GyEPI, GxEPI, ADCEPI), FSE (GyFSE, GxFSE, ADCFSE), radial (GyRadial,
  * GxRadial, ADCRadial) and spiral (GySpiral, GxSpiral, ADCSpiral) elements,
  * ExtBit; any other element is an MRS_ERR_ARG naming it.

@@ -525,7 +525,131 @@ static void rf_finish(RfOut &r, const Params &p, const Ctx &c) {
     duplicate({&r.amp, &r.freq, &r.phase, &r.coil}, r.t, p);
 }
 
+static bool adiabatic(const std::string &name) {
+    return name == "rfHyperbolicSecant" || name == "rfTanhTan" || name == "rfBIR" || name == "rfBIREF";
+}
+
+// The adiabatic pulses (rfHyperbolicSecant.m, rfTanhTan.m, rfBIR.m,
+// rfBIREF.m): amplitude MaxB1 (no flip-angle scaling), frequency sweep and
+// phase over one or more segments of the pulse; ends zeroed, duplicated.
+static RfOut rf_adiabatic(const std::string &name, const Params &p) {
+    RfOut r;
+    const double ts = p.num("tStart"), te = p.num("tEnd"), dt = p.num("dt"), b1 = p.num("MaxB1");
+    const double T = te - ts;  // the pulse on [0, T], shifted by tStart at the end
+    if (name == "rfHyperbolicSecant") {
+        r.t = linspace(ts, te, std::ceil((te - ts) / dt) + 1);
+    } else {
+        r.t = linspace(0, T, std::ceil(T / dt) + 1);
+    }
+    const size_t n = r.t.size();
+    if (n < 2) throw Error(name + ": fewer than 2 samples");
+    r.amp.assign(n, 0.0);
+    r.freq.assign(n, 0.0);
+    r.phase.assign(n, 0.0);
+    if (name == "rfHyperbolicSecant") {  // rfHyperbolicSecant.m:27-31
+        const double tbp = p.num("TBP"), ad = p.num("Adiab"), beta = (tbp / (te - ts)) * kPi / ad;
+        for (size_t k = 0; k < n; k++) {
+            const double x = beta * (r.t[k] - (te - ts) / 2 - ts);
+            r.amp[k] = b1 * (1 / std::cosh(x));
+            r.freq[k] = -beta * ad * std::tanh(x) / (2 * kPi);
+        }
+        r.phase.assign(n, p.num("rfPhase"));
+    } else if (name == "rfTanhTan") {  // rfTanhTan.m:25-40
+        const double zeta = 10, kappa = std::atan(20.0), a = p.num("TBP") * kPi / T;
+        for (size_t k = 0; k < n; k++) {
+            const bool first = r.t[k] < T / 2;
+            const double u = first ? r.t[k] : T - r.t[k];
+            r.amp[k] = b1 * std::tanh((2 * zeta * u) / T);
+            const double f = (std::tan(kappa * (1 - 2 * u / T)) / std::tan(kappa));
+            r.freq[k] = (first ? a : -a) * f / (2 * kPi);
+        }
+        r.phase.assign(n, p.num("rfPhase"));
+    } else if (name == "rfBIR") {  // rfBIR.m:28-79
+        const std::string flag = p.text("BIRFlag");
+        const double mf = p.num("MaxFreq");
+        if (flag == "BIR-1") {
+            const double z = kPi / T;
+            for (size_t k = 0; k < n; k++) {
+                const double t = r.t[k];
+                const bool first = t < T / 2;
+                r.freq[k] = (first ? mf : -mf) * std::sin(z * t);
+                r.phase[k] = first ? 0.0 : kPi / 2;
+                r.amp[k] = b1 * std::cos(z * t);
+            }
+        } else if (flag == "BIR-2") {
+            const double z = 2 * kPi / T;
+            for (size_t k = 0; k < n; k++) {
+                const double t = r.t[k];
+                r.phase[k] = t < T / 4 ? 0.0 : t < T / 2 ? kPi / 2 : kPi / 2 + kPi;
+                r.amp[k] = std::fabs(b1 * std::cos(z * t));
+                r.freq[k] = std::fabs(mf * std::sin(z * t));
+            }
+        } else if (flag == "BIR-4") {
+            const double lam = p.num("Lambda"), beta = p.num("Beta"), den = std::tan(beta) * 2 * kPi;
+            for (size_t k = 0; k < n; k++) {
+                const double t = r.t[k], q = 4 * t / T;
+                if (t < T / 4) {
+                    r.amp[k] = b1 * std::tanh(lam * (1 - q));
+                    r.freq[k] = std::tan(4 * beta * t / T) / den;
+                } else if (t < T / 2) {
+                    r.amp[k] = b1 * std::tanh(lam * (q - 1));
+                    r.freq[k] = std::tan(beta * (q - 2)) / den;
+                } else if (t < 3 * T / 4) {
+                    r.amp[k] = b1 * std::tanh(lam * (3 - q));
+                    r.freq[k] = std::tan(beta * (q - 2)) / den;
+                } else {
+                    r.amp[k] = b1 * std::tanh(lam * (q - 3));
+                    r.freq[k] = std::tan(beta * (q - 4)) / den;
+                }
+            }
+        } else {
+            throw Error("rfBIR: unknown BIRFlag " + flag);
+        }
+    } else {  // rfBIREF.m:27-72
+        const std::string flag = p.text("BIREFFlag");
+        const double mf = p.num("MaxFreq");
+        if (flag == "BIREF-1") {
+            const double z = kPi / T;
+            for (size_t k = 0; k < n; k++) {
+                const double t = r.t[k];
+                r.amp[k] = (t < T / 2 ? b1 : -b1) * std::sin(z * t);
+                r.freq[k] = mf * std::fabs(std::cos(z * t));
+            }
+        } else if (flag == "BIREF-2a") {
+            const double z = kPi / T;
+            for (size_t k = 0; k < n; k++) {
+                const double t = r.t[k];
+                r.freq[k] = (t < T / 2 ? mf : -mf) * std::sin(z * t);
+                r.amp[k] = b1 * std::fabs(std::cos(z * t));
+            }
+        } else if (flag == "BIREF-2b") {
+            const double z = 2 * kPi / T;
+            for (size_t k = 0; k < n; k++) {
+                const double t = r.t[k];
+                if (t < T / 2) {
+                    r.amp[k] = b1 * std::fabs(std::cos(z * t));
+                    r.freq[k] = (t < T / 4 ? mf : -mf) * std::sin(z * t);
+                } else {
+                    r.amp[k] = -b1 * std::cos(z * t);
+                    r.freq[k] = (t < 3 * T / 4 ? -mf : mf) * std::sin(z * t);
+                }
+            }
+        } else {
+            throw Error("rfBIREF: unknown BIREFFlag " + flag);
+        }
+    }
+    if (name != "rfHyperbolicSecant")
+        for (double &t : r.t) t = t + ts;
+    r.coil.assign(n, p.num("CoilID"));
+    r.amp.front() = r.amp.back() = 0;
+    r.freq.front() = r.freq.back() = 0;
+    r.phase.front() = r.phase.back() = 0;
+    duplicate({&r.amp, &r.freq, &r.phase, &r.coil}, r.t, p);
+    return r;
+}
+
 static RfOut rf_element(const std::string &name, const Params &p, const Ctx &c) {
+    if (adiabatic(name)) return rf_adiabatic(name, p);
     RfOut r;
     const double ts = p.num("tStart"), te = p.num("tEnd"), dt = p.num("dt");
     r.t = linspace(ts, te, std::ceil((te - ts) / dt) + 1);
@@ -966,6 +1090,62 @@ static GOut adc_spiral(const Params &p, const Ctx &c) {
     return o;
 }
 
+// {Gx,Gy,Gz}Trapezoid.m, {Gx,Gy,Gz}AreaTrapezoid.m and
+// {Gx,Gy,Gz}AreaTrapezoid2.m as elements of their line (ax = "Gx", "Gy",
+// "Gz"): a trapezoid of given amplitude, of given area over [tStart, tEnd],
+// or of given area at MaxGrad; duplicated. False if name is none of them.
+static bool g_trapezoid(const std::string &name, const std::string &ax, const Params &p, const Ctx &c, GOut &o) {
+    if (name == ax + "Trapezoid") {
+        const double ts = p.num("tStart"), te = p.num("tEnd"), tr = p.num("tRamp");
+        const double sr = std::max(2.0, p.num("sRamp"));
+        std_trap(ts - tr, te + tr, ts, te, p.num(ax + "Amp"), sr, 2, sr, o.amp, o.t);
+    } else if (name == ax + "AreaTrapezoid") {
+        o = area_trap(p.num("tStart"), p.num("tEnd"), p.num("Area"), c);
+    } else if (name == ax + "AreaTrapezoid2") {
+        o = area_trap2(p.num("tStart"), p.num("Area"), c);
+    } else {
+        return false;
+    }
+    duplicate({&o.amp}, o.t, p);
+    return true;
+}
+
+// GzSelective2.m: a slice-select plateau [t2Start, t2End] with optional
+// lobes of their own amplitude and duration before and after it, the lobe
+// ramps running into the plateau's amplitude.
+static GOut gz_selective2(const Params &p) {
+    const double t2s = p.num("t2Start"), t2e = p.num("t2End"), tr = p.num("tRamp");
+    const double tg1 = p.num("tGz1"), tg3 = p.num("tGz3");
+    const double a1 = p.num("Gz1Amp"), a2 = p.num("Gz2Amp"), a3 = p.num("Gz3Amp");
+    Vec g1, tt1, g3, tt3;
+    std_trap(0, tg1 + 2 * tr, tr, tg1 + tr, a1, 2, 2, 2, g1, tt1);
+    std_trap(0, tg3 + 2 * tr, tr, tg3 + tr, a3, 2, 2, 2, g3, tt3);
+    const Vec t2 = linspace(t2s, t2e, 2);
+    GOut o;
+    if (a1 != 0 && tg1 != 0) {
+        g1.back() = a2;
+        o.amp = g1;
+        for (double t : tt1) o.t.push_back(t2s - tt1.back() + t);
+    } else {
+        o.amp = {0.0};
+        o.t = {t2[0] - tr};
+    }
+    o.amp.push_back(a2);
+    o.amp.push_back(a2);
+    append(o.t, t2);
+    if (a3 != 0 && tg3 != 0) {
+        g3.front() = a2;
+        append(o.amp, g3);
+        for (double t : tt3) o.t.push_back(t2e + t);
+    } else {
+        o.amp.push_back(0.0);
+        o.t.push_back(o.t.back() + tr);
+    }
+    unique_pair(o.amp, o.t);
+    duplicate({&o.amp}, o.t, p);
+    return o;
+}
+
 // ExtBit.m / StdBit.m: one flagged point between two zeros.
 static GOut ext_bit(const Params &p, const Ctx &c) {
     const double ts = p.num("tStart");
@@ -1057,7 +1237,9 @@ static TrOut psd_tr(const Node &root, Ctx &c) {
                     GOut g;
                     if (el.name == "GzCartesian") g = g_cartesian_pe(p, c, true);
                     else if (el.name == "GzSelective") g = gz_selective(p);
-                    else throw Error("GzSS element " + el.name + " is not supported");
+                    else if (el.name == "GzSelective2") g = gz_selective2(p);
+                    else if (!g_trapezoid(el.name, "Gz", p, c, g))
+                        throw Error("GzSS element " + el.name + " is not supported");
                     append(L.Gz, g.amp);
                     append(L.GzT, g.t);
                 } else if (grp.name == "GyPE") {
@@ -1067,7 +1249,8 @@ static TrOut psd_tr(const Node &root, Ctx &c) {
                     else if (el.name == "GyFSE") g = gy_fse(p, c);
                     else if (el.name == "GyRadial") g = g_radial(p, c, true);
                     else if (el.name == "GySpiral") g = g_spiral(p, c, true);
-                    else throw Error("GyPE element " + el.name + " is not supported");
+                    else if (!g_trapezoid(el.name, "Gy", p, c, g))
+                        throw Error("GyPE element " + el.name + " is not supported");
                     append(L.Gy, g.amp);
                     append(L.GyT, g.t);
                 } else if (grp.name == "GxR") {
@@ -1077,7 +1260,8 @@ static TrOut psd_tr(const Node &root, Ctx &c) {
                     else if (el.name == "GxFSE") g = gx_fse(p, c);
                     else if (el.name == "GxRadial") g = g_radial(p, c, false);
                     else if (el.name == "GxSpiral") g = g_spiral(p, c, false);
-                    else throw Error("GxR element " + el.name + " is not supported");
+                    else if (!g_trapezoid(el.name, "Gx", p, c, g))
+                        throw Error("GxR element " + el.name + " is not supported");
                     append(L.Gx, g.amp);
                     append(L.GxT, g.t);
                 } else if (grp.name == "ADC") {

@@ -159,6 +159,8 @@ def params(el, c):
 
 # ------------------------------------------------------------- elements
 def rf_element(name, P, c):
+    if name in ADIABATIC:
+        return rf_adiabatic(name, P)
     ts, te, dt = P["tStart"], P["tEnd"], P["dt"]
     t = linspace(ts, te, math.ceil((te - ts) / dt) + 1)
     if name == "rfSinc":
@@ -191,6 +193,129 @@ def rf_element(name, P, c):
     freq[[0, -1]] = 0
     (amp, freq, phase, coil), t = duplicate([amp, freq, phase, coil], t, P)
     return amp, phase, freq, coil, t
+
+
+ADIABATIC = ("rfHyperbolicSecant", "rfTanhTan", "rfBIR", "rfBIREF")
+
+
+def rf_adiabatic(name, P):
+    """rfHyperbolicSecant.m, rfTanhTan.m, rfBIR.m, rfBIREF.m: MaxB1 amplitude,
+    no flip-angle scaling."""
+    ts, te, dt, b1 = P["tStart"], P["tEnd"], P["dt"], P["MaxB1"]
+    T = te - ts
+    t = linspace(ts, te, math.ceil((te - ts) / dt) + 1) if name == "rfHyperbolicSecant" else \
+        linspace(0.0, T, math.ceil(T / dt) + 1)
+    n = t.size
+    amp, freq, phase = np.zeros(n), np.zeros(n), np.zeros(n)
+    for k, x in enumerate(t):
+        if name == "rfHyperbolicSecant":
+            beta = (P["TBP"] / (te - ts)) * math.pi / P["Adiab"]
+            a = beta * (x - (te - ts) / 2 - ts)
+            amp[k] = b1 * (1 / math.cosh(a))
+            freq[k] = -beta * P["Adiab"] * math.tanh(a) / (2 * math.pi)
+            phase[k] = P["rfPhase"]
+        elif name == "rfTanhTan":
+            A = P["TBP"] * math.pi / T
+            first = x < T / 2
+            u = x if first else T - x
+            amp[k] = b1 * math.tanh((2 * 10 * u) / T)
+            f = math.tan(math.atan(20) * (1 - 2 * u / T)) / math.tan(math.atan(20))
+            freq[k] = (A if first else -A) * f / (2 * math.pi)
+            phase[k] = P["rfPhase"]
+        elif name == "rfBIR":
+            mf, flag = P["MaxFreq"], P["BIRFlag"]
+            if flag == "BIR-1":
+                z = math.pi / T
+                freq[k] = (mf if x < T / 2 else -mf) * math.sin(z * x)
+                phase[k] = 0.0 if x < T / 2 else math.pi / 2
+                amp[k] = b1 * math.cos(z * x)
+            elif flag == "BIR-2":
+                z = 2 * math.pi / T
+                phase[k] = 0.0 if x < T / 4 else (math.pi / 2 if x < T / 2 else math.pi / 2 + math.pi)
+                amp[k] = abs(b1 * math.cos(z * x))
+                freq[k] = abs(mf * math.sin(z * x))
+            elif flag == "BIR-4":
+                lam, beta = P["Lambda"], P["Beta"]
+                den = math.tan(beta) * 2 * math.pi
+                q = 4 * x / T
+                if x < T / 4:
+                    amp[k], freq[k] = b1 * math.tanh(lam * (1 - q)), math.tan(4 * beta * x / T) / den
+                elif x < T / 2:
+                    amp[k], freq[k] = b1 * math.tanh(lam * (q - 1)), math.tan(beta * (q - 2)) / den
+                elif x < 3 * T / 4:
+                    amp[k], freq[k] = b1 * math.tanh(lam * (3 - q)), math.tan(beta * (q - 2)) / den
+                else:
+                    amp[k], freq[k] = b1 * math.tanh(lam * (q - 3)), math.tan(beta * (q - 4)) / den
+            else:
+                raise ValueError(f"rfBIR: unknown BIRFlag {flag}")
+        else:
+            mf, flag = P["MaxFreq"], P["BIREFFlag"]
+            if flag == "BIREF-1":
+                z = math.pi / T
+                amp[k] = (b1 if x < T / 2 else -b1) * math.sin(z * x)
+                freq[k] = mf * abs(math.cos(z * x))
+            elif flag == "BIREF-2a":
+                z = math.pi / T
+                freq[k] = (mf if x < T / 2 else -mf) * math.sin(z * x)
+                amp[k] = b1 * abs(math.cos(z * x))
+            elif flag == "BIREF-2b":
+                z = 2 * math.pi / T
+                if x < T / 2:
+                    amp[k] = b1 * abs(math.cos(z * x))
+                    freq[k] = (mf if x < T / 4 else -mf) * math.sin(z * x)
+                else:
+                    amp[k] = -b1 * math.cos(z * x)
+                    freq[k] = (-mf if x < 3 * T / 4 else mf) * math.sin(z * x)
+            else:
+                raise ValueError(f"rfBIREF: unknown BIREFFlag {flag}")
+    if name != "rfHyperbolicSecant":
+        t = t + ts
+    coil = np.full(n, float(P["CoilID"]))
+    for v in (amp, freq, phase):
+        v[[0, -1]] = 0
+    (amp, freq, phase, coil), t = duplicate([amp, freq, phase, coil], t, P)
+    return amp, phase, freq, coil, t
+
+
+def g_trapezoid(name, ax, P, c):
+    """{Gx,Gy,Gz}Trapezoid.m, *AreaTrapezoid.m, *AreaTrapezoid2.m as elements."""
+    if name == ax + "Trapezoid":
+        sr = max(2.0, P["sRamp"])
+        g, t = std_trap(P["tStart"] - P["tRamp"], P["tEnd"] + P["tRamp"], P["tStart"], P["tEnd"], P[ax + "Amp"],
+                        sr, 2, sr)
+    elif name == ax + "AreaTrapezoid":
+        g, t = area_trap(P["tStart"], P["tEnd"], P["Area"], c)
+    elif name == ax + "AreaTrapezoid2":
+        g, t = area_trap2(P["tStart"], P["Area"], c)
+    else:
+        raise ValueError(f"{name} is not supported")
+    (g,), t = duplicate([g], t, P)
+    return g, t
+
+
+def gz_selective2(P):
+    """GzSelective2.m."""
+    t2s, t2e, tr = P["t2Start"], P["t2End"], P["tRamp"]
+    a1, a2, a3 = P["Gz1Amp"], P["Gz2Amp"], P["Gz3Amp"]
+    g1, tt1 = std_trap(0, P["tGz1"] + 2 * tr, tr, P["tGz1"] + tr, a1, 2, 2, 2)
+    g3, tt3 = std_trap(0, P["tGz3"] + 2 * tr, tr, P["tGz3"] + tr, a3, 2, 2, 2)
+    t2 = linspace(t2s, t2e, 2)
+    g, t = np.array([a2, a2]), t2
+    if a1 != 0 and P["tGz1"] != 0:
+        g1 = g1.copy()
+        g1[-1] = a2
+        g, t = np.r_[g1, g], np.r_[t2s - tt1[-1] + tt1, t]
+    else:
+        g, t = np.r_[0.0, g], np.r_[t[0] - tr, t]
+    if a3 != 0 and P["tGz3"] != 0:
+        g3 = g3.copy()
+        g3[0] = a2
+        g, t = np.r_[g, g3], np.r_[t, t2e + tt3]
+    else:
+        g, t = np.r_[g, 0.0], np.r_[t, t[-1] + tr]
+    t, g = unique_first(t, g)
+    (g,), t = duplicate([g], t, P)
+    return g, t
 
 
 def g_pe(P, c, z):
@@ -535,13 +660,20 @@ def psd_tr(root, c):
                             sl[k].append(v)
                     continue
                 if grp.tag == "GzSS":
-                    g, t = g_pe(P, c, True) if el.tag == "GzCartesian" else gz_selective(P)
+                    if el.tag == "GzCartesian":
+                        g, t = g_pe(P, c, True)
+                    elif el.tag == "GzSelective":
+                        g, t = gz_selective(P)
+                    elif el.tag == "GzSelective2":
+                        g, t = gz_selective2(P)
+                    else:
+                        g, t = g_trapezoid(el.tag, "Gz", P, c)
                     key = "Gz"
                 elif grp.tag == "GyPE":
-                    g, t = GY[el.tag](P, c)
+                    g, t = GY[el.tag](P, c) if el.tag in GY else g_trapezoid(el.tag, "Gy", P, c)
                     key = "Gy"
                 elif grp.tag == "GxR":
-                    g, t = GX[el.tag](P, c)
+                    g, t = GX[el.tag](P, c) if el.tag in GX else g_trapezoid(el.tag, "Gx", P, c)
                     key = "Gx"
                 elif grp.tag == "ADC":
                     g, t = ADCS[el.tag](P, c)

@@ -75,3 +75,48 @@ def spiral(gradient=0.01, shots=4.0, dt=4e-6):
     v = prescan.derive(dict(BASE, TR=40e-3, TE=2e-3, S_Lamda=2.1, S_SlewRate=150.0, S_SlewRate0=100.0,
                             S_Gradient=gradient, S_ShotNum=shots))
     return psd, v
+
+
+def rf_only(rf, gz='<GzTrapezoid DupSpacing="0" Duplicates="1" GzAmp="0" Notes="z" Switch="$1\'on\',\'off\'" '
+                 'sRamp="2" tEnd="VCtl.TR-0.5e-3" tRamp="10e-6" tStart="VCtl.TR-0.6e-3"/>',
+            gy='<GyAreaTrapezoid Area="0" DupSpacing="0" Duplicates="1" Notes="y" Switch="$1\'on\',\'off\'" '
+               'tEnd="VCtl.TR-0.5e-3" tStart="VCtl.TR-0.6e-3"/>',
+            gx='<GxAreaTrapezoid2 Area="0" DupSpacing="0" Duplicates="1" Notes="x" Switch="$1\'on\',\'off\'" '
+               'tStart="VCtl.TR-0.6e-3"/>'):
+    """A one-TR PSD of one rf element, zero-area gradient elements (the
+    lines may not be empty) and an ADC at the end of the TR."""
+    psd = (f'<?xml version="1.0" encoding="utf-8"?><MRiLabSeq Name="rf" Notes="one pulse"><CVs CV1="0"/>'
+           f'<Pulses Freq="1" Notes="TR" Switch="{ON}" TREnd="Inf" TRStart="1" tE="VCtl.TR" tS="0">'
+           f'<rf>{rf}</rf><GzSS>{gz}</GzSS><GyPE>{gy}</GyPE><GxR>{gx}</GxR>'
+           f'<ADC><ADCCartesian DupSpacing="0" Duplicates="1" Notes="adc" Switch="{ON}" '
+           f'tMiddle="VCtl.TR-0.3e-3"/></ADC>'
+           f'<Ext><ExtBit DupSpacing="0" Duplicates="1" Ext="1" Notes="reset k" Switch="{ON}" tStart="0"/></Ext>'
+           f'</Pulses></MRiLabSeq>')
+    v = prescan.derive(dict(BASE, TR=12e-3, SliceNum=1.0, ResFreq=4.0, ResPhase=2.0))
+    v["FirstPhNum"] = v["SecondPhNum"] = 1.0
+    return psd, v
+
+
+def hs(scale=1.0, t0=0.5e-3):
+    return (f'<rfHyperbolicSecant Adiab="4.9" AnchorTE="{OFF}" CoilID="1" DupSpacing="0" Duplicates="1" '
+            f'MaxB1="{14e-6 * scale}" Notes="HS" Switch="{ON}" TBP="10" dt="10e-6" rfPhase="0" tEnd="{t0 + 8e-3}" '
+            f'tStart="{t0}"/>')
+
+
+def tanhtan(scale=1.0):
+    return (f'<rfTanhTan AnchorTE="{OFF}" CoilID="1" DupSpacing="0" Duplicates="1" MaxB1="{9e-4 * scale}" '
+            f'Notes="tt" Switch="{ON}" TBP="50" dt="1e-6" rfPhase="0.2" tEnd="60e-6" tStart="10e-6"/>')
+
+
+def bir(flag, dups=1):
+    k = {"BIR-1": 1, "BIR-2": 2, "BIR-4": 3}[flag]
+    return (f'<rfBIR AnchorTE="{OFF}" BIRFlag="${k}\'BIR-1\',\'BIR-2\',\'BIR-4\'" Beta="1.4711" CoilID="1" '
+            f'DupSpacing="4e-3" Duplicates="{dups}" Lambda="10" MaxB1="10e-6" MaxFreq="10e3" Notes="bir" '
+            f'Switch="{ON}" dt="10e-6" tEnd="4e-3" tStart="1e-3"/>')
+
+
+def biref(flag):
+    k = {"BIREF-1": 1, "BIREF-2a": 2, "BIREF-2b": 3}[flag]
+    return (f'<rfBIREF AnchorTE="{OFF}" BIREFFlag="${k}\'BIREF-1\',\'BIREF-2a\',\'BIREF-2b\'" CoilID="1" '
+            f'DupSpacing="0" Duplicates="1" MaxB1="10e-6" MaxFreq="10e3" Notes="biref" Switch="{ON}" dt="10e-6" '
+            f'tEnd="3e-3" tStart="1e-3"/>')

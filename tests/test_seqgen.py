@@ -129,6 +129,44 @@ def test_echo_train_and_non_cartesian_elements(name, args):
     assert q["TRNum"] == v["FirstPhNum"] * v["SecondPhNum"]
 
 
+def _rf_and_gradient_elements():
+    import seq_techniques as T
+
+    on = "$1'on','off'"
+    rf = [T.hs(), T.hs(2.0), T.tanhtan(), T.bir("BIR-1"), T.bir("BIR-2", 3), T.bir("BIR-4"),
+          T.biref("BIREF-1"), T.biref("BIREF-2a"), T.biref("BIREF-2b")]
+    cases = [(r, {}) for r in rf]
+    g = {"gz": (f'<GzSelective2 DupSpacing="0.1e-3" Duplicates="2" Gz1Amp="4e-4" Gz2Amp="2e-4" Gz3Amp="-3e-4" '
+                f'Notes="s" Switch="{on}" t2End="9.5e-3" t2Start="9.3e-3" tGz1="0.2e-3" tGz3="0.1e-3" '
+                f'tRamp="40e-6"/>'),
+         "gy": (f'<GyTrapezoid DupSpacing="0" Duplicates="1" GyAmp="-7e-3" Notes="y" Switch="{on}" sRamp="4" '
+                f'tEnd="9.2e-3" tRamp="30e-6" tStart="9e-3"/>'),
+         "gx": (f'<GxAreaTrapezoid Area="-25" DupSpacing="0.5e-3" Duplicates="2" Notes="x" Switch="{on}" '
+                f'tEnd="9.2e-3" tStart="9e-3"/>')}
+    cases += [(T.hs(), {k: v}) for k, v in g.items()]
+    cases += [(T.hs(), {"gz": g["gz"].replace('Gz1Amp="4e-4"', 'Gz1Amp="0"'),
+                        "gy": f'<GyAreaTrapezoid2 Area="80" DupSpacing="0" Duplicates="1" Notes="y" Switch="{on}" '
+                              f'tStart="9e-3"/>',
+                        "gx": f'<GxTrapezoid DupSpacing="0" Duplicates="1" GxAmp="3e-3" Notes="x" Switch="{on}" '
+                              f'sRamp="1" tEnd="9.2e-3" tRamp="30e-6" tStart="9e-3"/>'})]
+    cases += [(T.hs(), {"gz": f'<GzAreaTrapezoid Area="12" DupSpacing="0" Duplicates="1" Notes="z" '
+                              f'Switch="{on}" tEnd="9.2e-3" tStart="9e-3"/>',
+                        "gy": f'<GyAreaTrapezoid Area="0" DupSpacing="0" Duplicates="1" Notes="y" Switch="{on}" '
+                              f'tEnd="9.2e-3" tStart="9e-3"/>'})]
+    return cases
+
+
+@pytest.mark.parametrize("case", range(len(_rf_and_gradient_elements())))
+def test_adiabatic_rf_trapezoid_and_selective2_elements(case):
+    """Adiabatic rf shapes (no flip-angle scaling), the plain and area
+    trapezoids on every axis and GzSelective2, in the one-pulse PSD of
+    tests/seq_techniques.py."""
+    import seq_techniques as T
+
+    rf, grads = _rf_and_gradient_elements()[case]
+    same(*T.rf_only(rf, **grads))
+
+
 @pytest.mark.parametrize("bad,msg", [
     ("<GxUser ", "GxUser"),
     ('tE="VCtl.TR"', "SE setting"),
