@@ -64,12 +64,24 @@ def cpu_baseline(n_tr: int, threads: int) -> dict:
     iso_steps = sc.n_iso * sc.n_steps()
     oracle.set_threads(threads, "f32")
     t0 = time.perf_counter()
-    oracle.scan(sc, "f32")
+    o = oracle.scan(sc, "f32")
     dt = time.perf_counter() - t0
     oracle.set_threads(0, "f32")
+    # the GPU engine on the same sample: parity at the full phantom size
+    import numpy as np
+    from mrilab_amd import engine
+    g = sc.copy()
+    engine.DoScanAtGPU(*g.structs)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    sig = lambda x: x.VSig["Sx"] + 1j * x.VSig["Sy"]
+    mag = lambda x: np.stack([x.VObj[k] for k in ("Mx", "My", "Mz")])
+    parity = {"signal": rel(sig(g), sig(o)), "magnetization": rel(mag(g), mag(o))}
+    log(f"cpu baseline sample: GPU vs oracle rel-L2 signal {parity['signal']:.2e} "
+        f"magnetization {parity['magnetization']:.2e}")
     return {"value": iso_steps / dt, "unit": "isochromat-steps/s", "cores": threads, "kind": "port",
             "sample": f"first {n_tr} TRs of the same phantom and sequence ({iso_steps:.3g} isochromat-steps, "
-                      f"{dt:.1f} s), fp32 restatement of DoScanAtCPU, OpenMP over slices"}
+                      f"{dt:.1f} s), fp32 restatement of DoScanAtCPU, OpenMP over slices",
+            "gpu_rel_l2_on_sample": parity}
 
 
 def batched_section(n_sets: int, plain_sample: int, reps: int) -> dict:

@@ -137,3 +137,17 @@ def test_small_launches_match_one_launch(gpu, max_adc):
     b = run_gpu(sc, max_adc_per_launch=max_adc)
     assert rel_l2(sig(b), sig(a)) < 1e-6
     assert rel_l2(mag(b), mag(a)) < 1e-6
+
+
+def test_full_phantom_first_tr(gpu):
+    """The 3-D bSSFP workload at its full size (128 x 128 x 64 voxels, 4
+    isochromats, 8 coils, 128 readout samples) for its first TR, against the
+    fp32 oracle: parity at the benchmarked shape, 2.1e9 isochromat-steps."""
+    sc = workloads.bssfp3d(n_tr=1)
+    print(f"case {sc.name} full phantom, 1 TR", flush=True)
+    g = run_gpu(sc)
+    o32 = oracle.scan(sc, "f32")
+    e_sig, e_mag = rel_l2(sig(g), sig(o32)), rel_l2(mag(g), mag(o32))
+    print(f"  signal {e_sig:.3e} magnetization {e_mag:.3e}", flush=True)
+    assert e_sig < 1e-4 and e_mag < 1e-4
+    assert_decoder_equal(g, o32)
