@@ -429,9 +429,21 @@ struct Params {
 };
 
 // ---------------------------------------------------------------- helpers
+// Largest sample count one element may produce: a bad dt, Duplicates or
+// spiral step (zero, NaN, huge) becomes an error instead of an allocation
+// of unbounded size.
+constexpr double kMaxPoints = 1 << 28;
+
+static int64_t checked_count(double nd, const char *what) {
+    if (!(nd < kMaxPoints)) throw Error(std::string(what) + ": sample count not finite or above 2^28");
+    return (int64_t)std::floor(nd);
+}
+
+static int echo_train(double etl) { return (int)checked_count(etl, "echo train length"); }
+
 // MATLAB linspace: d1 + (k*(d2 - d1))/(n - 1), with the ends set exactly.
 static Vec linspace(double d1, double d2, double nd) {
-    const int64_t n = (int64_t)std::floor(nd);
+    const int64_t n = checked_count(nd, "linspace");
     if (n < 1) return Vec();
     if (n == 1) return Vec{d2};
     Vec y((size_t)n);
@@ -493,7 +505,8 @@ static double b1_gain(const Vec &p, double dt, double fa, double gyro) {
 static void duplicate(std::vector<Vec *> vals, Vec &t, const Params &p) {
     const double dups = std::max(1.0, p.num("Duplicates")), sp = std::max(0.0, p.num("DupSpacing"));
     if (dups == 1 || sp == 0) return;
-    const int64_t nd = (int64_t)dups;
+    const int64_t nd = checked_count(dups * (double)std::max<size_t>(t.size(), 1), "Duplicates") > 0
+                           ? (int64_t)dups : 1;
     for (Vec *v : vals) {
         Vec o;
         for (int64_t d = 0; d < nd; d++) append(o, *v);
@@ -852,7 +865,7 @@ static GOut gx_epi(const Params &p, const Ctx &c) {
     const double tr = esp / 2 - th;
     GOut o;
     cat(o, pre, s1);
-    for (int i = 1; i <= (int)etl; i++) {
+    for (int i = 1; i <= echo_train(etl); i++) {
         const double ts = off + (i - 1) * esp + tr, te = ts + 2 * th;
         GOut g;
         std_trap(ts - tr, te + tr, ts, te, gx, 2, 2, 2, g.amp, g.t);
@@ -871,7 +884,7 @@ static GOut gy_epi(const Params &p, const Ctx &c) {
     cat(o, area_trap2(t1s, kmax - (c.phase_count - 1) * (dk / c.vctl("EPI_ShotNum")), c), s1);
     double off = (t2m + c.vctl("TEAnchorTime")) - (std::floor(etl / 2) - 0.5) * esp;
     if (echo_shift(c)) off = off + (c.phase_count - 1) * (esp / c.vctl("EPI_ShotNum"));
-    for (int i = 1; i <= (int)etl - 1; i++) cat(o, area_trap2(off + (i - 1) * esp, dk, c), s2);
+    for (int i = 1; i <= echo_train(etl) - 1; i++) cat(o, area_trap2(off + (i - 1) * esp, dk, c), s2);
     unique_pair(o.amp, o.t);
     return o;
 }
@@ -886,7 +899,7 @@ static GOut adc_epi(const Params &p, const Ctx &c) {
     if (echo_shift(c)) off = off + (c.phase_count - 1) * (esp / c.vctl("EPI_ShotNum"));
     const double tr = esp / 2 - th;
     GOut o;
-    for (int i = 1; i <= (int)etl; i++) {
+    for (int i = 1; i <= echo_train(etl); i++) {
         const double ts = off + (i - 1) * esp + tr;
         cat(o, adc_block(ts, ts + 2 * th, c.vctl("ResFreq"), c));
     }
@@ -904,7 +917,7 @@ static GOut gx_fse(const Params &p, const Ctx &c) {
     std_trap(t1s - tr, t1s + th + tr, t1s, t1s + th, gx * s1, 2, 2, 2, o.amp, o.t);
     const double esp = c.vctl("FSE_ESP"), etl = c.vctl("FSE_ETL");
     const double off = (t2m + c.vctl("TEAnchorTime")) - std::floor(etl / 2) * esp;
-    for (int i = 1; i <= (int)etl; i++) {
+    for (int i = 1; i <= echo_train(etl); i++) {
         const double m = off + (i - 1) * esp;
         GOut g;
         std_trap(m - th - tr, m + th + tr, m - th, m + th, gx * s2, 2, 2, 2, g.amp, g.t);
@@ -922,7 +935,7 @@ static GOut gy_fse(const Params &p, const Ctx &c) {
     const double dk = (2 * kmax) / etl;
     const double off = (tm + c.vctl("TEAnchorTime")) - std::floor(etl / 2) * esp;
     GOut o;
-    for (int i = 1; i <= (int)etl; i++) {
+    for (int i = 1; i <= echo_train(etl); i++) {
         const double k = -(kmax - (c.phase_count - 1) * (dk / c.vctl("FSE_ShotNum"))) + (i - 1) * dk;
         double ts = off + (i - 1) * esp - toff;
         cat(o, area_trap(ts, ts + g1, k * s1, c));
@@ -941,7 +954,7 @@ static GOut adc_fse(const Params &p, const Ctx &c) {
     const double esp = c.vctl("FSE_ESP"), etl = c.vctl("FSE_ETL");
     const double off = (tm + c.vctl("TEAnchorTime")) - std::floor(etl / 2) * esp;
     GOut o;
-    for (int i = 1; i <= (int)etl; i++) {
+    for (int i = 1; i <= echo_train(etl); i++) {
         const double m = off + (i - 1) * esp;
         GOut g;
         std_trap(m - th - tr, m + th + tr, m - th, m + th, 1, 2, c.vctl("ResFreq"), 2, g.amp, g.t);
@@ -1007,7 +1020,7 @@ static GOut adc_radial(const Params &p, const Ctx &c) {
 static Vec colon(double a, double d, double b) {
     const double q = (b - a) / d;
     if (!(q >= 0) || !std::isfinite(q)) return q == 0 ? Vec{a} : Vec();
-    const int64_t n = (int64_t)std::floor(q + 1e-10) + 1;
+    const int64_t n = checked_count(q + 1e-10, "spiral time grid") + 1;
     Vec y((size_t)n);
     for (int64_t k = 0; k < n; k++) y[(size_t)k] = a + (double)k * d;
     return y;

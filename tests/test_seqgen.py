@@ -184,3 +184,16 @@ def test_errors_name_the_cause(bad, msg):
         engine.compile_psd(psd, VCTL)
     with pytest.raises(engine.MRSError, match="XML"):
         engine.compile_psd(PSD.replace("</MRiLabSeq>", ""), VCTL)
+
+
+@pytest.mark.parametrize("field,value", [("dt", "0"), ("dt", "NaN"), ("Duplicates", "1e12")])
+def test_unbounded_sample_counts_are_errors(field, value):
+    """A zero or NaN rf step or an absurd duplicate count is an error, not an
+    allocation of unbounded size."""
+    import re
+
+    psd = re.sub(rf'(<rfSinc [^>]*?){field}="[^"]*"', rf'\g<1>{field}="{value}"', PSD, count=1)
+    if field == "Duplicates":
+        psd = psd.replace('DupSpacing="0" Duplicates', 'DupSpacing="1e-3" Duplicates', 1)
+    with pytest.raises(engine.MRSError, match="sample count"):
+        engine.compile_psd(psd, VCTL)
