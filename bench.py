@@ -361,6 +361,40 @@ def sar_average_section(sc, threads: int, grams: float = 10.0, cpu_slices=(28, 3
                     "sample": f"slices {s0}-{s1 - 1}, C restatement of DoSARAverageAtCPU, OpenMP over rows"}}
 
 
+def seqgen_section(sample_tr: int) -> dict:
+    """The sequence compiler (host code in the library) on the 3-D GRE PSD
+    of tests/golden at the benchmark's encoding (128 readout, 128 x 64
+    phase encodes: 8192 TRs), against the Python oracle of DoPulseGen on
+    its first TRs, with the lines of that sample compared."""
+    import numpy as np
+    from mrilab_amd import engine
+    from oracle import seqgen
+    here = os.path.dirname(os.path.abspath(__file__))
+    psd = open(os.path.join(here, "tests", "golden", "psd_gre3d_small.xml")).read()
+    v = dict(TR=8e-3, TE=4e-3, FlipAng=20.0, FOVFreq=0.256, FOVPhase=0.256, FOVSlice=0.128, ResFreq=129,
+             ResPhase=128, SliceNum=64, FirstPhNum=128, SecondPhNum=64, BandWidth=100e3, RFreq=0.256 / 128,
+             MinUpdRate=1e-6, TEAnchor="Middle", MultiTransmit="off")
+    engine.compile_psd(psd, dict(v, FirstPhNum=1, SecondPhNum=1))  # library loaded outside the timer
+    t0 = time.perf_counter()
+    q = engine.compile_psd(psd, v)
+    lib_s = time.perf_counter() - t0
+    n_tr, n_ts = q["TRNum"], q["tsLine"].size
+    vs = dict(v, FirstPhNum=sample_tr, SecondPhNum=1)
+    t0 = time.perf_counter()
+    o = seqgen.compile_psd(psd, vs)
+    cpu_s = time.perf_counter() - t0
+    a = engine.compile_psd(psd, vs)
+    equal = all(np.array_equal(a[k], o[k]) if k in ("utsLine", "tsLine", "flagsLine", "ADCLine", "ExtLine")
+                else np.allclose(a[k], o[k], rtol=1e-14, atol=1e-30) for k in engine.SEQ_LINES)
+    log(f"sequence compiler: {n_tr} TRs, {n_ts} time points in {lib_s:.2f} s; oracle {cpu_s / sample_tr * 1e3:.2f} "
+        f"ms per TR; sample lines equal: {equal}")
+    return {"workload": f"3-D GRE PSD (tests/golden/psd_gre3d_small.xml), 128 readout x 128 x 64 phase encodes: "
+                        f"{n_tr} TRs, {n_ts} time points", "seconds": lib_s, "trs_per_s": n_tr / lib_s,
+            "cores": 1, "sample_lines_equal_to_oracle": bool(equal),
+            "cpu": {"seconds_per_tr": cpu_s / sample_tr, "kind": "port", "cores": 1,
+                    "sample": f"first {sample_tr} TRs, Python restatement of DoPulseGen and the elements"}}
+
+
 def coil_sum_section(n_tr: int) -> dict:
     """Forward kernel time per launch on the first ``n_tr`` TRs of the same
     scan with each coil sum: the default MFMA projection and the wavefront
@@ -506,6 +540,7 @@ def main() -> int:
     plan.close()
     if args.tr is None:
         line["coil_sum"] = coil_sum_section(64)
+        line["seqgen"] = seqgen_section(64)
     if not args.no_batched:
         line["batched"] = batched_section(args.batch_sets, args.batch_plain_sample, 3)
     if not args.no_adjoint:
