@@ -483,11 +483,19 @@ typedef struct mrs_seq_lines {   /* caller buffers of the sizes mrs_seq_counts g
  * PSD's expressions name) and gyro = VObj.Gyro (rad/s/T).
  * Replaces: DoPulseGen(Simuh), Src/Main/DoPulseGen.m:2-117 with
  * Src/Main/DoWriteXML2m.m:11-337 and the Macro/SeqElem elements above
- * (the gradient and update-rate checks of DoPulseGen.m:119-134 are not
- * made). */
+ * (the gradient and update-rate checks of DoPulseGen.m:119-134 are
+ * mrs_seq_check). */
 mrs_status mrs_seq_compile(const char *psd_xml, const mrs_seq_param *vctl, int32_t n_params, double gyro,
                            mrs_seq **out);
 /* counts = {n_ts, n_uts, n_rf, n_Gz, n_Gy, n_Gx, n_ADC, n_Ext, TRs}. */
+/* The hardware checks DoPulseGen makes after compiling
+ * (Src/Main/DoPulseGen.m:119-134): DoGradChk.m (slew rate dG/dt of each
+ * gradient line within MaxSlewRate + 0.5 %, amplitude within MaxGrad +
+ * 0.5 %, GzSS, GyP, GxR in turn) then DoUpdRateChk.m (successive time
+ * points of each Tx coil's rf and of each gradient line at least
+ * 0.99 MinUpdRate apart, steps below eps/2 ignored). MRS_ERR_ARG names the
+ * first failing check; mrs_seq_compile does not call it. */
+mrs_status mrs_seq_check(const mrs_seq *seq, double max_grad, double max_slew_rate, double min_upd_rate);
 mrs_status mrs_seq_counts(const mrs_seq *seq, int64_t counts[9]);
 /* Copies the lines into the caller's buffers. */
 mrs_status mrs_seq_copy(const mrs_seq *seq, const mrs_seq_lines *dst);

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1475,6 +1476,72 @@ mrs_status mrs_seq_compile(const char *psd_xml, const mrs_seq_param *vctl, int32
     } catch (const std::exception &e) {
         return fail(MRS_ERR_ARG, std::string("PSD: ") + e.what());
     }
+}
+
+// DoGradChk.m and DoUpdRateChk.m on the compiled lines, in their order;
+// the first failing check names itself.
+mrs_status mrs_seq_check(const mrs_seq *s, double max_grad, double max_slew, double min_upd) {
+    if (!s) return fail(MRS_ERR_ARG, "NULL argument");
+    const size_t n = s->ts.size();
+    auto times = [&](int row) {  // tsLine(flagsLine(row, :) ~= 0)
+        std::vector<double> t;
+        for (size_t k = 0; k < n; k++)
+            if (s->flags[k * 6 + (size_t)row] != 0) t.push_back(s->ts[k]);
+        return t;
+    };
+    const double gtol = std::fabs(max_grad * 0.005), stol = std::fabs(max_slew * 0.005);
+    const char *gname[3] = {"GzSS", "GyP", "GxR"};
+    const std::vector<double> *gline[3] = {&s->Gz, &s->Gy, &s->Gx};
+    std::string why;
+    for (int g = 0; g < 3 && why.empty(); g++) {  // DoGradChk.m:8-61
+        const std::vector<double> t = times(g + 1), &a = *gline[g];
+        if (t.size() != a.size()) return fail(MRS_ERR_ARG, std::string(gname[g]) + " line and flags disagree");
+        for (size_t k = 0; k + 1 < a.size(); k++) {
+            const double sr = (a[k + 1] - a[k]) / (t[k + 1] - t[k]);
+            if ((std::fabs(sr) - max_slew) > stol) {
+                why = std::string(gname[g]) + " exceeds maximum allowed gradient slew rate";
+                break;
+            }
+        }
+        if (!why.empty()) break;
+        for (double x : a)
+            if ((std::fabs(x) - max_grad) > gtol) {
+                why = std::string(gname[g]) + " exceeds maximum allowed gradient amplitude";
+                break;
+            }
+    }
+    if (!why.empty()) return fail(MRS_ERR_ARG, "Gradient check failed: " + why);
+    const double utol = -std::fabs(min_upd * 0.01), half_eps = 0.5 * seqgen::kEps;
+    auto too_fast = [&](const std::vector<double> &t) {
+        for (size_t k = 0; k + 1 < t.size(); k++) {
+            const double dt = t[k + 1] - t[k];
+            if (dt < half_eps) continue;  // unique's rounding threshold; TR boundaries
+            if ((std::fabs(dt) - min_upd) < utol) return true;
+        }
+        return false;
+    };
+    {  // DoUpdRateChk.m:8-22, per Tx coil in ascending order
+        const std::vector<double> t = times(0);
+        if (t.size() != s->rfCoil.size()) return fail(MRS_ERR_ARG, "rf line and flags disagree");
+        std::vector<double> ids(s->rfCoil);
+        std::sort(ids.begin(), ids.end());
+        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+        for (double id : ids) {
+            std::vector<double> tc;
+            for (size_t k = 0; k < t.size(); k++)
+                if (s->rfCoil[k] == id) tc.push_back(t[k]);
+            if (too_fast(tc)) {
+                char b[96];
+                std::snprintf(b, sizeof b, "rf%g sequence line exceeds minimum update rate", id);
+                return fail(MRS_ERR_ARG, std::string("Update rate check failed: ") + b);
+            }
+        }
+    }
+    for (int g = 0; g < 3; g++)  // DoUpdRateChk.m:23-50
+        if (too_fast(times(g + 1)))
+            return fail(MRS_ERR_ARG, std::string("Update rate check failed: ") + gname[g] +
+                                         " sequence line exceeds minimum update rate");
+    return MRS_OK;
 }
 
 mrs_status mrs_seq_counts(const mrs_seq *s, int64_t counts[9]) {

@@ -46,7 +46,8 @@ EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanA
            "mrs_DoScanBatchAtGPU", "mrs_adjoint_create", "mrs_adjoint_forward", "mrs_adjoint_backward",
            "mrs_adjoint_get_info", "mrs_adjoint_destroy", "mrs_recon_dims", "mrs_DoCartRecon",
            "mrs_plan_recon", "mrs_DoAddNoise", "mrs_noise_scale", "mrs_DoCalSARAtGPU",
-           "mrs_DoSARAverageAtGPU", "mrs_seq_compile", "mrs_seq_counts", "mrs_seq_copy", "mrs_seq_destroy")
+           "mrs_DoSARAverageAtGPU", "mrs_seq_compile", "mrs_seq_counts", "mrs_seq_copy", "mrs_seq_destroy",
+           "mrs_seq_check")
 
 
 class MRSError(RuntimeError):
@@ -148,6 +149,8 @@ def load() -> C.CDLL:
         lib.mrs_seq_copy.restype = C.c_int
         lib.mrs_seq_destroy.argtypes = [C.c_void_p]
         lib.mrs_seq_destroy.restype = None
+        lib.mrs_seq_check.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        lib.mrs_seq_check.restype = C.c_int
         _lib = lib
         return lib
 
@@ -698,12 +701,14 @@ SEQ_LINES = ("utsLine", "tsLine", "rfAmpLine", "rfPhaseLine", "rfFreqLine", "rfC
              "GyAmpLine", "GxAmpLine", "ADCLine", "ExtLine", "flagsLine")
 
 
-def compile_psd(psd_xml: str, VCtl: dict, gyro: float = 267538030.3797068) -> dict:
+def compile_psd(psd_xml: str, VCtl: dict, gyro: float = 267538030.3797068, check: bool = False) -> dict:
     """VSeq lines of a PSD XML (its text) for the scan parameters ``VCtl``
     (numbers, numeric arrays, or strings for TEAnchor / MultiTransmit /
     DP_Flag), as DoPulseGen builds them (mrs_seq_compile). Returns a dict
     with the VSeq line names (flagsLine [6, n_ts] column-major) and
-    ``TRNum``, the number of TRs."""
+    ``TRNum``, the number of TRs. ``check`` makes DoPulseGen's gradient and
+    update-rate checks (mrs_seq_check with VCtl MaxGrad, MaxSlewRate,
+    MinUpdRate) and raises MRSError naming the first failure."""
     lib = load()
     keep, params = [], []
     for name, v in VCtl.items():
@@ -725,6 +730,9 @@ def compile_psd(psd_xml: str, VCtl: dict, gyro: float = 267538030.3797068) -> di
     h = C.c_void_p()
     _check(lib.mrs_seq_compile(psd_xml.encode(), arr, len(params), float(gyro), C.byref(h)))
     try:
+        if check:
+            num = lambda k: float(np.atleast_1d(VCtl[k])[0])
+            _check(lib.mrs_seq_check(h, num("MaxGrad"), num("MaxSlewRate"), num("MinUpdRate")))
         cnt = (C.c_int64 * 9)()
         _check(lib.mrs_seq_counts(h, cnt))
         n_ts, n_uts, n_rf, n_gz, n_gy, n_gx, n_adc, n_ext, n_tr = list(cnt)

@@ -763,3 +763,34 @@ def compile_psd(xml_text: str, vctl: dict, gyro: float = 267538030.3797068) -> d
             "rfFreqLine": cat["rfF"], "rfCoilLine": cat["rfC"], "GzAmpLine": cat["Gz"], "GyAmpLine": cat["Gy"],
             "GxAmpLine": cat["Gx"], "ADCLine": cat["ADC"], "ExtLine": cat["Ext"],
             "flagsLine": np.asfortranarray(np.concatenate(flags, axis=1)), "TRNum": n_tr}
+
+
+def check(q: dict, max_grad: float, max_slew: float, min_upd: float):
+    """DoGradChk.m then DoUpdRateChk.m on compiled lines: None, or the first
+    failing check's message."""
+    ts, fl = q["tsLine"], q["flagsLine"]
+    names = (("GzSS", "GzAmpLine", 1), ("GyP", "GyAmpLine", 2), ("GxR", "GxAmpLine", 3))
+    gtol, stol = abs(max_grad * 0.005), abs(max_slew * 0.005)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for name, line, row in names:
+            t, g = ts[fl[row] != 0], q[line]
+            sr = np.diff(g) / np.diff(t)
+            if np.any((np.abs(sr) - max_slew) > stol):
+                return f"Gradient check failed: {name} exceeds maximum allowed gradient slew rate"
+            if np.any((np.abs(g) - max_grad) > gtol):
+                return f"Gradient check failed: {name} exceeds maximum allowed gradient amplitude"
+    utol = -abs(min_upd * 0.01)
+
+    def too_fast(t):
+        dt = np.diff(t)
+        dt = dt[~(dt < 0.5 * EPS)]
+        return bool(np.any((np.abs(dt) - min_upd) < utol))
+
+    t = ts[fl[0] != 0]
+    for cid in np.unique(q["rfCoilLine"]):
+        if too_fast(t[q["rfCoilLine"] == cid]):
+            return f"Update rate check failed: rf{cid:g} sequence line exceeds minimum update rate"
+    for name, line, row in names:
+        if too_fast(ts[fl[row] != 0]):
+            return f"Update rate check failed: {name} sequence line exceeds minimum update rate"
+    return None

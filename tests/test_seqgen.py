@@ -197,3 +197,38 @@ def test_unbounded_sample_counts_are_errors(field, value):
         psd = psd.replace('DupSpacing="0" Duplicates', 'DupSpacing="1e-3" Duplicates', 1)
     with pytest.raises(engine.MRSError, match="sample count"):
         engine.compile_psd(psd, VCTL)
+
+
+def _check_cases():
+    import seq_techniques as T
+
+    on = "$1'on','off'"
+    trap = (f'<GxTrapezoid DupSpacing="0" Duplicates="1" GxAmp="10e-3" Notes="x" Switch="{on}" sRamp="2" '
+            f'tEnd="9.2e-3" tRamp="50e-6" tStart="9e-3"/>')  # slew 200 T/m/s, 10 mT/m
+    psd, v = T.rf_only(T.hs(), gx=trap)
+    return [(psd, v, None),
+            (psd, dict(v, MaxSlewRate=199.5), None),              # inside the 0.5 % tolerance
+            (psd, dict(v, MaxSlewRate=198.0), "GxR exceeds maximum allowed gradient slew rate"),
+            (psd, dict(v, MaxGrad=9.9e-3), "GxR exceeds maximum allowed gradient amplitude"),
+            (psd, dict(v, MinUpdRate=10.5e-6), "rf1 sequence line exceeds minimum update rate"),
+            (psd, dict(v, MinUpdRate=10.05e-6), None),            # within 1 % of the 10 us rf step
+            (*T.epi(), "GyP exceeds maximum allowed gradient slew rate"),
+            (*T.radial(), "GyP exceeds maximum allowed gradient slew rate"),
+            (*T.fse(), None), (*T.spiral(), None), (*T.spiral(0.5), None)]
+
+
+@pytest.mark.parametrize("case", range(11))
+def test_gradient_and_update_rate_checks(case):
+    """mrs_seq_check (DoGradChk.m, DoUpdRateChk.m) agrees with the oracle's
+    restatement: same verdict, same first failing check; thresholds at
+    MaxSlewRate/MaxGrad + 0.5 % and 0.99 MinUpdRate."""
+    psd, v, want = _check_cases()[case]
+    q = seqgen.compile_psd(psd, v)
+    got = seqgen.check(q, v["MaxGrad"], v["MaxSlewRate"], v["MinUpdRate"])
+    assert (got is None) == (want is None) and (want is None or want in got)
+    if want is None:
+        engine.compile_psd(psd, v, check=True)
+    else:
+        with pytest.raises(engine.MRSError, match=want):
+            engine.compile_psd(psd, v, check=True)
+    engine.compile_psd(psd, v)  # compiling alone makes no check
