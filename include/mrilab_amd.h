@@ -488,14 +488,17 @@ typedef struct mrs_seq_lines {   /* caller buffers of the sizes mrs_seq_counts g
 mrs_status mrs_seq_compile(const char *psd_xml, const mrs_seq_param *vctl, int32_t n_params, double gyro,
                            mrs_seq **out);
 /* counts = {n_ts, n_uts, n_rf, n_Gz, n_Gy, n_Gx, n_ADC, n_Ext, TRs}. */
-/* The hardware checks DoPulseGen makes after compiling
- * (Src/Main/DoPulseGen.m:119-134): DoGradChk.m (slew rate dG/dt of each
+/* The checks DoPulseGen makes after compiling (Src/Main/DoPulseGen.m:
+ * 119-134): DoMultiTransmitChk.m when VCtl.MultiTransmit is 'on' and
+ * tx_coil_num > 0 (no CoilID above tx_coil_num = VCoi.TxCoilNum, the
+ * master coil used), DoGradChk.m (slew rate dG/dt of each
  * gradient line within MaxSlewRate + 0.5 %, amplitude within MaxGrad +
  * 0.5 %, GzSS, GyP, GxR in turn) then DoUpdRateChk.m (successive time
  * points of each Tx coil's rf and of each gradient line at least
  * 0.99 MinUpdRate apart, steps below eps/2 ignored). MRS_ERR_ARG names the
  * first failing check; mrs_seq_compile does not call it. */
-mrs_status mrs_seq_check(const mrs_seq *seq, double max_grad, double max_slew_rate, double min_upd_rate);
+mrs_status mrs_seq_check(const mrs_seq *seq, double max_grad, double max_slew_rate, double min_upd_rate,
+                         int32_t tx_coil_num);
 mrs_status mrs_seq_counts(const mrs_seq *seq, int64_t counts[9]);
 /* Copies the lines into the caller's buffers. */
 mrs_status mrs_seq_copy(const mrs_seq *seq, const mrs_seq_lines *dst);

@@ -43,6 +43,8 @@ mrs_status fail(mrs_status st, const std::string &msg);
 struct mrs_seq {
     std::vector<double> uts, ts, rfAmp, rfPhase, rfFreq, rfCoil, Gz, Gy, Gx, ADC, Ext, flags;
     int32_t n_tr = 0;
+    bool multi_transmit = false;  // VCtl.MultiTransmit == 'on'
+    double master_tx = 1;         // VCtl.MasterTxCoil
 };
 
 namespace mrs {
@@ -1434,6 +1436,8 @@ static void compile(const Node &root, Ctx &c, mrs_seq &out) {
         s = s + 1;
     }
     if (c.vtext("MultiTransmit", "off") == "off") std::fill(out.rfCoil.begin(), out.rfCoil.end(), 1.0);
+    out.multi_transmit = c.vtext("MultiTransmit", "off") == "on";
+    out.master_tx = c.vctl("MasterTxCoil");
 }
 
 }  // namespace seqgen
@@ -1478,10 +1482,18 @@ mrs_status mrs_seq_compile(const char *psd_xml, const mrs_seq_param *vctl, int32
     }
 }
 
-// DoGradChk.m and DoUpdRateChk.m on the compiled lines, in their order;
-// the first failing check names itself.
-mrs_status mrs_seq_check(const mrs_seq *s, double max_grad, double max_slew, double min_upd) {
+// DoMultiTransmitChk.m, DoGradChk.m and DoUpdRateChk.m on the compiled
+// lines, in their order; the first failing check names itself.
+mrs_status mrs_seq_check(const mrs_seq *s, double max_grad, double max_slew, double min_upd, int32_t tx_coil_num) {
     if (!s) return fail(MRS_ERR_ARG, "NULL argument");
+    if (s->multi_transmit && tx_coil_num > 0 && !s->rfCoil.empty()) {  // DoMultiTransmitChk.m:7-18
+        const double mx = *std::max_element(s->rfCoil.begin(), s->rfCoil.end());
+        if ((double)tx_coil_num < mx)
+            return fail(MRS_ERR_ARG, "Multi-transmit check failed: the sequence uses more coil elements than "
+                                     "the Tx coil provides");
+        if (std::find(s->rfCoil.begin(), s->rfCoil.end(), s->master_tx) == s->rfCoil.end())
+            return fail(MRS_ERR_ARG, "Multi-transmit check failed: the master Tx coil element is not used");
+    }
     const size_t n = s->ts.size();
     auto times = [&](int row) {  // tsLine(flagsLine(row, :) ~= 0)
         std::vector<double> t;

@@ -149,7 +149,7 @@ def load() -> C.CDLL:
         lib.mrs_seq_copy.restype = C.c_int
         lib.mrs_seq_destroy.argtypes = [C.c_void_p]
         lib.mrs_seq_destroy.restype = None
-        lib.mrs_seq_check.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        lib.mrs_seq_check.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32]
         lib.mrs_seq_check.restype = C.c_int
         _lib = lib
         return lib
@@ -701,14 +701,16 @@ SEQ_LINES = ("utsLine", "tsLine", "rfAmpLine", "rfPhaseLine", "rfFreqLine", "rfC
              "GyAmpLine", "GxAmpLine", "ADCLine", "ExtLine", "flagsLine")
 
 
-def compile_psd(psd_xml: str, VCtl: dict, gyro: float = 267538030.3797068, check: bool = False) -> dict:
+def compile_psd(psd_xml: str, VCtl: dict, gyro: float = 267538030.3797068, check: bool = False,
+                tx_coil_num: int = 0) -> dict:
     """VSeq lines of a PSD XML (its text) for the scan parameters ``VCtl``
     (numbers, numeric arrays, or strings for TEAnchor / MultiTransmit /
     DP_Flag), as DoPulseGen builds them (mrs_seq_compile). Returns a dict
     with the VSeq line names (flagsLine [6, n_ts] column-major) and
-    ``TRNum``, the number of TRs. ``check`` makes DoPulseGen's gradient and
-    update-rate checks (mrs_seq_check with VCtl MaxGrad, MaxSlewRate,
-    MinUpdRate) and raises MRSError naming the first failure."""
+    ``TRNum``, the number of TRs. ``check`` makes DoPulseGen's checks
+    (mrs_seq_check: multi-transmit against ``tx_coil_num`` = VCoi.TxCoilNum
+    when it is > 0, then gradient and update rate with VCtl MaxGrad,
+    MaxSlewRate, MinUpdRate) and raises MRSError naming the first failure."""
     lib = load()
     keep, params = [], []
     for name, v in VCtl.items():
@@ -732,7 +734,7 @@ def compile_psd(psd_xml: str, VCtl: dict, gyro: float = 267538030.3797068, check
     try:
         if check:
             num = lambda k: float(np.atleast_1d(VCtl[k])[0])
-            _check(lib.mrs_seq_check(h, num("MaxGrad"), num("MaxSlewRate"), num("MinUpdRate")))
+            _check(lib.mrs_seq_check(h, num("MaxGrad"), num("MaxSlewRate"), num("MinUpdRate"), int(tx_coil_num)))
         cnt = (C.c_int64 * 9)()
         _check(lib.mrs_seq_counts(h, cnt))
         n_ts, n_uts, n_rf, n_gz, n_gy, n_gx, n_adc, n_ext, n_tr = list(cnt)

@@ -765,10 +765,16 @@ def compile_psd(xml_text: str, vctl: dict, gyro: float = 267538030.3797068) -> d
             "flagsLine": np.asfortranarray(np.concatenate(flags, axis=1)), "TRNum": n_tr}
 
 
-def check(q: dict, max_grad: float, max_slew: float, min_upd: float):
-    """DoGradChk.m then DoUpdRateChk.m on compiled lines: None, or the first
-    failing check's message."""
+def check(q: dict, max_grad: float, max_slew: float, min_upd: float, tx_coil_num: int = 0, vctl=None):
+    """DoMultiTransmitChk.m (with ``vctl`` and tx_coil_num > 0), DoGradChk.m
+    then DoUpdRateChk.m on compiled lines: None, or the first failing
+    check's message."""
     ts, fl = q["tsLine"], q["flagsLine"]
+    if vctl is not None and tx_coil_num > 0 and vctl.get("MultiTransmit") == "on" and q["rfCoilLine"].size:
+        if tx_coil_num < np.max(q["rfCoilLine"]):
+            return "Multi-transmit check failed: the sequence uses more coil elements than the Tx coil provides"
+        if not np.any(q["rfCoilLine"] == float(vctl.get("MasterTxCoil", 1.0))):
+            return "Multi-transmit check failed: the master Tx coil element is not used"
     names = (("GzSS", "GzAmpLine", 1), ("GyP", "GyAmpLine", 2), ("GxR", "GxAmpLine", 3))
     gtol, stol = abs(max_grad * 0.005), abs(max_slew * 0.005)
     with np.errstate(divide="ignore", invalid="ignore"):

@@ -232,3 +232,24 @@ def test_gradient_and_update_rate_checks(case):
         with pytest.raises(engine.MRSError, match=want):
             engine.compile_psd(psd, v, check=True)
     engine.compile_psd(psd, v)  # compiling alone makes no check
+
+
+@pytest.mark.parametrize("tx,master,want", [(2, 1.0, None), (1, 1.0, "more coil elements"),
+                                            (2, 3.0, "master Tx coil"), (0, 3.0, None)])
+def test_multitransmit_check(tx, master, want):
+    """DoMultiTransmitChk.m: with MultiTransmit 'on', no CoilID above
+    VCoi.TxCoilNum and the master coil among the rf lines' coils (tx 0:
+    the check is not made)."""
+    rf2 = ('<rfRect AnchorTE="$2\'on\',\'off\'" CoilID="2" DupSpacing="0" Duplicates="1" FA="10" Notes="x" '
+           'Switch="$1\'on\',\'off\'" dt="10e-6" rfFreq="0" rfPhase="0" tEnd="0.3e-3" tStart="0.1e-3"/>')
+    psd = PSD.replace("</rf>", rf2 + "</rf>", 1)
+    v = dict(VCTL, MultiTransmit="on", MasterTxCoil=master, MaxGrad=1.0, MaxSlewRate=1e6, FirstPhNum=2,
+             SecondPhNum=1)
+    q = seqgen.compile_psd(psd, v)
+    got = seqgen.check(q, 1.0, 1e6, 1e-6, tx, v)
+    assert (got is None) == (want is None) and (want is None or want in got)
+    if want is None:
+        engine.compile_psd(psd, v, check=True, tx_coil_num=tx)
+    else:
+        with pytest.raises(engine.MRSError, match=want):
+            engine.compile_psd(psd, v, check=True, tx_coil_num=tx)
