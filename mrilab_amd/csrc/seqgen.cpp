@@ -1484,7 +1484,19 @@ mrs_status mrs_seq_compile(const char *psd_xml, const mrs_seq_param *vctl, int32
 
 // DoMultiTransmitChk.m, DoGradChk.m and DoUpdRateChk.m on the compiled
 // lines, in their order; the first failing check names itself.
+static mrs_status seq_check(const mrs_seq *s, double max_grad, double max_slew, double min_upd,
+                            int32_t tx_coil_num);
+
 mrs_status mrs_seq_check(const mrs_seq *s, double max_grad, double max_slew, double min_upd, int32_t tx_coil_num) {
+    try {
+        return seq_check(s, max_grad, max_slew, min_upd, tx_coil_num);
+    } catch (const std::bad_alloc &) {
+        return fail(MRS_ERR_NOMEM, "sequence check: out of host memory");
+    }
+}
+
+static mrs_status seq_check(const mrs_seq *s, double max_grad, double max_slew, double min_upd,
+                            int32_t tx_coil_num) {
     if (!s) return fail(MRS_ERR_ARG, "NULL argument");
     if (s->multi_transmit && tx_coil_num > 0 && !s->rfCoil.empty()) {  // DoMultiTransmitChk.m:7-18
         const double mx = *std::max_element(s->rfCoil.begin(), s->rfCoil.end());
