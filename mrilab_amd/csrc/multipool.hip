@@ -9,7 +9,7 @@
 //             start-of-step magnetization of the other pools);
 //   SCHEME 1: the exact solution of the same linear Bloch-McConnell system
 //             over each held step, in fp64: the Taylor series of the action
-//             of exp(A*dt) on the state, with substeps of h*|A| <= 0.5 and the
+//             of exp(A*dt) on the state, with substeps of h*|A| <= 4 and the
 //             structured generator applied directly (3x3 rotation-relaxation
 //             blocks, scalar bound-pool rows, diagonal exchange couplings).
 // One thread owns one isochromat of one voxel with all its pools; all pool
