@@ -384,6 +384,16 @@ mrs_status mrs_recon_dims(const mrs_recon_desc *desc, int64_t dims[5]);
  * vsig->Sx/Sy and the VImg arrays are device memory of GPU 0.
  * Replaces: DoCartRecon, Src/Recon/Default/DoCartRecon.m:3-91 */
 mrs_status mrs_DoCartRecon(const mrs_vsig *vsig, const mrs_recon_desc *desc, mrs_vimg *img, int32_t device);
+/* EPI and FSE recon: the signal of an echo-train scan (per shot EPI_ETL or
+ * FSE_ETL echoes of ResFreq samples, SignalNum = ResFreq*ETL*ShotNum*
+ * SliceNum, TEPerTR 1) put in Cartesian order (echo e of shot p on phase
+ * line e*ShotNum + p; flip_even = 1 for EPI reverses the readout of every
+ * second echo), then the mrs_DoCartRecon pipeline with res_phase =
+ * ETL*ShotNum.
+ * Replaces: DoEPIRecon, Src/Recon/Default/DoEPIRecon.m:1-95 (flip_even 1)
+ *           DoFSERecon, Src/Recon/Default/DoFSERecon.m:1-90 (flip_even 0) */
+mrs_status mrs_DoEchoTrainRecon(const mrs_vsig *vsig, const mrs_recon_desc *desc, int32_t etl, int32_t shots,
+                                int32_t flip_even, mrs_vimg *img, int32_t device);
 /* The same from the k-space a plan's last run left in device memory (no
  * download of the signal). */
 mrs_status mrs_plan_recon(mrs_plan *plan, const mrs_recon_desc *desc, mrs_vimg *img, int32_t device);

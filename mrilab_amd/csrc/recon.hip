@@ -239,6 +239,31 @@ __global__ __launch_bounds__(256) void noise_kernel(double *sx, double *sy, int6
     }
 }
 
+// Echo-train k-space to Cartesian order (DoEPIRecon.m:8-26,
+// DoFSERecon.m:8-22): the signal of each shot is ETL echoes of R samples;
+// echo e (0-based) of shot p fills phase line e*ShotNum + p, and with
+// flip_even (EPI) the odd-indexed echoes, read out backwards, are
+// reversed along the readout. One thread per input value; the same
+// mapping for every coil and type block of N values.
+__global__ __launch_bounds__(256) void echo_train_reorder_kernel(const double *__restrict__ sx,
+                                                                 const double *__restrict__ sy, double *ox, double *oy,
+                                                                 int64_t total, int64_t N, int R, int etl, int shots,
+                                                                 int flip_even) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int64_t blk = i / N, n = i - blk * N;
+    const int64_t per = (int64_t)R * etl;  // samples of one shot
+    const int64_t ps = n / per;
+    const int a = (int)(n - ps * per);
+    const int p = (int)(ps % shots);
+    const int64_t sl = ps / shots;
+    const int e = a / R, m = a - e * R;
+    const int mo = (flip_even && (e & 1)) ? R - 1 - m : m;
+    const int64_t no = mo + (int64_t)R * ((int64_t)e * shots + p + (int64_t)etl * shots * sl);
+    ox[blk * N + no] = sx[i];
+    oy[blk * N + no] = sy[i];
+}
+
 struct RDev {
     void *p = nullptr;
     ~RDev() {
@@ -407,6 +432,52 @@ mrs_status mrs_recon_dims(const mrs_recon_desc *desc, int64_t dims[5]) {
     dims[3] = desc->rx;
     dims[4] = desc->te_per_tr;
     return MRS_OK;
+}
+
+mrs_status mrs_DoEchoTrainRecon(const mrs_vsig *vsig, const mrs_recon_desc *desc, int32_t etl, int32_t shots,
+                                int32_t flip_even, mrs_vimg *img, int32_t device) {
+    if (!vsig || !vsig->Sx || !vsig->Sy) return fail(MRS_ERR_ARG, "VSig.Sx/Sy is NULL");
+    if (!desc) return fail(MRS_ERR_ARG, "recon descriptor is NULL");
+    if (etl < 1 || shots < 1) return fail(MRS_ERR_ARG, "echo train length and shot count must be positive");
+    if (desc->te_per_tr != 1) return fail(MRS_ERR_ARG, "echo-train recon takes TEPerTR = 1");
+    if ((int64_t)desc->res_phase != (int64_t)etl * shots)
+        return fail(MRS_ERR_ARG, "res_phase must be the echo train length times the shot count");
+    if (desc->res_freq < 1 || desc->slice_num < 1 || desc->rx < 1 || desc->types < 1)
+        return fail(MRS_ERR_ARG, "recon sizes must be positive");
+    const int64_t N = vsig->SignalNum;
+    if (N != (int64_t)desc->res_freq * etl * shots * desc->slice_num)
+        return fail(MRS_ERR_ARG, "SignalNum must be ResFreq * ETL * ShotNum * SliceNum");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (ndev <= 0) return fail(MRS_ERR_DEVICE, "no HIP device visible");
+    HIP_TRY(hipSetDevice(0));
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct S {
+        hipStream_t s;
+        ~S() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    const int64_t total = N * desc->rx * desc->types;
+    const size_t bytes = (size_t)total * sizeof(double);
+    RDev ix, iy, ox, oy;
+    mrs_status rc;
+    const double *sx = vsig->Sx, *sy = vsig->Sy;
+    if (!device) {
+        if ((rc = rmalloc(ix, bytes)) || (rc = rmalloc(iy, bytes))) return rc;
+        HIP_TRY(hipMemcpyAsync(ix.p, vsig->Sx, bytes, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(iy.p, vsig->Sy, bytes, hipMemcpyHostToDevice, st));
+        sx = (const double *)ix.p;
+        sy = (const double *)iy.p;
+    }
+    if ((rc = rmalloc(ox, bytes)) || (rc = rmalloc(oy, bytes))) return rc;
+    const int64_t nblk = (total + 255) / 256;
+    if (nblk > INT32_MAX) return fail(MRS_ERR_ARG, "signal too large");
+    echo_train_reorder_kernel<<<dim3((unsigned)nblk), dim3(256), 0, st>>>(
+        sx, sy, (double *)ox.p, (double *)oy.p, total, N, desc->res_freq, etl, shots, flip_even ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    rc = recon_device((const double *)ox.p, (const double *)oy.p, N, desc, img, device != 0, st);
+    HIP_TRY(hipStreamSynchronize(st));  // the temporaries are freed on return
+    return rc;
 }
 
 mrs_status mrs_DoCartRecon(const mrs_vsig *vsig, const mrs_recon_desc *desc, mrs_vimg *img, int32_t device) {

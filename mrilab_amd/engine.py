@@ -44,7 +44,7 @@ EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanA
            "mrs_plan_fetch", "mrs_plan_get_info", "mrs_plan_destroy", "mrs_batch_create", "mrs_batch_run",
            "mrs_batch_sync", "mrs_batch_fetch", "mrs_batch_get_info", "mrs_batch_destroy",
            "mrs_DoScanBatchAtGPU", "mrs_adjoint_create", "mrs_adjoint_forward", "mrs_adjoint_backward",
-           "mrs_adjoint_get_info", "mrs_adjoint_destroy", "mrs_recon_dims", "mrs_DoCartRecon",
+           "mrs_adjoint_get_info", "mrs_adjoint_destroy", "mrs_recon_dims", "mrs_DoCartRecon", "mrs_DoEchoTrainRecon",
            "mrs_plan_recon", "mrs_DoAddNoise", "mrs_noise_scale", "mrs_DoCalSARAtGPU",
            "mrs_DoSARAverageAtGPU", "mrs_seq_compile", "mrs_seq_counts", "mrs_seq_copy", "mrs_seq_destroy",
            "mrs_seq_check")
@@ -129,6 +129,9 @@ def load() -> C.CDLL:
         lib.mrs_recon_dims.restype = C.c_int
         lib.mrs_DoCartRecon.argtypes = [P(abi.VSigC), P(abi.ReconDescC), P(abi.VImgC), C.c_int32]
         lib.mrs_DoCartRecon.restype = C.c_int
+        lib.mrs_DoEchoTrainRecon.argtypes = [P(abi.VSigC), P(abi.ReconDescC), C.c_int32, C.c_int32, C.c_int32,
+                                             P(abi.VImgC), C.c_int32]
+        lib.mrs_DoEchoTrainRecon.restype = C.c_int
         lib.mrs_plan_recon.argtypes = [C.c_void_p, P(abi.ReconDescC), P(abi.VImgC), C.c_int32]
         lib.mrs_plan_recon.restype = C.c_int
         lib.mrs_DoAddNoise.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_uint64, C.c_int32]
@@ -547,10 +550,12 @@ def _recon_call(call, desc, device: bool) -> dict:
     return {k: v.reshape(dims, order="F") for k, v in out.items()}
 
 
-def DoCartRecon(Sx, Sy, desc, device: bool = False) -> dict:
+def DoCartRecon(Sx, Sy, desc, device: bool = False, echo_train=None) -> dict:
     """VImg from VSig.Sx/Sy (numpy float64 arrays, or float64 CUDA tensors
     with ``device``); arrays [d0, d1, d2, Rx, TEPerTR] in MATLAB order
-    (numpy order "F"; torch tensors with the dims reversed)."""
+    (numpy order "F"; torch tensors with the dims reversed).
+    ``echo_train=(etl, shots, epi)`` reconstructs an EPI (epi True) or FSE
+    scan as DoEPIRecon/DoFSERecon do (mrs_DoEchoTrainRecon)."""
     lib = load()
     if device:
         import torch
@@ -566,8 +571,13 @@ def DoCartRecon(Sx, Sy, desc, device: bool = False) -> dict:
         n = sx.size // (desc.rx * desc.types)
         vs = abi.VSigC(Sx=sx.ctypes.data, Sy=sy.ctypes.data, SignalNum=n)
         keep = (sx, sy)
-    out = _recon_call(lambda img, dev: lib.mrs_DoCartRecon(C.byref(vs), C.byref(desc), C.byref(img), dev),
-                      desc, device)
+    if echo_train is None:
+        call = lambda img, dev: lib.mrs_DoCartRecon(C.byref(vs), C.byref(desc), C.byref(img), dev)
+    else:
+        etl, shots, epi = echo_train
+        call = lambda img, dev: lib.mrs_DoEchoTrainRecon(C.byref(vs), C.byref(desc), int(etl), int(shots),
+                                                         1 if epi else 0, C.byref(img), dev)
+    out = _recon_call(call, desc, device)
     del keep
     return out
 

@@ -89,6 +89,25 @@ def cart_recon(Sx, Sy, res_freq: int, res_phase: int, slice_num: int, te_per_tr:
     return out
 
 
+def echo_train_order(Sx, Sy, res_freq: int, etl: int, shots: int, slice_num: int, rx: int, types: int, epi: bool):
+    """DoEPIRecon.m:8-26 / DoFSERecon.m:8-22: the signal of each shot (ETL
+    echoes of res_freq samples) into Cartesian order, echo e of shot p on
+    line (e-1)*ShotNum + p, EPI's even echoes reversed along the readout;
+    per coil and type, so cart_recon then sums the types."""
+    shape = (res_freq * etl, shots, slice_num, rx, types)
+    out = []
+    for S in (Sx, Sy):
+        X = np.asarray(S, dtype=np.float64).reshape(shape, order="F")
+        Y = np.zeros((res_freq, etl * shots, slice_num, rx, types))
+        for e in range(1, etl + 1):
+            blk = X[(e - 1) * res_freq:e * res_freq]
+            if epi and (-1) ** (e - 1) != 1:
+                blk = blk[::-1]
+            Y[:, (e - 1) * shots:e * shots] = blk
+        out.append(Y.ravel(order="F"))
+    return out[0], out[1]
+
+
 def noise_scale(bandwidth, noise_level, nex, b0, rfreq, rphase, rslice, res_freq, res_phase, slice_num) -> float:
     """Standard deviation of the k-space noise, DoAddNoise.m:8-16."""
     bw_ref, b0_ref, nex_ref, adc_ref, vol_ref, noise_ref = 1e3, 1.5, 1.0, 1e4, 1e-9, 1.0

@@ -110,3 +110,29 @@ def test_noise_device_pointers_and_determinism(gpu):
     engine.DoAddNoise(hx, hy, 1.0, 99)
     assert np.array_equal(a.cpu().numpy(), hx)
     assert np.array_equal(b.cpu().numpy(), hy)
+
+
+@pytest.mark.parametrize("epi,etl,shots,sl,rx,types", [(True, 8, 2, 2, 1, 1), (True, 6, 3, 1, 2, 2),
+                                                       (False, 4, 2, 2, 3, 1), (False, 6, 1, 1, 1, 2)])
+@pytest.mark.parametrize("device", [False, True])
+def test_echo_train_recon_matches_oracle(gpu, epi, etl, shots, sl, rx, types, device):
+    """mrs_DoEchoTrainRecon (DoEPIRecon / DoFSERecon) against the oracle's
+    reorder followed by its Cartesian recon."""
+    rf = 17
+    print(f"case echo train epi={epi} etl={etl} shots={shots} sl={sl} rx={rx} types={types} "
+          f"device={device}", flush=True)
+    sx, sy = _signal(rf, etl * shots, sl, 1, types, rx, seed=etl * 7 + shots)
+    d = engine.recon_desc(rf, etl * shots, sl, 1, rx, types, scan_plane="Axial", freq_dir="R/L")
+    if device:
+        import torch
+        out = engine.DoCartRecon(torch.from_numpy(sx).cuda(), torch.from_numpy(sy).cuda(), d, device=True,
+                                 echo_train=(etl, shots, epi))
+        out = {k: v.cpu().numpy() if hasattr(v, "cpu") else v for k, v in out.items()}
+    else:
+        out = engine.DoCartRecon(sx, sy, d, echo_train=(etl, shots, epi))
+    ox, oy = orc.echo_train_order(sx, sy, rf, etl, shots, sl, rx, types, epi)
+    want = orc.cart_recon(ox, oy, rf, etl * shots, sl, 1, rx, types, swap=orc.swap_xy("Axial", "R/L"))
+    if device:  # tensors come back with the MATLAB dims reversed
+        out = {k: np.transpose(v) for k, v in out.items()}
+    assert out["Real"].shape == want["Real"].shape
+    _check(out, want, f"echo train {epi} {etl} {shots}")

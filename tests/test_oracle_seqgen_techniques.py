@@ -194,3 +194,27 @@ def test_spiral_slew_limited_and_sample_count():
     t = adc_times(q)
     assert all(x.size == math.floor(tacq * v["BandWidth"]) for x in t)
     assert n_tr == int(v["S_ShotNum"]) * int(v["SliceNum"])
+
+
+@pytest.mark.parametrize("name", ["epi", "fse"])
+def test_echo_train_recon_order_puts_k_space_on_the_grid(name):
+    """The recon oracle's DoEPIRecon/DoFSERecon reorder (oracle/recon.py
+    echo_train_order), applied to the k-space positions the oracle scan
+    recorded, gives the Cartesian grid DoCartRecon expects: kx increasing
+    along the readout on every line (EPI's reversed echoes flipped) and
+    line j at ky = (j - ResPhase/2)/FOVPhase."""
+    from oracle import recon as orc
+
+    psd, v = getattr(T, name)()
+    q, n_tr, kx, ky, kz = scan_k(psd, v)
+    R, sl = int(v["ResFreq"]), int(v["SliceNum"])
+    etl = int(v["EPI_ETL"] if name == "epi" else v["FSE_ETL"])
+    shots = int(v["EPI_ShotNum"] if name == "epi" else v["FSE_ShotNum"])
+    KX, KY = orc.echo_train_order(kx, ky, R, etl, shots, sl, 1, 1, name == "epi")
+    KX = KX.reshape((R, etl * shots, sl), order="F")
+    KY = KY.reshape((R, etl * shots, sl), order="F")
+    kxmax = (1 / v["RFreq"]) / 2
+    m = np.arange(R)[:, None, None]
+    j = np.arange(etl * shots)[None, :, None]
+    np.testing.assert_allclose(KX, np.broadcast_to(kxmax * (2 * m / (R - 1) - 1), KX.shape), atol=1e-6)
+    np.testing.assert_allclose(KY, np.broadcast_to((j - v["ResPhase"] / 2) / v["FOVPhase"], KY.shape), atol=1e-6)
