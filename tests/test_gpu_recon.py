@@ -1,6 +1,7 @@
-"""GPU Cartesian recon and k-space noise (mrs_DoCartRecon, mrs_plan_recon,
-mrs_DoAddNoise) against the oracle (oracle/recon.py, pinned by
-tests/test_oracle_recon.py).
+"""GPU Cartesian, EPI and FSE recon and k-space noise (mrs_DoCartRecon,
+mrs_DoEchoTrainRecon, mrs_plan_recon, mrs_DoAddNoise) against the oracle
+(oracle/recon.py, pinned by tests/test_oracle_recon.py; its echo-train
+reorder by tests/test_oracle_seqgen_techniques.py).
 
 Recon runs in fp64 on hipFFT; the bar is 1e-12 relative L2 on every VImg
 array (the FFTs' rounding differs from numpy's by ulps). Noise: the Philox
