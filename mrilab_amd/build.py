@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -28,9 +29,11 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found; the engine needs ROCm's hipcc to build")
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compiles every source (in parallel, ``jobs`` at a time: default the
+    host's cores, at most 16) and links the library."""
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     common = ["-O3", "-fPIC", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
               "-Wall", "-Wno-unused-result"]
     for src in HIP_SOURCES + CXX_SOURCES:
@@ -41,8 +44,13 @@ def build(verbose: bool = False) -> str:
             cmd = [hipcc(), *common, "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    n = jobs or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=n) as pool:
+        for r in list(pool.map(lambda c: subprocess.run(c), cmds)):
+            if r.returncode != 0:
+                raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = LIB + ".tmp"
     rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc()))), "lib")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, f"-L{rocm_lib}", "-lhipfft",

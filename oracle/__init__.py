@@ -29,6 +29,30 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def _configure(lib: C.CDLL) -> C.CDLL:
+    P = C.POINTER
+    lib.orc_DoScanAtCPU.argtypes = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC),
+                                    P(abi.VSeqC), P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC)]
+    lib.orc_DoScanAtCPU.restype = C.c_int
+    lib.orc_DoGMScan.argtypes = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC),
+                                 P(abi.VSeqC), P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC), C.c_int,
+                                 C.c_int]
+    lib.orc_DoGMScan.restype = C.c_int
+    lib.orc_superlorentzian.argtypes = [C.c_double, C.c_double]
+    lib.orc_superlorentzian.restype = C.c_double
+    if hasattr(lib, "orc_DoCalSAR"):
+        lib.orc_DoCalSAR.argtypes = [P(abi.SarObjC), P(abi.VSeqC), P(abi.VVarC), P(abi.SarSigC)]
+        lib.orc_DoCalSAR.restype = C.c_int
+    if hasattr(lib, "orc_DoSARAverage"):
+        D = C.POINTER(C.c_double)
+        lib.orc_DoSARAverage.argtypes = [D, D, D, C.POINTER(C.c_int64), C.c_double, C.c_double, C.c_double,
+                                         C.c_double, D, D, C.c_int64, C.c_int64]
+        lib.orc_DoSARAverage.restype = C.c_int
+    lib.orc_set_threads.argtypes = [C.c_int]
+    lib.orc_set_relax_mode.argtypes = [C.c_int]
+    return lib
+
+
 def _lib(precision: str) -> C.CDLL:
     if precision not in ("f32", "f64"):
         raise ValueError("precision must be 'f32' or 'f64'")
@@ -36,27 +60,20 @@ def _lib(precision: str) -> C.CDLL:
         path = os.path.join(_BUILD, f"liborc_{precision}.so")
         if not os.path.exists(path):
             build()
-        lib = C.CDLL(path)
-        P = C.POINTER
-        lib.orc_DoScanAtCPU.argtypes = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC),
-                                        P(abi.VSeqC), P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC)]
-        lib.orc_DoScanAtCPU.restype = C.c_int
-        lib.orc_DoGMScan.argtypes = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC),
-                                     P(abi.VSeqC), P(abi.VVarC), P(abi.VSigC), P(abi.CallbacksC), C.c_int,
-                                     C.c_int]
-        lib.orc_DoGMScan.restype = C.c_int
-        lib.orc_superlorentzian.argtypes = [C.c_double, C.c_double]
-        lib.orc_superlorentzian.restype = C.c_double
-        lib.orc_DoCalSAR.argtypes = [P(abi.SarObjC), P(abi.VSeqC), P(abi.VVarC), P(abi.SarSigC)]
-        lib.orc_DoCalSAR.restype = C.c_int
-        D = C.POINTER(C.c_double)
-        lib.orc_DoSARAverage.argtypes = [D, D, D, C.POINTER(C.c_int64), C.c_double, C.c_double, C.c_double,
-                                         C.c_double, D, D, C.c_int64, C.c_int64]
-        lib.orc_DoSARAverage.restype = C.c_int
-        lib.orc_set_threads.argtypes = [C.c_int]
-        lib.orc_set_relax_mode.argtypes = [C.c_int]
-        _libs[precision] = lib
+        _libs[precision] = _configure(C.CDLL(path))
     return _libs[precision]
+
+
+def build_mutant(mutation: int, out_dir: str, precision: str = "f64") -> C.CDLL:
+    """Compiles bloch_oracle.c with planted mistake ``mutation`` (the
+    ORC_MUTATION list in its header) into ``out_dir`` and loads it. Used
+    only by the oracle's own tests, to show that a pin catches the mistake."""
+    real = "double" if precision == "f64" else "float"
+    path = os.path.join(out_dir, f"liborc_mut{mutation}_{precision}.so")
+    subprocess.run(["gcc", "-O1", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+                    f"-DORC_REAL={real}", f"-DORC_MUTATION={int(mutation)}", "-o", path,
+                    os.path.join(_HERE, "bloch_oracle.c"), "-lm"], check=True)
+    return _configure(C.CDLL(path))
 
 
 def set_threads(n: int, precision: str = "f32") -> None:
@@ -69,10 +86,11 @@ def set_relax_mode(mode: int, precision: str = "f32") -> None:
     _lib(precision).orc_set_relax_mode(int(mode))
 
 
-def scan(sc, precision: str = "f64", callbacks=None, inplace: bool = False):
-    """Run the oracle scan; returns the Scan with VObj/VVar/VSig outputs."""
+def scan(sc, precision: str = "f64", callbacks=None, inplace: bool = False, lib=None):
+    """Run the oracle scan; returns the Scan with VObj/VVar/VSig outputs.
+    ``lib``: a library from build_mutant in place of the default one."""
     out = sc if inplace else sc.copy()
-    lib = _lib(precision)
+    lib = lib or _lib(precision)
     P = abi.pack_scan(*out.structs)
     cb = C.pointer(callbacks) if callbacks is not None else None
     rc = lib.orc_DoScanAtCPU(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi), C.byref(P.vctl),
@@ -91,14 +109,14 @@ def k_planes(variant: str, ty: int) -> int:
     return 2 * (ty - 1) if variant == "CEST" else ty * ty
 
 
-def gm_scan(sc, precision: str = "f64", scheme: int = 0, inplace: bool = False, variant: str = "GM"):
+def gm_scan(sc, precision: str = "f64", scheme: int = 0, inplace: bool = False, variant: str = "GM", lib=None):
     """Multi-pool scan: variant "GM" (BlochKernelGM.cuh), "MT"
     (BlochKernelMT.cuh), "CEST" (BlochKernelCEST.cuh) or "ME"
     (BlochKernelME.cuh) with VObj.K in that kernel's layout; scheme 0 =
     the kernel's operator split, 1 = fp64 matrix exponential of the same
     generator."""
     out = sc if inplace else sc.copy()
-    lib = _lib(precision)
+    lib = lib or _lib(precision)
     ty = int(out.VObj["TypeNum"])
     P = abi.pack_scan(*out.structs, multipool=True, k_planes=k_planes(variant, ty))
     rc = lib.orc_DoGMScan(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi), C.byref(P.vctl),

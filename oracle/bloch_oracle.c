@@ -34,6 +34,22 @@ typedef ORC_REAL real;
 
 #define ORC_PI 3.14159265359 /* DoScanAtCPU.cpp:63 */
 
+/* Planted mistakes, for the oracle's own tests only: tests/test_oracle_pins.py
+ * compiles a copy of this file with -DORC_MUTATION=n and checks that one of
+ * the closed-form pins fails on it. Every library that checks the engine is
+ * built with the default 0, where each ORC_MUT(n) below is constant false.
+ *   1 multi-Tx phase from atan2(re, im) instead of atan2(im, re)
+ *   2 multi-Tx rf frequency added for the first active coil only
+ *   3 multi-Tx rf frequency of a silent coil (amplitude 0) added
+ *   4 bound-pool saturation W summed over active Tx coils, not averaged
+ *   5 split exchange with the end-of-step state of pools updated earlier
+ *   6 MT bound pool exchanging with water's end-of-step Mz
+ *   7 ReleaseK (Ext 4) leaving Kx unrestored */
+#ifndef ORC_MUTATION
+#define ORC_MUTATION 0
+#endif
+#define ORC_MUT(n) (ORC_MUTATION == (n))
+
 static int g_threads = 0; /* 0: min(slices, MaxThreadNum) like the reference */
 static int g_relax = 0;   /* 0: reference order; 1: expm1 form (error study only) */
 
@@ -161,19 +177,22 @@ static void orc_bloch_slice(real Gyro, const double *CS, int SpinNum,
                 b2 = b2 * -1;
             } else {                                                /* :73-121 */
                 real b5 = 0, b6 = 0;
+                int nact = 0;
                 for (int i = 0; i < TxCoilNum; i++) {
                     real A = (real)rfAmp[i], P = (real)rfPhase[i], F = (real)rfFreq[i];
+                    if (ORC_MUT(3) && A == 0) dW = dW + (real)((double)(F * 2) * ORC_PI);
                     if (A != 0) {
                         if (A < 0) { A = (real)fabs((double)A); P = (real)((double)P + ORC_PI); }
                         real m = (real)TxCoilmg[v + i * SpinMxSliceNum * SpinMxNum] * A;
                         real ph = (real)TxCoilpe[v + i * SpinMxSliceNum * SpinMxNum] + P;
                         real re = m * r_cos(ph), im = m * r_sin(ph);   /* PolarToCart */
                         b5 = b5 + re; b6 = b6 + im;
-                        dW = dW + (real)((double)(F * 2) * ORC_PI);
+                        if (!(ORC_MUT(2) && nact > 0)) dW = dW + (real)((double)(F * 2) * ORC_PI);
+                        nact++;
                     }
                 }
                 b1 = r_sqrt(b5 * b5 + b6 * b6);                     /* CartToPolar */
-                b2 = r_atan2(b6, b5);
+                b2 = ORC_MUT(1) ? r_atan2(b5, b6) : r_atan2(b6, b5);
                 b1 = b1 * Gyro;
                 b2 = b2 * -1;
             }
@@ -308,6 +327,7 @@ static double orc_gm_W(const orc_ctx *C, int64_t v, int t, double T2b, const dou
         W += (double)((real)ORC_PI * bg * bg * (real)gc->G);
         n++;
     }
+    if (ORC_MUT(4)) return W;
     return n ? W / n : 0.0;
 }
 
@@ -331,17 +351,20 @@ static int orc_rf_field(const orc_ctx *C, int64_t v, const double *rfAmp, const 
         *dW = *dW + (real)((double)(F * 2) * ORC_PI);
     } else {
         real b5 = 0, b6 = 0;
+        int nact = 0;
         for (int i = 0; i < Tx; i++) {
             real A = (real)rfAmp[i], P = (real)rfPhase[i], F = (real)rfFreq[i];
+            if (ORC_MUT(3) && A == 0) *dW = *dW + (real)((double)(F * 2) * ORC_PI);
             if (A == 0) continue;
             if (A < 0) { A = (real)fabs((double)A); P = (real)((double)P + ORC_PI); }
             real m = (real)C->vcoi->TxCoilmg[v + (int64_t)i * C->nvox] * A;
             real ph = (real)C->vcoi->TxCoilpe[v + (int64_t)i * C->nvox] + P;
             b5 = b5 + m * r_cos(ph); b6 = b6 + m * r_sin(ph);
-            *dW = *dW + (real)((double)(F * 2) * ORC_PI);
+            if (!(ORC_MUT(2) && nact > 0)) *dW = *dW + (real)((double)(F * 2) * ORC_PI);
+            nact++;
         }
         b1 = r_sqrt(b5 * b5 + b6 * b6);
-        b2 = r_atan2(b6, b5);
+        b2 = ORC_MUT(1) ? r_atan2(b5, b6) : r_atan2(b6, b5);
     }
     *b1o = b1 * Gyro;
     *b2o = b2 * -1;
@@ -496,6 +519,7 @@ static void orc_gm_split_voxel(orc_ctx *C, int64_t v, int s, real dt, real gz, r
             my = by * kout + ky;
         }
         mz = bz * kout + kz + m0;
+        if (ORC_MUT(5)) { if (flag == 0) { Bx[t] = mx; By[t] = my; } Bz[t] = mz; }
         if (C->wMx) {
             if (flag == 0) { C->wMx[si[t]] = mx; C->wMy[si[t]] = my; }
             C->wMz[si[t]] = mz;
@@ -606,7 +630,7 @@ static void orc_wc_split_voxel(orc_ctx *C, int64_t v, int s, real dt, real gz, r
         const real e1k = r_exp(-dt / T1 - dt * k21);
         const real m0 = (Rho * (1 - r_exp(-dt / T1))) / (real)S;
         const real ek = 1 - r_exp(-dt * k12);
-        oz[1] = bz * e1k + m0 + Bz[0] * ek;
+        oz[1] = bz * e1k + m0 + (ORC_MUT(6) ? oz[0] : Bz[0]) * ek;
     }
     for (int t = 0; t < Ty; t++) {
         if (C->wMx) {
@@ -751,7 +775,10 @@ static int orc_ext(orc_ctx *c, double ext)
     case 1: V->Kz = 0; V->Ky = 0; V->Kx = 0; return 0;              /* Plugin_ResetK.m:7-9 */
     case 2: V->Kz = -V->Kz; V->Ky = -V->Ky; V->Kx = -V->Kx; return 0; /* Plugin_ReverseK.m */
     case 3: V->KzKeeper = V->Kz; V->KyKeeper = V->Ky; V->KxKeeper = V->Kx; return 0; /* LockK */
-    case 4: V->Kz = V->KzKeeper; V->Ky = V->KyKeeper; V->Kx = V->KxKeeper; return 0; /* ReleaseK */
+    case 4:                                                       /* Plugin_ReleaseK.m */
+        V->Kz = V->KzKeeper; V->Ky = V->KyKeeper;
+        if (!ORC_MUT(7)) V->Kx = V->KxKeeper;
+        return 0;
     case 6:                                                       /* Plugin_IdealSpoiler.m:10-11 */
         for (int64_t i = 0; i < ntot; i++) {
             if (c->wMx) { c->wMx[i] = 0; c->wMy[i] = 0; }
