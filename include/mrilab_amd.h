@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define MRS_ABI_VERSION 1
+#define MRS_ABI_VERSION 2
 
 typedef enum mrs_status {
     MRS_OK = 0,
@@ -114,16 +114,37 @@ typedef struct mrs_vsig {
     double *Muts;             /* RunMode 1 only: [n_uts] */
 } mrs_vsig;
 
-/* ---- Ext callbacks (DoScanAtCPU.cpp:356-409, DoScanAtGPU.cu:565-659)
+/* ---- Ext callbacks (DoScanAtCPU.cpp:356-409, DoScanAtGPU.cu:479-662)
  * Ext 1-4 (k reset/reverse/lock/release), 6 (ideal spoiler) and 7 (rfRef)
  * are executed by the engine itself. Any other nonzero Ext code (5 timer,
  * 8 motion, 9 real-time recon, negative user codes) calls ext() if it is
- * set, and is a no-op otherwise. Before ext(), gpu_fetch() decides whether
- * the magnetization is copied to VObj.Mx/My/Mz on the host first and back
- * to the device afterwards (DoGPUFetch.m). Both return 0 on success. */
+ * set, and is a no-op otherwise; both return 0 on success.
+ *
+ * At the call (DoScanAtGPU.cu:483-560): every step before the Ext event has
+ * run, and VSig.Sx/Sy and VSig.Kx/Ky/Kz hold all samples acquired so far
+ * (added with +=, as the scan adds the later ones at its end); vvar is the
+ * decoder state at the event, and changes to it steer the rest of the scan.
+ * When the state is fetched (DoGPUFetch.m), VObj.Mx/My/Mz hold the current
+ * magnetization before ext() and, after it, everything the scan reads from
+ * the host is read again from the (possibly re-pointed) arrays of
+ * mrs_ext_state: VObj.Mx/My/Mz/Rho/T1/T2/K, VMag.dB0/dWRnd/G*grid and the
+ * Tx and Rx coil maps (DoScanAtGPU.cu:596-659, pointers re-fetched as
+ * :617-659 re-fetches them). Whether it is fetched: mrs_DoScanAtCPU always
+ * (the CPU engine's plugins share its arrays, DoScanAtCPU.cpp:366-384); the
+ * GPU entry points ask gpu_fetch() if it is set, else follow DoGPUFetch.m
+ * (codes 6 and 8 fetch). */
+typedef struct mrs_ext_state {
+    mrs_vobj *vobj;           /* the caller's VObj */
+    mrs_vmag *vmag;           /* the scan's VMag (a copy; its pointers may be re-pointed) */
+    mrs_vcoi *vcoi;           /* the scan's VCoi (likewise) */
+    mrs_vsig *vsig;           /* the caller's VSig */
+    int32_t fetched;          /* 1 when VObj.Mx/My/Mz hold the current state */
+    int32_t reserved;
+} mrs_ext_state;
+
 typedef struct mrs_callbacks {
     int (*gpu_fetch)(void *user, double ext, int *fetch);
-    int (*ext)(void *user, double ext, mrs_vvar *vvar, mrs_vobj *vobj);
+    int (*ext)(void *user, double ext, mrs_vvar *vvar, mrs_ext_state *state);
     void *user;
 } mrs_callbacks;
 

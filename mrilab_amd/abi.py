@@ -75,8 +75,14 @@ class VSigC(C.Structure):
                 ("Mx", AP), ("My", AP), ("Mz", AP), ("Muts", AP)]
 
 
+class ExtStateC(C.Structure):
+    """mrs_ext_state: what an Ext callback may read and re-point."""
+    _fields_ = [("vobj", C.POINTER(VObjC)), ("vmag", C.POINTER(VMagC)), ("vcoi", C.POINTER(VCoiC)),
+                ("vsig", C.POINTER(VSigC)), ("fetched", C.c_int32), ("reserved", C.c_int32)]
+
+
 GPU_FETCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_int))
-EXT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(VVarC), C.POINTER(VObjC))
+EXT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(VVarC), C.POINTER(ExtStateC))
 
 
 class CallbacksC(C.Structure):

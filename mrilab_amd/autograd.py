@@ -12,10 +12,17 @@ import torch
 
 
 class BlochExcitationLoss(torch.autograd.Function):
+    """The adjoint handle differentiates the forward pass it ran last. Each
+    forward records the handle's generation; a backward whose forward is no
+    longer the last one (two losses on one handle, or a forward call in
+    between) first runs its forward again on the saved inputs."""
+
     @staticmethod
     def forward(ctx, rf, T1, T2, dB0, adj):
         loss = adj.forward(rf, T1, T2, dB0)
         ctx.adj = adj
+        ctx.generation = adj.generation
+        ctx.save_for_backward(rf, T1, T2, dB0)
         ctx.shapes = (rf.shape, T1.shape, T2.shape, dB0.shape)
         ctx.device = rf.device
         return torch.tensor(loss, dtype=torch.float64, device=rf.device)
@@ -23,9 +30,12 @@ class BlochExcitationLoss(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         dev = ctx.device
+        adj = ctx.adj
+        if adj.generation != ctx.generation:
+            adj.forward(*ctx.saved_tensors)
         g = [torch.empty(s, dtype=torch.float32, device=dev) for s in ctx.shapes]
         torch.cuda.synchronize(dev)
-        ctx.adj.backward(out=g)
+        adj.backward(out=g)
         scale = grad_out.to(torch.float32)
         return g[0] * scale, g[1] * scale, g[2] * scale, g[3] * scale, None
 

@@ -153,6 +153,33 @@ void compress_runs(Decoded &d, int64_t max_adc_run) {
     }
 }
 
+const char *check_lines(const mrs_vseq &S, const mrs_vvar &V, int tx) {
+    double sum[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t i = 0; i < S.n_ts; i++)
+        for (int k = 0; k < 6; k++) {
+            const double f = S.flagsLine[i * 6 + k];
+            if (!(f >= 0) || f != std::floor(f) || f > 1e9) return "VSeq.flagsLine entries must be non-negative integers";
+            sum[k] += f;
+        }
+    const int64_t start[6] = {V.rfi, V.Gzi, V.Gyi, V.Gxi, V.ADCi, V.Exti};
+    const int64_t len[6] = {S.n_rf, S.n_Gz, S.n_Gy, S.n_Gx, S.n_ADC, S.n_Ext};
+    static const char *msg[6] = {
+        "VVar.rfi plus the rf flags of VSeq.flagsLine exceeds the length of VSeq.rfAmpLine",
+        "VVar.Gzi plus the Gz flags of VSeq.flagsLine exceeds the length of VSeq.GzAmpLine",
+        "VVar.Gyi plus the Gy flags of VSeq.flagsLine exceeds the length of VSeq.GyAmpLine",
+        "VVar.Gxi plus the Gx flags of VSeq.flagsLine exceeds the length of VSeq.GxAmpLine",
+        "VVar.ADCi plus the ADC flags of VSeq.flagsLine exceeds the length of VSeq.ADCLine",
+        "VVar.Exti plus the Ext flags of VSeq.flagsLine exceeds the length of VSeq.ExtLine"};
+    for (int k = 0; k < 6; k++)
+        if (start[k] < 0 || (double)start[k] + sum[k] > (double)len[k]) return msg[k];
+    for (int64_t j = 0; j < S.n_rf; j++) {
+        const double c = S.rfCoilLine[j];
+        if (!(c >= 1 && c <= tx) || c != std::floor(c)) return "VSeq.rfCoilLine names a coil outside 1..TxCoilNum";
+    }
+    if (V.utsi < 0 || V.utsi >= S.n_uts) return "VVar.utsi is outside VSeq.utsLine";
+    return nullptr;
+}
+
 void Decoder::flush_tail(Decoded &out) {
     // a trailing Ext 6 changes the final state only
     if (spoil_pending_) emit(out, false, false, false);

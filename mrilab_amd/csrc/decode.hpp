@@ -67,6 +67,15 @@ inline int64_t rec_adc_samples(uint32_t flags) {
     return (flags & REC_ADCRUN) ? 1 + (int64_t)(flags >> REC_REP_SHIFT) : 1;
 }
 
+// Checks that decoding VSeq from the state in vvar stays inside the lines:
+// every flagsLine entry a finite non-negative integer; per line, the start
+// index in VVar plus the flags that advance it within the line's length
+// (rf: rfi + the sum of the rf row <= n_rf; Gz/Gy/Gx/ADC/Ext likewise);
+// every rfCoilLine value an integer in 1..tx; 0 <= utsi < n_uts. The
+// reference reads past the arrays on such input (DoScanAtCPU.cpp:291-409).
+// Returns nullptr when the lines are consistent, else what is wrong.
+const char *check_lines(const mrs_vseq &seq, const mrs_vvar &vvar, int tx);
+
 // Stop reasons of Decoder::run.
 enum DecodeStop { DECODE_END = 0, DECODE_CALLBACK = 1 };
 

@@ -414,6 +414,7 @@ struct mrs_plan {
     int spt = 1, nrx = 1, block = 256, ngroups = 1, gx = 1, nblk = 1, nv = 2;
     int64_t max_adc = 0;
     int64_t signal_num = 0;
+    int64_t sig_from = 0;     // samples below it were already added into VSig (Ext callbacks)
     float gyro = 0.f;
     float cs[kMaxTypes] = {0};
     // device memory
@@ -520,6 +521,7 @@ mrs_status mrs::check_args(const mrs_vobj *vo, const mrs_vmag *vm, const mrs_vco
         return fail(MRS_ERR_ARG, "RunMode 1 needs VSig.Mx/My/Mz/Muts");
     if (vt->MultiTransmit && (vt->MasterTxCoil < 1 || vt->MasterTxCoil > vc->TxCoilNum))
         return fail(MRS_ERR_ARG, "VCtl.MasterTxCoil outside 1..TxCoilNum");
+    if (const char *bad = check_lines(*vs, *vv, vc->TxCoilNum)) return fail(MRS_ERR_ARG, bad);
     return MRS_OK;
 }
 
@@ -889,6 +891,61 @@ void mrs::copy_vvar_out(const mrs_vvar &src, const std::vector<double> *rf, int 
     }
 }
 
+// Adds the device signal of samples [from, to) of every type and coil into
+// VSig.Sx/Sy (index t*Rx*N + c*N + n).
+static mrs_status add_signal(mrs_plan *P, mrs_vsig *vsig, int64_t from, int64_t to) {
+    if (to <= from) return MRS_OK;
+    const int64_t N = P->signal_num, n = to - from;
+    std::vector<double> hx((size_t)n), hy((size_t)n);
+    for (int t = 0; t < P->Ty; t++)
+        for (int c = 0; c < P->Rx; c++) {
+            const int64_t off = (int64_t)t * P->Rx * N + (int64_t)c * N + from;
+            HIP_TRY(hipMemcpy(hx.data(), (const double *)P->Sx.p + off, n * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(hy.data(), (const double *)P->Sy.p + off, n * sizeof(double), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; i++) {
+                vsig->Sx[off + i] += hx[(size_t)i];
+                vsig->Sy[off + i] += hy[(size_t)i];
+            }
+        }
+    return MRS_OK;
+}
+
+// Uploads the current magnetization and every map the kernels read from the
+// host structs into the plan's device buffers, after an Ext callback that
+// had the state fetched (DoScanAtGPU.cu:633-659: VObj, VMag and VCoi
+// arrays, and VObj.K for the multi-pool engines, DoGMScanAtGPU.cu:714).
+static mrs_status refresh_maps(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm, const mrs_vcoi *vc) {
+    const size_t vb = (size_t)P->nvox * sizeof(float), sb = (size_t)P->ntot * sizeof(float);
+    struct {
+        DevBuf *d;
+        const void *h;
+        size_t n;
+        const char *name;
+    } L[] = {{&P->M[0], vo->Mx, sb, "VObj.Mx"},          {&P->M[1], vo->My, sb, "VObj.My"},
+             {&P->M[2], vo->Mz, sb, "VObj.Mz"},          {&P->dWRnd, vm->dWRnd, sb, "VMag.dWRnd"},
+             {&P->Rho, vo->Rho, vb * P->Ty, "VObj.Rho"}, {&P->T1, vo->T1, vb * P->Ty, "VObj.T1"},
+             {&P->T2, vo->T2, vb * P->Ty, "VObj.T2"},    {&P->dB0, vm->dB0, vb, "VMag.dB0"},
+             {&P->gxg, vm->Gxgrid, vb, "VMag.Gxgrid"},   {&P->gyg, vm->Gygrid, vb, "VMag.Gygrid"},
+             {&P->gzg, vm->Gzgrid, vb, "VMag.Gzgrid"},   {&P->txmg, vc->TxCoilmg, vb * P->Tx, "VCoi.TxCoilmg"},
+             {&P->txpe, vc->TxCoilpe, vb * P->Tx, "VCoi.TxCoilpe"}};
+    for (auto &l : L) {
+        if (!l.h) return fail(MRS_ERR_CALLBACK, std::string("the Ext callback left ") + l.name + " NULL");
+        HIP_TRY(hipMemcpy(l.d->p, l.h, l.n, hipMemcpyHostToDevice));
+    }
+    if (!P->rx_default) {
+        if (!vc->RxCoilx || !vc->RxCoily) return fail(MRS_ERR_CALLBACK, "the Ext callback left a VCoi.RxCoil map NULL");
+        HIP_TRY(hipMemcpy(P->rxx.p, vc->RxCoilx, vb * P->Rx, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P->rxy.p, vc->RxCoily, vb * P->Rx, hipMemcpyHostToDevice));
+    }
+    if (P->model >= 1) {
+        if (!vo->K) return fail(MRS_ERR_CALLBACK, "the Ext callback left VObj.K NULL");
+        std::vector<float> rates;
+        canonical_rates(vo->K, P->variant, P->Ty, P->nvox, rates);
+        HIP_TRY(hipMemcpy(P->K.p, rates.data(), rates.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return MRS_OK;
+}
+
 extern "C" {
 
 const char *mrs_last_error(void) { return g_err.c_str(); }
@@ -984,20 +1041,14 @@ mrs_status mrs_plan_fetch(mrs_plan *P, mrs_vobj *vobj, mrs_vvar *vvar, mrs_vsig 
             HIP_TRY(hipMemcpy(dst[k], P->M[k].p, P->ntot * sizeof(float), hipMemcpyDeviceToHost));
     }
     if (vsig) {
-        const size_t nsig = (size_t)P->signal_num * P->Ty * P->Rx;
-        std::vector<double> hx(nsig), hy(nsig);
-        if (nsig) {
-            HIP_TRY(hipMemcpy(hx.data(), P->Sx.p, nsig * sizeof(double), hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(hy.data(), P->Sy.p, nsig * sizeof(double), hipMemcpyDeviceToHost));
-        }
-        for (size_t i = 0; i < nsig; i++) {
-            vsig->Sx[i] += hx[i];
-            vsig->Sy[i] += hy[i];
-        }
+        mrs_status rc2 = add_signal(P, vsig, P->sig_from, P->signal_num);
+        if (rc2) return rc2;
+        // the decoded k-space holds the samples from sig_from on
+        const size_t k0 = (size_t)P->sig_from;
         for (size_t n = 0; n < P->dec.kx.size(); n++) {
-            vsig->Kz[n] += P->dec.kz[n];
-            vsig->Ky[n] += P->dec.ky[n];
-            vsig->Kx[n] += P->dec.kx[n];
+            vsig->Kz[k0 + n] += P->dec.kz[n];
+            vsig->Ky[k0 + n] += P->dec.ky[n];
+            vsig->Kx[k0 + n] += P->dec.kx[n];
         }
         if (P->run_mode == 1) {
             const size_t ncap = (size_t)P->ntot * P->n_uts;
@@ -1063,9 +1114,13 @@ void mrs_plan_destroy(mrs_plan *P) {
 
 // One call with callbacks: decode until an Ext code that needs the caller,
 // run the steps decoded so far, hand over to the callback, continue.
+// cpu_engine: the state is always fetched, as the CPU engine's plugins share
+// its arrays (DoScanAtCPU.cpp:360-405); otherwise gpu_fetch() decides, or
+// DoGPUFetch.m's table when it is not set.
 static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                                       const mrs_vctl *vctl, const mrs_vseq *vseq, mrs_vvar *vvar,
-                                      mrs_vsig *vsig, const mrs_callbacks *cb, const mrs_options *opt) {
+                                      mrs_vsig *vsig, const mrs_callbacks *cb, const mrs_options *opt,
+                                      bool cpu_engine) {
     mrs_plan *P = new (std::nothrow) mrs_plan();
     if (!P) return fail(MRS_ERR_NOMEM, "host allocation failed");
     struct Guard {
@@ -1075,9 +1130,13 @@ static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, cons
     mrs_status rc = plan_setup(P, vobj, vmag, vcoi, vctl, vseq, vsig, opt);
     if (rc) return rc;
     if ((rc = reset_state(P))) return rc;
+    // the callback may re-point VMag and VCoi arrays (MATLAB reallocates the
+    // globals it edits); later re-uploads read these copies
+    mrs_vmag vm_cur = *vmag;
+    mrs_vcoi vc_cur = *vcoi;
     Decoder dec(vobj, vcoi, vctl, vseq, vvar, true);
     Decoded all;  // accumulated k-space, muts and capture slots across chunks
-    int64_t adc_base = 0;
+    int64_t adc_base = 0, flushed = 0;
     for (;;) {
         P->dec = Decoded();
         P->dec.tx = P->Tx;
@@ -1112,28 +1171,46 @@ static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, cons
         all.cap_slot.insert(all.cap_slot.end(), P->dec.cap_slot.begin(), P->dec.cap_slot.end());
         all.muts = P->dec.muts;
         if (stop == DECODE_END) break;
-        // callback: DoGPUFetch.m decides whether the state travels to the host
+        // the samples so far go into VSig before the callback (DoScanAtGPU.cu:527-557)
+        if ((rc = add_signal(P, vsig, flushed, adc_base))) return rc;
+        for (int64_t n = flushed; n < adc_base; n++) {
+            vsig->Kz[n] += all.kz[(size_t)n];
+            vsig->Ky[n] += all.ky[(size_t)n];
+            vsig->Kx[n] += all.kx[(size_t)n];
+        }
+        flushed = adc_base;
         const double ext = dec.ext_code();
         int fetch = 0;
-        if (cb->gpu_fetch && cb->gpu_fetch(cb->user, ext, &fetch) != 0)
-            return fail(MRS_ERR_CALLBACK, "gpu_fetch callback failed");
+        if (cpu_engine) {
+            fetch = 1;
+        } else if (cb->gpu_fetch) {  // DoGPUFetch.m
+            if (cb->gpu_fetch(cb->user, ext, &fetch) != 0) return fail(MRS_ERR_CALLBACK, "gpu_fetch callback failed");
+        } else {
+            fetch = ext == 6 || ext == 8;  // DoGPUFetch.m:12-15
+        }
         if (fetch) {
             float *dst[3] = {vobj->Mx, vobj->My, vobj->Mz};
             for (int k = 0; k < 3; k++)
                 HIP_TRY(hipMemcpy(dst[k], P->M[k].p, P->ntot * sizeof(float), hipMemcpyDeviceToHost));
         }
-        if (cb->ext && cb->ext(cb->user, ext, &dec.vvar(), vobj) != 0)
+        mrs_ext_state es;
+        std::memset(&es, 0, sizeof es);
+        es.vobj = vobj;
+        es.vmag = &vm_cur;
+        es.vcoi = &vc_cur;
+        es.vsig = vsig;
+        es.fetched = fetch != 0;
+        dec.vvar().gpuFetch = fetch != 0;
+        if (cb->ext && cb->ext(cb->user, ext, &dec.vvar(), &es) != 0)
             return fail(MRS_ERR_CALLBACK, "Extended process encounters ERROR!");
-        if (fetch) {
-            const float *src[3] = {vobj->Mx, vobj->My, vobj->Mz};
-            for (int k = 0; k < 3; k++)
-                HIP_TRY(hipMemcpy(P->M[k].p, src[k], P->ntot * sizeof(float), hipMemcpyHostToDevice));
-        }
+        dec.vvar().gpuFetch = 0;  // DoScanAtGPU.cu:686-687
+        if (fetch && (rc = refresh_maps(P, vobj, &vm_cur, &vc_cur))) return rc;
     }
-    // outputs
-    P->dec.kx.swap(all.kx);
-    P->dec.ky.swap(all.ky);
-    P->dec.kz.swap(all.kz);
+    // outputs: the samples from `flushed` on
+    P->sig_from = flushed;
+    P->dec.kx.assign(all.kx.begin() + flushed, all.kx.end());
+    P->dec.ky.assign(all.ky.begin() + flushed, all.ky.end());
+    P->dec.kz.assign(all.kz.begin() + flushed, all.kz.end());
     P->dec.cap_slot.swap(all.cap_slot);
     P->dec.muts.swap(all.muts);
     P->vvar_final = dec.vvar();
@@ -1144,19 +1221,25 @@ static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, cons
     return mrs_plan_fetch(P, vobj, vvar, vsig);
 }
 
-mrs_status mrs_DoScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
-                           const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
-                           const mrs_options *opt) {
+static mrs_status scan_entry(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                             const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
+                             const mrs_options *opt, bool cpu_engine) {
     mrs_status rc = check_args(vobj, vmag, vcoi, vctl, vseq, vvar, vsig);
     if (rc) return rc;
     if (cb && (cb->ext || cb->gpu_fetch))
-        return scan_with_callbacks(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt);
+        return scan_with_callbacks(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt, cpu_engine);
     mrs_plan *P = nullptr;
     if ((rc = mrs_plan_create(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, opt, &P))) return rc;
     rc = mrs_plan_run(P);
     if (!rc) rc = mrs_plan_fetch(P, vobj, vvar, vsig);
     mrs_plan_destroy(P);
     return rc;
+}
+
+mrs_status mrs_DoScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
+                           const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
+                           const mrs_options *opt) {
+    return scan_entry(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt, false);
 }
 
 mrs_status mrs_DoGMScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
@@ -1200,7 +1283,7 @@ mrs_status mrs_DoMEScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vco
 mrs_status mrs_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi, const mrs_vctl *vctl,
                            const mrs_vseq *vseq, mrs_vvar *vvar, mrs_vsig *vsig, const mrs_callbacks *cb,
                            const mrs_options *opt) {
-    return mrs_DoScanAtGPU(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt);
+    return scan_entry(vobj, vmag, vcoi, vctl, vseq, vvar, vsig, cb, opt, true);
 }
 
 }  // extern "C"

@@ -170,30 +170,35 @@ def make_options(block_size=0, spins_per_thread=0, max_adc_per_launch=0, multipo
                         precision_mode=precision_mode, timing=timing, model=model, coil_sum=coil_sum)
 
 
-def _callbacks(ext: Optional[Callable], gpu_fetch: Optional[Callable]):
-    """Wraps Python callables as mrs_callbacks. ``ext(code, vvar_struct,
-    vobj_struct) -> None`` may raise to signal an error; ``gpu_fetch(code)
-    -> bool`` says whether the magnetization is synced to VObj first."""
+def make_callbacks(ext: Optional[Callable] = None, gpu_fetch: Optional[Callable] = None):
+    """Wraps Python callables as mrs_callbacks. ``ext(code, vvar, state)``
+    gets the VVarC struct (edits steer the rest of the scan) and the
+    ExtStateC struct (``state.fetched`` says whether VObj.Mx/My/Mz hold the
+    current magnetization; VSig holds the samples acquired so far); it may
+    edit the scan's numpy arrays in place, which the engine reads again when
+    the state was fetched, and may raise to signal an error. ``gpu_fetch(code)
+    -> bool`` decides the fetch on the GPU entry points (DoGPUFetch.m); without
+    it codes 6 and 8 fetch."""
     if ext is None and gpu_fetch is None:
         return None, ()
 
-    def _ext(user, code, vvar, vobj):
+    def _ext(user, code, vvar, state):
         try:
             if ext is not None:
-                ext(code, vvar.contents, vobj.contents)
+                ext(code, vvar.contents, state.contents)
             return 0
         except Exception:  # noqa: BLE001 - any error maps to MRS_ERR_CALLBACK
             return 1
 
     def _fetch(user, code, out):
         try:
-            out[0] = int(bool(gpu_fetch(code))) if gpu_fetch is not None else 0
+            out[0] = int(bool(gpu_fetch(code)))
             return 0
         except Exception:  # noqa: BLE001
             return 1
 
     f_ext = abi.EXT_FN(_ext)
-    f_fetch = abi.GPU_FETCH_FN(_fetch)
+    f_fetch = abi.GPU_FETCH_FN(_fetch) if gpu_fetch is not None else abi.GPU_FETCH_FN()
     cb = abi.CallbacksC(gpu_fetch=f_fetch, ext=f_ext, user=None)
     return cb, (f_ext, f_fetch)
 
@@ -211,7 +216,7 @@ def _scan(fn: str, VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, ext=None, gpu_fetch
     lib = load()
     kp = k_planes(model, int(VObj["TypeNum"])) if multipool else None
     P = abi.pack_scan(VObj, VMag, VCoi, VCtl, VSeq, VVar, VSig, multipool=multipool, k_planes=kp)
-    cb, keep = _callbacks(ext, gpu_fetch)
+    cb, keep = make_callbacks(ext, gpu_fetch)
     rc = getattr(lib, fn)(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi), C.byref(P.vctl),
                           C.byref(P.vseq), C.byref(P.vvar), C.byref(P.vsig),
                           C.byref(cb) if cb is not None else None,
@@ -441,6 +446,7 @@ class Adjoint:
                              grad=self._grad.ctypes.data_as(C.POINTER(C.c_float)),
                              target=self._target.ctypes.data_as(C.POINTER(C.c_float)))
         self._h = C.c_void_p()
+        self.generation = 0   # forward passes run; the backward differentiates the last
         P = self._packed
         _check(self._lib.mrs_adjoint_create(C.byref(P.vobj), C.byref(P.vmag), C.byref(P.vcoi), C.byref(ex),
                                             C.byref(options) if options is not None else None,
@@ -457,6 +463,7 @@ class Adjoint:
             torch.cuda.synchronize()
         loss = C.c_double()
         mxy = np.zeros((self.nvox, 2)) if want_mxy else None
+        self.generation += 1
         _check(self._lib.mrs_adjoint_forward(self._h, ptrs[0][0], ptrs[1][0], ptrs[2][0], ptrs[3][0],
                                              C.c_int32(dev.pop()), C.byref(loss),
                                              mxy.ctypes.data_as(C.POINTER(C.c_double)) if want_mxy else None))

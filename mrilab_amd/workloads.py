@@ -660,6 +660,101 @@ def as_variant(sc: Scan, variant: str, self_rate: float = 0.0) -> Scan:
     return out
 
 
+def multi_transmit(sc: Scan, scale, phase, freq=None, seed: int = 7, silent_after=None,
+                   master: int = 1) -> Scan:
+    """Copy of a single-Tx scan driven by len(scale) transmit coils in
+    parallel (VCtl.MultiTransmit 'on'): every rf sample becomes one sample
+    per coil at the same time point, coil c with amplitude scale[c] times the
+    original, phase + phase[c] and frequency + freq[c] (Hz), in the order a
+    PSD's rf elements of coils 1..Tx fire together (DoPulseGen.m:42-117,
+    rfCoilLine = coil). Coil c's B1+ map is a smooth random magnitude in
+    [0.7, 1.3] with a linear phase ramp. ``silent_after = (c, j)`` sets coil
+    c's amplitude to 0 from rf sample j on (a coil switched off)."""
+    out = sc.copy()
+    ntx = len(scale)
+    freq = np.zeros(ntx) if freq is None else np.asarray(freq, float)
+    q = out.VSeq
+    ts, fl = q["tsLine"], q["flagsLine"]
+    cols = np.nonzero(fl[0] == 1)[0]
+    n_rf = q["rfAmpLine"].size
+    if cols.size != n_rf:
+        raise ValueError("expected one rf flag per rf sample")
+    # each rf column becomes ntx columns at the same time point
+    rep = np.ones(ts.size, dtype=np.int64)
+    rep[cols] = ntx
+    q["tsLine"] = np.repeat(ts, rep)
+    q["flagsLine"] = np.asfortranarray(np.repeat(fl, rep, axis=1))
+    amp = np.empty((n_rf, ntx)); ph = np.empty((n_rf, ntx)); fr = np.empty((n_rf, ntx))
+    for c in range(ntx):
+        amp[:, c] = q["rfAmpLine"] * scale[c]
+        ph[:, c] = q["rfPhaseLine"] + phase[c]
+        fr[:, c] = q["rfFreqLine"] + freq[c]
+    # a PSD's elements zero their first and last samples (rf*.m): keep the
+    # phase and frequency of a zero-amplitude sample at 0 as they do
+    zero = q["rfAmpLine"] == 0
+    ph[zero] = 0.0
+    fr[zero] = 0.0
+    if silent_after is not None:
+        c, j = silent_after
+        amp[j:, c] = 0.0
+    q["rfAmpLine"] = amp.ravel()
+    q["rfPhaseLine"] = ph.ravel()
+    q["rfFreqLine"] = fr.ravel()
+    q["rfCoilLine"] = np.tile(np.arange(1, ntx + 1, dtype=np.float64), n_rf)
+    for k in ("tsLine", "rfAmpLine", "rfPhaseLine", "rfFreqLine", "rfCoilLine"):
+        q[k] = np.asfortranarray(q[k], dtype=np.float64)
+    shape = out.VObj["Mz"].shape[:3] if out.VObj["Mz"].ndim >= 3 else out.VObj["Mz"].shape[:2] + (1,)
+    R, C_, Z = (tuple(shape) + (1, 1, 1))[:3]
+    rng = np.random.default_rng(seed)
+    yy, xx, zz = np.meshgrid(np.linspace(-1, 1, R), np.linspace(-1, 1, C_), np.linspace(-1, 1, Z), indexing="ij")
+    mg = np.empty((R, C_, Z, ntx))
+    pe = np.empty((R, C_, Z, ntx))
+    for c in range(ntx):
+        a = rng.uniform(-1, 1, 4)
+        mg[..., c] = 1.0 + 0.3 * np.tanh(a[0] * xx + a[1] * yy + a[2] * zz + a[3])
+        b = rng.uniform(-1.5, 1.5, 3)
+        pe[..., c] = b[0] * xx + b[1] * yy + b[2] * zz
+    out.VCoi["TxCoilmg"] = np.asfortranarray(mg.astype(np.float32))
+    out.VCoi["TxCoilpe"] = np.asfortranarray(pe.astype(np.float32))
+    out.VCoi["TxCoilNum"] = np.int32(ntx)
+    out.VCoi["TxCoilDefault"] = np.float64(0.0)
+    out.VCtl["MultiTransmit"] = "on"
+    out.VCtl["MasterTxCoil"] = int(master)
+    for k in ("rfAmp", "rfPhase", "rfFreq"):
+        out.VVar[k] = np.zeros(ntx)
+    out.name = sc.name + f"_Tx{ntx}"
+    return out
+
+
+def two_species(sc: Scan, cs_hz: float = -440.0, seed: int = 11) -> Scan:
+    """Copy of a single-type scan with a second species (TypeNum 2, e.g.
+    fat at cs_hz from water, VCtl.CS = [0, cs_hz], as DoPreScan.m:120 sets it
+    from VObj.ChemShift): its own Rho ~ U[0, 0.5], T1 ~ U[0.2, 0.4] s, T2 ~
+    U[0.04, 0.1] s, the same intravoxel spread, signal per type."""
+    out = sc.copy()
+    rng = np.random.default_rng(seed)
+    o = out.VObj
+    shp = o["Rho"].shape[:3] if o["Rho"].ndim >= 3 else o["Rho"].shape[:2] + (1,)
+    S = int(o["SpinNum"])
+    rho2 = rng.uniform(0.0, 0.5, shp).astype(np.float32)
+    t1 = rng.uniform(0.2, 0.4, shp).astype(np.float32)
+    t2 = rng.uniform(0.04, 0.1, shp).astype(np.float32)
+    for k, extra in (("Rho", rho2), ("T1", t1), ("T2", t2)):
+        o[k] = np.asfortranarray(np.stack([o[k].reshape(shp), extra], axis=3))
+    mz2 = np.repeat((rho2 / np.float32(S))[..., None], S, axis=3)
+    zero = np.zeros_like(mz2)
+    for k, extra in (("Mx", zero), ("My", zero), ("Mz", mz2)):
+        o[k] = np.asfortranarray(np.stack([o[k].reshape(shp + (S,)), extra], axis=4).astype(np.float32))
+    d = out.VMag["dWRnd"].reshape(shp + (S,))
+    out.VMag["dWRnd"] = np.asfortranarray(np.stack([d, d], axis=4).astype(np.float32))
+    o["TypeNum"] = np.int32(2)
+    out.VCtl["CS"] = np.array([float(np.asarray(out.VCtl["CS"]).ravel()[0]), float(cs_hz)])
+    n = int(out.VSig["SignalNum"]) * 2 * int(out.VCoi["RxCoilNum"])
+    out.VSig["Sx"], out.VSig["Sy"] = np.zeros(n), np.zeros(n)
+    out.name = sc.name + "_Ty2"
+    return out
+
+
 def sar_scan(base: Scan | None = None, n_tx: int = 2, n_samples: int = 8, seed: int = 5,
              body: bool = False) -> Scan:
     """A scan with the SAR engine's fields (DoCalSARAtCPU.cpp:115-166) on

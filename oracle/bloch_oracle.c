@@ -253,7 +253,7 @@ static double orc_sum(const real *x, int64_t n)
 }
 
 typedef struct orc_ctx {
-    mrs_vobj *vobj; const mrs_vmag *vmag; const mrs_vcoi *vcoi; const mrs_vctl *vctl;
+    mrs_vobj *vobj; mrs_vmag *vmag; mrs_vcoi *vcoi; const mrs_vctl *vctl;
     const mrs_vseq *vseq; mrs_vvar *vvar; mrs_vsig *vsig; const mrs_callbacks *cb;
     int64_t nvox;   /* R*C*Z */
     int64_t nxy;    /* R*C */
@@ -796,7 +796,15 @@ static int orc_ext(orc_ctx *c, double ext)
                     c->vobj->Mx[i] = (float)c->wMx[i]; c->vobj->My[i] = (float)c->wMy[i];
                     c->vobj->Mz[i] = (float)c->wMz[i];
                 }
-            int rc = c->cb->ext(c->cb->user, ext, V, c->vobj);
+            /* the CPU engine's plugins share its arrays: VObj holds the
+             * current state, VSig the samples so far, and the arrays are
+             * read again through the (possibly re-pointed) structs after
+             * the call (DoScanAtCPU.cpp:360-405) */
+            mrs_ext_state st;
+            memset(&st, 0, sizeof st);
+            st.vobj = c->vobj; st.vmag = c->vmag; st.vcoi = c->vcoi; st.vsig = c->vsig;
+            st.fetched = 1;
+            int rc = c->cb->ext(c->cb->user, ext, V, &st);
             if (c->wMx)
                 for (int64_t i = 0; i < ntot; i++) {
                     c->wMx[i] = c->vobj->Mx[i]; c->wMy[i] = c->vobj->My[i]; c->wMz[i] = c->vobj->Mz[i];
@@ -830,7 +838,11 @@ static int orc_scan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
     orc_ctx C;
     memset(&C, 0, sizeof C);
     C.variant = variant;
-    C.vobj = vobj; C.vmag = vmag; C.vcoi = vcoi; C.vctl = vctl; C.vseq = vseq;
+    /* Ext callbacks may re-point VMag and VCoi arrays: the scan reads them
+     * through these copies */
+    mrs_vmag vmag_cur = *vmag;
+    mrs_vcoi vcoi_cur = *vcoi;
+    C.vobj = vobj; C.vmag = &vmag_cur; C.vcoi = &vcoi_cur; C.vctl = vctl; C.vseq = vseq;
     C.vvar = vvar; C.vsig = vsig; C.cb = cb;
     const int64_t SpinMxNum = (int64_t)vobj->dims[0] * vobj->dims[1];
     const int64_t SpinMxSliceNum = vobj->dims[2] > 0 ? vobj->dims[2] : 1;
@@ -947,7 +959,7 @@ static int orc_scan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                                                                             : orc_pool_flag(&C, Typei) == 0));
                 if (sample) {                                        /* :445-491 */
                     for (int c = 0; c < Rx; c++) {
-                        const float *Cx = vcoi->RxCoilx + (int64_t)c * C.nvox, *Cy = vcoi->RxCoily + (int64_t)c * C.nvox;
+                        const float *Cx = C.vcoi->RxCoilx + (int64_t)c * C.nvox, *Cy = C.vcoi->RxCoily + (int64_t)c * C.nvox;
                         const real cr = (real)cos(-vvar->rfRef), msr = (real)-sin(-vvar->rfRef);
                         const real sr = (real)sin(-vvar->rfRef);
                         for (int64_t v = 0; v < C.nvox; v++) {
@@ -986,9 +998,9 @@ static int orc_scan(mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
                     orc_bloch_slice(Gyro, vctl->CS, S,
                                     vobj->Rho + po, vobj->T1 + po, vobj->T2 + po,
                                     vobj->Mz + base + so, vobj->My + base + so, vobj->Mx + base + so,
-                                    vmag->dB0 + so, vmag->dWRnd + base + so,
-                                    vmag->Gzgrid + so, vmag->Gygrid + so, vmag->Gxgrid + so,
-                                    vcoi->TxCoilmg + so, vcoi->TxCoilpe + so,
+                                    C.vmag->dB0 + so, C.vmag->dWRnd + base + so,
+                                    C.vmag->Gzgrid + so, C.vmag->Gygrid + so, C.vmag->Gxgrid + so,
+                                    C.vcoi->TxCoilmg + so, C.vcoi->TxCoilpe + so,
                                     rdt, vvar->rfAmp, vvar->rfPhase, vvar->rfFreq, gz, gy, gx,
                                     Typei, SpinMxNum, SpinMxSliceNum, Tx,
                                     C.wMx ? C.wMx + base + so : NULL,

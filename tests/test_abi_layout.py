@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PAIRS = [
     ("mrs_vobj", abi.VObjC), ("mrs_vmag", abi.VMagC), ("mrs_vcoi", abi.VCoiC), ("mrs_vctl", abi.VCtlC),
     ("mrs_vseq", abi.VSeqC), ("mrs_vvar", abi.VVarC), ("mrs_vsig", abi.VSigC),
-    ("mrs_callbacks", abi.CallbacksC), ("mrs_options", abi.OptionsC), ("mrs_plan_info", abi.PlanInfoC),
+    ("mrs_callbacks", abi.CallbacksC), ("mrs_ext_state", abi.ExtStateC), ("mrs_options", abi.OptionsC), ("mrs_plan_info", abi.PlanInfoC),
     ("mrs_batch_info", abi.BatchInfoC), ("mrs_excitation", abi.ExcitationC),
     ("mrs_adjoint_info", abi.AdjointInfoC), ("mrs_recon_desc", abi.ReconDescC), ("mrs_vimg", abi.VImgC),
     ("mrs_sar_obj", abi.SarObjC), ("mrs_sar_sig", abi.SarSigC), ("mrs_seq_param", abi.SeqParamC),

@@ -131,3 +131,37 @@ def test_batch_entry_fails_loudly_without_gpu():
     scans = workloads.fingerprint_batch(n_sets=2, tr_min=5, tr_max=8)
     with pytest.raises(engine.MRSError):
         engine.DoScanBatchAtGPU([s.structs for s in scans])
+
+
+@pytest.mark.parametrize("edit,match", [
+    (lambda sc: sc.VSeq["rfCoilLine"].__setitem__(3, 2.0), "rfCoilLine"),
+    (lambda sc: sc.VSeq["rfCoilLine"].__setitem__(3, 0.0), "rfCoilLine"),
+    (lambda sc: sc.VSeq["flagsLine"].__setitem__((0, 5), 0.5), "non-negative integers"),
+    (lambda sc: sc.VSeq["flagsLine"].__setitem__((4, 7), -1.0), "non-negative integers"),
+    (lambda sc: sc.VVar.__setitem__("rfi", np.int32(3)), "VVar.rfi"),
+    (lambda sc: sc.VVar.__setitem__("ADCi", np.int32(-1)), "VVar.ADCi"),
+    (lambda sc: sc.VVar.__setitem__("utsi", np.int32(10 ** 6)), "VVar.utsi"),
+])
+def test_scan_rejects_lines_it_would_read_past(edit, match):
+    """Sequence lines that would make the decoder index past an array (a Tx
+    coil outside 1..TxCoilNum, fractional or negative flags, start indices
+    past the line) are MRS_ERR_ARG before any device work; the reference
+    reads out of bounds there (DoScanAtCPU.cpp:291-409)."""
+    if not os.path.exists(engine.LIB_PATH):
+        pytest.skip("library not built")
+    sc = workloads.gre2d(n_pe=2, n_ro=8)
+    edit(sc)
+    with pytest.raises(engine.MRSError, match=match) as e:
+        engine.DoScanAtGPU(*sc.structs)
+    assert e.value.status == 1
+
+
+def test_scan_rejects_more_rf_flags_than_samples():
+    if not os.path.exists(engine.LIB_PATH):
+        pytest.skip("library not built")
+    sc = workloads.gre2d(n_pe=2, n_ro=8)
+    n = sc.VSeq["rfAmpLine"].size
+    for k in ("rfAmpLine", "rfPhaseLine", "rfFreqLine", "rfCoilLine"):
+        sc.VSeq[k] = np.asfortranarray(sc.VSeq[k][: n - 1])
+    with pytest.raises(engine.MRSError, match="VVar.rfi plus the rf flags"):
+        engine.DoScanAtGPU(*sc.structs)
