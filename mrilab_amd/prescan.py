@@ -36,6 +36,12 @@ def _value(v: str):
     try:
         return float(v)
     except ValueError:
+        pass
+    # a list of numbers, as the panel's str2num reads "5e-3 5e-3" (ME_TEs)
+    parts = v.replace(",", " ").split()
+    try:
+        return [float(x) for x in parts] if len(parts) > 1 else v
+    except ValueError:
         return v
 
 

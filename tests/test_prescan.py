@@ -41,3 +41,10 @@ def test_radial_and_spiral_square_pixels():
     s = prescan.vctl_from_files(ATTRS, '<Spiral S_Gradient="500e-3" S_Lamda="2.1" S_ShotNum="8" '
                                        'S_SlewRate="150" S_SlewRate0="100"/>', SliceNum=1.0)
     assert s["FirstPhNum"] == 8 and s["SecondPhNum"] == 1 and s["FOVSlice"] == 6e-3
+
+
+def test_number_list_attribute():
+    """A space-separated list (MultiEcho.xml's ME_TEs) is read as numbers,
+    as the panel's str2num reads it."""
+    a = prescan.read_attrs('<?xml version="1.0"?><MultiEcho ME_TEs="5e-3 7.5e-3"/>')
+    assert a["ME_TEs"] == [5e-3, 7.5e-3]
