@@ -1,0 +1,11 @@
+// DoScanAtGPU: MRiLab's MEX entry point (Lib/src/engine/GPUEngine/DoScanAtGPU.cu:76), on the engine's C ABI
+// (mrs_DoScanAtGPU). Exports only mexFunction; MATLAB calls it with no arguments.
+#include "mrs_mex_gateway.hpp"
+
+void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
+    (void)nlhs;
+    (void)plhs;
+    (void)nrhs;
+    (void)prhs;
+    mrs_mex::run(mrs_DoScanAtGPU, true, 0);
+}
