@@ -194,7 +194,7 @@ mrs_status mrs_DoScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
                            const mrs_options *opt);
 
 /* Generalized multi-pool scan (free and bound pools with exchange, MT
- * saturation), TypeNum 2..4. VObj.K is [R,C,Z,TypeNum,TypeNum] with
+ * saturation), TypeNum 2..6 (5 and 6 pools with up to 4 receive coils). VObj.K is [R,C,Z,TypeNum,TypeNum] with
  * K(:,:,:,a,b) the rate from pool a into pool b; VObj.TypeFlag is 0 for a
  * free pool and 1 for a bound pool. mrs_options.multipool_scheme picks the
  * reference's operator split (0) or the fp64 matrix exponential (1).
@@ -217,7 +217,7 @@ mrs_status mrs_DoMTScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
                              const mrs_options *opt);
 
 /* CEST: pool 0 water, 1 MT bound pool, 2..TypeNum-1 CEST pools (TypeNum
- * 3..4). VObj.K is [R,C,Z,2*(TypeNum-1)]: plane 1 water -> MT, plane TypeNum
+ * 3..6). VObj.K is [R,C,Z,2*(TypeNum-1)]: plane 1 water -> MT, plane TypeNum
  * MT -> water, plane t water -> CEST pool t, plane t+TypeNum-1 CEST pool t
  * -> water (1-based planes); signal is water only.
  * Replaces: mexFunction, Lib/src/engine/GPUEngine/DoCESTScanAtGPU.cu:76 */
@@ -272,6 +272,34 @@ typedef struct mrs_plan_info {
 } mrs_plan_info;
 mrs_status mrs_plan_get_info(const mrs_plan *plan, mrs_plan_info *info);
 void mrs_plan_destroy(mrs_plan *plan);
+
+/* ---------------------------------------------------------------------------
+ * Batched multi-pool sequences: one phantom under n_seq independent
+ * sequences, e.g. the saturation offsets of a Z-spectrum, each simulated
+ * from VObj's magnetization. No reference counterpart: MRiLab runs such
+ * offsets as successive TRs of one scan (DoGMScanAtGPU.cu:76), the state of
+ * one carrying into the next. Every (voxel, sequence) pair is a thread of
+ * one launch grid (launch groups of whole sequences when their ADC samples
+ * exceed one launch's partial buffer), so the device is filled even when the
+ * phantom alone is one wave per SIMD. options.model picks the engine (1 GM,
+ * 2 MT, 3 CEST, 4 ME, VObj.K in that engine's layout) and
+ * options.multipool_scheme the scheme; RunMode 0, no Ext callbacks. Each
+ * sequence's outputs are what the single-scan entry point writes for that
+ * sequence alone. Run, sync, info and destroy with mrs_plan_*.
+ * ------------------------------------------------------------------------- */
+mrs_status mrs_multipool_batch_create(int32_t n_seq, const mrs_vobj *vobj, const mrs_vmag *vmag,
+                                      const mrs_vcoi *vcoi, const mrs_vctl *vctl, const mrs_vseq *const *vseq,
+                                      const mrs_vvar *const *vvar, const mrs_vsig *const *vsig,
+                                      const mrs_options *options, mrs_plan **out);
+/* Per sequence q: final magnetization into vobj[q]->Mx/My/Mz, signal and
+ * k-space added into vsig[q], decoder state into vvar[q] (any array, or
+ * entry, may be NULL). */
+mrs_status mrs_multipool_batch_fetch(mrs_plan *plan, mrs_vobj *const *vobj, mrs_vvar *const *vvar,
+                                     mrs_vsig *const *vsig);
+/* create + run + fetch (final states into vobj_out) + destroy. */
+mrs_status mrs_DoMultiPoolBatchAtGPU(int32_t n_seq, const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
+                                     const mrs_vctl *vctl, const mrs_vseq *const *vseq, mrs_vvar *const *vvar,
+                                     mrs_vsig *const *vsig, mrs_vobj *const *vobj_out, const mrs_options *options);
 
 /* ---------------------------------------------------------------------------
  * Batched scans. No reference counterpart: MRiLab's batch runner

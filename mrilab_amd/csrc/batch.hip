@@ -55,6 +55,7 @@ struct BatchScan {
     int32_t n_tiles;
     float cs, gyro;
     int32_t pad_;
+    int64_t relax_off;  // single-voxel scan: first record's relaxation factors, else -1
 };
 
 struct BatchParams {
@@ -66,6 +67,7 @@ struct BatchParams {
     float *partial;                // [part][NV]
     const BatchScan *scans;
     const int32_t *tile_scan, *tile_idx;  // per work item, longest scan first
+    const float *relax;                   // [record][2] of single-voxel scans (relax_off)
     int32_t n_tiles;
 };
 
@@ -138,6 +140,9 @@ __global__ __launch_bounds__(256) void bloch_batch_kernel(BatchParams p, const u
         const TxMaps txm{p.txmg, p.txpe, vg, nvox, 1};
         const crec_t rec = (crec_t)rec_g + d.rec_off;
         const int n_rec = d.n_rec;
+        // a single-voxel scan (one T1, T2 for the wave) reads its relaxation
+        // factors per record instead of evaluating exp per lane
+        const float *rlx = d.relax_off >= 0 ? p.relax + 2 * d.relax_off : nullptr;
         const int64_t part0 = d.part_off + ti;
         const int ntile = d.n_tiles;
         int64_t adc = 0;
@@ -167,7 +172,7 @@ __global__ __launch_bounds__(256) void bloch_batch_kernel(BatchParams p, const u
                 if ((lane & (G - 1)) == 0) p.partial[(part0 + adc * ntile) * NV + lane / G] = val[0];
                 adc++;
             }
-            if (!(flags & REC_NOSTEP)) bloch_step<SPT, false, PREC>(st, k, q, flags, txm);
+            if (!(flags & REC_NOSTEP)) bloch_step<SPT, false, PREC>(st, k, q, flags, txm, rlx ? rlx + 2 * r : nullptr);
         }
         if (valid) {
 #pragma unroll
@@ -306,7 +311,7 @@ struct mrs_batch {
     std::vector<int64_t> steps;
     int64_t total_iso = 0, total_adc = 0, total_iso_steps = 0, n_tiles = 0;
     BDev M0[3], M[3], dWRnd, Rho, T1, T2, dB0, gx, gy, gz, txmg, txpe, rxx, rxy;
-    BDev rec, scans, tile_scan, tile_idx, partial, adc_ref, samp_scan, Sx, Sy;
+    BDev rec, scans, tile_scan, tile_idx, partial, adc_ref, samp_scan, Sx, Sy, relax;
     size_t partial_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -382,6 +387,7 @@ static BatchParams batch_params(mrs_batch *B) {
     p.scans = (const BatchScan *)B->scans.p;
     p.tile_scan = (const int32_t *)B->tile_scan.p;
     p.tile_idx = (const int32_t *)B->tile_idx.p;
+    p.relax = (const float *)B->relax.p;
     p.n_tiles = (int32_t)B->n_tiles;
     return p;
 }
@@ -606,6 +612,38 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
         std::vector<uint32_t> words((size_t)std::max<int64_t>(recw, 2));
         std::vector<double> ref((size_t)std::max<int64_t>(adc, 1));
         std::vector<int32_t> ss((size_t)std::max<int64_t>(adc, 1));
+        // relaxation factors of single-voxel scans per record (the factors
+        // depend only on dt and the voxel's T1, T2), as bloch_step rounds them
+        std::vector<float> rlx;
+        for (int i = 0; i < n; i++) {
+            BatchScan &d = B->hscan[i];
+            d.relax_off = -1;
+            if (d.nvox != 1) continue;
+            d.relax_off = (int64_t)rlx.size() / 2;
+            const float t1 = vobj[i]->T1[0], t2 = vobj[i]->T2[0];
+            const std::vector<uint32_t> &w = B->dec[i].rec;
+            for (int64_t r = 0; r < B->dec[i].n_rec(); r++) {
+                float dt;
+                std::memcpy(&dt, &w[(size_t)r * 8], 4);
+                float f2, f1;
+                if (B->prec == 0) {  // BlochKernel.h:348-368 in fp32, exp rounded from fp64
+                    float a2 = (1.0f / t2) * dt;
+                    a2 = -a2;
+                    if (a2 < -200.f) a2 = -200.f;
+                    float a1 = (1.0f / t1) * dt;
+                    a1 = -a1;
+                    if (a1 < -200.f) a1 = -200.f;
+                    f2 = (float)std::exp((double)a2);
+                    f1 = (float)std::exp((double)a1);
+                } else {
+                    f2 = (float)(-std::expm1(-(double)dt / (double)t2));
+                    f1 = (float)(-std::expm1(-(double)dt / (double)t1));
+                }
+                rlx.push_back(f2);
+                rlx.push_back(f1);
+            }
+        }
+        if ((rc = bupload(B->relax, rlx, st))) return rc;
         for (int i = 0; i < n; i++) {
             const BatchScan &d = B->hscan[i];
             std::copy(B->dec[i].rec.begin(), B->dec[i].rec.end(), words.begin() + d.rec_off);

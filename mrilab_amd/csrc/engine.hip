@@ -437,6 +437,17 @@ struct mrs_plan {
     int flag[kGmMaxTypes] = {0};
     std::vector<hipEvent_t> tev;  // per launch: fwd begin, fwd end, reduce end
     double fwd_ms = 0.0, red_ms = 0.0;
+    // batched sequences (mrs_multipool_batch_*): one phantom, nseq record
+    // streams, each from the initial state; 0 = one scan
+    int nseq = 0;
+    DevBuf seq_tab, vox_perm, Mout[3];
+    std::vector<int32_t> seq_r0, seq_r1, seq_slot;    // records; first partial slot in its group
+    std::vector<int64_t> seq_nadc, seq_adc_off;       // samples; offset in the adc_ref table
+    std::vector<int64_t> seq_signum, seq_sig_off;     // VSig.SignalNum; offset in Sx/Sy
+    std::vector<int> grp_begin, grp_end;              // launch groups of whole sequences
+    std::vector<std::vector<double>> seq_k[3];        // k-space per sequence (x, y, z)
+    std::vector<mrs_vvar> seq_vvar;
+    std::vector<std::vector<double>> seq_rf[3];
     ~mrs_plan() {
         for (auto e : tev) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -586,7 +597,7 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
         P->variant = P->model - 1;
         if (!vo->K) return fail(MRS_ERR_ARG, "multi-pool scan needs VObj.K");
         if (vo->TypeNum < 2 || vo->TypeNum > kGmMaxTypes)
-            return fail(MRS_ERR_UNSUPPORTED, "multi-pool engine supports TypeNum 2..4");
+            return fail(MRS_ERR_UNSUPPORTED, "multi-pool engine supports TypeNum 2.." + std::to_string(kGmMaxTypes));
         if (P->variant == kVarGM && !vo->TypeFlag) return fail(MRS_ERR_ARG, "the GM engine needs VObj.TypeFlag");
         if (P->variant == kVarMT && vo->TypeNum != 2)
             return fail(MRS_ERR_ARG, "the MT engine takes two pools (free water, bound)");
@@ -634,7 +645,7 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     P->nrx = nrx;
     P->nv = P->model >= 1 ? gm_values_per_isochromat(P->Ty, nrx) : 2 * nrx;
     if (P->model >= 1 && (nrx > 8 || !pick_gm(P->Ty, nrx, P->scheme, P->variant)))
-        return fail(MRS_ERR_UNSUPPORTED, "multi-pool engine supports up to 8 receive coils");
+        return fail(MRS_ERR_UNSUPPORTED, "multi-pool engine supports up to 8 receive coils (4 with 5 or 6 pools)");
     const size_t budget = (size_t)512 << 20;  // partial-buffer budget
     int64_t max_adc = (int64_t)(budget / ((size_t)P->nblk * P->nv * sizeof(float)));
     max_adc = std::max<int64_t>(kRedBatch, std::min<int64_t>(max_adc, 65536));
@@ -733,8 +744,8 @@ static mrs_status upload_program(mrs_plan *P) {
     return MRS_OK;
 }
 
-// Multi-pool launches: one thread per (voxel, isochromat), all pools.
-static mrs_status execute_gm(mrs_plan *P) {
+// Kernel parameters of the plan's multi-pool launches.
+static GmParams gm_params(mrs_plan *P) {
     GmParams gp;
     std::memset(&gp, 0, sizeof gp);
     gp.Mx = (float *)P->M[0].p;
@@ -768,6 +779,12 @@ static mrs_status execute_gm(mrs_plan *P) {
     }
     gp.variant = P->variant;
     gp.self_x = P->variant == kVarME;
+    return gp;
+}
+
+// Multi-pool launches: one thread per (voxel, isochromat), all pools.
+static mrs_status execute_gm(mrs_plan *P) {
+    GmParams gp = gm_params(P);
     gm_launch_fn fn = pick_gm(P->Ty, P->nrx, P->scheme, P->variant);
     const dim3 grid(P->gx, P->ngroups), block(P->block);
     const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
@@ -891,6 +908,58 @@ void mrs::copy_vvar_out(const mrs_vvar &src, const std::vector<double> *rf, int 
     }
 }
 
+// One run of a batched multi-pool plan: per launch group one launch over
+// (voxel tile, sequence) blocks, then the signal reduction of each sequence.
+static mrs_status run_batch(mrs_plan *P) {
+    hipStream_t st = P->stream;
+    const size_t nsig = (size_t)std::max<int64_t>(1, P->seq_sig_off.back());
+    HIP_TRY(hipMemsetAsync(P->Sx.p, 0, nsig * sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(P->Sy.p, 0, nsig * sizeof(double), st));
+    HIP_TRY(hipEventRecord(P->ev0, st));
+    GmParams gp = gm_params(P);
+    for (int k = 0; k < 3; k++) {
+        (&gp.Mx)[k] = (float *)P->M0[k].p;  // every sequence starts from the initial state
+        gp.Mout[k] = (float *)P->Mout[k].p;
+    }
+    gp.seq_tab = (const int32_t *)P->seq_tab.p;
+    gp.vox_perm = (const int32_t *)P->vox_perm.p;
+    gp.tiles = P->gx;
+    gm_launch_fn fn = pick_gm(P->Ty, P->nrx, P->scheme, P->variant);
+    const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
+    const size_t ng = P->grp_begin.size();
+    if (P->timing) {
+        while (P->tev.size() < 3 * ng) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            P->tev.push_back(e);
+        }
+    }
+    const int nblk = P->gx * P->ngroups;
+    for (size_t g = 0; g < ng; g++) {
+        gp.seq_base = P->grp_begin[g];
+        gp.nseq_l = P->grp_end[g] - P->grp_begin[g];
+        const dim3 grid((unsigned)(P->gx * gp.nseq_l), P->ngroups), block(P->block);
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * g], st));
+        fn(grid, block, lds, st, gp, (const uint32_t *)P->rec.p, 0, 0);
+        HIP_TRY(hipGetLastError());
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * g + 1], st));
+        for (int q = P->grp_begin[g]; q < P->grp_end[g]; q++) {
+            const int64_t na = P->seq_nadc[q];
+            if (na <= 0) continue;
+            launch_signal_reduce(2 * P->nrx, dim3((unsigned)na, P->Ty), st,
+                                 (const float *)P->partial.p + (int64_t)P->seq_slot[q] * nblk * P->nv, nblk, 0, nblk,
+                                 P->nv, 2 * P->nrx, P->Rx, P->rx_default,
+                                 (const double *)P->adc_ref.p + P->seq_adc_off[q], 0, P->seq_signum[q],
+                                 (double *)P->Sx.p + P->seq_sig_off[q], (double *)P->Sy.p + P->seq_sig_off[q]);
+            HIP_TRY(hipGetLastError());
+        }
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * g + 2], st));
+    }
+    HIP_TRY(hipEventRecord(P->ev1, st));
+    P->ran = true;
+    return MRS_OK;
+}
+
 // Adds the device signal of samples [from, to) of every type and coil into
 // VSig.Sx/Sy (index t*Rx*N + c*N + n).
 static mrs_status add_signal(mrs_plan *P, mrs_vsig *vsig, int64_t from, int64_t to) {
@@ -999,6 +1068,7 @@ mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs
 mrs_status mrs_plan_run(mrs_plan *P) {
     if (!P) return fail(MRS_ERR_ARG, "plan is NULL");
     HIP_TRY(hipSetDevice(P->device));
+    if (P->nseq) return run_batch(P);
     mrs_status rc = reset_state(P);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(P->ev0, P->stream));
@@ -1017,7 +1087,8 @@ mrs_status mrs_plan_sync(mrs_plan *P) {
         if (hipEventElapsedTime(&ms, P->ev0, P->ev1) == hipSuccess) P->last_ms = ms;
         if (P->timing) {
             double f = 0.0, r = 0.0;
-            for (size_t l = 0; l < P->launch_r0.size() && 3 * l + 2 < P->tev.size(); l++) {
+            const size_t nl = P->nseq ? P->grp_begin.size() : P->launch_r0.size();
+            for (size_t l = 0; l < nl && 3 * l + 2 < P->tev.size(); l++) {
                 float a = 0.f, b = 0.f;
                 if (hipEventElapsedTime(&a, P->tev[3 * l], P->tev[3 * l + 1]) == hipSuccess) f += a;
                 if (hipEventElapsedTime(&b, P->tev[3 * l + 1], P->tev[3 * l + 2]) == hipSuccess) r += b;
@@ -1033,6 +1104,7 @@ mrs_status mrs_plan_sync(mrs_plan *P) {
 
 mrs_status mrs_plan_fetch(mrs_plan *P, mrs_vobj *vobj, mrs_vvar *vvar, mrs_vsig *vsig) {
     if (!P) return fail(MRS_ERR_ARG, "plan is NULL");
+    if (P->nseq) return fail(MRS_ERR_ARG, "a batched multi-pool plan is fetched with mrs_multipool_batch_fetch");
     mrs_status rc = mrs_plan_sync(P);
     if (rc) return rc;
     if (vobj) {
@@ -1093,7 +1165,7 @@ mrs_status mrs_plan_get_info(const mrs_plan *P, mrs_plan_info *info) {
     info->n_steps = P->dec.n_steps;
     info->n_adc = (int64_t)P->dec.adc_ref.size();
     info->n_iso = P->ntot;
-    info->n_launches = (int64_t)P->launch_r0.size();
+    info->n_launches = (int64_t)(P->nseq ? P->grp_begin.size() : P->launch_r0.size());
     info->n_rf_steps = P->dec.n_rf;
     info->n_cached_steps = P->dec.n_same;
     info->block_size = P->block;
@@ -1110,6 +1182,193 @@ void mrs_plan_destroy(mrs_plan *P) {
     (void)hipSetDevice(P->device);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     delete P;
+}
+
+mrs_status mrs_multipool_batch_create(int32_t n_seq, const mrs_vobj *vobj, const mrs_vmag *vmag,
+                                      const mrs_vcoi *vcoi, const mrs_vctl *vctl, const mrs_vseq *const *vseq,
+                                      const mrs_vvar *const *vvar, const mrs_vsig *const *vsig,
+                                      const mrs_options *opt, mrs_plan **out) {
+    if (!out) return fail(MRS_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n_seq < 1 || !vseq || !vvar || !vsig) return fail(MRS_ERR_ARG, "n_seq must be positive, with VSeq/VVar/VSig arrays");
+    if (!opt || opt->model < 1 || opt->model > 4)
+        return fail(MRS_ERR_ARG, "a batched multi-pool plan needs options.model 1..4 (GM, MT, CEST, ME)");
+    mrs_status rc;
+    for (int q = 0; q < n_seq; q++) {
+        if (!vseq[q] || !vvar[q] || !vsig[q]) return fail(MRS_ERR_ARG, "NULL VSeq/VVar/VSig of a sequence");
+        if ((rc = check_args(vobj, vmag, vcoi, vctl, vseq[q], vvar[q], vsig[q]))) return rc;
+    }
+    if (vctl->RunMode != 0) return fail(MRS_ERR_UNSUPPORTED, "batched sequences run with RunMode 0");
+    mrs_plan *P = new (std::nothrow) mrs_plan();
+    if (!P) return fail(MRS_ERR_NOMEM, "host allocation failed");
+    struct Guard {
+        mrs_plan *p;
+        ~Guard() {
+            if (p) mrs_plan_destroy(p);
+        }
+    } guard{P};
+    if ((rc = plan_setup(P, vobj, vmag, vcoi, vctl, vseq[0], vsig[0], opt))) return rc;
+    P->nseq = n_seq;
+    P->dec = Decoded();
+    P->dec.tx = P->Tx;
+    P->dec.coil_words = 4;
+    P->dec.stride = rec_stride(P->Tx, 4);
+    int64_t sig_off = 0;
+    for (int q = 0; q < n_seq; q++) {
+        Decoded d;
+        d.tx = P->Tx;
+        d.coil_words = 4;
+        d.stride = P->dec.stride;
+        Decoder dec(vobj, vcoi, vctl, vseq[q], vvar[q], false);
+        dec.run(d);
+        dec.flush_tail(d);
+        compress_runs(d, 0);
+        if ((int64_t)d.adc_ref.size() > vsig[q]->SignalNum)
+            return fail(MRS_ERR_ARG, "VSig.SignalNum of sequence " + std::to_string(q) +
+                                         " is smaller than its number of ADC samples");
+        P->seq_r0.push_back((int32_t)P->dec.n_rec());
+        P->dec.rec.insert(P->dec.rec.end(), d.rec.begin(), d.rec.end());
+        if (P->dec.rec.size() >= ((size_t)1 << 31)) return fail(MRS_ERR_UNSUPPORTED, "more than 2^31 record words");
+        P->seq_r1.push_back((int32_t)P->dec.n_rec());
+        P->seq_adc_off.push_back((int64_t)P->dec.adc_ref.size());
+        P->seq_nadc.push_back((int64_t)d.adc_ref.size());
+        P->dec.adc_ref.insert(P->dec.adc_ref.end(), d.adc_ref.begin(), d.adc_ref.end());
+        P->dec.n_steps += d.n_steps;
+        P->dec.n_rf += d.n_rf;
+        P->dec.n_same += d.n_same;
+        P->seq_k[0].push_back(std::move(d.kx));
+        P->seq_k[1].push_back(std::move(d.ky));
+        P->seq_k[2].push_back(std::move(d.kz));
+        P->seq_vvar.push_back(dec.vvar());
+        P->seq_rf[0].emplace_back(dec.vvar().rfAmp, dec.vvar().rfAmp + P->Tx);
+        P->seq_rf[1].emplace_back(dec.vvar().rfPhase, dec.vvar().rfPhase + P->Tx);
+        P->seq_rf[2].emplace_back(dec.vvar().rfFreq, dec.vvar().rfFreq + P->Tx);
+        P->seq_signum.push_back(vsig[q]->SignalNum);
+        P->seq_sig_off.push_back(sig_off);
+        sig_off += (int64_t)vsig[q]->SignalNum * P->Ty * P->Rx;
+    }
+    P->seq_sig_off.push_back(sig_off);
+    // launch groups: whole sequences whose block partials fit the budget
+    const int nblk = P->gx * P->ngroups;
+    const int64_t budget = opt->max_adc_per_launch > 0
+                               ? (int64_t)opt->max_adc_per_launch
+                               : ((int64_t)512 << 20) / ((int64_t)nblk * P->nv * (int64_t)sizeof(float));
+    int64_t used = 0, most = 1;
+    P->seq_slot.assign(n_seq, 0);
+    P->grp_begin.push_back(0);
+    for (int q = 0; q < n_seq; q++) {
+        const int64_t na = P->seq_nadc[q];
+        if (na > budget)
+            return fail(MRS_ERR_UNSUPPORTED, "sequence " + std::to_string(q) +
+                                                 " has more ADC samples than one batched launch holds; scan it alone");
+        if (used + na > budget) {
+            P->grp_end.push_back(q);
+            P->grp_begin.push_back(q);
+            used = 0;
+        }
+        P->seq_slot[q] = (int32_t)used;
+        used += na;
+        most = std::max(most, used);
+    }
+    P->grp_end.push_back(n_seq);
+    hipStream_t st = P->stream;
+    for (int k = 0; k < 2; k++) {
+        DevBuf &b = k ? P->Sy : P->Sx;
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        HIP_TRY(hipMalloc(&b.p, (size_t)std::max<int64_t>(1, sig_off) * sizeof(double)));
+    }
+    const size_t words = P->dec.rec.size();
+    HIP_TRY(hipMalloc(&P->rec.p, std::max<size_t>(words, 8) * 4));
+    if (words) HIP_TRY(hipMemcpyAsync(P->rec.p, P->dec.rec.data(), words * 4, hipMemcpyHostToDevice, st));
+    const size_t nadc = P->dec.adc_ref.size();
+    HIP_TRY(hipMalloc(&P->adc_ref.p, std::max<size_t>(nadc, 1) * sizeof(double)));
+    if (nadc)
+        HIP_TRY(hipMemcpyAsync(P->adc_ref.p, P->dec.adc_ref.data(), nadc * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMalloc(&P->partial.p, (size_t)most * nblk * P->nv * sizeof(float)));
+    std::vector<int32_t> tab((size_t)4 * n_seq, 0);
+    for (int q = 0; q < n_seq; q++) {
+        tab[4 * q] = P->seq_r0[q];
+        tab[4 * q + 1] = P->seq_r1[q];
+        tab[4 * q + 2] = P->seq_slot[q];
+    }
+    HIP_TRY(hipMalloc(&P->seq_tab.p, tab.size() * sizeof(int32_t)));
+    HIP_TRY(hipMemcpy(P->seq_tab.p, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    for (int k = 0; k < 3; k++) HIP_TRY(hipMalloc(&P->Mout[k].p, (size_t)n_seq * P->ntot * sizeof(float)));
+    if (P->scheme == 1) {
+        // the exact scheme's cost grows with a voxel's fastest rate (its
+        // Taylor substeps): order the voxels costliest first, so the waves
+        // that hold them share tiles and are dispatched first
+        std::vector<float> rates;
+        canonical_rates(vobj->K, P->variant, P->Ty, P->nvox, rates);
+        std::vector<float> cost((size_t)P->nvox, 0.f);
+        for (int a = 0; a < P->Ty * P->Ty; a++)
+            for (int64_t v = 0; v < P->nvox; v++) cost[v] += std::fabs(rates[(size_t)a * P->nvox + v]);
+        std::vector<int32_t> perm((size_t)P->nvox);
+        for (int64_t v = 0; v < P->nvox; v++) perm[v] = (int32_t)v;
+        std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+        HIP_TRY(hipMalloc(&P->vox_perm.p, perm.size() * sizeof(int32_t)));
+        HIP_TRY(hipMemcpy(P->vox_perm.p, perm.data(), perm.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    *out = P;
+    guard.p = nullptr;
+    return MRS_OK;
+}
+
+mrs_status mrs_multipool_batch_fetch(mrs_plan *P, mrs_vobj *const *vobj, mrs_vvar *const *vvar,
+                                     mrs_vsig *const *vsig) {
+    if (!P) return fail(MRS_ERR_ARG, "plan is NULL");
+    if (!P->nseq) return fail(MRS_ERR_ARG, "not a batched multi-pool plan (use mrs_plan_fetch)");
+    mrs_status rc = mrs_plan_sync(P);
+    if (rc) return rc;
+    for (int q = 0; q < P->nseq; q++) {
+        if (vobj && vobj[q]) {
+            float *dst[3] = {vobj[q]->Mx, vobj[q]->My, vobj[q]->Mz};
+            for (int k = 0; k < 3; k++) {
+                if (!dst[k]) return fail(MRS_ERR_ARG, "NULL VObj.Mx/My/Mz of a sequence");
+                HIP_TRY(hipMemcpy(dst[k], (const float *)P->Mout[k].p + (size_t)q * P->ntot,
+                                  P->ntot * sizeof(float), hipMemcpyDeviceToHost));
+            }
+        }
+        if (vsig && vsig[q]) {
+            const int64_t n = P->seq_sig_off[q + 1] - P->seq_sig_off[q];
+            std::vector<double> hx((size_t)n), hy((size_t)n);
+            if (n) {
+                HIP_TRY(hipMemcpy(hx.data(), (const double *)P->Sx.p + P->seq_sig_off[q], n * sizeof(double),
+                                  hipMemcpyDeviceToHost));
+                HIP_TRY(hipMemcpy(hy.data(), (const double *)P->Sy.p + P->seq_sig_off[q], n * sizeof(double),
+                                  hipMemcpyDeviceToHost));
+            }
+            for (int64_t i = 0; i < n; i++) {
+                vsig[q]->Sx[i] += hx[(size_t)i];
+                vsig[q]->Sy[i] += hy[(size_t)i];
+            }
+            for (size_t i = 0; i < P->seq_k[0][q].size(); i++) {
+                vsig[q]->Kx[i] += P->seq_k[0][q][i];
+                vsig[q]->Ky[i] += P->seq_k[1][q][i];
+                vsig[q]->Kz[i] += P->seq_k[2][q][i];
+            }
+        }
+        if (vvar && vvar[q]) {
+            const std::vector<double> rf[3] = {P->seq_rf[0][q], P->seq_rf[1][q], P->seq_rf[2][q]};
+            copy_vvar_out(P->seq_vvar[q], rf, P->Tx, vvar[q]);
+        }
+    }
+    return MRS_OK;
+}
+
+mrs_status mrs_DoMultiPoolBatchAtGPU(int32_t n_seq, const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs_vcoi *vcoi,
+                                     const mrs_vctl *vctl, const mrs_vseq *const *vseq, mrs_vvar *const *vvar,
+                                     mrs_vsig *const *vsig, mrs_vobj *const *vobj_out, const mrs_options *opt) {
+    mrs_plan *P = nullptr;
+    mrs_status rc = mrs_multipool_batch_create(n_seq, vobj, vmag, vcoi, vctl, vseq, (const mrs_vvar *const *)vvar,
+                                               (const mrs_vsig *const *)vsig, opt, &P);
+    if (rc) return rc;
+    rc = mrs_plan_run(P);
+    if (!rc) rc = mrs_multipool_batch_fetch(P, vobj_out, vvar, vsig);
+    mrs_plan_destroy(P);
+    return rc;
 }
 
 // One call with callbacks: decode until an Ext code that needs the caller,

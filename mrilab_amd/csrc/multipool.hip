@@ -58,6 +58,14 @@ __device__ T superlorentzian(double dw, double t2b) {
 // longest Taylor series of one exact substep (h*|A| <= 4 needs about 35 terms)
 constexpr int kTaylorMax = 48;
 
+// signal values per isochromat and ADC sample: 2*nrx per free pool, padded
+// to a power of two
+#define NVT_OF(TY, NRX) (gm_nvt((TY), (NRX)))
+constexpr int gm_nvt(int ty, int nrx) {
+    return (2 * nrx * ty <= 2) ? 2 : (2 * nrx * ty <= 4) ? 4 : (2 * nrx * ty <= 8) ? 8 : (2 * nrx * ty <= 16) ? 16
+           : (2 * nrx * ty <= 32) ? 32 : 64;
+}
+
 template <int TY, typename T>
 struct GmState {
     T mx[TY], my[TY], mz[TY];
@@ -71,8 +79,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     GmParams p, const uint32_t *__restrict__ rec_g, int r0, int r1) {
     using T = typename std::conditional<SCHEME == 0, float, double>::type;
     constexpr int NV = 2 * NRX;
-    constexpr int NVT = (NV * TY <= 2) ? 2 : (NV * TY <= 4) ? 4 : (NV * TY <= 8) ? 8 : (NV * TY <= 16) ? 16
-                                                                 : (NV * TY <= 32) ? 32 : 64;
+    constexpr int NVT = NVT_OF(TY, NRX);
     static_assert(NV * TY <= 64, "too many signal values per isochromat");
     extern __shared__ float red[];  // [kRedBatch][nwaves][NVT]
     const crec_t rec = (crec_t)rec_g;
@@ -83,11 +90,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     const int nwaves = blockDim.x >> 6;
     const int64_t nvox = p.nvox;
     const int s = blockIdx.y;
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = v < nvox;
-    const int64_t vv = valid ? v : nvox - 1;
-    const int nblk = gridDim.x * gridDim.y;
-    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+    // batched sequences: block x is (tile, sequence), sequence-minor so the
+    // costliest tiles of every sequence are dispatched first
+    const bool batched = p.seq_tab != nullptr;
+    const int tile = batched ? (int)blockIdx.x / p.nseq_l : (int)blockIdx.x;
+    const int sq = batched ? p.seq_base + (int)blockIdx.x % p.nseq_l : 0;
+    if (batched) {
+        r0 = p.seq_tab[4 * sq];
+        r1 = p.seq_tab[4 * sq + 1];
+    }
+    float *const partial = batched ? p.partial + (int64_t)p.seq_tab[4 * sq + 2] * p.tiles * gridDim.y * NVT_OF(TY, NRX)
+                                   : p.partial;
+    const int64_t vt = (int64_t)tile * blockDim.x + threadIdx.x;
+    const bool valid = vt < nvox;
+    const int64_t vv = !valid ? (p.vox_perm ? p.vox_perm[nvox - 1] : nvox - 1) : (p.vox_perm ? p.vox_perm[vt] : vt);
+    const int nblk = (batched ? p.tiles : (int)gridDim.x) * gridDim.y;
+    const int blk = blockIdx.y * (batched ? p.tiles : (int)gridDim.x) + tile;
 
     const float x = p.gx[vv], y = p.gy[vv], z = p.gz[vv];
     const float dB0 = p.dB0[vv];
@@ -195,7 +213,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
             if ((lane & (G - 1)) == 0) red[(slot * nwaves + wave) * NVT + lane / G] = val[0];
             if (++slot == kRedBatch) {
                 __syncthreads();
-                flush_partials(red, kRedBatch, nwaves, NVT, p.partial, adc_batch0, nblk, blk, NVT);
+                flush_partials(red, kRedBatch, nwaves, NVT, partial, adc_batch0, nblk, blk, NVT);
                 __syncthreads();
                 adc_batch0 += kRedBatch;
                 slot = 0;
@@ -609,14 +627,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     }
     if (slot > 0) {
         __syncthreads();
-        flush_partials(red, slot, nwaves, NVT, p.partial, adc_batch0, nblk, blk, NVT);
+        flush_partials(red, slot, nwaves, NVT, partial, adc_batch0, nblk, blk, NVT);
     }
     if (valid) {
+        const int64_t ob = batched ? (int64_t)sq * nvox * p.S * TY : 0;
+        float *ox = batched ? p.Mout[0] + ob : p.Mx;
+        float *oy = batched ? p.Mout[1] + ob : p.My;
+        float *oz = batched ? p.Mout[2] + ob : p.Mz;
 #pragma unroll
         for (int t = 0; t < TY; t++) {
-            p.Mx[sidx[t]] = (float)st.mx[t];
-            p.My[sidx[t]] = (float)st.my[t];
-            p.Mz[sidx[t]] = (float)st.mz[t];
+            ox[sidx[t]] = (float)st.mx[t];
+            oy[sidx[t]] = (float)st.my[t];
+            oz[sidx[t]] = (float)st.mz[t];
         }
     }
 }
@@ -640,7 +662,9 @@ static gm_launch_fn pick_nrx(int nrx, int scheme, int variant) {
         case 1: return pick_scheme<TY, 1>(scheme, variant);
         case 2: return pick_scheme<TY, 2>(scheme, variant);
         case 4: return pick_scheme<TY, 4>(scheme, variant);
-        case 8: return pick_scheme<TY, 8>(scheme, variant);
+        case 8:
+            if constexpr (2 * 8 * TY <= 64) return pick_scheme<TY, 8>(scheme, variant);
+            return nullptr;
     }
     return nullptr;
 }
@@ -652,6 +676,8 @@ gm_launch_fn pick_gm(int ty, int nrx, int scheme, int variant) {
         case 2: return pick_nrx<2>(nrx, scheme, variant);
         case 3: return pick_nrx<3>(nrx, scheme, variant);
         case 4: return pick_nrx<4>(nrx, scheme, variant);
+        case 5: return pick_nrx<5>(nrx, scheme, variant);
+        case 6: return pick_nrx<6>(nrx, scheme, variant);
     }
     return nullptr;
 }

@@ -7,7 +7,7 @@
 
 namespace mrs {
 
-constexpr int kGmMaxTypes = 4;
+constexpr int kGmMaxTypes = 6;
 
 struct GmParams {
     float *Mx, *My, *Mz;                 // [nvox, S, Ty]
@@ -23,6 +23,17 @@ struct GmParams {
     int flag[kGmMaxTypes];               // 0 free, 1 bound, per type
     int variant;                         // kVarGM, kVarMT, kVarCEST, kVarME
     int self_x;                          // ME: a nonzero diagonal of K exchanges too
+    // Batched sequences (one phantom, many independent record streams):
+    // seq_tab[4*q .. 4*q+3] = {first record, end record, first ADC slot of
+    // the launch's partial buffer, 0} of sequence q; nullptr for one scan.
+    // In batched launches block x maps to (tile x / nseq_l, sequence
+    // seq_base + x % nseq_l), each sequence starts from Mx/My/Mz and its
+    // final state goes to Mout[c] + q * nvox * S * Ty; vox_perm[i] is the
+    // voxel of thread i of a tile row (costliest voxels first).
+    const int32_t *seq_tab;
+    const int32_t *vox_perm;
+    float *Mout[3];
+    int seq_base, nseq_l, tiles;  // tiles: voxel tiles of one sequence (grid x / nseq_l)
 };
 
 // Multi-pool layouts of the reference engines (the K layout each entry point
@@ -31,7 +42,8 @@ enum { kVarGM = 0, kVarMT = 1, kVarCEST = 2, kVarME = 3 };
 
 typedef void (*gm_launch_fn)(dim3, dim3, size_t, hipStream_t, const GmParams &, const uint32_t *, int, int);
 
-// Kernel for TypeNum ty (2..4), padded coil count nrx (1, 2, 4, 8), scheme
+// Kernel for TypeNum ty (2..6; 5 and 6 with at most 4 padded receive
+// coils), padded coil count nrx (1, 2, 4, 8), scheme
 // (0 split fp32, 1 exact fp64) and variant; nullptr if unsupported.
 gm_launch_fn pick_gm(int ty, int nrx, int scheme, int variant);
 // Signal values each isochromat contributes per ADC sample (power of two).

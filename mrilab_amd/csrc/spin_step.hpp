@@ -81,13 +81,25 @@ __device__ __forceinline__ void free_step(SpinState<SPT> &st, float m0s) {
 // Applies the step of record q (flags already read; REC_NOSTEP excluded by
 // the caller). GEN: any Tx coil count (TxMaps), else one Tx coil (VoxelConst
 // weight). PREC: 0 = the reference's fp32 relaxation rounding, 1 = accurate.
+// relax: when not null (wave-uniform), the record's relaxation factors
+// computed on the host for a single-voxel scan -- {e2, e1} (PREC 0) or
+// {d2, d1} (PREC 1), rounded exactly as the branch below rounds them.
 template <int SPT, bool GEN, int PREC>
 __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst &k, crec_t q, uint32_t flags,
-                                           const TxMaps &txm) {
+                                           const TxMaps &txm, const float *relax = nullptr) {
     const float dt = __uint_as_float(q[0]);
     if ((flags & REC_NEWDT) || !st.have_e) {
         st.have_e = true;
-        if constexpr (PREC == 0) {
+        if (relax) {
+            if constexpr (PREC == 0) {
+                st.e2 = relax[0];
+                st.e1 = relax[1];
+                st.m0e = k.m0s * (1.0f - st.e1);
+            } else {
+                st.d2 = relax[0];
+                st.d1 = relax[1];
+            }
+        } else if constexpr (PREC == 0) {
             // BlochKernel.h:348-368 in fp32: E = exp(max(-dt/T, -200)), with
             // exp rounded correctly from fp64
             float a2 = (1.0f / k.t2) * dt;

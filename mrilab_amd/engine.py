@@ -47,7 +47,7 @@ EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanA
            "mrs_adjoint_get_info", "mrs_adjoint_destroy", "mrs_recon_dims", "mrs_DoCartRecon", "mrs_DoEchoTrainRecon",
            "mrs_plan_recon", "mrs_DoAddNoise", "mrs_noise_scale", "mrs_DoCalSARAtGPU",
            "mrs_DoSARAverageAtGPU", "mrs_seq_compile", "mrs_seq_counts", "mrs_seq_copy", "mrs_seq_destroy",
-           "mrs_seq_check")
+           "mrs_seq_check", "mrs_multipool_batch_create", "mrs_multipool_batch_fetch", "mrs_DoMultiPoolBatchAtGPU")
 
 
 class MRSError(RuntimeError):
@@ -154,6 +154,15 @@ def load() -> C.CDLL:
         lib.mrs_seq_destroy.restype = None
         lib.mrs_seq_check.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32]
         lib.mrs_seq_check.restype = C.c_int
+        one = [P(abi.VObjC), P(abi.VMagC), P(abi.VCoiC), P(abi.VCtlC)]
+        lib.mrs_multipool_batch_create.argtypes = ([C.c_int32] + one + [PP(abi.VSeqC), PP(abi.VVarC), PP(abi.VSigC),
+                                                    P(abi.OptionsC), P(C.c_void_p)])
+        lib.mrs_multipool_batch_create.restype = C.c_int
+        lib.mrs_multipool_batch_fetch.argtypes = [C.c_void_p, PP(abi.VObjC), PP(abi.VVarC), PP(abi.VSigC)]
+        lib.mrs_multipool_batch_fetch.restype = C.c_int
+        lib.mrs_DoMultiPoolBatchAtGPU.argtypes = ([C.c_int32] + one + [PP(abi.VSeqC), PP(abi.VVarC), PP(abi.VSigC),
+                                                  PP(abi.VObjC), P(abi.OptionsC)])
+        lib.mrs_DoMultiPoolBatchAtGPU.restype = C.c_int
         _lib = lib
         return lib
 
@@ -306,6 +315,71 @@ class Plan:
         when ``device``."""
         return _recon_call(lambda img, dev: self._lib.mrs_plan_recon(self._h, C.byref(desc), C.byref(img), dev),
                            desc, device)
+
+    def info(self) -> dict:
+        inf = abi.PlanInfoC()
+        _check(self._lib.mrs_plan_get_info(self._h, C.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in abi.PlanInfoC._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.mrs_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class MultiPoolBatch:
+    """One phantom under several independent sequences on the GPU
+    (mrs_multipool_batch_*): ``scans`` share VObj/VMag/VCoi/VCtl (those of
+    scans[0] are used) and each brings its VSeq/VVar/VSig, e.g. the offsets of
+    workloads.mt_cest(per_offset=True). ``model`` is "GM", "MT", "CEST" or
+    "ME" (VObj.K in that engine's layout). No reference counterpart."""
+
+    def __init__(self, scans, model: str = "GM", options=None):
+        self._lib = load()
+        self.scans = list(scans)
+        m = MODELS[model]
+        opt = abi.OptionsC.from_buffer_copy(options if options is not None else make_options())
+        opt.model = m
+        self._kp = k_planes(m, int(self.scans[0].VObj["TypeNum"]))
+        self._packs = [abi.pack_scan(*sc.structs, multipool=True, k_planes=self._kp) for sc in self.scans]
+        P0 = self._packs[0]
+        self._args = [_ptr_array(abi.VSeqC, [p.vseq for p in self._packs]),
+                      _ptr_array(abi.VVarC, [p.vvar for p in self._packs]),
+                      _ptr_array(abi.VSigC, [p.vsig for p in self._packs])]
+        self._h = C.c_void_p()
+        _check(self._lib.mrs_multipool_batch_create(C.c_int32(len(self._packs)), C.byref(P0.vobj), C.byref(P0.vmag),
+                                                    C.byref(P0.vcoi), C.byref(P0.vctl), *self._args,
+                                                    C.byref(opt), C.byref(self._h)))
+
+    def run(self) -> None:
+        _check(self._lib.mrs_plan_run(self._h))
+
+    def sync(self) -> None:
+        _check(self._lib.mrs_plan_sync(self._h))
+
+    def fetch(self, out_scans) -> None:
+        """Sequence q's final magnetization, signal and decoder state go into
+        out_scans[q] (Scans of the same shapes, each with its own VObj
+        magnetization arrays)."""
+        packs = [abi.pack_scan(*sc.structs, multipool=True, k_planes=self._kp) for sc in out_scans]
+        a_obj = _ptr_array(abi.VObjC, [p.vobj for p in packs])
+        a_var = _ptr_array(abi.VVarC, [p.vvar for p in packs])
+        a_sig = _ptr_array(abi.VSigC, [p.vsig for p in packs])
+        _check(self._lib.mrs_multipool_batch_fetch(self._h, a_obj, a_var, a_sig))
+        for p, sc in zip(packs, out_scans):
+            abi.unpack_vvar(p, sc.VVar)
 
     def info(self) -> dict:
         inf = abi.PlanInfoC()

@@ -378,7 +378,7 @@ def bssfp3d(shape=(128, 128, 64), n_ro=128, n_pe_y=None, n_pe_z=None, spins=4, n
 
 
 def mt_cest(shape=(256, 256, 1), n_offsets=40, ppm_range=50.0, b0=3.0, sat_time=2.0, pulse=0.02,
-            gap=0.02, sat_fa=300.0, dt=10e-6, seed=2, t2b=10e-6, spins=1, n_coils=1) -> Scan:
+            gap=0.02, sat_fa=300.0, dt=10e-6, seed=2, t2b=10e-6, spins=1, n_coils=1, per_offset=False):
     """Three-pool exchange with MT and CEST-style saturation.
 
     Pools: 0 free water, 1 myelin water (free), 2 MT bound pool
@@ -392,6 +392,10 @@ def mt_cest(shape=(256, 256, 1), n_offsets=40, ppm_range=50.0, b0=3.0, sat_time=
     K is [R,C,Z,3,3]: K(a,b) is the rate from pool a into pool b.
     ``spins`` > 1 spreads each voxel's isochromats over +-25 Hz (dWRnd, same
     for every pool); ``n_coils`` > 1 adds Gaussian receive coils.
+    ``per_offset``: instead of one scan whose TRs run the offsets in turn,
+    a list of scans of one TR each (one offset, from equilibrium), sharing
+    the phantom's VObj/VMag/VCoi/VCtl dicts, for the batched multi-pool
+    entry point.
     """
     rng = np.random.default_rng(seed)
     R, C_, Z = shape
@@ -436,21 +440,30 @@ def mt_cest(shape=(256, 256, 1), n_offsets=40, ppm_range=50.0, b0=3.0, sat_time=
         tr.add_ext(times[[n_sat, n_sat + 1, n_sat + 2]], [0, 6, 0])          # ideal spoiler before readout
         tr.add_ext(times[[n - 4, n - 3, n - 2]], [0, 6, 0])                  # and after it
         trs.append(tr.finalize())
-    seq = psd.concat(trs)
+    seq = psd.concat(trs if not per_offset else trs[:1])
     res = (2e-3, 2e-3, 5e-3)
     dWRnd = np.zeros(shape + (spins, 3))
     for q in range(spins):
         dWRnd[..., q, :] = 2 * np.pi * 25.0 * (2 * (q + 0.5) / spins - 1)
     rx = gaussian_coils(shape, res, n_coils, rng) if n_coils > 1 else None
     sc = make_structs(Rho, T1, T2, dB0, res, spin_num=spins, dWRnd=dWRnd, rx=rx, seq=seq,
-                      tr_num=n_offsets)
+                      tr_num=1 if per_offset else n_offsets)
     sc.VObj["K"] = np.asfortranarray(K.astype(np.float32))
     sc.VObj["TypeFlag"] = np.array([0.0, 0.0, 1.0])
     sc.VCtl["GM_Flag"] = "on"
     sc.VCtl["CS"] = np.zeros(3)
     sc.name = f"mt_cest_{R}x{C_}x{Z}_off{n_offsets}_steps{n}"
     sc.offsets_hz = offsets
-    return sc
+    if not per_offset:
+        return sc
+    scans = []
+    for j, tr in enumerate(trs):
+        seq_j = psd.concat([tr])
+        o = Scan(sc.VObj, sc.VMag, sc.VCoi, sc.VCtl, seq_j, copy.deepcopy(sc.VVar), copy.deepcopy(sc.VSig),
+                 name=f"{sc.name}_offset{j}")
+        o.offset_hz = float(offsets[j])
+        scans.append(o)
+    return scans
 
 
 def _fingerprint_template(t_rf, te, delta):

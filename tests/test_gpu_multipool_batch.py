@@ -1,0 +1,144 @@
+"""GPU parity of the batched multi-pool sequences (mrs_multipool_batch_*)
+and of 5- and 6-pool exchange, against the CPU oracle.
+
+Batched: one phantom under several saturation offsets, each offset a
+sequence of its own simulated from equilibrium; every offset's signal,
+final magnetization and decoder state must equal the oracle's scan of that
+offset alone (fp32 oracle for the operator split, fp64 for the exact
+scheme; 1e-4 rel-L2 as for single scans), and the single-scan GPU entry
+point's to fp32 summation order (the exact scheme orders the voxels by cost,
+which changes only the order of the coil sums).
+
+Five and six pools (the reference's GM, CEST and ME engines take any
+TypeNum, DoGMScanAtGPU.cu:376): GM, ME and CEST layouts, both schemes.
+
+Every scan here ends with a 90 deg readout, an ideal spoiler and a few
+steps, so the final magnetization is only the Mz regrown in about 25 steps
+(a few 1e-4 of the equilibrium norm): its error is measured against the
+larger of its own norm and the initial (equilibrium) state's, since an
+absolute error at fp32 rounding of the pre-readout state is already 1e-4
+of the final norm (the fp32 and fp64 oracles differ by 6e-5 there).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from mrilab_amd import engine, workloads
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def sig(sc):
+    return sc.VSig["Sx"] + 1j * sc.VSig["Sy"]
+
+
+def mag(sc):
+    return np.concatenate([sc.VObj["Mx"].ravel(), sc.VObj["My"].ravel(), sc.VObj["Mz"].ravel()])
+
+
+def mag_err(g, ref, init):
+    d = np.linalg.norm(mag(g) - mag(ref))
+    return float(d / max(np.linalg.norm(mag(ref)), np.linalg.norm(mag(init))))
+
+
+def pools(sc, idx, variant="GM", t1_scale=None):
+    """The scan with pools idx (GM layout), then K in the variant's layout."""
+    out = workloads.select_pools(sc, idx, t1_scale=t1_scale)
+    return workloads.as_variant(out, variant, self_rate=3.0 if variant == "ME" else 0.0)
+
+
+BATCH_CASES = [
+    ("GM3_split", "GM", 0, None, dict(spins=1, n_coils=1)),
+    ("GM3_exact_c2", "GM", 1, None, dict(spins=2, n_coils=2)),
+    ("MT2_split", "MT", 0, [0, 2], dict(spins=1, n_coils=1)),
+    ("CEST4_split", "CEST", 0, [0, 2, 1, 1], dict(spins=1, n_coils=1)),
+    ("ME3_exact", "ME", 1, [0, 1, 1], dict(spins=1, n_coils=1)),
+]
+
+
+@pytest.mark.parametrize("name,variant,scheme,idx,kw", BATCH_CASES, ids=[c[0] for c in BATCH_CASES])
+def test_batched_offsets_match_oracle_per_offset(gpu, name, variant, scheme, idx, kw):
+    base = workloads.mt_cest(shape=(5, 4, 1), n_offsets=3, sat_time=0.04, pulse=0.02, gap=0.0, sat_fa=200.0,
+                             per_offset=True, **kw)
+    if idx is not None:
+        first = pools(base[0], idx, variant)
+        scans = [workloads.Scan(first.VObj, first.VMag, first.VCoi, first.VCtl, s.VSeq, s.VVar, first.VSig,
+                                name=s.name).copy() for s in base]
+        # fresh signal arrays per offset (select_pools resized VSig for the pools)
+    else:
+        scans = [s.copy() for s in base]
+    print(f"case {name}: {len(scans)} offsets, TypeNum {int(scans[0].VObj['TypeNum'])}", flush=True)
+    opt = engine.make_options(multipool_scheme=scheme)
+    outs = [s.copy() for s in scans]
+    with engine.MultiPoolBatch(scans, model=variant, options=opt) as b:
+        b.run()
+        b.fetch(outs)
+        info = b.info()
+    assert info["n_launches"] >= 1
+    prec = "f32" if scheme == 0 else "f64"
+    for q, (s, g) in enumerate(zip(scans, outs)):
+        ref = oracle.gm_scan(s, prec, scheme, variant=variant)
+        e_s, e_m = rel_l2(sig(g), sig(ref)), mag_err(g, ref, s)
+        print(f"  offset {q}: vs oracle signal {e_s:.3e} magnetization {e_m:.3e}", flush=True)
+        assert e_s < 1e-4 and e_m < 1e-4
+        for k in ("utsi", "rfi", "ADCi", "Exti", "TRCount"):
+            assert int(g.VVar[k]) == int(ref.VVar[k]), k
+        np.testing.assert_array_equal(g.VSig["Kx"], ref.VSig["Kx"])
+        one = s.copy()
+        getattr(engine, f"Do{variant}ScanAtGPU")(*one.structs, options=opt)
+        assert rel_l2(sig(g), sig(one)) < 1e-6
+        assert rel_l2(mag(g), mag(one)) < 1e-6
+
+
+def test_batched_offsets_in_launch_groups(gpu):
+    """With at most one ADC sample per launch (options.max_adc_per_launch),
+    each offset runs in a launch group of its own; the results equal the
+    one-group run bit for bit."""
+    scans = workloads.mt_cest(shape=(6, 6, 1), n_offsets=3, sat_time=0.02, pulse=0.01, gap=0.0, sat_fa=90.0,
+                              per_offset=True)
+    res = []
+    for cap in (0, 1):
+        outs = [s.copy() for s in scans]
+        with engine.MultiPoolBatch(scans, options=engine.make_options(max_adc_per_launch=cap)) as b:
+            b.run()
+            b.fetch(outs)
+            n = b.info()["n_launches"]
+        print(f"  max_adc_per_launch {cap}: {n} launch groups", flush=True)
+        assert n == (1 if cap == 0 else 3)
+        res.append(outs)
+    for a, b in zip(*res):
+        np.testing.assert_array_equal(sig(a), sig(b))
+        np.testing.assert_array_equal(mag(a), mag(b))
+
+
+FIVE = [
+    ("GM5_split", "GM", 0, [0, 1, 1, 2, 2], [1.0, 1.0, 2.0, 1.0, 1.0], 1),
+    ("GM5_exact_c4", "GM", 1, [0, 1, 1, 2, 2], [1.0, 1.0, 2.0, 1.0, 1.0], 4),
+    ("GM6_exact", "GM", 1, [0, 1, 1, 2, 0, 2], [1.0, 1.0, 2.0, 1.0, 0.5, 1.0], 1),
+    ("ME5_split", "ME", 0, [0, 1, 1, 0, 1], [1.0, 1.0, 2.0, 0.7, 0.5], 2),
+    ("ME5_exact", "ME", 1, [0, 1, 1, 0, 1], [1.0, 1.0, 2.0, 0.7, 0.5], 1),
+    ("CEST5_split", "CEST", 0, [0, 2, 1, 1, 1], [1.0, 1.0, 1.0, 2.0, 0.5], 1),
+    ("CEST5_exact", "CEST", 1, [0, 2, 1, 1, 1], [1.0, 1.0, 1.0, 2.0, 0.5], 1),
+]
+
+
+@pytest.mark.parametrize("name,variant,scheme,idx,t1s,coils", FIVE, ids=[c[0] for c in FIVE])
+def test_five_and_six_pools(gpu, name, variant, scheme, idx, t1s, coils):
+    base = workloads.mt_cest(shape=(4, 3, 1), n_offsets=2, sat_time=0.04, pulse=0.02, gap=0.0, sat_fa=200.0,
+                             n_coils=coils)
+    sc = pools(base, idx, variant, t1_scale=t1s)
+    print(f"case {name}: TypeNum {int(sc.VObj['TypeNum'])}, Rx {coils}", flush=True)
+    ref = oracle.gm_scan(sc, "f32" if scheme == 0 else "f64", scheme, variant=variant)
+    g = sc.copy()
+    getattr(engine, f"Do{variant}ScanAtGPU")(*g.structs, options=engine.make_options(multipool_scheme=scheme))
+    e_s, e_m = rel_l2(sig(g), sig(ref)), mag_err(g, ref, sc)
+    print(f"  signal {e_s:.3e} magnetization {e_m:.3e}", flush=True)
+    assert e_s < 1e-4 and e_m < 1e-4
+
+
+def test_five_pools_eight_coils_is_refused(gpu):
+    base = workloads.mt_cest(shape=(3, 3, 1), n_offsets=1, sat_time=0.02, pulse=0.01, gap=0.0, n_coils=8)
+    sc = pools(base, [0, 1, 1, 2, 2])
+    with pytest.raises(engine.MRSError, match="4 with 5 or 6 pools"):
+        engine.DoGMScanAtGPU(*sc.copy().structs)

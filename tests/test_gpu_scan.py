@@ -151,3 +151,10 @@ def test_full_phantom_first_tr(gpu):
     print(f"  signal {e_sig:.3e} magnetization {e_mag:.3e}", flush=True)
     assert e_sig < 1e-4 and e_mag < 1e-4
     assert_decoder_equal(g, o32)
+    # error against the fp64 oracle, reference and accurate modes
+    o64 = oracle.scan(sc, "f64")
+    a = run_gpu(sc, precision=1)
+    e = [rel_l2(sig(g), sig(o64)), rel_l2(mag(g), mag(o64)), rel_l2(sig(a), sig(o64)), rel_l2(mag(a), mag(o64))]
+    print("  vs fp64 oracle: reference mode signal %.3e magnetization %.3e; accurate mode %.3e %.3e" % tuple(e),
+          flush=True)
+    assert max(e) < 1e-4
