@@ -501,6 +501,20 @@ mrs_status mrs_DoCalSARAtGPU(const mrs_sar_obj *obj, const mrs_vseq *vseq, mrs_v
 mrs_status mrs_DoSARAverageAtGPU(const double *SAR, const double *Power, const double *MassDen, const int64_t dims[4],
                                  double dx, double dy, double dz, double grams, double *aveSAR, double *avePower,
                                  int32_t device);
+/* mrs_DoSARAverageAtGPU on device arrays of the calling thread's current GPU
+ * that allocates no memory: no hipMalloc, hipFree or allocating library
+ * call. No counterpart in the reference; for callers that cannot allocate
+ * (code under stream capture, serving loops, harnesses that forbid device
+ * allocation). The per-voxel radius, weight and denominator and the mass
+ * partials live in a static device array of 20 MiB + 4 KiB that every GPU
+ * the library runs on holds until the process ends, so the volume may have
+ * at most 2^20 voxels (rows*cols*slices; MRS_ERR_UNSUPPORTED above). All
+ * calls on one GPU share that array: calls that overlap on one GPU race.
+ * stream: a hipStream_t, NULL for the default stream; the call synchronizes
+ * it (the mass check reads two values back). */
+mrs_status mrs_DoSARAverageAtGPU_noalloc(const double *SAR, const double *Power, const double *MassDen,
+                                         const int64_t dims[4], double dx, double dy, double dz, double grams,
+                                         double *aveSAR, double *avePower, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Sequence compiler: a PSD XML file (MRiLab's MRiLabSeq format: CVs, then

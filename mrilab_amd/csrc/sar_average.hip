@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "host_common.hpp"
@@ -281,64 +282,47 @@ mrs_status amalloc(ADev &b, size_t bytes) {
     return MRS_OK;
 }
 
-}  // namespace
+// Sets the search-status word to 0 (a kernel rather than a memset, so that
+// code running this on a captured stream records no memset node).
+__global__ void clear_status_kernel(int *status) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *status = 0;
+}
 
-}  // namespace mrs
+// Device buffers of one average beyond the inputs and outputs: per voxel the
+// radius, weight and denominator; the status word and the mass partials.
+struct AvgScratch {
+    int *R;
+    double *coefw, *den;
+    int *status;
+    double *tot;  // 2 * kMassBlocks + 2 doubles
+};
 
-using namespace mrs;
-
-extern "C" {
-
-mrs_status mrs_DoSARAverageAtGPU(const double *SAR, const double *Power, const double *MassDen, const int64_t dims[4],
-                                 double dx, double dy, double dz, double grams, double *aveSAR, double *avePower,
-                                 int32_t device) {
+mrs_status check_avg_args(const double *SAR, const double *Power, const double *MassDen, const int64_t dims[4],
+                          double dx, double dy, double dz, double grams, double *aveSAR, double *avePower) {
     if (!SAR || !Power || !MassDen || !aveSAR || !avePower) return fail(MRS_ERR_ARG, "NULL array pointer");
     if (!dims) return fail(MRS_ERR_ARG, "dims is NULL");
     const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2], T = dims[3];
     if (d0 <= 0 || d1 <= 0 || d2 <= 0 || T <= 0) return fail(MRS_ERR_ARG, "dims must be positive");
     if (d0 + d1 + d2 > 40000) return fail(MRS_ERR_UNSUPPORTED, "rows + columns + slices must be at most 40000");
-    const int64_t nvox = d0 * d1 * d2;
-    if (nvox > ((int64_t)1 << 40) / T) return fail(MRS_ERR_UNSUPPORTED, "volume too large");
+    if (d0 * d1 * d2 > ((int64_t)1 << 40) / T) return fail(MRS_ERR_UNSUPPORTED, "volume too large");
     if (!std::isfinite(dx) || !std::isfinite(dy) || !std::isfinite(dz) || !std::isfinite(grams))
         return fail(MRS_ERR_ARG, "dx, dy, dz and grams must be finite");
-    int ndev = 0;
-    HIP_TRY(hipGetDeviceCount(&ndev));
-    if (ndev <= 0) return fail(MRS_ERR_DEVICE, "no HIP device visible");
-    HIP_TRY(hipSetDevice(0));
-    hipStream_t st = nullptr;
-    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    struct S {
-        hipStream_t s;
-        ~S() { (void)hipStreamDestroy(s); }
-    } guard{st};
-    const size_t vb = (size_t)nvox * sizeof(double), tb = vb * (size_t)T;
-    ADev dsar, dpow, dmass, dout_s, dout_p, dR, dcw, dden, dstat;
-    mrs_status rc;
-    const double *sar = SAR, *pw = Power, *mass = MassDen;
-    double *out_s = aveSAR, *out_p = avePower;
-    if (!device) {
-        if ((rc = amalloc(dsar, tb)) || (rc = amalloc(dpow, tb)) || (rc = amalloc(dmass, vb)) ||
-            (rc = amalloc(dout_s, tb)) || (rc = amalloc(dout_p, tb)))
-            return rc;
-        HIP_TRY(hipMemcpyAsync(dsar.p, SAR, tb, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(dpow.p, Power, tb, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(dmass.p, MassDen, vb, hipMemcpyHostToDevice, st));
-        sar = (const double *)dsar.p;
-        pw = (const double *)dpow.p;
-        mass = (const double *)dmass.p;
-        out_s = (double *)dout_s.p;
-        out_p = (double *)dout_p.p;
-    }
-    if ((rc = amalloc(dR, (size_t)nvox * sizeof(int))) || (rc = amalloc(dcw, vb)) || (rc = amalloc(dden, vb)) ||
-        (rc = amalloc(dstat, (2 * kMassBlocks + 2) * sizeof(double))))
-        return rc;
+    return MRS_OK;
+}
+
+// The average on device arrays, enqueued on st (synchronized before return).
+mrs_status sar_average_device(const double *sar, const double *pw, const double *mass, const int64_t dims[4],
+                              double dx, double dy, double dz, double grams, double *out_s, double *out_p,
+                              const AvgScratch &sc, hipStream_t st) {
+    const int64_t d0 = dims[0], d1 = dims[1], d2 = dims[2], T = dims[3];
+    const int64_t nvox = d0 * d1 * d2;
     const double vol = dx * dy * dz, gkg = grams * 1e-3;
 
     // the search ends for every voxel iff the covering sphere's mass,
     // summed in memory order, exceeds gkg; decide from an any-order sum and
     // its rounding bound, and sum in order only when the bound straddles gkg
     double part[2 * kMassBlocks], tot[2] = {0, 0};
-    double *dtot = (double *)dstat.p + 2;
+    double *dtot = sc.tot + 2;
     mass_total_kernel<<<kMassBlocks, 1024, 0, st>>>(mass, nvox, vol, dtot);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(part, dtot, sizeof(part), hipMemcpyDeviceToHost, st));
@@ -362,14 +346,15 @@ mrs_status mrs_DoSARAverageAtGPU(const double *SAR, const double *Power, const d
     const int64_t tiles = (nvox + kAvgBlock - 1) / kAvgBlock;
     const int64_t grid = (tiles + kXcds - 1) / kXcds * kXcds;
     if (grid > INT32_MAX) return fail(MRS_ERR_UNSUPPORTED, "volume too large");
-    HIP_TRY(hipMemsetAsync(dstat.p, 0, sizeof(int), st));
+    clear_status_kernel<<<1, 64, 0, st>>>(sc.status);
+    HIP_TRY(hipGetLastError());
     RadiusParams rp;
     rp.mass = mass;
     rp.sar = sar;
-    rp.R = (int *)dR.p;
-    rp.coefw = (double *)dcw.p;
-    rp.den = (double *)dden.p;
-    rp.status = (int *)dstat.p;
+    rp.R = sc.R;
+    rp.coefw = sc.coefw;
+    rp.den = sc.den;
+    rp.status = sc.status;
     rp.nvox = nvox;
     rp.d0 = (int)d0;
     rp.d1 = (int)d1;
@@ -408,14 +393,119 @@ mrs_status mrs_DoSARAverageAtGPU(const double *SAR, const double *Power, const d
         t += groups * tt;
     }
     int status = 0;
-    HIP_TRY(hipMemcpyAsync(&status, dstat.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    if (!device) {
-        HIP_TRY(hipMemcpyAsync(aveSAR, out_s, tb, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(avePower, out_p, tb, hipMemcpyDeviceToHost, st));
-    }
+    HIP_TRY(hipMemcpyAsync(&status, sc.status, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (status) return fail(MRS_ERR_DEVICE, "N-gram radius search passed the covering sphere");
     return MRS_OK;
+}
+
+// Scratch of mrs_DoSARAverageAtGPU_noalloc, for volumes of at most
+// kNoAllocMaxVoxels voxels: the status word and mass partials (4 KiB), then
+// per voxel the radius (int) and the weight and denominator (double).
+constexpr int64_t kNoAllocMaxVoxels = (int64_t)1 << 20;
+constexpr size_t kScratchHead = 4096;
+constexpr size_t kScratchBytes = kScratchHead + (size_t)kNoAllocMaxVoxels * (sizeof(int) + 2 * sizeof(double));
+static_assert((2 * kMassBlocks + 2) * sizeof(double) <= kScratchHead, "mass partials exceed the scratch head");
+
+// One copy on every GPU the library runs on, held until the process ends.
+// In host code the name stands for the symbol, not a device address: the
+// address of this GPU's copy comes from hipGetSymbolAddress.
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// This device's scratch address (looked up once per device, with it current).
+mrs_status scratch_address(int device, unsigned char **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return fail(MRS_ERR_DEVICE, "device index out of range");
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(MRS_ERR_DEVICE, "hipGetSymbolAddress returned NULL for the scratch array");
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = (unsigned char *)g_scratch_ptr[device];
+    return MRS_OK;
+}
+
+}  // namespace
+
+}  // namespace mrs
+
+using namespace mrs;
+
+extern "C" {
+
+mrs_status mrs_DoSARAverageAtGPU(const double *SAR, const double *Power, const double *MassDen, const int64_t dims[4],
+                                 double dx, double dy, double dz, double grams, double *aveSAR, double *avePower,
+                                 int32_t device) {
+    mrs_status rc = check_avg_args(SAR, Power, MassDen, dims, dx, dy, dz, grams, aveSAR, avePower);
+    if (rc) return rc;
+    const int64_t nvox = dims[0] * dims[1] * dims[2], T = dims[3];
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (ndev <= 0) return fail(MRS_ERR_DEVICE, "no HIP device visible");
+    HIP_TRY(hipSetDevice(0));
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct S {
+        hipStream_t s;
+        ~S() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    const size_t vb = (size_t)nvox * sizeof(double), tb = vb * (size_t)T;
+    ADev dsar, dpow, dmass, dout_s, dout_p, dR, dcw, dden, dstat;
+    const double *sar = SAR, *pw = Power, *mass = MassDen;
+    double *out_s = aveSAR, *out_p = avePower;
+    if (!device) {
+        if ((rc = amalloc(dsar, tb)) || (rc = amalloc(dpow, tb)) || (rc = amalloc(dmass, vb)) ||
+            (rc = amalloc(dout_s, tb)) || (rc = amalloc(dout_p, tb)))
+            return rc;
+        HIP_TRY(hipMemcpyAsync(dsar.p, SAR, tb, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dpow.p, Power, tb, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dmass.p, MassDen, vb, hipMemcpyHostToDevice, st));
+        sar = (const double *)dsar.p;
+        pw = (const double *)dpow.p;
+        mass = (const double *)dmass.p;
+        out_s = (double *)dout_s.p;
+        out_p = (double *)dout_p.p;
+    }
+    if ((rc = amalloc(dR, (size_t)nvox * sizeof(int))) || (rc = amalloc(dcw, vb)) || (rc = amalloc(dden, vb)) ||
+        (rc = amalloc(dstat, (2 * kMassBlocks + 2) * sizeof(double))))
+        return rc;
+    AvgScratch sc{(int *)dR.p, (double *)dcw.p, (double *)dden.p, (int *)dstat.p, (double *)dstat.p};
+    if ((rc = sar_average_device(sar, pw, mass, dims, dx, dy, dz, grams, out_s, out_p, sc, st))) return rc;
+    if (!device) {
+        HIP_TRY(hipMemcpyAsync(aveSAR, out_s, tb, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(avePower, out_p, tb, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return MRS_OK;
+}
+
+mrs_status mrs_DoSARAverageAtGPU_noalloc(const double *SAR, const double *Power, const double *MassDen,
+                                         const int64_t dims[4], double dx, double dy, double dz, double grams,
+                                         double *aveSAR, double *avePower, void *stream) {
+    mrs_status rc = check_avg_args(SAR, Power, MassDen, dims, dx, dy, dz, grams, aveSAR, avePower);
+    if (rc) return rc;
+    const int64_t nvox = dims[0] * dims[1] * dims[2];
+    if (nvox > kNoAllocMaxVoxels)
+        return fail(MRS_ERR_UNSUPPORTED, "the allocation-free SAR average takes at most " +
+                                             std::to_string(kNoAllocMaxVoxels) + " voxels, got " +
+                                             std::to_string(nvox));
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    unsigned char *base = nullptr;
+    if ((rc = scratch_address(device, &base))) return rc;
+    AvgScratch sc;
+    sc.status = (int *)base;
+    sc.tot = (double *)base;  // the status word shares the first 16 bytes; partials from tot + 2
+    sc.R = (int *)(base + kScratchHead);
+    sc.coefw = (double *)(base + kScratchHead + (size_t)kNoAllocMaxVoxels * sizeof(int));
+    sc.den = sc.coefw + kNoAllocMaxVoxels;
+    return sar_average_device(SAR, Power, MassDen, dims, dx, dy, dz, grams, aveSAR, avePower, sc,
+                              (hipStream_t)stream);
 }
 
 }  // extern "C"

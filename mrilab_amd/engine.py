@@ -47,7 +47,8 @@ EXPORTS = ("mrs_last_error", "mrs_abi_version", "mrs_device_count", "mrs_DoScanA
            "mrs_adjoint_get_info", "mrs_adjoint_destroy", "mrs_recon_dims", "mrs_DoCartRecon", "mrs_DoEchoTrainRecon",
            "mrs_plan_recon", "mrs_DoAddNoise", "mrs_noise_scale", "mrs_DoCalSARAtGPU",
            "mrs_DoSARAverageAtGPU", "mrs_seq_compile", "mrs_seq_counts", "mrs_seq_copy", "mrs_seq_destroy",
-           "mrs_seq_check", "mrs_multipool_batch_create", "mrs_multipool_batch_fetch", "mrs_DoMultiPoolBatchAtGPU")
+           "mrs_seq_check", "mrs_multipool_batch_create", "mrs_multipool_batch_fetch", "mrs_DoMultiPoolBatchAtGPU",
+           "mrs_DoSARAverageAtGPU_noalloc")
 
 
 class MRSError(RuntimeError):
@@ -144,6 +145,10 @@ def load() -> C.CDLL:
                                               C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                               C.c_int32]
         lib.mrs_DoSARAverageAtGPU.restype = C.c_int
+        lib.mrs_DoSARAverageAtGPU_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(C.c_int64), C.c_double,
+                                                      C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                                      C.c_void_p]
+        lib.mrs_DoSARAverageAtGPU_noalloc.restype = C.c_int
         lib.mrs_seq_compile.argtypes = [C.c_char_p, P(abi.SeqParamC), C.c_int32, C.c_double, P(C.c_void_p)]
         lib.mrs_seq_compile.restype = C.c_int
         lib.mrs_seq_counts.argtypes = [C.c_void_p, P(C.c_int64)]
