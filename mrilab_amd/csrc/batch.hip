@@ -142,12 +142,39 @@ __global__ __launch_bounds__(256) void bloch_batch_kernel(BatchParams p, const u
         const int n_rec = d.n_rec;
         // a single-voxel scan (one T1, T2 for the wave) reads its relaxation
         // factors per record instead of evaluating exp per lane
-        const float *rlx = d.relax_off >= 0 ? p.relax + 2 * d.relax_off : nullptr;
+        const bool have_rlx = d.relax_off >= 0;
+        const crec_t rlx = (crec_t)p.relax + 2 * (have_rlx ? d.relax_off : 0);
         const int64_t part0 = d.part_off + ti;
         const int ntile = d.n_tiles;
         int64_t adc = 0;
+        // Each scan has a record stream of its own, so its records are not
+        // shared through the caches with other waves: every record (8 words,
+        // one scalar load) and its relaxation pair are fetched one step
+        // ahead, and the step waits only for data requested a step before.
+        uint32_t nx[8];
+        uint32_t nr[2] = {0u, 0u};
+        if (n_rec > 0) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) nx[j] = rec[j];
+            if (have_rlx) {
+                nr[0] = rlx[0];
+                nr[1] = rlx[1];
+            }
+        }
         for (int r = 0; r < n_rec; ++r) {
-            const crec_t q = rec + r * 8;
+            uint32_t q[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) q[j] = nx[j];
+            const RelaxIn rin{have_rlx, __uint_as_float(nr[0]), __uint_as_float(nr[1])};
+            if (r + 1 < n_rec) {
+                const crec_t qn = rec + (r + 1) * 8;
+#pragma unroll
+                for (int j = 0; j < 8; j++) nx[j] = qn[j];
+                if (have_rlx) {
+                    nr[0] = rlx[2 * (r + 1)];
+                    nr[1] = rlx[2 * (r + 1) + 1];
+                }
+            }
             const uint32_t flags = q[5];
             if (flags & REC_SPOIL) {
                 __asm__ volatile("");
@@ -172,7 +199,7 @@ __global__ __launch_bounds__(256) void bloch_batch_kernel(BatchParams p, const u
                 if ((lane & (G - 1)) == 0) p.partial[(part0 + adc * ntile) * NV + lane / G] = val[0];
                 adc++;
             }
-            if (!(flags & REC_NOSTEP)) bloch_step<SPT, false, PREC>(st, k, q, flags, txm, rlx ? rlx + 2 * r : nullptr);
+            if (!(flags & REC_NOSTEP)) bloch_step<SPT, false, PREC>(st, k, (const uint32_t *)q, flags, txm, rin);
         }
         if (valid) {
 #pragma unroll

@@ -78,26 +78,34 @@ __device__ __forceinline__ void free_step(SpinState<SPT> &st, float m0s) {
     }
 }
 
+// Relaxation factors of one record computed on the host for a single-voxel
+// scan (one T1, T2 for the wave): {e2, e1} (PREC 0) or {d2, d1} (PREC 1),
+// rounded exactly as the device branch rounds them; on = false evaluates
+// them per lane.
+struct RelaxIn {
+    bool on;
+    float a, b;
+};
+
 // Applies the step of record q (flags already read; REC_NOSTEP excluded by
-// the caller). GEN: any Tx coil count (TxMaps), else one Tx coil (VoxelConst
-// weight). PREC: 0 = the reference's fp32 relaxation rounding, 1 = accurate.
-// relax: when not null (wave-uniform), the record's relaxation factors
-// computed on the host for a single-voxel scan -- {e2, e1} (PREC 0) or
-// {d2, d1} (PREC 1), rounded exactly as the branch below rounds them.
-template <int SPT, bool GEN, int PREC>
-__device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst &k, crec_t q, uint32_t flags,
-                                           const TxMaps &txm, const float *relax = nullptr) {
+// the caller). q is any word accessor: a scalar-path pointer into the record
+// stream, or a record already held in registers. GEN: any Tx coil count
+// (TxMaps), else one Tx coil (VoxelConst weight). PREC: 0 = the reference's
+// fp32 relaxation rounding, 1 = accurate.
+template <int SPT, bool GEN, int PREC, typename Q>
+__device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst &k, Q q, uint32_t flags,
+                                           const TxMaps &txm, RelaxIn relax = RelaxIn{false, 0.f, 0.f}) {
     const float dt = __uint_as_float(q[0]);
     if ((flags & REC_NEWDT) || !st.have_e) {
         st.have_e = true;
-        if (relax) {
+        if (relax.on) {
             if constexpr (PREC == 0) {
-                st.e2 = relax[0];
-                st.e1 = relax[1];
+                st.e2 = relax.a;
+                st.e1 = relax.b;
                 st.m0e = k.m0s * (1.0f - st.e1);
             } else {
-                st.d2 = relax[0];
-                st.d1 = relax[1];
+                st.d2 = relax.a;
+                st.d1 = relax.b;
             }
         } else if constexpr (PREC == 0) {
             // BlochKernel.h:348-368 in fp32: E = exp(max(-dt/T, -200)), with
@@ -150,14 +158,19 @@ __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst 
         for (int s = 0; s < SPT; s++) {
             // rotation by |Omega|*dt about -Omega, Omega = (bx, by, w): the
             // matrix of BlochKernel.h:146-306 in Rodrigues form
+            // (the rotation is applied once, so the hardware reciprocal
+            // square root, 1 ulp, and the 1.5 ulp fp32 sincos keep the
+            // step within a few ulp of the correctly rounded one; below
+            // |Omega| = 1e-15 rad/s the rotation, under 1e-15 rad per
+            // second of dt, is skipped, which keeps rsq off subnormals)
             const float w = (st.wb[s] + gv) + wrf;
             const float om2 = b2 + w * w;
-            if (om2 > 0.f) {
-                const float om = sqrtf(om2);
-                const float inv = 1.0f / om;
+            if (om2 > 1e-30f) {
+                const float inv = __builtin_amdgcn_rsqf(om2);
+                const float om = om2 * inv;
                 const float nx = bx * inv, ny = by * inv, nz = w * inv;
                 float sn, cs;
-                sincos_cr(om * dt, &sn, &cs);
+                sincos_fast(om * dt, &sn, &cs);
                 const float ax = st.mx[s], ay = st.my[s], az = st.mz[s];
                 const float kk = (1.f - cs) * (nx * ax + ny * ay + nz * az);
                 st.mx[s] = ax * cs - (ny * az - nz * ay) * sn + nx * kk;

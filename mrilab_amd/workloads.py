@@ -378,7 +378,8 @@ def bssfp3d(shape=(128, 128, 64), n_ro=128, n_pe_y=None, n_pe_z=None, spins=4, n
 
 
 def mt_cest(shape=(256, 256, 1), n_offsets=40, ppm_range=50.0, b0=3.0, sat_time=2.0, pulse=0.02,
-            gap=0.02, sat_fa=300.0, dt=10e-6, seed=2, t2b=10e-6, spins=1, n_coils=1, per_offset=False):
+            gap=0.02, sat_fa=300.0, dt=10e-6, seed=2, t2b=10e-6, spins=1, n_coils=1, per_offset=False,
+            crop=None, offset_index=None):
     """Three-pool exchange with MT and CEST-style saturation.
 
     Pools: 0 free water, 1 myelin water (free), 2 MT bound pool
@@ -396,6 +397,10 @@ def mt_cest(shape=(256, 256, 1), n_offsets=40, ppm_range=50.0, b0=3.0, sat_time=
     a list of scans of one TR each (one offset, from equilibrium), sharing
     the phantom's VObj/VMag/VCoi/VCtl dicts, for the batched multi-pool
     entry point.
+    ``crop``: (row slice, column slice) of the phantom drawn at ``shape``
+    (the same tissue values as the full phantom's voxels there, receive
+    coils of the cropped grid); ``offset_index``: keep only these offsets
+    (indices into the full list).
     """
     rng = np.random.default_rng(seed)
     R, C_, Z = shape
@@ -414,6 +419,14 @@ def mt_cest(shape=(256, 256, 1), n_offsets=40, ppm_range=50.0, b0=3.0, sat_time=
         K[..., b, a] = k_ab * f[..., a] / f[..., b]               # detailed balance
     gbar = GYRO / (2 * np.pi)
     offsets = np.linspace(-ppm_range, ppm_range, n_offsets) * 1e-6 * gbar * b0   # Hz
+    if offset_index is not None:
+        offsets = offsets[list(offset_index)]
+        n_offsets = len(offsets)
+    if crop is not None:
+        sl = (crop[0], crop[1])
+        f, Rho, T1, T2, dB0, K = f[sl], Rho[sl], T1[sl], T2[sl], dB0[sl], K[sl]
+        shape = f.shape[:3]
+        R, C_, Z = shape
     n_pulse = int(round(pulse / dt))
     n_gap = int(round(gap / dt))
     period = n_pulse + n_gap

@@ -14,6 +14,7 @@ import pytest
 import oracle
 from mrilab_amd import psd, workloads
 from mrilab_amd.psd import GYRO
+from tests.conftest import adc_capture_slots
 
 PI = 3.14159265359
 
@@ -232,40 +233,12 @@ def test_signal_is_brute_force_coil_sum():
     seq = sc.VSeq
     sc2 = workloads.with_run_mode1(sc)
     out = oracle.scan(sc2, "f64")
-    uts = seq["utsLine"]
-    # find ADC time points: replay the decoder's ADC bookkeeping through Muts
+    adc_slots = adc_capture_slots(seq)
     N = int(out.VSig["SignalNum"])
     mx, my = out.VSig["Mx"], out.VSig["My"]
     C = sc2.VCoi["RxCoilx"].astype(np.float64) + 1j * sc2.VCoi["RxCoily"].astype(np.float64)
     s = out.VSig["Sx"] + 1j * out.VSig["Sy"]
     s = s.reshape(3, N)
-    # The state captured at slot u is the state after the step ending at uts[u];
-    # ADC at the step starting at uts[u] uses the state captured at slot u.
-    adc_slots = []
-    utsi = 0
-    k = 0
-    ts, fl = seq["tsLine"], seq["flagsLine"]
-    i = 0
-    adci = 0
-    while i < ts.size - 1:
-        f = fl[:, i].copy()
-        j = i
-        while j + 1 < ts.size and ts[j + 1] == ts[j]:
-            f += fl[:, j + 1]
-            j += 1
-            if j == ts.size - 1:
-                break
-        i = j + 1
-        cur = utsi
-        utsi = min(utsi + 1, uts.size - 1)
-        dtv = uts[utsi] - uts[cur]
-        adc = 0
-        if f[4] == 1:
-            adc = seq["ADCLine"][adci]
-            adci += 1
-        if dtv > 0 and adc == 1:
-            adc_slots.append(cur)
-        k += 1
     assert len(adc_slots) == N
     rfref = np.pi * (np.arange(2) % 2)
     for n, u in enumerate(adc_slots):
