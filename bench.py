@@ -8,21 +8,26 @@ Workload (one "step" = one full scan of it, inputs resident in HBM):
   on a 10 us grid (500 steps), 128 x 64 phase encodes x 128 readout
   samples = 8192 TRs = 4.096e6 steps = 1.72e13 isochromat-steps.
 
-Prints one JSON line with isochromat-steps/s, the forward kernel's
-roofline under the algorithmic-bytes convention (per isochromat-step:
-24 B state read+write, 4 B dWRnd, 36/S B per-voxel maps, 8*Rx/S B of coil
-maps on ADC steps) and the fp32 CPU oracle timed on the first TRs of the
-same phantom on this host's cores.
+Prints one JSON line with isochromat-steps/s, the forward kernel's roofline
+and the fp32 CPU oracle timed on the first TRs of the same phantom on this
+host's cores, then sections for the other paths (batched fingerprinting,
+adjoint, multi-pool at its full size, recon, SAR, sequence compiler).
 
-The kernel keeps the state in registers for a whole launch, so the bytes
-it really moves (rocprofv3 FETCH_SIZE + WRITE_SIZE, stored in
-profiles/bloch_fwd_bssfp3d_traffic.json) are a small fraction of the
-algorithmic bytes and the algorithmic-byte rate can exceed HBM peak. The
-line therefore also carries the measured HBM rate and the VALU roofline:
-algorithmic fp32 flops per isochromat-step = 10 (cached free-precession
-rotation 6, transverse and longitudinal relaxation 4) + 35 more on rf steps
-(Rodrigues rotation; plus sqrt, reciprocal and sin/cos) + 8 per receive
-coil on ADC steps (complex product with the coil map and accumulation).
+The forward kernel keeps each thread's isochromats in registers for a whole
+launch (16 TRs), so HBM carries little (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+about 77 GB/s) and the kernel is bound by VALU instruction issue. The
+roofline is that of issue: busy 4-cycle issue slots (VALU instructions less
+the dual-issued pairs, from rocprofv3 counters of this kernel build in
+profiles/bloch_fwd_bssfp3d_counters.json) over the slots of the live launch
+time at the peak clock. Beside it: the counted fp32 flops against 157.3
+TFLOP/s, the measured HBM rate against 8 TB/s, and, labelled as a
+convention, the bytes a step-at-a-time engine would move (24 B state
+read+write, 4 B dWRnd, 36/S B per-voxel maps, 8*Rx/S B of coil maps on ADC
+steps per isochromat-step) and a flop model (10 per free step, +35 per rf
+step, +8 per coil on ADC steps).
+
+Profile: python bench.py --write-profile profiles/bloch_fwd_bssfp3d_counters.json
+DB... turns rocprofv3 databases of this command into that file.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -114,7 +119,10 @@ def batched_section(n_sets: int, plain_sample: int, reps: int) -> dict:
     b.fetch()  # into the structs given at create, in place as MRiLab's globals
     t_fetch = time.perf_counter() - t0
     b.close()
-    # plain per-scan launches on a sample, end to end (what BatchSim.m does per scan)
+    # plain per-scan launches, end to end (what BatchSim.m does per scan): all
+    # scans, or an evenly spaced sample whose mean time is scaled to all
+    if plain_sample <= 0 or plain_sample >= n_sets:
+        plain_sample = n_sets
     idx = np.linspace(0, n_sets - 1, plain_sample).astype(int)
     plain = []
     errs = []
@@ -126,7 +134,9 @@ def batched_section(n_sets: int, plain_sample: int, reps: int) -> dict:
         a = o.VSig["Sx"] + 1j * o.VSig["Sy"]
         c = outs[i].VSig["Sx"] + 1j * outs[i].VSig["Sy"]
         errs.append(float(np.linalg.norm(c - a) / max(np.linalg.norm(a), 1e-30)))
-    plain_total = float(np.mean(plain)) * n_sets
+        if len(plain) % 1024 == 0:
+            log(f"batched: {len(plain)}/{plain_sample} plain per-scan calls, {np.sum(plain):.1f} s")
+    plain_total = float(np.sum(plain)) if plain_sample == n_sets else float(np.mean(plain)) * n_sets
     batched_e2e = t_create + wall_run + t_fetch
     iso_steps = info["n_iso_steps"]
     return {"workload": f"{n_sets} single-voxel fingerprinting scans x 64 isochromats, 500-1500 TRs of 5 steps "
@@ -136,7 +146,9 @@ def batched_section(n_sets: int, plain_sample: int, reps: int) -> dict:
             "end_to_end_s": batched_e2e, "create_s": t_create, "decode_ms": info["decode_ms"],
             "tune_ms": info["tune_ms"], "library_create_ms": info["create_ms"], "fetch_s": t_fetch,
             "plain_per_scan_ms": float(np.mean(plain)) * 1e3, "plain_sample": int(plain_sample),
-            "plain_total_s_estimated": plain_total,
+            "plain_total_s": plain_total,
+            "plain_total_kind": "measured, every scan" if plain_sample == n_sets else
+                                f"extrapolated from {plain_sample} evenly spaced scans",
             "speedup": plain_total / wall_run, "speedup_end_to_end": plain_total / batched_e2e,
             "max_rel_l2_vs_plain": max(errs),
             "shape": {"spins_per_thread": info["spins_per_thread"], "block_size": info["block_size"],
@@ -182,46 +194,79 @@ def adjoint_section(reps: int) -> dict:
             "spins_per_thread": info["spins_per_thread"]}
 
 
-def multipool_section(sat_split: float, sat_exact: float, cpu_shape: int, threads: int) -> dict:
-    """Multi-pool exchange with MT saturation on the 256x256 3-pool phantom
-    (free water, myelin water, bound pool with a super-Lorentzian line):
-    the first of the 40 saturation offsets (-50 ppm), its Gaussian pulse
-    train cut to ``sat_split`` s (operator split, fp32) and ``sat_exact`` s
-    (fp64 matrix exponential), then readout. Device time per scan, and the
-    CPU oracle's split scheme on a ``cpu_shape``^2 crop for comparison."""
+def multipool_section(threads: int, sat_time: float = 2.0) -> dict:
+    """Multi-pool exchange with MT saturation, the full configuration: the
+    256x256 3-pool phantom (free water, myelin water, bound pool with a
+    super-Lorentzian line) under 40 saturation offsets over +-50 ppm at 3 T,
+    each a ``sat_time`` s train of 300 deg Gaussian pulses (20 ms on, 20 ms
+    off) then readout, every offset from equilibrium. The 40 offsets run as
+    one batched launch (offsets x voxels as the parallel axis), in the fp32
+    operator split and in the fp64 exact scheme; device time of a whole run.
+    Beside it: both schemes against the oracle on a 2x2-voxel crop at 4 of
+    the offsets over the full train, and the oracle's split scheme timed on
+    a crop (CPU baseline)."""
+    import numpy as np
     import oracle
     from mrilab_amd import engine, workloads
-    out = {"workload": "256x256 voxels x 3 pools (Dirichlet(8,1,1) fractions, k~U[1,30]/s, T2b 10 us), first "
-                       "saturation offset (-50 ppm at 3 T) of 300 deg Gaussian pulses (20 ms on, 20 ms off), "
-                       "10 us steps"}
-    for scheme, sat in ((0, sat_split), (1, sat_exact)):
-        sc = workloads.mt_cest(shape=(256, 256, 1), n_offsets=1, sat_time=sat)
-        o = sc.copy()
-        opt = engine.make_options(multipool_scheme=scheme, timing=1, model=1)
-        with engine.Plan(*o.structs, options=opt) as p:
-            p.run()
-            p.sync()
-            t0 = time.perf_counter()
-            p.run()
-            p.sync()
-            el = time.perf_counter() - t0
-            info = p.info()
-        vs = 256 * 256 * info["n_steps"]
+    t0 = time.perf_counter()
+    scans = workloads.mt_cest(sat_time=sat_time, per_offset=True)
+    n_off = len(scans)
+    log(f"multipool: {n_off} offsets x {scans[0].n_steps()} steps built ({time.perf_counter() - t0:.1f} s)")
+    out = {"workload": f"256x256 voxels x 3 pools (Dirichlet(8,1,1) fractions, k~U[1,30]/s, T2b 10 us), {n_off} "
+                       f"saturation offsets over +-50 ppm at 3 T, each {sat_time:g} s of 300 deg Gaussian pulses "
+                       "(20 ms on, 20 ms off) then readout, 10 us steps, offsets batched in one launch"}
+    for scheme in (0, 1):
         name = "split" if scheme == 0 else "exact"
-        out[name] = {"saturation_s": sat, "steps": info["n_steps"], "ms_per_scan": el * 1e3,
-                     "kernel_ms": info["fwd_kernel_ms"], "voxel_steps_per_s": vs / el,
-                     # 40 offsets x a 2 s train: time grows with the train length
-                     "full_config_s_estimated": 40 * (2.0 / sat) * el}
-        log(f"multipool {name}: {info['n_steps']} steps, {el * 1e3:.1f} ms, {vs / el:.3e} voxel-steps/s")
-    sc = workloads.mt_cest(shape=(cpu_shape, cpu_shape, 1), n_offsets=1, sat_time=sat_split)
+        t0 = time.perf_counter()
+        b = engine.MultiPoolBatch(scans, options=engine.make_options(multipool_scheme=scheme, timing=1))
+        t_create = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b.run()
+        b.sync()
+        el = time.perf_counter() - t0
+        info = b.info()
+        b.close()
+        vs = 256 * 256 * info["n_steps"]
+        out[name] = {"precision": "fp32" if scheme == 0 else "fp64", "steps_all_offsets": info["n_steps"],
+                     "seconds": el, "kernel_seconds": info["fwd_kernel_ms"] / 1e3, "create_s": t_create,
+                     "launches": info["n_launches"], "voxel_steps_per_s": vs / el}
+        log(f"multipool {name}: {info['n_steps']} steps over {n_off} offsets in {el:.2f} s "
+            f"({vs / el:.3e} voxel-steps/s, {info['n_launches']} launches)")
+    # parity on a crop over the full train, 4 offsets
+    idx = [0, 13, 26, 39]
+    crop = workloads.mt_cest(sat_time=sat_time, per_offset=True, crop=(slice(100, 102), slice(60, 62)),
+                             offset_index=idx)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    sig = lambda x: x.VSig["Sx"] + 1j * x.VSig["Sy"]
+    par = {}
+    for scheme in (0, 1):
+        outs = [c.copy() for c in crop]
+        with engine.MultiPoolBatch(crop, options=engine.make_options(multipool_scheme=scheme)) as b:
+            b.run()
+            b.fetch(outs)
+        r64 = [oracle.gm_scan(c, "f64", scheme) for c in crop]
+        if scheme == 1:
+            par["exact_vs_fp64_oracle"] = max(rel(sig(g), sig(r)) for g, r in zip(outs, r64))
+        else:
+            # an fp32 evaluation of the split drifts from its fp64 evaluation
+            # over 200,000 steps; the fp32 oracle's drift is the yardstick
+            r32 = [oracle.gm_scan(c, "f32", 0) for c in crop]
+            par["split_vs_fp64_split"] = max(rel(sig(g), sig(r)) for g, r in zip(outs, r64))
+            par["fp32_oracle_vs_fp64_split"] = max(rel(sig(a), sig(r)) for a, r in zip(r32, r64))
+            par["split_vs_fp32_oracle"] = max(rel(sig(g), sig(a)) for g, a in zip(outs, r32))
+    out["crop_parity"] = {"sample": f"2x2 voxels, offsets {idx}, full train", "max_signal_rel_l2": par}
+    log(f"multipool crop parity: {par}")
+    # CPU baseline: the oracle's split scheme on a 32x32 crop, first offset, full train
+    cpu = workloads.mt_cest(sat_time=sat_time, per_offset=True, crop=(slice(0, 32), slice(0, 32)),
+                            offset_index=[0])[0]
     oracle.set_threads(threads, "f32")
     t0 = time.perf_counter()
-    oracle.gm_scan(sc, precision="f32", scheme=0)
+    oracle.gm_scan(cpu, precision="f32", scheme=0)
     el = time.perf_counter() - t0
     oracle.set_threads(0, "f32")
-    vs = cpu_shape * cpu_shape * sc.n_steps()
+    vs = 32 * 32 * cpu.n_steps()
     out["cpu_split"] = {"value": vs / el, "unit": "voxel-steps/s", "cores": threads, "kind": "port",
-                        "sample": f"{cpu_shape}x{cpu_shape} crop, {sc.n_steps()} steps ({el:.1f} s), fp32 "
+                        "sample": f"32x32 crop, first offset ({cpu.n_steps()} steps, {el:.1f} s), fp32 "
                                   "restatement of BlochKernelGM.cuh's split, OpenMP over voxels"}
     log(f"multipool cpu split: {vs / el:.3e} voxel-steps/s on {threads} threads")
     return out
@@ -425,13 +470,137 @@ def coil_sum_section(n_tr: int) -> dict:
     return out
 
 
-def traffic_from_profile() -> float | None:
-    """HBM bytes per forward launch measured by rocprofv3 (profiles/)."""
-    path = os.path.join(ROOT, "profiles", "bloch_fwd_bssfp3d_traffic.json")
-    if not os.path.exists(path):
+# The forward kernel's per-launch hardware counters (rocprofv3 --pmc) live in
+# profiles/bloch_fwd_bssfp3d_counters.json, written by --write-profile from
+# the databases of the commands it lists. The kernel's instruction counts are
+# fixed by its code and by the launch (16 TRs of this workload), so they hold
+# for every launch of the scan; the file records a hash of the sources the
+# kernel is compiled from and is ignored when they have changed since.
+COUNTERS_JSON = os.path.join(ROOT, "profiles", "bloch_fwd_bssfp3d_counters.json")
+KERNEL_SOURCES = ("mrilab_amd/csrc/engine.hip", "mrilab_amd/csrc/spin_step.hpp", "mrilab_amd/csrc/device_common.hpp",
+                  "mrilab_amd/csrc/decode.hpp", "mrilab_amd/csrc/decode.cpp")
+FWD_KERNEL = "bloch_fwd_kernel<4, 8, false, 0, true>"
+SIMDS = 1024                # 256 CUs x 4 SIMDs
+PEAK_CLOCK_HZ = 2.4e9       # MI355X peak engine clock
+
+
+def kernel_sources_hash() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def _pmc_per_dispatch(db: str) -> dict:
+    """{dispatch: (kernel name, duration ms, {counter: value summed over instances})}."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    out = {}
+    for disp, name, cname, val, t0, t1 in con.execute(
+            "select dispatch_id, kernel_name, counter_name, value, start, end from counters_collection"):
+        k, dur, vals = out.setdefault(disp, (name, (t1 - t0) * 1e-6, {}))
+        vals[cname] = vals.get(cname, 0.0) + val
+    return out
+
+
+def write_profile(out_path: str, dbs: list) -> None:
+    """Forward-kernel counter profile from rocprofv3 databases: --pmc runs
+    (any counters), a FETCH_SIZE and a WRITE_SIZE run made with
+    --calibrate-copy (the 256 MiB device copy scales the byte counters), and
+    a --kernel-trace run; per launch, the last launch of the kernel in each
+    database (the first carries the cold caches)."""
+    import sqlite3
+    import statistics
+    prof = {"kernel": FWD_KERNEL, "sources_sha256_16": kernel_sources_hash(),
+            "workload": "3-D bSSFP 128x128x64 x 4 isochromats, 8 Rx coils; one launch = 16 TRs = 8000 steps, "
+                        "2048 ADC samples, 3.355e10 isochromat-steps", "counters": {}}
+    for db in dbs:
+        con = sqlite3.connect(db)
+        tables = {r[0] for r in con.execute("select name from sqlite_master")}
+        if "counters_collection" in tables and con.execute("select count(*) from counters_collection").fetchone()[0]:
+            per = _pmc_per_dispatch(db)
+            fwd = [d for d in sorted(per) if FWD_KERNEL in per[d][0]]
+            last = per[fwd[-1]]
+            prof["counters"].update(last[2])
+            prof.setdefault("pmc_launch_ms", {})[os.path.basename(db)] = last[1]
+            if "GRBM_GUI_ACTIVE" in last[2]:  # summed over the 8 XCDs
+                prof.setdefault("clock_hz", {})[os.path.basename(db)] = last[2]["GRBM_GUI_ACTIVE"] / 8 / (last[1] / 1e3)
+            copies = [per[d] for d in sorted(per) if "copyBuffer" in per[d][0] and
+                      max(per[d][2].values()) > 100000]
+            for cname in ("FETCH_SIZE", "WRITE_SIZE"):
+                if cname in last[2] and copies:
+                    prof[cname + "_copy_kib"] = statistics.median(c[2][cname] for c in copies)
+        elif "kernels" in tables:
+            d = [(t1 - t0) * 1e-6 for name, t0, t1 in con.execute("select name, start, end from kernels")
+                 if FWD_KERNEL in name]
+            prof["trace"] = {"launches": len(d), "avg_ms": sum(d) / len(d), "median_ms": statistics.median(d),
+                             "min_ms": min(d), "max_ms": max(d)}
+    c = prof["counters"]
+    copy_bytes = 256 << 20
+    if "FETCH_SIZE" in c and "WRITE_SIZE" in c:
+        ff = copy_bytes / 1024 / prof["FETCH_SIZE_copy_kib"]
+        wf = copy_bytes / 1024 / prof["WRITE_SIZE_copy_kib"]
+        prof["traffic"] = {"fetch_factor": ff, "write_factor": wf, "read_bytes": c["FETCH_SIZE"] * 1024 * ff,
+                           "write_bytes": c["WRITE_SIZE"] * 1024 * wf}
+        prof["traffic"]["bytes_per_launch"] = prof["traffic"]["read_bytes"] + prof["traffic"]["write_bytes"]
+    prof["commands"] = [a for a in os.environ.get("PROFILE_COMMANDS", "").split(";;") if a]
+    with open(out_path, "w") as f:
+        json.dump(prof, f, indent=1)
+    print(json.dumps(prof, indent=1))
+
+
+def load_profile() -> dict | None:
+    if not os.path.exists(COUNTERS_JSON):
         return None
-    with open(path) as f:
-        return float(json.load(f)["bytes_per_launch"])
+    with open(COUNTERS_JSON) as f:
+        prof = json.load(f)
+    if prof.get("sources_sha256_16") != kernel_sources_hash():
+        log("profiles/bloch_fwd_bssfp3d_counters.json was measured on other kernel sources: roofline counters "
+            "omitted")
+        return None
+    return prof
+
+
+def roofline(prof: dict | None, avg_launch_s: float, ab: float, iso_steps_per_launch: float, flop: float) -> dict:
+    """The forward kernel is bound by VALU issue. Each SIMD issues at most one
+    VALU instruction per 4-cycle slot, or two (dual issue, counted by
+    SQ_ACTIVE_INST_VALU2); busy slots per launch = SQ_INSTS_VALU -
+    SQ_ACTIVE_INST_VALU2. frac = busy slots / (SIMDs x peak clock x live
+    launch time / 4); achieved and peak are the same ratio in instructions:
+    the kernel's issue rate, and the rate at which every slot issues with
+    this kernel's dual-issue share."""
+    out = {"bound": "valu", "achieved": None, "peak": None, "unit": "G VALU wave-instructions/s", "frac": None,
+           "traffic": None, "kernel": FWD_KERNEL, "avg_launch_ms": avg_launch_s * 1e3}
+    alg_gbs = ab * iso_steps_per_launch / avg_launch_s / 1e9
+    out["algorithmic_bytes_convention"] = {
+        "bytes_per_isochromat_step": ab, "GBs": alg_gbs, "x_hbm_peak": alg_gbs / HBM_PEAK_GBS,
+        "note": "bytes a step-at-a-time engine would move; the state stays in registers, so this is not HBM work"}
+    out["model_flops"] = {"flop_per_isochromat_step": flop,
+                          "TFLOPs": flop * iso_steps_per_launch / avg_launch_s / 1e12}
+    if prof is None:
+        return out
+    c = prof["counters"]
+    if "SQ_INSTS_VALU" in c and "SQ_ACTIVE_INST_VALU2" in c:
+        insts, dual = c["SQ_INSTS_VALU"], c["SQ_ACTIVE_INST_VALU2"]
+        slots = SIMDS * PEAK_CLOCK_HZ * avg_launch_s / 4
+        out["achieved"] = insts / avg_launch_s / 1e9
+        out["peak"] = SIMDS * PEAK_CLOCK_HZ / 4 * insts / (insts - dual) / 1e9
+        out["frac"] = (insts - dual) / slots
+        out["valu"] = {"instructions_per_launch": insts, "dual_issue_quad_cycles": dual,
+                       "dual_issued_share": 2 * dual / insts,
+                       "instructions_over_single_issue_slots": insts / slots}
+    if "SQ_INSTS_VALU_FLOPS_FP32" in c:
+        tf = c["SQ_INSTS_VALU_FLOPS_FP32"] * 64 / avg_launch_s / 1e12
+        out["counted_fp32"] = {"TFLOPs": tf, "peak": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS}
+    if "traffic" in prof:
+        t = prof["traffic"]["bytes_per_launch"]
+        out["traffic"] = t
+        out["hbm"] = {"GBs": t / avg_launch_s / 1e9, "peak": HBM_PEAK_GBS, "frac": t / avg_launch_s / 1e9 / HBM_PEAK_GBS}
+    if "trace" in prof:
+        out["rocprof_trace_avg_ms"] = prof["trace"]["avg_ms"]
+    return out
 
 
 def main() -> int:
@@ -448,10 +617,16 @@ def main() -> int:
     ap.add_argument("--no-recon", action="store_true", help="skip the image reconstruction section")
     ap.add_argument("--no-sar", action="store_true", help="skip the local SAR section")
     ap.add_argument("--batch-sets", type=int, default=4096)
-    ap.add_argument("--batch-plain-sample", type=int, default=256)
+    ap.add_argument("--batch-plain-sample", type=int, default=0,
+                    help="plain per-scan calls timed for the batched speed-up (0: every scan)")
+    ap.add_argument("--write-profile", nargs="+", metavar=("OUT_JSON", "DB"),
+                    help="write the forward kernel's counter profile from rocprofv3 databases and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.write_profile:
+        write_profile(args.write_profile[0], args.write_profile[1:])
+        return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this engine runs the scan on one MI355X and has no "
@@ -502,11 +677,7 @@ def main() -> int:
     ab = algorithmic_bytes(spins, rx, info["n_adc"] / info["n_steps"])
     n_launch = info["n_launches"]
     avg_launch_s = fwd_ms / 1e3 / (args.steps * n_launch)
-    ab_per_launch = iso_steps / n_launch * ab
-    achieved = ab_per_launch / avg_launch_s / 1e9
-    traffic = traffic_from_profile()
     flop = algorithmic_flops(info["n_rf_steps"] / info["n_steps"], info["n_adc"] / info["n_steps"], rx)
-    tflops = iso_steps / n_launch * flop / avg_launch_s / 1e12
     line = {
         "metric": METRIC,
         "value": value,
@@ -524,15 +695,7 @@ def main() -> int:
                                "TR 5 ms = 500 x 10 us steps, 128x64 phase encodes x 128 readout",
                    "isochromats": info["n_iso"], "steps": info["n_steps"], "adc_samples": info["n_adc"],
                    "launches_per_step": n_launch, "precision_mode": "reference"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "convention": f"algorithmic bytes {ab:.2f} B per isochromat-step",
-                     "kernel": "bloch_fwd_kernel<4,8>", "avg_launch_ms": avg_launch_s * 1e3,
-                     "measured_hbm_GBs": None if traffic is None else traffic / avg_launch_s / 1e9,
-                     "valu": {"achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                              "frac": tflops / FP32_PEAK_TFLOPS,
-                              "frac_nonpacked": tflops / FP32_NONPACKED_PEAK_TFLOPS,
-                              "flop_per_isochromat_step": flop}},
+        "roofline": roofline(load_profile(), avg_launch_s, ab, iso_steps / n_launch, flop),
         "kernel_ms_per_step": {"bloch_fwd": fwd_ms / args.steps, "signal_reduce": red_ms / args.steps},
     }
     if not args.no_recon and args.tr is None:
@@ -547,7 +710,7 @@ def main() -> int:
         line["adjoint"] = adjoint_section(3)
     if not args.no_multipool:
         mp_threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-        line["multipool"] = multipool_section(0.2, 0.04, 64, mp_threads)
+        line["multipool"] = multipool_section(mp_threads)
     if not args.no_sar:
         sar_threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         line["sar"] = sar_section(sar_threads)

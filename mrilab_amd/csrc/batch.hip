@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <numeric>
 #include <string>
@@ -636,51 +637,99 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
         HIP_TRY(hipStreamSynchronize(st));
     }
     {
-        std::vector<uint32_t> words((size_t)std::max<int64_t>(recw, 2));
+        // the record streams, concatenated without zero-filling (every word
+        // is copied from a scan's stream), scans copied in parallel
+        std::unique_ptr<uint32_t[]> words(new (std::nothrow) uint32_t[(size_t)std::max<int64_t>(recw, 2)]);
+        if (!words) return fail(MRS_ERR_NOMEM, "host allocation of the record streams failed");
         std::vector<double> ref((size_t)std::max<int64_t>(adc, 1));
         std::vector<int32_t> ss((size_t)std::max<int64_t>(adc, 1));
         // relaxation factors of single-voxel scans per record (the factors
-        // depend only on dt and the voxel's T1, T2), as bloch_step rounds them
-        std::vector<float> rlx;
+        // depend only on dt and the voxel's T1, T2), as bloch_step rounds
+        // them; scans in parallel, each with a small cache keyed on dt (a
+        // train repeats a few step durations)
+        int64_t nrlx = 0;
         for (int i = 0; i < n; i++) {
             BatchScan &d = B->hscan[i];
             d.relax_off = -1;
             if (d.nvox != 1) continue;
-            d.relax_off = (int64_t)rlx.size() / 2;
+            d.relax_off = nrlx / 2;
+            nrlx += 2 * B->dec[i].n_rec();
+        }
+        std::vector<float> rlx((size_t)nrlx);
+        auto relax_scan = [&](int i) {
+            const BatchScan &d = B->hscan[i];
+            if (d.relax_off < 0) return;
             const float t1 = vobj[i]->T1[0], t2 = vobj[i]->T2[0];
             const std::vector<uint32_t> &w = B->dec[i].rec;
+            float *out = rlx.data() + 2 * d.relax_off;
+            constexpr int kCache = 8;
+            uint32_t key[kCache];
+            float v2[kCache], v1[kCache];
+            int nk = 0, next = 0;
             for (int64_t r = 0; r < B->dec[i].n_rec(); r++) {
-                float dt;
-                std::memcpy(&dt, &w[(size_t)r * 8], 4);
-                float f2, f1;
-                if (B->prec == 0) {  // BlochKernel.h:348-368 in fp32, exp rounded from fp64
-                    float a2 = (1.0f / t2) * dt;
-                    a2 = -a2;
-                    if (a2 < -200.f) a2 = -200.f;
-                    float a1 = (1.0f / t1) * dt;
-                    a1 = -a1;
-                    if (a1 < -200.f) a1 = -200.f;
-                    f2 = (float)std::exp((double)a2);
-                    f1 = (float)std::exp((double)a1);
-                } else {
-                    f2 = (float)(-std::expm1(-(double)dt / (double)t2));
-                    f1 = (float)(-std::expm1(-(double)dt / (double)t1));
+                const uint32_t kdt = w[(size_t)r * 8];
+                int hit = -1;
+                for (int c = 0; c < nk; c++)
+                    if (key[c] == kdt) hit = c;
+                if (hit < 0) {
+                    float dt;
+                    std::memcpy(&dt, &kdt, 4);
+                    float f2, f1;
+                    if (B->prec == 0) {  // BlochKernel.h:348-368 in fp32, exp rounded from fp64
+                        float a2 = (1.0f / t2) * dt;
+                        a2 = -a2;
+                        if (a2 < -200.f) a2 = -200.f;
+                        float a1 = (1.0f / t1) * dt;
+                        a1 = -a1;
+                        if (a1 < -200.f) a1 = -200.f;
+                        f2 = (float)std::exp((double)a2);
+                        f1 = (float)std::exp((double)a1);
+                    } else {
+                        f2 = (float)(-std::expm1(-(double)dt / (double)t2));
+                        f1 = (float)(-std::expm1(-(double)dt / (double)t1));
+                    }
+                    hit = next;
+                    key[hit] = kdt;
+                    v2[hit] = f2;
+                    v1[hit] = f1;
+                    next = (next + 1) % kCache;
+                    if (nk < kCache) nk++;
                 }
-                rlx.push_back(f2);
-                rlx.push_back(f1);
+                out[2 * r] = v2[hit];
+                out[2 * r + 1] = v1[hit];
             }
+        };
+        {
+            const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nth; t++)
+                pool.emplace_back([&, t] {
+                    for (int i = t; i < n; i += nth) relax_scan(i);
+                });
+            for (auto &th : pool) th.join();
         }
         if ((rc = bupload(B->relax, rlx, st))) return rc;
-        for (int i = 0; i < n; i++) {
-            const BatchScan &d = B->hscan[i];
-            std::copy(B->dec[i].rec.begin(), B->dec[i].rec.end(), words.begin() + d.rec_off);
-            std::copy(B->dec[i].adc_ref.begin(), B->dec[i].adc_ref.end(), ref.begin() + d.adc_off);
-            std::fill(ss.begin() + d.adc_off, ss.begin() + d.adc_off + B->dec[i].adc_ref.size(), i);
-            std::vector<uint32_t>().swap(B->dec[i].rec);  // keep only the host-side outputs
+        {
+            const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nth; t++)
+                pool.emplace_back([&, t] {
+                    for (int i = t; i < n; i += nth) {
+                        const BatchScan &d = B->hscan[i];
+                        std::copy(B->dec[i].rec.begin(), B->dec[i].rec.end(), words.get() + d.rec_off);
+                        std::copy(B->dec[i].adc_ref.begin(), B->dec[i].adc_ref.end(), ref.begin() + d.adc_off);
+                        std::fill(ss.begin() + d.adc_off, ss.begin() + d.adc_off + B->dec[i].adc_ref.size(), i);
+                    }
+                });
+            for (auto &th : pool) th.join();
         }
-        if ((rc = bupload(B->rec, words, st)) || (rc = bupload(B->adc_ref, ref, st)) ||
-            (rc = bupload(B->samp_scan, ss, st)))
-            return rc;
+        for (int i = 0; i < n; i++) std::vector<uint32_t>().swap(B->dec[i].rec);  // keep only the host-side outputs
+        {
+            const size_t bytes = (size_t)std::max<int64_t>(recw, 2) * sizeof(uint32_t);
+            HIP_TRY(hipMalloc(&B->rec.p, bytes));
+            HIP_TRY(hipMemcpyAsync(B->rec.p, words.get(), bytes, hipMemcpyHostToDevice, st));
+        }
+        if ((rc = bupload(B->adc_ref, ref, st)) || (rc = bupload(B->samp_scan, ss, st))) return rc;
         HIP_TRY(hipMalloc(&B->Sx.p, std::max<int64_t>(sig, 1) * sizeof(double)));
         HIP_TRY(hipMalloc(&B->Sy.p, std::max<int64_t>(sig, 1) * sizeof(double)));
         HIP_TRY(hipMemsetAsync(B->Sx.p, 0, std::max<int64_t>(sig, 1) * sizeof(double), st));

@@ -14,7 +14,7 @@ import pytest
 
 import oracle
 from mrilab_amd import engine, workloads
-from tests.conftest import rel_l2
+from tests.conftest import adc_capture_slots, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -158,3 +158,41 @@ def test_full_phantom_first_tr(gpu):
     print("  vs fp64 oracle: reference mode signal %.3e magnetization %.3e; accurate mode %.3e %.3e" % tuple(e),
           flush=True)
     assert max(e) < 1e-4
+
+
+def test_coil_sum_error_bound(gpu):
+    """Error of the in-block fp32 coil sum (the MFMA projection, 8 coils, 4
+    isochromats per thread, 256-thread blocks), measured against an fp64
+    sum of the very fp32 states the kernel summed (RunMode-1 capture).
+
+    Within a block the sum is plain fp32: 3 adds of a lane's isochromats, one
+    product rounding, at most 128 accumulations in the MFMA chain over the
+    wave's 64 voxels x 2 components, and 3 adds of the block's 4 waves in
+    LDS; across blocks the partials are added with Neumaier compensation in
+    fp64. So |S - S_exact| <= 135 u sum_terms |m| |C| (u = 2^-24, first
+    order), about 8.0e-6 of the sum of the terms' magnitudes."""
+    sc = workloads.bssfp3d(shape=(16, 16, 8), n_ro=16, spins=4, n_coils=8, n_tr=4, tr_steps=120,
+                           fov=(0.064, 0.064, 0.032))
+    rm = workloads.with_run_mode1(sc)
+    g = rm.copy()
+    engine.DoScanAtGPU(*g.structs)
+    slots = adc_capture_slots(sc.VSeq)
+    N = int(g.VSig["SignalNum"])
+    assert slots.size == N
+    C = (sc.VCoi["RxCoilx"].astype(np.float64) + 1j * sc.VCoi["RxCoily"].astype(np.float64))  # [R,C,Z,8]
+    s = (g.VSig["Sx"] + 1j * g.VSig["Sy"]).reshape(8, N)
+    mx, my = g.VSig["Mx"], g.VSig["My"]
+    worst = 0.0
+    for n, u in enumerate(slots):
+        m = mx[..., u].astype(np.float64) + 1j * my[..., u].astype(np.float64)  # [R,C,Z,S,1]
+        msum = m.sum(axis=(3, 4))
+        mabs = np.abs(m).sum(axis=(3, 4))
+        ref = np.pi * ((n // 16) % 2)
+        for c in range(8):
+            exact = np.sum(msum * np.conj(C[..., c])) * np.exp(-1j * ref)
+            scale = np.sum(mabs * np.abs(C[..., c]))
+            worst = max(worst, abs(s[c, n] - exact) / scale)
+    bound = 135 * 2.0 ** -24
+    print(f"coil sum: worst |S - S_exact| / sum|terms| = {worst:.2e} over {N} samples x 8 coils "
+          f"(bound {bound:.2e})", flush=True)
+    assert worst < bound
