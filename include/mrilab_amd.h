@@ -177,7 +177,9 @@ int32_t mrs_abi_version(void);
 /* Number of visible HIP devices, or a negative value on error. */
 int32_t mrs_device_count(void);
 
-/* Single-pool scan with DoScanAtCPU semantics, run on the GPU.
+/* Single-pool scan with DoScanAtCPU semantics, run on the GPU. Any number
+ * of receive coils: a kernel pass sums up to 32; with more, the scan's
+ * program runs once per group of 32 coils from the same state.
  * Replaces: mexFunction, Lib/src/engine/CPUEngine/DoScanAtCPU.cpp:70 */
 mrs_status mrs_DoScanAtCPU(mrs_vobj *vobj, const mrs_vmag *vmag,
                            const mrs_vcoi *vcoi, const mrs_vctl *vctl,
@@ -194,7 +196,9 @@ mrs_status mrs_DoScanAtGPU(mrs_vobj *vobj, const mrs_vmag *vmag,
                            const mrs_options *opt);
 
 /* Generalized multi-pool scan (free and bound pools with exchange, MT
- * saturation), TypeNum 2..6 (5 and 6 pools with up to 4 receive coils). VObj.K is [R,C,Z,TypeNum,TypeNum] with
+ * saturation), TypeNum 2..6, any number of receive coils (a kernel pass
+ * sums 8, or 4 with 5 or 6 pools; more coils re-run the scan per group of
+ * coils from the same state). VObj.K is [R,C,Z,TypeNum,TypeNum] with
  * K(:,:,:,a,b) the rate from pool a into pool b; VObj.TypeFlag is 0 for a
  * free pool and 1 for a bound pool. mrs_options.multipool_scheme picks the
  * reference's operator split (0) or the fp64 matrix exponential (1).
@@ -283,7 +287,8 @@ void mrs_plan_destroy(mrs_plan *plan);
  * exceed one launch's partial buffer), so the device is filled even when the
  * phantom alone is one wave per SIMD. options.model picks the engine (1 GM,
  * 2 MT, 3 CEST, 4 ME, VObj.K in that engine's layout) and
- * options.multipool_scheme the scheme; RunMode 0, no Ext callbacks. Each
+ * options.multipool_scheme the scheme; RunMode 0, no Ext callbacks, at
+ * most one kernel pass of receive coils (8, or 4 with 5 or 6 pools). Each
  * sequence's outputs are what the single-scan entry point writes for that
  * sequence alone. Run, sync, info and destroy with mrs_plan_*.
  * ------------------------------------------------------------------------- */

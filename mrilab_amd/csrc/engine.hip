@@ -267,7 +267,8 @@ template <int NV>
 __global__ __launch_bounds__(64) void signal_reduce_kernel(const float *__restrict__ partial, int nblk, int bstep,
                                                            int bcount, int vstride, int vstep, int Rx,
                                                            int rx_default, const double *__restrict__ adc_ref,
-                                                           int64_t adc0, int64_t N, double *Sx, double *Sy) {
+                                                           int64_t adc0, int64_t N, double *Sx, double *Sy,
+                                                           int c0, int rx_total) {
     __shared__ double tot[NV];
     const int a = blockIdx.x;
     const int t = blockIdx.y;
@@ -312,8 +313,8 @@ __global__ __launch_bounds__(64) void signal_reduce_kernel(const float *__restri
             re = r2;
             im = i2;
         }
-        Sx[(int64_t)t * Rx * N + (int64_t)c * N + n] += re;
-        Sy[(int64_t)t * Rx * N + (int64_t)c * N + n] += im;
+        Sx[(int64_t)t * rx_total * N + (int64_t)(c0 + c) * N + n] += re;
+        Sy[(int64_t)t * rx_total * N + (int64_t)(c0 + c) * N + n] += im;
     }
 }
 
@@ -382,10 +383,10 @@ static launch_fn pick_fwd(int spt, int nrx, bool gen, int prec, bool mfma) {
 
 void launch_signal_reduce(int nv, dim3 grid, hipStream_t st, const float *partial, int nblk, int bstep,
                           int bcount, int vstride, int vstep, int Rx, int rxd, const double *ref, int64_t adc0,
-                          int64_t N, double *Sx, double *Sy) {
+                          int64_t N, double *Sx, double *Sy, int c0, int rx_total) {
 #define MRS_RED(NV_)                                                                                         \
     signal_reduce_kernel<NV_><<<grid, 64, 0, st>>>(partial, nblk, bstep, bcount, vstride, vstep, Rx, rxd, ref, \
-                                                   adc0, N, Sx, Sy)
+                                                   adc0, N, Sx, Sy, c0, rx_total)
     switch (nv) {
         case 2: MRS_RED(2); break;
         case 4: MRS_RED(4); break;
@@ -412,13 +413,17 @@ struct mrs_plan {
     int64_t nvox = 0, ntot = 0;
     int rx_default = 1, precision = 0, run_mode = 0;
     int spt = 1, nrx = 1, block = 256, ngroups = 1, gx = 1, nblk = 1, nv = 2;
+    // receive coils per kernel pass and the number of passes: a scan with
+    // more coils than one kernel sums re-runs its program once per group
+    // of coils, from the same starting state (Msave)
+    int rx_grp = 1, n_rxg = 1;
     int64_t max_adc = 0;
     int64_t signal_num = 0;
     int64_t sig_from = 0;     // samples below it were already added into VSig (Ext callbacks)
     float gyro = 0.f;
     float cs[kMaxTypes] = {0};
     // device memory
-    DevBuf M0[3], M[3], dWRnd, Rho, T1, T2, dB0, gxg, gyg, gzg, txmg, txpe, rxx, rxy;
+    DevBuf M0[3], M[3], Msave[3], dWRnd, Rho, T1, T2, dB0, gxg, gyg, gzg, txmg, txpe, rxx, rxy;
     DevBuf rec, cap_slot, adc_ref, partial, Sx, Sy, cap[3];
     size_t rec_cap_words = 0, adc_cap = 0;
     int64_t n_uts = 0;
@@ -521,7 +526,6 @@ mrs_status mrs::check_args(const mrs_vobj *vo, const mrs_vmag *vm, const mrs_vco
         return fail(MRS_ERR_ARG, "VObj dims, SpinNum and TypeNum must be positive");
     if (vo->TypeNum > kMaxTypes) return fail(MRS_ERR_UNSUPPORTED, "TypeNum above 16");
     if (vc->TxCoilNum <= 0 || vc->RxCoilNum <= 0) return fail(MRS_ERR_ARG, "coil counts must be positive");
-    if (vc->RxCoilDefault == 0 && vc->RxCoilNum > 32) return fail(MRS_ERR_UNSUPPORTED, "more than 32 receive coils");
     if (!vt->CS) return fail(MRS_ERR_ARG, "VCtl.CS is NULL");
     if (!vs->utsLine || !vs->tsLine || !vs->flagsLine || !vs->rfAmpLine || !vs->rfPhaseLine || !vs->rfFreqLine ||
         !vs->rfCoilLine || !vs->GzAmpLine || !vs->GyAmpLine || !vs->GxAmpLine || !vs->ADCLine || !vs->ExtLine)
@@ -639,13 +643,18 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     const int gy = P->model >= 1 ? P->ngroups : P->Ty * P->ngroups;
     if (gy > 65535) return fail(MRS_ERR_UNSUPPORTED, "TypeNum*SpinNum/spins_per_thread too large");
     P->nblk = P->gx * gy;
-    int nrx = 1;
+    // coils per kernel pass: 32 for the single-pool kernel, 8 for the
+    // multi-pool kernel (4 with 5 or 6 pools, whose signal values per
+    // isochromat would exceed a wavefront)
     const int rxn = P->rx_default ? 1 : P->Rx;
-    while (nrx < rxn) nrx <<= 1;
+    P->rx_grp = std::min(rxn, P->model >= 1 ? (P->Ty <= 4 ? 8 : 4) : 32);
+    P->n_rxg = (rxn + P->rx_grp - 1) / P->rx_grp;
+    int nrx = 1;
+    while (nrx < P->rx_grp) nrx <<= 1;
     P->nrx = nrx;
     P->nv = P->model >= 1 ? gm_values_per_isochromat(P->Ty, nrx) : 2 * nrx;
-    if (P->model >= 1 && (nrx > 8 || !pick_gm(P->Ty, nrx, P->scheme, P->variant)))
-        return fail(MRS_ERR_UNSUPPORTED, "multi-pool engine supports up to 8 receive coils (4 with 5 or 6 pools)");
+    if (P->model >= 1 && !pick_gm(P->Ty, nrx, P->scheme, P->variant))
+        return fail(MRS_ERR_UNSUPPORTED, "no multi-pool kernel for this pool and coil count");
     const size_t budget = (size_t)512 << 20;  // partial-buffer budget
     int64_t max_adc = (int64_t)(budget / ((size_t)P->nblk * P->nv * sizeof(float)));
     max_adc = std::max<int64_t>(kRedBatch, std::min<int64_t>(max_adc, 65536));
@@ -782,42 +791,54 @@ static GmParams gm_params(mrs_plan *P) {
     return gp;
 }
 
-// Multi-pool launches: one thread per (voxel, isochromat), all pools.
-static mrs_status execute_gm(mrs_plan *P) {
+// Receive coils of pass g: the first coil and the count.
+static int group_c0(const mrs_plan *P, int g) { return g * P->rx_grp; }
+static int group_rx(const mrs_plan *P, int g) {
+    return P->rx_default ? P->Rx : std::min(P->rx_grp, P->Rx - g * P->rx_grp);
+}
+
+// Multi-pool launches: one thread per (voxel, isochromat), all pools;
+// receive coils of pass g.
+static mrs_status execute_gm(mrs_plan *P, int g) {
     GmParams gp = gm_params(P);
+    gp.rxx += (size_t)group_c0(P, g) * P->nvox;
+    gp.rxy += (size_t)group_c0(P, g) * P->nvox;
+    gp.Rx = group_rx(P, g);
     gm_launch_fn fn = pick_gm(P->Ty, P->nrx, P->scheme, P->variant);
     const dim3 grid(P->gx, P->ngroups), block(P->block);
     const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float);
     const uint32_t *rec = (const uint32_t *)P->rec.p;
     const size_t nl = P->launch_r0.size();
+    const size_t e0 = 3 * nl * (size_t)g;  // timing events of this pass
     if (P->timing) {
-        while (P->tev.size() < 3 * nl) {
+        while (P->tev.size() < e0 + 3 * nl) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
             P->tev.push_back(e);
         }
     }
     for (size_t l = 0; l < nl; l++) {
-        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l], P->stream));
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[e0 + 3 * l], P->stream));
         fn(grid, block, lds, P->stream, gp, rec, (int)P->launch_r0[l], (int)P->launch_r1[l]);
         HIP_TRY(hipGetLastError());
-        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 1], P->stream));
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[e0 + 3 * l + 1], P->stream));
         const int64_t na = P->launch_nadc[l];
         if (na > 0) {
             launch_signal_reduce(2 * P->nrx, dim3((unsigned)na, P->Ty), P->stream, (const float *)P->partial.p,
-                                 P->nblk, 0, P->nblk, P->nv, 2 * P->nrx, P->Rx, P->rx_default,
+                                 P->nblk, 0, P->nblk, P->nv, 2 * P->nrx, gp.Rx, P->rx_default,
                                  (const double *)P->adc_ref.p, P->launch_adc0[l], P->signal_num, (double *)P->Sx.p,
-                                 (double *)P->Sy.p);
+                                 (double *)P->Sy.p, group_c0(P, g), P->Rx);
             HIP_TRY(hipGetLastError());
         }
-        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 2], P->stream));
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[e0 + 3 * l + 2], P->stream));
     }
     return MRS_OK;
 }
 
-// Runs every launch of the uploaded program on the plan stream.
-// adc_base: index of the program's first ADC sample in the scan.
-static mrs_status execute(mrs_plan *P, int64_t adc_base) {
+// Runs every launch of the uploaded program on the plan stream for the
+// receive coils of pass g. adc_base: index of the program's first ADC
+// sample in the scan.
+static mrs_status execute_group(mrs_plan *P, int64_t adc_base, int g) {
     FwdParams fp;
     std::memset(&fp, 0, sizeof fp);
     fp.Mx = (float *)P->M[0].p;
@@ -833,8 +854,8 @@ static mrs_status execute(mrs_plan *P, int64_t adc_base) {
     fp.gz = (const float *)P->gzg.p;
     fp.txmg = (const float *)P->txmg.p;
     fp.txpe = (const float *)P->txpe.p;
-    fp.rxx = (const float *)P->rxx.p;
-    fp.rxy = (const float *)P->rxy.p;
+    fp.rxx = (const float *)P->rxx.p + (size_t)group_c0(P, g) * P->nvox;
+    fp.rxy = (const float *)P->rxy.p + (size_t)group_c0(P, g) * P->nvox;
     fp.partial = (float *)P->partial.p;
     fp.capx = (float *)P->cap[0].p;
     fp.capy = (float *)P->cap[1].p;
@@ -844,14 +865,14 @@ static mrs_status execute(mrs_plan *P, int64_t adc_base) {
     fp.S = P->S;
     fp.Ty = P->Ty;
     fp.Tx = P->Tx;
-    fp.Rx = P->Rx;
+    fp.Rx = group_rx(P, g);
     fp.ngroups = P->ngroups;
     fp.rx_default = P->rx_default;
     fp.precision = P->precision;
     fp.stride = P->dec.stride;
     fp.gyro = P->gyro;
     for (int t = 0; t < P->Ty; t++) fp.cs[t] = P->cs[t];
-    if (P->model >= 1) return execute_gm(P);
+    if (P->model >= 1) return execute_gm(P, g);
     const bool mfma = use_mfma(P->nrx, P->coil_sum);
     launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1, P->precision != 0, mfma);
     const dim3 grid(P->gx, P->Ty * P->ngroups), block(P->block);
@@ -859,29 +880,49 @@ static mrs_status execute(mrs_plan *P, int64_t adc_base) {
                        (mfma ? (size_t)(P->block / 64) * kMfmaWaveFloat2 * sizeof(float2) : 0);
     const uint32_t *rec = (const uint32_t *)P->rec.p;
     const size_t nl = P->launch_r0.size();
+    const size_t e0 = 3 * nl * (size_t)g;  // timing events of this pass
     if (P->timing) {
-        while (P->tev.size() < 3 * nl) {
+        while (P->tev.size() < e0 + 3 * nl) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
             P->tev.push_back(e);
         }
     }
     for (size_t l = 0; l < nl; l++) {
-        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l], P->stream));
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[e0 + 3 * l], P->stream));
         fn(grid, block, lds, P->stream, fp, rec, (int)P->launch_r0[l], (int)P->launch_r1[l]);
         HIP_TRY(hipGetLastError());
-        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 1], P->stream));
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[e0 + 3 * l + 1], P->stream));
         const int64_t na = P->launch_nadc[l];
         if (na > 0) {
             launch_signal_reduce(P->nv, dim3((unsigned)na, P->Ty), P->stream, (const float *)P->partial.p, P->nblk,
-                                 P->gx * P->ngroups, P->gx * P->ngroups, P->nv, 0, P->Rx, P->rx_default,
+                                 P->gx * P->ngroups, P->gx * P->ngroups, P->nv, 0, fp.Rx, P->rx_default,
                                  (const double *)P->adc_ref.p, P->launch_adc0[l], P->signal_num, (double *)P->Sx.p,
-                                 (double *)P->Sy.p);
+                                 (double *)P->Sy.p, group_c0(P, g), P->Rx);
             HIP_TRY(hipGetLastError());
         }
-        if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * l + 2], P->stream));
+        if (P->timing) HIP_TRY(hipEventRecord(P->tev[e0 + 3 * l + 2], P->stream));
     }
     (void)adc_base;
+    return MRS_OK;
+}
+
+// Runs the uploaded program: once, or once per group of receive coils, each
+// pass from the state the program starts from.
+static mrs_status execute(mrs_plan *P, int64_t adc_base) {
+    if (P->n_rxg == 1) return execute_group(P, adc_base, 0);
+    const size_t bytes = (size_t)P->ntot * sizeof(float);
+    for (int k = 0; k < 3; k++) {
+        if (!P->Msave[k].p) HIP_TRY(hipMalloc(&P->Msave[k].p, bytes));
+        HIP_TRY(hipMemcpyAsync(P->Msave[k].p, P->M[k].p, bytes, hipMemcpyDeviceToDevice, P->stream));
+    }
+    mrs_status rc;
+    for (int g = 0; g < P->n_rxg; g++) {
+        if (g > 0)
+            for (int k = 0; k < 3; k++)
+                HIP_TRY(hipMemcpyAsync(P->M[k].p, P->Msave[k].p, bytes, hipMemcpyDeviceToDevice, P->stream));
+        if ((rc = execute_group(P, adc_base, g))) return rc;
+    }
     return MRS_OK;
 }
 
@@ -950,7 +991,8 @@ static mrs_status run_batch(mrs_plan *P) {
                                  (const float *)P->partial.p + (int64_t)P->seq_slot[q] * nblk * P->nv, nblk, 0, nblk,
                                  P->nv, 2 * P->nrx, P->Rx, P->rx_default,
                                  (const double *)P->adc_ref.p + P->seq_adc_off[q], 0, P->seq_signum[q],
-                                 (double *)P->Sx.p + P->seq_sig_off[q], (double *)P->Sy.p + P->seq_sig_off[q]);
+                                 (double *)P->Sx.p + P->seq_sig_off[q], (double *)P->Sy.p + P->seq_sig_off[q], 0,
+                                 P->Rx);
             HIP_TRY(hipGetLastError());
         }
         if (P->timing) HIP_TRY(hipEventRecord(P->tev[3 * g + 2], st));
@@ -1087,7 +1129,7 @@ mrs_status mrs_plan_sync(mrs_plan *P) {
         if (hipEventElapsedTime(&ms, P->ev0, P->ev1) == hipSuccess) P->last_ms = ms;
         if (P->timing) {
             double f = 0.0, r = 0.0;
-            const size_t nl = P->nseq ? P->grp_begin.size() : P->launch_r0.size();
+            const size_t nl = P->nseq ? P->grp_begin.size() : P->launch_r0.size() * (size_t)P->n_rxg;
             for (size_t l = 0; l < nl && 3 * l + 2 < P->tev.size(); l++) {
                 float a = 0.f, b = 0.f;
                 if (hipEventElapsedTime(&a, P->tev[3 * l], P->tev[3 * l + 1]) == hipSuccess) f += a;
@@ -1208,6 +1250,9 @@ mrs_status mrs_multipool_batch_create(int32_t n_seq, const mrs_vobj *vobj, const
         }
     } guard{P};
     if ((rc = plan_setup(P, vobj, vmag, vcoi, vctl, vseq[0], vsig[0], opt))) return rc;
+    if (P->n_rxg > 1)
+        return fail(MRS_ERR_UNSUPPORTED, "batched multi-pool sequences take up to " + std::to_string(P->rx_grp) +
+                                             " receive coils");
     P->nseq = n_seq;
     P->dec = Decoded();
     P->dec.tx = P->Tx;

@@ -137,11 +137,14 @@ def test_five_and_six_pools(gpu, name, variant, scheme, idx, t1s, coils):
     assert e_s < 1e-4 and e_m < 1e-4
 
 
-def test_five_pools_eight_coils_is_refused(gpu):
-    base = workloads.mt_cest(shape=(3, 3, 1), n_offsets=1, sat_time=0.02, pulse=0.01, gap=0.0, n_coils=8)
-    sc = pools(base, [0, 1, 1, 2, 2])
-    with pytest.raises(engine.MRSError, match="4 with 5 or 6 pools"):
-        engine.DoGMScanAtGPU(*sc.copy().structs)
+def test_five_pools_eight_coils_batched_is_refused(gpu):
+    """A kernel pass of 5 or 6 pools sums 4 receive coils; the single-scan
+    entry points run more coils in passes, the batched sequences refuse."""
+    scans = workloads.mt_cest(shape=(3, 3, 1), n_offsets=2, sat_time=0.02, pulse=0.01, gap=0.0, n_coils=8,
+                              per_offset=True)
+    five = [pools(s, [0, 1, 1, 2, 2]) for s in scans]
+    with pytest.raises(engine.MRSError, match="up to 4 receive coils"):
+        engine.MultiPoolBatch(five)
 
 
 @pytest.mark.parametrize("scheme", [0, 1])
