@@ -194,7 +194,7 @@ def adjoint_section(reps: int) -> dict:
             "spins_per_thread": info["spins_per_thread"]}
 
 
-def multipool_section(threads: int, sat_time: float = 2.0) -> dict:
+def multipool_section(threads: int, sat_time: float = 2.0, extras: bool = True) -> dict:
     """Multi-pool exchange with MT saturation, the full configuration: the
     256x256 3-pool phantom (free water, myelin water, bound pool with a
     super-Lorentzian line) under 40 saturation offsets over +-50 ppm at 3 T,
@@ -232,6 +232,8 @@ def multipool_section(threads: int, sat_time: float = 2.0) -> dict:
                      "launches": info["n_launches"], "voxel_steps_per_s": vs / el}
         log(f"multipool {name}: {info['n_steps']} steps over {n_off} offsets in {el:.2f} s "
             f"({vs / el:.3e} voxel-steps/s, {info['n_launches']} launches)")
+    if not extras:
+        return out
     # parity on a crop over the full train, 4 offsets
     idx = [0, 13, 26, 39]
     crop = workloads.mt_cest(sat_time=sat_time, per_offset=True, crop=(slice(100, 102), slice(60, 62)),
@@ -616,6 +618,10 @@ def main() -> int:
     ap.add_argument("--no-multipool", action="store_true", help="skip the multi-pool section")
     ap.add_argument("--no-recon", action="store_true", help="skip the image reconstruction section")
     ap.add_argument("--no-sar", action="store_true", help="skip the local SAR section")
+    ap.add_argument("--multipool-sat", type=float, default=2.0,
+                    help="saturation train per offset of the multi-pool section, seconds")
+    ap.add_argument("--multipool-timing-only", action="store_true",
+                    help="multi-pool section: only the batched runs of both schemes")
     ap.add_argument("--batch-sets", type=int, default=4096)
     ap.add_argument("--batch-plain-sample", type=int, default=0,
                     help="plain per-scan calls timed for the batched speed-up (0: every scan)")
@@ -710,7 +716,7 @@ def main() -> int:
         line["adjoint"] = adjoint_section(3)
     if not args.no_multipool:
         mp_threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-        line["multipool"] = multipool_section(mp_threads)
+        line["multipool"] = multipool_section(mp_threads, args.multipool_sat, not args.multipool_timing_only)
     if not args.no_sar:
         sar_threads = min(64, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         line["sar"] = sar_section(sar_threads)
