@@ -57,6 +57,15 @@ __device__ T superlorentzian(double dw, double t2b) {
 
 // longest Taylor series of one exact substep (h*|A| <= 4 needs about 35 terms)
 constexpr int kTaylorMax = 48;
+// 1/k for the Taylor coefficients h/k (a uniform index: read through the
+// scalar cache instead of a division per term)
+__constant__ const double kInvK[kTaylorMax + 1] = {
+    0.0, 1.0 / 1, 1.0 / 2, 1.0 / 3, 1.0 / 4, 1.0 / 5, 1.0 / 6, 1.0 / 7, 1.0 / 8, 1.0 / 9, 1.0 / 10,
+    1.0 / 11, 1.0 / 12, 1.0 / 13, 1.0 / 14, 1.0 / 15, 1.0 / 16, 1.0 / 17, 1.0 / 18, 1.0 / 19,
+    1.0 / 20, 1.0 / 21, 1.0 / 22, 1.0 / 23, 1.0 / 24, 1.0 / 25, 1.0 / 26, 1.0 / 27, 1.0 / 28,
+    1.0 / 29, 1.0 / 30, 1.0 / 31, 1.0 / 32, 1.0 / 33, 1.0 / 34, 1.0 / 35, 1.0 / 36, 1.0 / 37,
+    1.0 / 38, 1.0 / 39, 1.0 / 40, 1.0 / 41, 1.0 / 42, 1.0 / 43, 1.0 / 44, 1.0 / 45, 1.0 / 46,
+    1.0 / 47, 1.0 / 48};
 
 // signal values per isochromat and ADC sample: 2*nrx per free pool, padded
 // to a power of two
@@ -145,6 +154,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
 #pragma unroll
         for (int t = 1; t < TY; t++) skip[t] = skip[0];
     }
+    // exact scheme: the constant parts of each pool's rows of the generator
+    // (BlochKernelGM.cuh's rates, exchange K, T1/T2), arranged so that a
+    // Taylor term is the same straight-line code for every pool: a skipped
+    // pool has all-zero rows (its state stays as it is), a bound pool zero
+    // transverse rows, and MT/CEST water exchanges no transverse
+    // magnetization with the bound pool (xm)
+    using ET = typename std::conditional<SCHEME == 1, double, float>::type;
+    ET kz[TY][TY];   // kz[t][q]: inflow rate q -> t into Mz of t
+    ET kt[TY][TY];   // the same into Mx, My of t (zero unless t is a free pool)
+    ET dxc[TY], dzc[TY], affc[TY], fm[TY], bm[TY], xm[TY], rsum[TY];
+    if constexpr (SCHEME == 1) {
+#pragma unroll
+        for (int t = 0; t < TY; t++) {
+            const bool fr = p.flag[t] == 0;
+            const bool on = !skip[t];
+            double ko = 0.0, kox = 0.0, kin = 0.0;
+#pragma unroll
+            for (int q = 0; q < TY; q++) {
+                if (q == t) continue;  // self-exchange cancels exactly
+                ko += kmat[t][q];
+                if (!(wcv && p.flag[q] != 0)) kox += kmat[t][q];  // MT/CEST: no transverse loss to bound
+                kin += kmat[q][t];
+            }
+            const double r1 = 1.0 / (double)t1[t];
+            const double r2 = fr ? 1.0 / (double)t2[t] : 0.0;
+#pragma unroll
+            for (int q = 0; q < TY; q++) {
+                kz[t][q] = (on && q != t) ? (double)kmat[q][t] : 0.0;
+                kt[t][q] = fr ? kz[t][q] : 0.0;
+            }
+            dxc[t] = (on && fr) ? r2 + kox : 0.0;
+            dzc[t] = on ? r1 + ko : 0.0;
+            affc[t] = on ? r1 * (double)rho[t] / p.S : 0.0;
+            fm[t] = (on && fr) ? 1.0 : 0.0;
+            bm[t] = (on && !fr) ? 1.0 : 0.0;
+            xm[t] = (wcv && !fr) ? 0.0 : 1.0;
+            rsum[t] = on ? r1 + r2 + ko + kin : 0.0;
+        }
+    }
+    (void)kz;
+    (void)kt;
+    (void)dxc;
+    (void)dzc;
+    (void)affc;
+    (void)fm;
+    (void)bm;
+    (void)xm;
+    (void)rsum;
     float crx[NRX], cry[NRX];
 #pragma unroll
     for (int c = 0; c < NRX; c++) {
@@ -449,41 +506,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                 st.mz[1] = bzb * e1[1] + m0[1] + Bz[0] * kout[1];
             }
         } else {
-            // exact step: x <- exp(A*dt) x with the affine M0 term
-            double om[TY], r1d[TY], r2d[TY], kod[TY], kodx[TY], aff[TY];
+            // exact step: x <- exp(A*dt) x with the affine M0 term; per step
+            // the rotation of each free pool (zero for others) and the bound
+            // pools' saturation
+            double omt[TY], bxt[TY], byt[TY], dzs[TY];
             double nrm = 0.0;
-            const double bxd = bx, byd = by;
 #pragma unroll
             for (int t = 0; t < TY; t++) {
-                om[t] = 0;
-                r1d[t] = 0;
-                r2d[t] = 0;
-                kod[t] = 0;
-                kodx[t] = 0;
-                aff[t] = 0;
-                if (skip[t]) continue;
-                double ko = 0.0, kox = 0.0, kin = 0.0;
-#pragma unroll
-                for (int qq = 0; qq < TY; qq++) {
-                    if (qq == t) continue;  // self-exchange cancels exactly
-                    ko += kmat[t][qq];
-                    if (!(wcv && p.flag[qq] == 1)) kox += kmat[t][qq];  // MT/CEST: no transverse loss to bound
-                    kin += kmat[qq][t];
-                }
-                kod[t] = ko;
-                kodx[t] = kox;
-                r1d[t] = 1.0 / (double)t1[t];
-                r2d[t] = 1.0 / (double)t2[t];
-                aff[t] = r1d[t] * (double)rho[t] / p.S;
-                double rowmax;
-                if (p.flag[t] == 0) {
-                    om[t] = (double)wb[t] + (double)gv + (rf ? (double)wrf : 0.0);
-                    rowmax = fabs(om[t]) + fabs(bxd) + fabs(byd) + r1d[t] + r2d[t] + ko + kin;
-                } else {
-                    r2d[t] = 0.0;
-                    rowmax = (double)wsat[t] + r1d[t] + ko + kin;
-                }
-                nrm = fmax(nrm, rowmax);
+                omt[t] = fm[t] * ((double)wb[t] + (double)gv + (rf ? (double)wrf : 0.0));
+                bxt[t] = fm[t] * (double)bx;
+                byt[t] = fm[t] * (double)by;
+                dzs[t] = dzc[t] + bm[t] * (double)wsat[t];
+                // the row sum of pool t's rows bounds |A| (max row sum)
+                nrm = fmax(nrm, rsum[t] + fabs(omt[t]) + fabs(bxt[t]) + fabs(byt[t]) + bm[t] * (double)wsat[t]);
             }
             // substeps of h*|A| <= 4; the series runs until every active lane's
             // term is below 1e-13 of its state (at most kTaylorMax terms): the
@@ -496,33 +531,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
             const double hn = h * nrm;
             double amax = 0.0;
 #pragma unroll
-            for (int t = 0; t < TY; t++) amax = fmax(amax, skip[t] ? 0.0 : fabs(aff[t]));
-            auto apply = [&](const double *in, double *out) {
+            for (int t = 0; t < TY; t++) amax = fmax(amax, fabs(affc[t]));
+            auto apply = [&](const double *in, double *out) __attribute__((always_inline)) {
+                double ixm[TY], iym[TY];
+#pragma unroll
+                for (int q = 0; q < TY; q++) {
+                    ixm[q] = in[3 * q] * xm[q];
+                    iym[q] = in[3 * q + 1] * xm[q];
+                }
 #pragma unroll
                 for (int t = 0; t < TY; t++) {
-                    double sx = 0, sy = 0, sz = 0;
-#pragma unroll
-                    for (int qq = 0; qq < TY; qq++) {
-                        if (qq == t) continue;
-                        const double kk = kmat[qq][t];
-                        if (!(wcv && p.flag[qq] == 1)) {
-                            sx += kk * in[3 * qq];
-                            sy += kk * in[3 * qq + 1];
-                        }
-                        sz += kk * in[3 * qq + 2];
-                    }
                     const double ix = in[3 * t], iy = in[3 * t + 1], iz = in[3 * t + 2];
-                    if (skip[t]) {
-                        out[3 * t] = out[3 * t + 1] = out[3 * t + 2] = 0.0;
-                    } else if (p.flag[t] == 0) {
-                        out[3 * t] = om[t] * iy - byd * iz - (r2d[t] + kodx[t]) * ix + sx;
-                        out[3 * t + 1] = bxd * iz - om[t] * ix - (r2d[t] + kodx[t]) * iy + sy;
-                        out[3 * t + 2] = byd * ix - bxd * iy - (r1d[t] + kod[t]) * iz + sz;
-                    } else {
-                        out[3 * t] = 0.0;
-                        out[3 * t + 1] = 0.0;
-                        out[3 * t + 2] = -((double)wsat[t] + r1d[t] + kod[t]) * iz + sz;
+                    double ox = omt[t] * iy - byt[t] * iz - dxc[t] * ix;
+                    double oy = bxt[t] * iz - omt[t] * ix - dxc[t] * iy;
+                    double oz = byt[t] * ix - bxt[t] * iy - dzs[t] * iz;
+#pragma unroll
+                    for (int q = 0; q < TY; q++) {
+                        if (q == t) continue;
+                        ox += kt[t][q] * ixm[q];
+                        oy += kt[t][q] * iym[q];
+                        oz += kz[t][q] * in[3 * q + 2];
                     }
+                    out[3 * t] = ox;
+                    out[3 * t + 1] = oy;
+                    out[3 * t + 2] = oz;
                 }
             };
             // x <- exp(A*dt) x + (exp(A*dt) - I) A^-1 b, substep by substep:
@@ -536,7 +568,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                     const double tol = 1e-13 * (xmax + h * amax);
                     apply(xv, nt);
 #pragma unroll
-                    for (int t = 0; t < TY; t++) nt[3 * t + 2] += skip[t] ? 0.0 : aff[t];
+                    for (int t = 0; t < TY; t++) nt[3 * t + 2] += affc[t];
 #pragma unroll
                     for (int i = 0; i < 3 * TY; i++) {
                         term[i] = h * nt[i];
@@ -544,7 +576,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                     }
                     for (int kk = 2; kk <= kTaylorMax; kk++) {
                         apply(term, nt);
-                        const double f = h / kk;
+                        const double f = h * kInvK[kk];
                         double tm = 0.0;
 #pragma unroll
                         for (int i = 0; i < 3 * TY; i++) {
