@@ -351,13 +351,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                                float &az) __attribute__((always_inline)) {
             const float w = rf ? (wbt + gv) + wrf : wbt + gv;
             if (rf) {
+                // applied once: the hardware reciprocal square root and the
+                // 1.5 ulp fp32 sincos (as the single-pool rf step); below
+                // |Omega| = 1e-15 rad/s the rotation is skipped
                 const float om2 = (bx * bx + by * by) + w * w;
-                if (om2 > 0.f) {
-                    const float om = sqrtf(om2);
-                    const float inv = 1.0f / om;
+                if (om2 > 1e-30f) {
+                    const float inv = __builtin_amdgcn_rsqf(om2);
+                    const float om = om2 * inv;
                     const float nx = bx * inv, ny = by * inv, nz = w * inv;
                     float sn, cs;
-                    sincos_cr(om * dt, &sn, &cs);
+                    sincos_fast(om * dt, &sn, &cs);
                     const float kk = (1.f - cs) * (nx * ax + ny * ay + nz * az);
                     const float rx = ax * cs - (ny * az - nz * ay) * sn + nx * kk;
                     const float ry = ay * cs - (nz * ax - nx * az) * sn + ny * kk;
