@@ -224,13 +224,46 @@ void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, i
         // an ADC run is this step and then (sample, step) pairs with the
         // cached rotation; every other record is one pass of this loop
         const bool run = (flags & REC_ADCRUN) != 0;
-        const int npass = run ? 1 + (int)(flags >> REC_REP_SHIFT) : 1;
-        for (int i = 0; i < npass; i++) {
-            if (flags & REC_ADC) adc_sample();
-            if (i > 0)
+        if (flags & REC_ADC) adc_sample();
+        if (!(flags & REC_NOSTEP))
+            bloch_step<SPT, GEN, PREC>(st, k, q, run ? flags & ((1u << REC_REP_SHIFT) - 1) : flags, txm);
+        if (run) {
+            int left = (int)(flags >> REC_REP_SHIFT);  // (sample, step) pairs after this step
+            if constexpr (MF) {
+                // whole batches of staged samples: the sample counter, the
+                // projection and the flush test once per batch
+                while (left > 0 && (slot & (kMfmaRows - 1)) != 0) {
+                    adc_sample();
+                    free_step<SPT, PREC>(st, k.m0s);
+                    left--;
+                }
+                for (; left >= kMfmaRows; left -= kMfmaRows) {
+#pragma unroll
+                    for (int j = 0; j < kMfmaRows; j++) {
+                        float sx = st.mx[0], sy = st.my[0];
+#pragma unroll
+                        for (int s = 1; s < SPT; s++) {
+                            sx += st.mx[s];
+                            sy += st.my[s];
+                        }
+                        msum[lane * kMfmaPad + j] = make_float2(sx, sy);
+                        free_step<SPT, PREC>(st, k.m0s);
+                    }
+                    slot += kMfmaRows;
+                    mfma_project(slot - kMfmaRows, kMfmaRows);
+                    if (slot == kRedBatch) {
+                        __syncthreads();
+                        flush_partials(red, kRedBatch, nwaves, NV, p.partial, adc_batch0, nblk, blk, NV);
+                        __syncthreads();
+                        adc_batch0 += kRedBatch;
+                        slot = 0;
+                    }
+                }
+            }
+            for (; left > 0; left--) {
+                adc_sample();
                 free_step<SPT, PREC>(st, k.m0s);
-            else if (!(flags & REC_NOSTEP))
-                bloch_step<SPT, GEN, PREC>(st, k, q, run ? flags & ((1u << REC_REP_SHIFT) - 1) : flags, txm);
+            }
         }
         if (GEN && (flags & REC_CAPTURE) && valid) {  // DoScanAtCPU.cpp:544-552
             const int64_t base = (int64_t)p.cap_slot[r] * nvox * p.S * p.Ty;
