@@ -138,20 +138,26 @@ __device__ __forceinline__ float relax_mz_ref(float mz, float e1, float m0e) {
 // Block-level flush of buffered per-wave ADC values: sums the waves of the
 // block in LDS order and writes one partial per (sample, value).
 // MFMA coil projection (forward kernel, 4 or 8 receive coils). Each wave
-// stages its lanes' per-sample spin sums (sum Mx, sum My) in LDS, kMfmaRows
-// samples at a time, stored voxel-major with a padded row of kMfmaPad
-// float2 (conflict-free writes across lanes and reads along samples), next
-// to its 64 voxels' coil maps (kMfmaCoils float2 (Cx, Cy) per voxel). Every
-// kMfmaRows samples one 16x16x4 f32 MFMA chain forms
-//   D[sample][2c]   = sum_v (sx Cx_vc + sy Cy_vc)
-//   D[sample][2c+1] = sum_v (sy Cx_vc - sx Cy_vc)
-// with K running over (voxel, component): 32 MFMAs reduce the wave's 64
-// voxels for 8 samples and 8 coils (rows 8..15 of the 16-row tile repeat
-// rows 0..7 and are dropped).
+// stages its lanes' per-sample spin sums in LDS, kMfmaRows samples at a
+// time: row 2s holds sum Mx and row 2s+1 sum My of staged sample s, one
+// float per voxel, rows kMfmaStride floats apart (the 4-float pad keeps the
+// 16-byte row reads of 8 lanes on distinct banks). Every kMfmaRows samples
+// one chain of 16 16x16x4 f32 MFMAs, K running over the wave's 64 voxels,
+// forms the real products
+//   P[2s+p][2c+q] = sum_v m_p(s, v) C_q(v, c),  m_0 = sum Mx, m_1 = sum My,
+//                                              C_0 = Cx, C_1 = Cy
+// with the B operand (the coil maps) stored in LDS in the order the MFMA
+// steps read it. Every one of the 256 products is used: lane (g, j) holds
+// rows 4g..4g+3 of column j, and one exchange with the neighbouring lane
+// gives
+//   Re_c(s) = P[2s][2c] + P[2s+1][2c+1] = sum_v (sx Cx + sy Cy)
+//   Im_c(s) = P[2s+1][2c] - P[2s][2c+1] = sum_v (sy Cx - sx Cy)
+// in column 2c (Re) and 2c+1 (Im), samples 2g and 2g+1.
 constexpr int kMfmaRows = 8;
-constexpr int kMfmaPad = kMfmaRows + 1;
+constexpr int kMfmaStride = 64 + 4;
 constexpr int kMfmaCoils = 8;
-constexpr int kMfmaWaveFloat2 = 64 * kMfmaPad + 64 * kMfmaCoils;  // per wave: samples, then coil maps
+// per wave: 2 kMfmaRows staged rows, then 16 x 64 B-operand floats
+constexpr int kMfmaWaveFloats = 2 * kMfmaRows * kMfmaStride + 16 * 64;
 
 typedef float mfma_v4f __attribute__((ext_vector_type(4)));
 
@@ -160,6 +166,7 @@ __device__ __forceinline__ void flush_partials(const float *red, int nslot, int 
     for (int i = threadIdx.x; i < nslot * nv; i += blockDim.x) {
         const int sl = i / nv, j = i - sl * nv;
         float acc = 0.f;
+#pragma unroll 4
         for (int w = 0; w < nwaves; w++) acc += red[(sl * nwaves + w) * nv + j];
         partial[((adc0 + sl) * nblk + blk) * vstride + j] = acc;
     }

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPT <= 4 ? 
 void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, int r1) {
     constexpr int NV = 2 * NRX;
     constexpr bool MF = MFC && (NRX == 4 || NRX == 8);
-    extern __shared__ float red[];  // [kRedBatch][nwaves][NV], then (MF) kMfmaWaveFloat2 float2 per wave
+    extern __shared__ float red[];  // [kRedBatch][nwaves][NV], then (MF) kMfmaWaveFloats floats per wave
     const crec_t rec = (crec_t)rec_g;
     const int stride = GEN ? p.stride : 8;
     const int lane = threadIdx.x & 63;
@@ -128,12 +128,19 @@ void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, i
 #pragma unroll
         for (int c = 0; c < NRX; c++) crx[c] = cry[c] = 0.f;
     }
-    float2 *const msum = MF ? (float2 *)(red + kRedBatch * nwaves * NV) + wave * kMfmaWaveFloat2 : nullptr;
-    float2 *const mcoil = MF ? msum + 64 * kMfmaPad : nullptr;
+    float *const mstage = MF ? red + kRedBatch * nwaves * NV + wave * kMfmaWaveFloats : nullptr;
+    float *const mcoil = MF ? mstage + 2 * kMfmaRows * kMfmaStride : nullptr;
     if constexpr (MF) {
+        // B operand: MFMA step m of lane (g, j) = 16 g + j reads the map of
+        // voxel 16 g + m for column j (Cx of coil j/2 for even j, Cy for odd
+        // j), at float ((m/4) * 64 + lane) * 4 + m % 4; this lane is voxel
+        // 16 g + m with g = lane / 16, m = lane % 16
+        const int g = lane >> 4, m = lane & 15;
 #pragma unroll
-        for (int c = 0; c < kMfmaCoils; c++)
-            mcoil[lane * kMfmaCoils + c] = c < NRX ? make_float2(crx[c], cry[c]) : make_float2(0.f, 0.f);
+        for (int j = 0; j < 16; j++) {
+            const int c = j >> 1;
+            mcoil[((m >> 2) * 64 + 16 * g + j) * 4 + (m & 3)] = c < NRX ? ((j & 1) ? cry[c] : crx[c]) : 0.f;
+        }
     }
     k.wtx_r = k.wtx_i = 0.f;
     if (!GEN || p.Tx == 1) {
@@ -154,23 +161,27 @@ void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, i
             // this wave's LDS writes precede its reads (LDS is in order per wave)
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            const int g = lane >> 4, j = lane & 15, srow = lane & (kMfmaRows - 1), c = j >> 1;
-            const bool im = (j & 1) != 0;
+            const int g = lane >> 4, j = lane & 15;
+            // A: row j of the staged samples, voxels 16 g .. 16 g + 15
+            const float4 *const arow = (const float4 *)(mstage + j * kMfmaStride + 16 * g);
+            const float4 *const brow = (const float4 *)(mcoil + lane * 4);
             mfma_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-            for (int kk = 0; kk < 16; kk++) {
-                const int vox = 4 * kk + g;
-                const float2 a = msum[vox * kMfmaPad + srow];
-                const float2 w = mcoil[vox * kMfmaCoils + c];
-                const float b0 = im ? -w.y : w.x, b1 = im ? w.x : w.y;
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b0, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b1, acc, 0, 0, 0);
-            }
-            // lane holds D[4g + i][j], i = 0..3
-            if (g < kMfmaRows / 4 && j < NV) {
 #pragma unroll
-                for (int i = 0; i < 4; i++)
-                    if (4 * g + i < nrows) red[((row0 + 4 * g + i) * nwaves + wave) * NV + j] = acc[i];
+            for (int qq = 0; qq < 4; qq++) {
+                const float4 a = arow[qq], b = brow[qq * 64];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, acc, 0, 0, 0);
+            }
+            // lane holds P[4g + i][j], i = 0..3: rows 2s, 2s+1 of samples
+            // s = 2g (i = 0, 1) and 2g + 1 (i = 2, 3); the neighbouring
+            // lane (j ^ 1) holds the other column of the same coil
+            const float sgn = (j & 1) ? -1.f : 1.f;
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const float o = dpp_partner<1>(acc[2 * t + 1]) + sgn * acc[2 * t];
+                if (j < NV && 2 * g + t < nrows) red[((row0 + 2 * g + t) * nwaves + wave) * NV + j] = o;
             }
         }
     };
@@ -193,7 +204,9 @@ void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, i
                 sy += st.my[s];
             }
             if constexpr (MF) {
-                msum[lane * kMfmaPad + (slot & (kMfmaRows - 1))] = make_float2(sx, sy);
+                const int row = 2 * (slot & (kMfmaRows - 1));
+                mstage[row * kMfmaStride + lane] = sx;
+                mstage[(row + 1) * kMfmaStride + lane] = sy;
                 if ((++slot & (kMfmaRows - 1)) == 0) mfma_project(slot - kMfmaRows, kMfmaRows);
                 if (slot == kRedBatch) {
                     __syncthreads();
@@ -246,7 +259,8 @@ void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, i
                             sx += st.mx[s];
                             sy += st.my[s];
                         }
-                        msum[lane * kMfmaPad + j] = make_float2(sx, sy);
+                        mstage[2 * j * kMfmaStride + lane] = sx;
+                        mstage[(2 * j + 1) * kMfmaStride + lane] = sy;
                         free_step<SPT, PREC>(st, k.m0s);
                     }
                     slot += kMfmaRows;
@@ -910,7 +924,7 @@ static mrs_status execute_group(mrs_plan *P, int64_t adc_base, int g) {
     launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1, P->precision != 0, mfma);
     const dim3 grid(P->gx, P->Ty * P->ngroups), block(P->block);
     const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float) +
-                       (mfma ? (size_t)(P->block / 64) * kMfmaWaveFloat2 * sizeof(float2) : 0);
+                       (mfma ? (size_t)(P->block / 64) * kMfmaWaveFloats * sizeof(float) : 0);
     const uint32_t *rec = (const uint32_t *)P->rec.p;
     const size_t nl = P->launch_r0.size();
     const size_t e0 = 3 * nl * (size_t)g;  // timing events of this pass
