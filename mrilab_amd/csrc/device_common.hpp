@@ -35,13 +35,14 @@ __device__ __forceinline__ void sincos_cr(float xf, float *sp, float *cp) {
                        r2 * (-1.0 / 39916800 + r2 * (1.0 / 6227020800.0))))));
     const double cp_ = 1.0 + r2 * (-0.5 + r2 * (1.0 / 24 + r2 * (-1.0 / 720 + r2 * (1.0 / 40320 +
                        r2 * (-1.0 / 3628800 + r2 * (1.0 / 479001600.0 + r2 * (-1.0 / 87178291200.0)))))));
+    // quadrant: rounding commutes with negation, so the swap and the signs
+    // are applied to the fp32 values (selects, not branches on fp64 pairs)
     const int q = (int)(((long long)k) & 3);
-    double s = sp_, c = cp_;
-    if (q == 1) { s = cp_; c = -sp_; }
-    else if (q == 2) { s = -sp_; c = -cp_; }
-    else if (q == 3) { s = -cp_; c = sp_; }
-    *sp = (float)s;
-    *cp = (float)c;
+    const float sf = (float)sp_, cf = (float)cp_;
+    const float so = (q & 1) ? cf : sf;
+    const float co = (q & 1) ? sf : cf;
+    *sp = (q & 2) ? -so : so;
+    *cp = ((q + 1) & 2) ? -co : co;
 }
 
 // sin and cos of an fp32 angle in fp32, within 1.5 ulp: Cody-Waite
