@@ -154,7 +154,10 @@ typedef struct mrs_options {
     int32_t spins_per_thread; /* 0 = auto */
     int32_t max_adc_per_launch; /* ADC samples per kernel launch, 0 = auto */
     int32_t multipool_scheme; /* 0 = operator split as DoGMScanAtGPU,
-                                 1 = exact FP64 matrix exponential */
+                                 1 = exact FP64 matrix exponential, blocks
+                                 of steps that repeat (saturation trains)
+                                 applied as the block's exact affine map,
+                                 2 = exact FP64, every step stepped */
     int32_t precision_mode;   /* 0 = reference: relaxation factors rounded
                                  as BlochKernel.h:348-368 rounds them;
                                  1 = accurate: expm1-based relaxation */

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 
 namespace mrs {
 
@@ -151,6 +152,76 @@ void compress_runs(Decoded &d, int64_t max_adc_run) {
         if (free_step && (f >> REC_REP_SHIFT) == 0 && !next_same) f |= REC_ONCE;
         next_same = (f & REC_SAME) != 0;
     }
+}
+
+int64_t fold_periods(Decoded &d, int64_t min_repeats, int64_t max_block) {
+    const int st = d.stride;
+    const int64_t n = d.n_rec();
+    if (n < 2 * min_repeats || !d.cap_slot.empty()) return 0;
+    const uint32_t ignore = REC_SAME | REC_NEWDT | REC_ONCE;
+    const uint32_t barrier = REC_ADC | REC_SPOIL | REC_CAPTURE | REC_LOOP;
+    const uint32_t *w = d.rec.data();
+    auto word = [&](int64_t r, int k) { return k == 5 ? w[r * st + 5] & ~ignore : w[r * st + k]; };
+    auto equal = [&](int64_t a, int64_t b) {
+        for (int k = 0; k < st; k++)
+            if (word(a, k) != word(b, k)) return false;
+        return true;
+    };
+    // next[r]: the next record with the same content as r (hash, then compare)
+    std::vector<int64_t> next((size_t)n, -1);
+    {
+        std::unordered_map<uint64_t, int64_t> last;
+        last.reserve((size_t)n);
+        for (int64_t r = n - 1; r >= 0; r--) {
+            uint64_t h = 1469598103934665603ull;
+            for (int k = 0; k < st; k++) h = (h ^ word(r, k)) * 1099511628211ull;
+            auto it = last.find(h);
+            if (it != last.end() && equal(r, it->second)) next[r] = it->second;
+            last[h] = r;
+        }
+    }
+    // barrier records before r: a block [a, b) is loop-free of them when
+    // bar[b] == bar[a]
+    std::vector<int64_t> bar((size_t)n + 1, 0);
+    for (int64_t r = 0; r < n; r++) bar[r + 1] = bar[r] + ((w[r * st + 5] & barrier) != 0);
+    std::vector<uint32_t> out;
+    out.reserve(d.rec.size());
+    int64_t loops = 0;
+    int64_t r = 0;
+    while (r < n) {
+        int64_t best_len = 0, best_rep = 0;
+        int tries = 0;
+        for (int64_t p = next[r]; p > r && p - r <= max_block && tries < 8; p = next[p], tries++) {
+            const int64_t L = p - r;
+            if (bar[p] != bar[r]) break;  // longer blocks hold the same barrier
+            int64_t N = 1;
+            while (r + (N + 1) * L <= n && bar[r + (N + 1) * L] == bar[r]) {
+                bool same = true;
+                for (int64_t i = 0; i < L && same; i++) same = equal(r + i, r + N * L + i);
+                if (!same) break;
+                N++;
+            }
+            if (N >= min_repeats && N * L > best_rep * best_len) {
+                best_len = L;
+                best_rep = N;
+            }
+            if (N >= min_repeats) break;
+        }
+        if (best_rep >= min_repeats && best_rep - 1 <= (int64_t)REC_REP_MAX) {
+            std::vector<uint32_t> head((size_t)st, 0);
+            head[5] = REC_LOOP | REC_NOSTEP | ((uint32_t)(best_rep - 1) << REC_REP_SHIFT);
+            head[6] = (uint32_t)best_len;
+            out.insert(out.end(), head.begin(), head.end());
+            out.insert(out.end(), w + r * st, w + (r + best_len) * st);
+            r += best_rep * best_len;
+            loops++;
+        } else {
+            out.insert(out.end(), w + r * st, w + (r + 1) * st);
+            r++;
+        }
+    }
+    if (loops) d.rec.swap(out);
+    return loops;
 }
 
 const char *check_lines(const mrs_vseq &S, const mrs_vvar &V, int tx) {

@@ -20,14 +20,16 @@ enum : uint32_t {
     REC_NEWDT = 1u << 6,    // dt differs from the previous step's
     REC_ONCE = 1u << 7,     // free step whose rotation no later step reuses
     REC_ADCRUN = 1u << 8,   // ADC run: the repeats are (ADC sample, free step) pairs
+    REC_LOOP = 1u << 9,     // loop head (with REC_NOSTEP): the next word-6 records repeat
 };
-// Bits 9..31 of the flags word: the repeat count. For a plain record, the
+// Bits 10..31 of the flags word: the repeat count. For a plain record, the
 // number of further plain free steps with this record's dt and fields (no
 // ADC, rf, spoiler or capture) that follow it; for an REC_ADCRUN record,
 // the number of further steps that each take an ADC sample first and then
-// step with the same dt and fields. compress_runs() folds them.
-constexpr int REC_REP_SHIFT = 9;
-constexpr uint32_t REC_REP_MAX = (1u << 23) - 1;
+// step with the same dt and fields. compress_runs() folds them. For a loop
+// head, the number of further passes over its block (fold_periods()).
+constexpr int REC_REP_SHIFT = 10;
+constexpr uint32_t REC_REP_MAX = (1u << 22) - 1;
 
 // Record layout, 32-bit words:
 //   0 dt (s)            1 gamma*Gz (rad/s/m)  2 gamma*Gy   3 gamma*Gx
@@ -61,6 +63,14 @@ struct Decoded {
 // most max_adc_run samples per record) instead of plain steps; only
 // kernels that execute REC_ADCRUN may be given such a stream.
 void compress_runs(Decoded &d, int64_t max_adc_run = 0);
+// Periodic blocks for the exact multi-pool kernel: a block of records that
+// recurs at least min_repeats times in a row, holds no ADC sample, spoiler,
+// capture or loop, and is at most max_block records long, is kept once
+// behind a loop head (REC_LOOP | REC_NOSTEP, further passes in the repeat
+// bits, the block length in word 6). Records compare equal when every word
+// matches except the flag bits that only the split scheme reads (REC_SAME,
+// REC_NEWDT, REC_ONCE). Returns the number of loops made.
+int64_t fold_periods(Decoded &d, int64_t min_repeats, int64_t max_block);
 // ADC samples a record takes.
 inline int64_t rec_adc_samples(uint32_t flags) {
     if (!(flags & REC_ADC)) return 0;

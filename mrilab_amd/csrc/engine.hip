@@ -493,6 +493,11 @@ struct mrs_plan {
     // streams, each from the initial state; 0 = one scan
     int nseq = 0;
     DevBuf seq_tab, vox_perm, Mout[3];
+    // exact multi-pool scheme: loop records (fold_periods) and the
+    // per-thread slots their kernels keep a block's map in
+    int64_t n_loops = 0;
+    DevBuf loop_cols;
+    size_t loop_cols_n = 0;
     std::vector<int32_t> seq_r0, seq_r1, seq_slot;    // records; first partial slot in its group
     std::vector<int64_t> seq_nadc, seq_adc_off;       // samples; offset in the adc_ref table
     std::vector<int64_t> seq_signum, seq_sig_off;     // VSig.SignalNum; offset in Sx/Sy
@@ -644,6 +649,7 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     P->scheme = opt ? opt->multipool_scheme : 0;
     P->coil_sum = opt ? opt->coil_sum : 0;
     if (P->model < 0 || P->model > 4) return fail(MRS_ERR_ARG, "unknown engine model");
+    if (P->scheme < 0 || P->scheme > 2) return fail(MRS_ERR_ARG, "multipool_scheme must be 0, 1 or 2");
     if (P->model >= 1) {
         P->variant = P->model - 1;
         if (!vo->K) return fail(MRS_ERR_ARG, "multi-pool scan needs VObj.K");
@@ -753,6 +759,33 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
 }
 
 // Uploads the decoded records of P->dec (replacing earlier ones).
+// Exact multi-pool scheme (multipool_scheme 1): periodic blocks of the
+// record stream become loop records when the block recurs often enough to
+// repay running the block once per basis state of the 3*Ty-dimensional
+// state (plus the zero state); scheme 2 steps every record. The slots a
+// launch's threads keep a block's map in ((3 Ty + 2) * 3 Ty doubles each)
+// are capped at kLoopColsMax bytes; a plan that would need more steps
+// every record.
+constexpr double kLoopColsMax = 8.0 * (1 << 30);
+static void fold_exact(const mrs_plan *P, Decoded &d, int64_t threads, int64_t *loops) {
+    if (P->model < 1 || P->scheme != 1) return;
+    const double D = 3.0 * P->Ty;
+    if ((double)threads * (D + 2) * D * sizeof(double) > kLoopColsMax) return;
+    *loops += fold_periods(d, 3 * (int64_t)P->Ty + 2, (int64_t)1 << 20);
+}
+
+// Slots for loop records: (3 Ty + 2) * 3 Ty doubles per thread of a launch.
+static mrs_status ensure_loop_cols(mrs_plan *P, int64_t threads) {
+    if (!P->n_loops) return MRS_OK;
+    const int64_t D = 3 * (int64_t)P->Ty;
+    const size_t need = (size_t)(threads * (D + 2) * D);
+    if (need <= P->loop_cols_n && P->loop_cols.p) return MRS_OK;
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    HIP_TRY(dalloc<double>(P->loop_cols, need));
+    P->loop_cols_n = need;
+    return MRS_OK;
+}
+
 static mrs_status upload_program(mrs_plan *P) {
     hipStream_t st = P->stream;
     const size_t words = P->dec.rec.size();
@@ -835,6 +868,7 @@ static GmParams gm_params(mrs_plan *P) {
     }
     gp.variant = P->variant;
     gp.self_x = P->variant == kVarME;
+    gp.loop_cols = P->n_loops ? (double *)P->loop_cols.p : nullptr;
     return gp;
 }
 
@@ -1134,6 +1168,7 @@ mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs
     dec.run(P->dec);
     dec.flush_tail(P->dec);
     compress_runs(P->dec, P->model == 0 ? P->max_adc : 0);
+    fold_exact(P, P->dec, (int64_t)P->gx * P->ngroups * P->block, &P->n_loops);
     if ((int64_t)P->dec.adc_ref.size() > P->signal_num) {
         delete P;
         return fail(MRS_ERR_ARG, "VSig.SignalNum is smaller than the number of ADC samples");
@@ -1142,7 +1177,7 @@ mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs
     P->rf_final[0].assign(dec.vvar().rfAmp, dec.vvar().rfAmp + P->Tx);
     P->rf_final[1].assign(dec.vvar().rfPhase, dec.vvar().rfPhase + P->Tx);
     P->rf_final[2].assign(dec.vvar().rfFreq, dec.vvar().rfFreq + P->Tx);
-    if ((rc = upload_program(P))) {
+    if ((rc = upload_program(P)) || (rc = ensure_loop_cols(P, (int64_t)P->gx * P->ngroups * P->block))) {
         delete P;
         return rc;
     }
@@ -1315,6 +1350,7 @@ mrs_status mrs_multipool_batch_create(int32_t n_seq, const mrs_vobj *vobj, const
         dec.run(d);
         dec.flush_tail(d);
         compress_runs(d, 0);
+        fold_exact(P, d, (int64_t)P->gx * n_seq * P->ngroups * P->block, &P->n_loops);
         if ((int64_t)d.adc_ref.size() > vsig[q]->SignalNum)
             return fail(MRS_ERR_ARG, "VSig.SignalNum of sequence " + std::to_string(q) +
                                          " is smaller than its number of ADC samples");
@@ -1363,6 +1399,11 @@ mrs_status mrs_multipool_batch_create(int32_t n_seq, const mrs_vobj *vobj, const
         most = std::max(most, used);
     }
     P->grp_end.push_back(n_seq);
+    {
+        int most_seq = 1;
+        for (size_t g = 0; g < P->grp_begin.size(); g++) most_seq = std::max(most_seq, P->grp_end[g] - P->grp_begin[g]);
+        if ((rc = ensure_loop_cols(P, (int64_t)P->gx * most_seq * P->ngroups * P->block))) return rc;
+    }
     hipStream_t st = P->stream;
     for (int k = 0; k < 2; k++) {
         DevBuf &b = k ? P->Sy : P->Sx;
@@ -1387,7 +1428,7 @@ mrs_status mrs_multipool_batch_create(int32_t n_seq, const mrs_vobj *vobj, const
     HIP_TRY(hipMalloc(&P->seq_tab.p, tab.size() * sizeof(int32_t)));
     HIP_TRY(hipMemcpy(P->seq_tab.p, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     for (int k = 0; k < 3; k++) HIP_TRY(hipMalloc(&P->Mout[k].p, (size_t)n_seq * P->ntot * sizeof(float)));
-    if (P->scheme == 1) {
+    if (P->scheme != 0) {
         // the exact scheme's cost grows with a voxel's fastest rate (its
         // Taylor substeps): order the voxels costliest first, so the waves
         // that hold them share tiles and are dispatched first
@@ -1497,9 +1538,11 @@ static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, cons
         const DecodeStop stop = dec.run(P->dec);
         if (stop == DECODE_END) dec.flush_tail(P->dec);
         compress_runs(P->dec, P->model == 0 ? P->max_adc : 0);
+        fold_exact(P, P->dec, (int64_t)P->gx * P->ngroups * P->block, &P->n_loops);
         if (adc_base + (int64_t)P->dec.adc_ref.size() > P->signal_num)
             return fail(MRS_ERR_ARG, "VSig.SignalNum is smaller than the number of ADC samples");
-        if ((rc = upload_program(P))) return rc;
+        if ((rc = upload_program(P)) || (rc = ensure_loop_cols(P, (int64_t)P->gx * P->ngroups * P->block)))
+            return rc;
         // the launches of this chunk write samples adc_base + [0, n)
         for (auto &a : P->launch_adc0) a += adc_base;
         {

@@ -90,6 +90,47 @@ struct GmState {
 constexpr int gm_waves_min(int ty, int scheme) { return scheme == 0 ? (ty <= 3 ? 3 : 1) : (ty <= 3 ? 2 : 1); }
 constexpr int gm_waves_max(int ty, int scheme) { return scheme == 0 ? (ty <= 3 ? 3 : 2) : 2; }
 
+// Loop records of the exact scheme (kept out of line: they run once per
+// basis state and loop, and inlined they would hold registers across the
+// whole step loop). Entry e of a thread's slots is at e * gs.
+// Column b of the block's map: the basis image less the constant c (b = 0
+// stores c itself).
+template <int D>
+__device__ __attribute__((noinline)) void loop_store_column(double *colA, double *colC, int64_t gs, int b,
+                                                            const double *sv) {
+    if (b == 0) {
+        for (int i = 0; i < D; i++) colC[(int64_t)i * gs] = sv[i];
+    } else {
+        for (int i = 0; i < D; i++) colA[((int64_t)(b - 1) * D + i) * gs] = sv[i] - colC[(int64_t)i * gs];
+    }
+}
+
+// The waiting state mapped n times by x -> A x + c, into xs.
+template <int D>
+__device__ __attribute__((noinline)) void loop_apply(const double *colA, const double *colC, const double *keep,
+                                                     int64_t gs, int n, double *xs) {
+    double x[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) x[i] = keep[(int64_t)i * gs];
+    for (int k = 0; k < n; k++) {
+        double y[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) y[i] = colC[(int64_t)i * gs];
+#pragma unroll 1
+        for (int b = 0; b < D; b++) {
+            double xb = x[0];
+#pragma unroll
+            for (int i = 1; i < D; i++) xb = b == i ? x[i] : xb;
+#pragma unroll
+            for (int i = 0; i < D; i++) y[i] += xb * colA[((int64_t)b * D + i) * gs];
+        }
+#pragma unroll
+        for (int i = 0; i < D; i++) x[i] = y[i];
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) xs[i] = x[i];
+}
+
 template <int TY, int NRX, int SCHEME, bool WC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_min(TY, SCHEME),
                                                                      gm_waves_max(TY, SCHEME)))) void bloch_gm_kernel(
@@ -244,54 +285,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
     float fsn[TY], fcs[TY];
     bool have_frot = false;
 
-    for (int r = r0; r < r1; ++r) {
-        const crec_t q = rec + r * stride;
-        const uint32_t flags = q[5];
-        if (flags & REC_SPOIL) {  // Plugin_IdealSpoiler.m:10-11 zeroes every pool
-#pragma unroll
-            for (int t = 0; t < TY; t++) {
-                st.mx[t] = 0;
-                st.my[t] = 0;
-            }
-        }
-        if (flags & REC_ADC) {  // free pools only (BlochKernelGM.cuh:192)
-            float val[NVT];
-#pragma unroll
-            for (int j = 0; j < NVT; j++) val[j] = 0.f;
-#pragma unroll
-            for (int t = 0; t < TY; t++) {
-                if (WC || wcv) {
-                    if (t != 0) continue;  // water only (BlochKernelMT.cuh:230)
-                } else if (p.flag[t] != 0) {
-                    continue;
-                }
-                const float sx = (valid && !skip[t]) ? (float)st.mx[t] : 0.f;
-                const float sy = (valid && !skip[t]) ? (float)st.my[t] : 0.f;
-#pragma unroll
-                for (int c = 0; c < NRX; c++) {
-                    val[t * NV + 2 * c] = sx * crx[c] + sy * cry[c];
-                    val[t * NV + 2 * c + 1] = sy * crx[c] - sx * cry[c];
-                }
-            }
-            WaveSum<NVT, 32>::run(val, lane);
-            constexpr int G = 64 / NVT;
-            if ((lane & (G - 1)) == 0) red[(slot * nwaves + wave) * NVT + lane / G] = val[0];
-            if (++slot == kRedBatch) {
-                __syncthreads();
-                flush_partials(red, kRedBatch, nwaves, NVT, partial, adc_batch0, nblk, blk, NVT);
-                __syncthreads();
-                adc_batch0 += kRedBatch;
-                slot = 0;
-            }
-        }
-        if (flags & REC_NOSTEP) continue;
-        const float dt = __uint_as_float(q[0]);
-        const bool rf = (flags & REC_RF) != 0;
-        const float gv = z * __uint_as_float(q[1]) + y * __uint_as_float(q[2]) + x * __uint_as_float(q[3]);
-        const float wrf = __uint_as_float(q[4]);
+    // the step of record q: dt, gradient phase rate, rf frequency, effective
+    // B1 (gamma-scaled) and the bound pools' saturation rates
+    auto rec_fields = [&](const crec_t q, uint32_t flags, float &dt, bool &rf, float &gv, float &wrf, float &bx,
+                          float &by, float (&wsat)[TY]) __attribute__((always_inline)) {
+        dt = __uint_as_float(q[0]);
+        rf = (flags & REC_RF) != 0;
+        gv = z * __uint_as_float(q[1]) + y * __uint_as_float(q[2]) + x * __uint_as_float(q[3]);
+        wrf = __uint_as_float(q[4]);
         // effective B1 (gamma-scaled) and bound-pool saturation rates
-        float bx = 0.f, by = 0.f;
-        float wsat[TY];
+        bx = 0.f;
+        by = 0.f;
 #pragma unroll
         for (int t = 0; t < TY; t++) wsat[t] = 0.f;
         if (rf) {
@@ -335,6 +339,293 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
             }
             (void)wsum;
         }
+    };
+    // exact step of record q (not a no-step record) on the state sv (pool t
+    // at sv[3t .. 3t+2]), its folded repeats included
+    auto exact_record = [&](const crec_t q, uint32_t flags, double *sv) __attribute__((always_inline)) {
+        if constexpr (SCHEME == 1) {
+            float dt, gv, wrf, bx, by;
+            bool rf;
+            float wsat[TY];
+            rec_fields(q, flags, dt, rf, gv, wrf, bx, by, wsat);
+            const int nrep = 1 + (int)(flags >> REC_REP_SHIFT);
+            // exact step: x <- exp(A*dt) x with the affine M0 term; per step
+            // the rotation of each free pool (zero for others) and the bound
+            // pools' saturation
+            double omt[TY], bxt[TY], byt[TY], dzs[TY];
+            double nrm = 0.0;
+#pragma unroll
+            for (int t = 0; t < TY; t++) {
+                omt[t] = fm[t] * ((double)wb[t] + (double)gv + (rf ? (double)wrf : 0.0));
+                bxt[t] = fm[t] * (double)bx;
+                byt[t] = fm[t] * (double)by;
+                dzs[t] = dzc[t] + bm[t] * (double)wsat[t];
+                // the row sum of pool t's rows bounds |A| (max row sum)
+                nrm = fmax(nrm, rsum[t] + fabs(omt[t]) + fabs(bxt[t]) + fabs(byt[t]) + bm[t] * (double)wsat[t]);
+            }
+            // substeps of h*|A| <= 4; the series runs until every active lane's
+            // term is below 1e-13 of its state (at most kTaylorMax terms): the
+            // truncation adds at most 1e-13 relative per step, 2e-8 over the
+            // 200,000 steps of a 2 s saturation train, under the fp32
+            // rounding of the state kept between launches and of the output
+            int nsub = (int)ceil(nrm * (double)dt * 0.25);
+            if (nsub < 1) nsub = 1;
+            const double h = (double)dt / nsub;
+            const double hn = h * nrm;
+            double amax = 0.0;
+#pragma unroll
+            for (int t = 0; t < TY; t++) amax = fmax(amax, fabs(affc[t]));
+            auto apply = [&](const double *in, double *out) __attribute__((always_inline)) {
+                double ixm[TY], iym[TY];
+#pragma unroll
+                for (int q = 0; q < TY; q++) {
+                    ixm[q] = in[3 * q] * xm[q];
+                    iym[q] = in[3 * q + 1] * xm[q];
+                }
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    const double ix = in[3 * t], iy = in[3 * t + 1], iz = in[3 * t + 2];
+                    double ox = omt[t] * iy - byt[t] * iz - dxc[t] * ix;
+                    double oy = bxt[t] * iz - omt[t] * ix - dxc[t] * iy;
+                    double oz = byt[t] * ix - bxt[t] * iy - dzs[t] * iz;
+#pragma unroll
+                    for (int q = 0; q < TY; q++) {
+                        if (q == t) continue;
+                        ox += kt[t][q] * ixm[q];
+                        oy += kt[t][q] * iym[q];
+                        oz += kz[t][q] * in[3 * q + 2];
+                    }
+                    out[3 * t] = ox;
+                    out[3 * t + 1] = oy;
+                    out[3 * t + 2] = oz;
+                }
+            };
+            // x <- exp(A*dt) x + (exp(A*dt) - I) A^-1 b, substep by substep:
+            // x += sum_k h^k/k! A^(k-1) (A x + b)
+            auto step_exact = [&](double *xv) __attribute__((always_inline)) {
+                for (int sub = 0; sub < nsub; sub++) {
+                    double term[3 * TY], nt[3 * TY];
+                    double xmax = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 3 * TY; i++) xmax = fmax(xmax, fabs(xv[i]));
+                    const double tol = 1e-13 * (xmax + h * amax);
+                    apply(xv, nt);
+#pragma unroll
+                    for (int t = 0; t < TY; t++) nt[3 * t + 2] += affc[t];
+#pragma unroll
+                    for (int i = 0; i < 3 * TY; i++) {
+                        term[i] = h * nt[i];
+                        xv[i] += term[i];
+                    }
+                    for (int kk = 2; kk <= kTaylorMax; kk++) {
+                        apply(term, nt);
+                        const double f = h * kInvK[kk];
+                        double tm = 0.0;
+#pragma unroll
+                        for (int i = 0; i < 3 * TY; i++) {
+                            term[i] = f * nt[i];
+                            xv[i] += term[i];
+                            tm = fmax(tm, fabs(term[i]));
+                        }
+                        // terms shrink monotonically once kk exceeds h*|A|
+                        if (__all((double)kk > hn && tm <= tol)) break;
+                    }
+                }
+            };
+            if (!rf && nrep > 1) {
+                // held free step: the map of one step is affine and the same for
+                // every repeat; take its columns from basis states (transverse
+                // columns are complex: without rf, Mx + iMy of the free pools
+                // evolve complex-linearly) and apply it nrep times
+                double cz[TY];   // the affine constant (longitudinal only)
+                double tr[TY][TY], ti[TY][TY], lz[TY][TY];
+                // basis b = 0: zero state; 1 + t: unit Mx of pool t; 1 + TY + t:
+                // unit Mz of pool t. Every basis state is stepped (skipped and
+                // bound pools included), so the columns are those of the direct
+                // step; one copy of the step code, entries stored by
+                // compile-time index.
+#pragma unroll 1
+                for (int b = 0; b <= 2 * TY; b++) {
+                    const bool bz = b > TY;
+                    const int tb = bz ? b - 1 - TY : b - 1;
+                    const int ib = bz ? 3 * tb + 2 : 3 * tb;
+                    double e[3 * TY];
+#pragma unroll
+                    for (int i = 0; i < 3 * TY; i++) e[i] = (b > 0 && i == ib) ? 1.0 : 0.0;
+                    step_exact(e);
+#pragma unroll
+                    for (int q = 0; q < TY; q++) {
+                        if (b == 0) cz[q] = e[3 * q + 2];
+#pragma unroll
+                        for (int t = 0; t < TY; t++) {
+                            if (b > 0 && t == tb) {
+                                if (bz) {
+                                    lz[q][t] = e[3 * q + 2] - cz[q];
+                                } else {
+                                    tr[q][t] = e[3 * q];
+                                    ti[q][t] = e[3 * q + 1];
+                                }
+                            }
+                        }
+                    }
+                }
+                for (int k = 0; k < nrep; k++) {
+                    double ox[TY], oy[TY], oz[TY];
+#pragma unroll
+                    for (int t = 0; t < TY; t++) {
+                        ox[t] = sv[3 * t];
+                        oy[t] = sv[3 * t + 1];
+                        oz[t] = sv[3 * t + 2];
+                    }
+#pragma unroll
+                    for (int q = 0; q < TY; q++) {
+                        double ax = 0.0, ay = 0.0, az = cz[q];
+#pragma unroll
+                        for (int t = 0; t < TY; t++) {
+                            ax += tr[q][t] * ox[t] - ti[q][t] * oy[t];
+                            ay += tr[q][t] * oy[t] + ti[q][t] * ox[t];
+                            az += lz[q][t] * oz[t];
+                        }
+                        sv[3 * q] = ax;
+                        sv[3 * q + 1] = ay;
+                        sv[3 * q + 2] = az;
+                    }
+                }
+            } else {
+                for (int k = 0; k < nrep; k++) step_exact(sv);
+            }
+        }
+    };
+    // loop records (exact scheme): the block [loop_start, loop_end] runs for
+    // basis state loop_b (-1: no loop open); this thread's slots of
+    // loop_cols hold A ((3 TY)^2 entries), c (3 TY) and the waiting state
+    // (3 TY), entry e at e * (threads of the grid)
+    int loop_start = 0, loop_end = -1, loop_n = 0, loop_b = -1;
+    const int64_t lgs = (int64_t)gridDim.x * gridDim.y * blockDim.x;
+    double *const lcolA = p.loop_cols ? p.loop_cols + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x +
+                                             threadIdx.x
+                                      : nullptr;
+    double *const lcolC = lcolA ? lcolA + (int64_t)9 * TY * TY * lgs : nullptr;
+    double *const lkeep = lcolC ? lcolC + (int64_t)3 * TY * lgs : nullptr;
+    (void)loop_start;
+    (void)loop_end;
+    (void)loop_n;
+    (void)loop_b;
+    (void)lkeep;
+    for (int r = r0; r < r1; ++r) {
+        const crec_t q = rec + r * stride;
+        const uint32_t flags = q[5];
+        if (flags & REC_SPOIL) {  // Plugin_IdealSpoiler.m:10-11 zeroes every pool
+#pragma unroll
+            for (int t = 0; t < TY; t++) {
+                st.mx[t] = 0;
+                st.my[t] = 0;
+            }
+        }
+        if (flags & REC_ADC) {  // free pools only (BlochKernelGM.cuh:192)
+            float val[NVT];
+#pragma unroll
+            for (int j = 0; j < NVT; j++) val[j] = 0.f;
+#pragma unroll
+            for (int t = 0; t < TY; t++) {
+                if (WC || wcv) {
+                    if (t != 0) continue;  // water only (BlochKernelMT.cuh:230)
+                } else if (p.flag[t] != 0) {
+                    continue;
+                }
+                const float sx = (valid && !skip[t]) ? (float)st.mx[t] : 0.f;
+                const float sy = (valid && !skip[t]) ? (float)st.my[t] : 0.f;
+#pragma unroll
+                for (int c = 0; c < NRX; c++) {
+                    val[t * NV + 2 * c] = sx * crx[c] + sy * cry[c];
+                    val[t * NV + 2 * c + 1] = sy * crx[c] - sx * cry[c];
+                }
+            }
+            WaveSum<NVT, 32>::run(val, lane);
+            constexpr int G = 64 / NVT;
+            if ((lane & (G - 1)) == 0) red[(slot * nwaves + wave) * NVT + lane / G] = val[0];
+            if (++slot == kRedBatch) {
+                __syncthreads();
+                flush_partials(red, kRedBatch, nwaves, NVT, partial, adc_batch0, nblk, blk, NVT);
+                __syncthreads();
+                adc_batch0 += kRedBatch;
+                slot = 0;
+            }
+        }
+        if constexpr (SCHEME == 1) {
+            constexpr int D = 3 * TY;
+            if (flags & REC_LOOP) {
+                // the block's affine map x -> A x + c: c is the image of the
+                // zero state, column b of A the image of unit vector b less
+                // c; the block runs once per basis state (the state itself
+                // waits in loop_cols), then the map is applied N times
+                loop_start = r + 1;
+                loop_end = r + (int)q[6];
+                loop_n = 1 + (int)(flags >> REC_REP_SHIFT);
+                loop_b = 0;
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    lkeep[(int64_t)(3 * t) * lgs] = st.mx[t];
+                    lkeep[(int64_t)(3 * t + 1) * lgs] = st.my[t];
+                    lkeep[(int64_t)(3 * t + 2) * lgs] = st.mz[t];
+                    st.mx[t] = st.my[t] = st.mz[t] = 0.0;
+                }
+                continue;
+            }
+            if (!(flags & REC_NOSTEP)) {
+                double sv[D];
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    sv[3 * t] = st.mx[t];
+                    sv[3 * t + 1] = st.my[t];
+                    sv[3 * t + 2] = st.mz[t];
+                }
+                exact_record(q, flags, sv);
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    st.mx[t] = sv[3 * t];
+                    st.my[t] = sv[3 * t + 1];
+                    st.mz[t] = sv[3 * t + 2];
+                }
+            }
+            if (loop_b >= 0 && r == loop_end) {
+                double sv[D];
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    sv[3 * t] = st.mx[t];
+                    sv[3 * t + 1] = st.my[t];
+                    sv[3 * t + 2] = st.mz[t];
+                }
+                loop_store_column<D>(lcolA, lcolC, lgs, loop_b, sv);
+                loop_b++;
+                if (loop_b <= D) {
+                    // the next basis state through the block
+#pragma unroll
+                    for (int t = 0; t < TY; t++) {
+                        st.mx[t] = (3 * t == loop_b - 1) ? 1.0 : 0.0;
+                        st.my[t] = (3 * t + 1 == loop_b - 1) ? 1.0 : 0.0;
+                        st.mz[t] = (3 * t + 2 == loop_b - 1) ? 1.0 : 0.0;
+                    }
+                    r = loop_start - 1;
+                    continue;
+                }
+                double xs[D];
+                loop_apply<D>(lcolA, lcolC, lkeep, lgs, loop_n, xs);
+#pragma unroll
+                for (int t = 0; t < TY; t++) {
+                    st.mx[t] = xs[3 * t];
+                    st.my[t] = xs[3 * t + 1];
+                    st.mz[t] = xs[3 * t + 2];
+                }
+                loop_b = -1;
+            }
+            continue;
+        }
+        if (flags & REC_NOSTEP) continue;
+        float dt, gv, wrf, bx, by;
+        bool rf;
+        float wsat[TY];
+        rec_fields(q, flags, dt, rf, gv, wrf, bx, by, wsat);
         const int nrep = 1 + (int)(flags >> REC_REP_SHIFT);
         if constexpr (SCHEME == 0) {
             if (rf) {
@@ -507,167 +798,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                 float bzb = Bz[1];
                 if (rf) bzb = Bz[1] * expf_cr(-wsat[1] * dt);
                 st.mz[1] = bzb * e1[1] + m0[1] + Bz[0] * kout[1];
-            }
-        } else {
-            // exact step: x <- exp(A*dt) x with the affine M0 term; per step
-            // the rotation of each free pool (zero for others) and the bound
-            // pools' saturation
-            double omt[TY], bxt[TY], byt[TY], dzs[TY];
-            double nrm = 0.0;
-#pragma unroll
-            for (int t = 0; t < TY; t++) {
-                omt[t] = fm[t] * ((double)wb[t] + (double)gv + (rf ? (double)wrf : 0.0));
-                bxt[t] = fm[t] * (double)bx;
-                byt[t] = fm[t] * (double)by;
-                dzs[t] = dzc[t] + bm[t] * (double)wsat[t];
-                // the row sum of pool t's rows bounds |A| (max row sum)
-                nrm = fmax(nrm, rsum[t] + fabs(omt[t]) + fabs(bxt[t]) + fabs(byt[t]) + bm[t] * (double)wsat[t]);
-            }
-            // substeps of h*|A| <= 4; the series runs until every active lane's
-            // term is below 1e-13 of its state (at most kTaylorMax terms): the
-            // truncation adds at most 1e-13 relative per step, 2e-8 over the
-            // 200,000 steps of a 2 s saturation train, under the fp32
-            // rounding of the state kept between launches and of the output
-            int nsub = (int)ceil(nrm * (double)dt * 0.25);
-            if (nsub < 1) nsub = 1;
-            const double h = (double)dt / nsub;
-            const double hn = h * nrm;
-            double amax = 0.0;
-#pragma unroll
-            for (int t = 0; t < TY; t++) amax = fmax(amax, fabs(affc[t]));
-            auto apply = [&](const double *in, double *out) __attribute__((always_inline)) {
-                double ixm[TY], iym[TY];
-#pragma unroll
-                for (int q = 0; q < TY; q++) {
-                    ixm[q] = in[3 * q] * xm[q];
-                    iym[q] = in[3 * q + 1] * xm[q];
-                }
-#pragma unroll
-                for (int t = 0; t < TY; t++) {
-                    const double ix = in[3 * t], iy = in[3 * t + 1], iz = in[3 * t + 2];
-                    double ox = omt[t] * iy - byt[t] * iz - dxc[t] * ix;
-                    double oy = bxt[t] * iz - omt[t] * ix - dxc[t] * iy;
-                    double oz = byt[t] * ix - bxt[t] * iy - dzs[t] * iz;
-#pragma unroll
-                    for (int q = 0; q < TY; q++) {
-                        if (q == t) continue;
-                        ox += kt[t][q] * ixm[q];
-                        oy += kt[t][q] * iym[q];
-                        oz += kz[t][q] * in[3 * q + 2];
-                    }
-                    out[3 * t] = ox;
-                    out[3 * t + 1] = oy;
-                    out[3 * t + 2] = oz;
-                }
-            };
-            // x <- exp(A*dt) x + (exp(A*dt) - I) A^-1 b, substep by substep:
-            // x += sum_k h^k/k! A^(k-1) (A x + b)
-            auto step_exact = [&](double *xv) __attribute__((always_inline)) {
-                for (int sub = 0; sub < nsub; sub++) {
-                    double term[3 * TY], nt[3 * TY];
-                    double xmax = 0.0;
-#pragma unroll
-                    for (int i = 0; i < 3 * TY; i++) xmax = fmax(xmax, fabs(xv[i]));
-                    const double tol = 1e-13 * (xmax + h * amax);
-                    apply(xv, nt);
-#pragma unroll
-                    for (int t = 0; t < TY; t++) nt[3 * t + 2] += affc[t];
-#pragma unroll
-                    for (int i = 0; i < 3 * TY; i++) {
-                        term[i] = h * nt[i];
-                        xv[i] += term[i];
-                    }
-                    for (int kk = 2; kk <= kTaylorMax; kk++) {
-                        apply(term, nt);
-                        const double f = h * kInvK[kk];
-                        double tm = 0.0;
-#pragma unroll
-                        for (int i = 0; i < 3 * TY; i++) {
-                            term[i] = f * nt[i];
-                            xv[i] += term[i];
-                            tm = fmax(tm, fabs(term[i]));
-                        }
-                        // terms shrink monotonically once kk exceeds h*|A|
-                        if (__all((double)kk > hn && tm <= tol)) break;
-                    }
-                }
-            };
-            if (!rf && nrep > 1) {
-                // held free step: the map of one step is affine and the same for
-                // every repeat; take its columns from basis states (transverse
-                // columns are complex: without rf, Mx + iMy of the free pools
-                // evolve complex-linearly) and apply it nrep times
-                double cz[TY];   // the affine constant (longitudinal only)
-                double tr[TY][TY], ti[TY][TY], lz[TY][TY];
-                // basis b = 0: zero state; 1 + t: unit Mx of pool t; 1 + TY + t:
-                // unit Mz of pool t. Every basis state is stepped (skipped and
-                // bound pools included), so the columns are those of the direct
-                // step; one copy of the step code, entries stored by
-                // compile-time index.
-#pragma unroll 1
-                for (int b = 0; b <= 2 * TY; b++) {
-                    const bool bz = b > TY;
-                    const int tb = bz ? b - 1 - TY : b - 1;
-                    const int ib = bz ? 3 * tb + 2 : 3 * tb;
-                    double e[3 * TY];
-#pragma unroll
-                    for (int i = 0; i < 3 * TY; i++) e[i] = (b > 0 && i == ib) ? 1.0 : 0.0;
-                    step_exact(e);
-#pragma unroll
-                    for (int q = 0; q < TY; q++) {
-                        if (b == 0) cz[q] = e[3 * q + 2];
-#pragma unroll
-                        for (int t = 0; t < TY; t++) {
-                            if (b > 0 && t == tb) {
-                                if (bz) {
-                                    lz[q][t] = e[3 * q + 2] - cz[q];
-                                } else {
-                                    tr[q][t] = e[3 * q];
-                                    ti[q][t] = e[3 * q + 1];
-                                }
-                            }
-                        }
-                    }
-                }
-                for (int k = 0; k < nrep; k++) {
-                    double ox[TY], oy[TY], oz[TY];
-#pragma unroll
-                    for (int t = 0; t < TY; t++) {
-                        ox[t] = st.mx[t];
-                        oy[t] = st.my[t];
-                        oz[t] = st.mz[t];
-                    }
-#pragma unroll
-                    for (int q = 0; q < TY; q++) {
-                        double ax = 0.0, ay = 0.0, az = cz[q];
-#pragma unroll
-                        for (int t = 0; t < TY; t++) {
-                            ax += tr[q][t] * ox[t] - ti[q][t] * oy[t];
-                            ay += tr[q][t] * oy[t] + ti[q][t] * ox[t];
-                            az += lz[q][t] * oz[t];
-                        }
-                        st.mx[q] = ax;
-                        st.my[q] = ay;
-                        st.mz[q] = az;
-                    }
-                }
-            } else {
-                for (int k = 0; k < nrep; k++) {
-                    double xv[3 * TY];
-#pragma unroll
-                    for (int t = 0; t < TY; t++) {
-                        xv[3 * t] = st.mx[t];
-                        xv[3 * t + 1] = st.my[t];
-                        xv[3 * t + 2] = st.mz[t];
-                    }
-                    step_exact(xv);
-#pragma unroll
-                    for (int t = 0; t < TY; t++) {
-                        st.mx[t] = xv[3 * t];
-                        st.my[t] = xv[3 * t + 1];
-                        st.mz[t] = xv[3 * t + 2];
-                    }
-                }
             }
         }
     }

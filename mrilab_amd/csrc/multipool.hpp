@@ -34,6 +34,10 @@ struct GmParams {
     const int32_t *vox_perm;
     float *Mout[3];
     int seq_base, nseq_l, tiles;  // tiles: voxel tiles of one sequence (grid x / nseq_l)
+    // Exact scheme with loop records: (3 Ty + 2) * 3 Ty doubles per thread of
+    // the launch grid (entry e of thread i at e * threads + i); nullptr when
+    // the record stream has no loops.
+    double *loop_cols;
 };
 
 // Multi-pool layouts of the reference engines (the K layout each entry point

@@ -182,3 +182,54 @@ def test_full_train_offsets_on_crop(gpu, scheme):
                   f"vs fp32 oracle {e_g32:.3e}; magnetization vs fp64 {e_m:.3e}", flush=True)
             assert e_s < 2 * e_o + 1e-5
             assert e_m < 1e-4
+
+
+def test_loop_records_match_stepwise_exact(gpu):
+    """The exact scheme folds a saturation train's repeated periods into loop
+    records (the block's affine map from the state basis, applied once per
+    period; multipool_scheme 1) unless asked to step every record
+    (multipool_scheme 2). Both are the same fp64 exponential: on the full
+    train (50 periods of 4000 steps) the batched offsets, a single scan
+    whose TRs run two offsets in turn (two loops in one stream), and the
+    two-pool MT engine (its 10-step rectangular readout pulse folds too, as
+    a one-record block) agree with the stepwise run to fp32 output rounding
+    and with the fp64 oracle."""
+    crop = (slice(100, 102), slice(60, 62))
+    scans = workloads.mt_cest(per_offset=True, crop=crop, offset_index=[0, 20, 39])
+    res = {}
+    for scheme in (1, 2):
+        outs = [s.copy() for s in scans]
+        with engine.MultiPoolBatch(scans, options=engine.make_options(multipool_scheme=scheme, timing=1)) as b:
+            b.run()
+            b.fetch(outs)
+            res[scheme] = (outs, b.info()["fwd_kernel_ms"])
+    print(f"case batched crop: folded {res[1][1]:.1f} ms, stepwise {res[2][1]:.1f} ms", flush=True)
+    for s, a, b in zip(scans, res[1][0], res[2][0]):
+        e_s, e_m = rel_l2(sig(a), sig(b)), mag_err(a, b, s)
+        ref = oracle.gm_scan(s, "f64", 1)
+        e_o = rel_l2(sig(a), sig(ref))
+        print(f"  offset {s.offset_hz:+.0f} Hz: folded vs stepwise signal {e_s:.2e} magnetization {e_m:.2e}, "
+              f"vs fp64 oracle {e_o:.2e}", flush=True)
+        assert e_s < 1e-6 and e_m < 1e-6 and e_o < 1e-6
+    one = workloads.mt_cest(shape=(256, 256, 1), crop=(slice(10, 12), slice(30, 31)), offset_index=[5, 30])
+    out = {}
+    for scheme in (1, 2):
+        g = one.copy()
+        engine.DoGMScanAtGPU(*g.structs, options=engine.make_options(multipool_scheme=scheme))
+        out[scheme] = g
+    ref = oracle.gm_scan(one, "f64", 1)
+    e_s, e_o = rel_l2(sig(out[1]), sig(out[2])), rel_l2(sig(out[1]), sig(ref))
+    e_m = mag_err(out[1], ref, one)
+    print(f"case single scan, 2 offsets as TRs: folded vs stepwise {e_s:.2e}, vs fp64 oracle signal {e_o:.2e} "
+          f"magnetization {e_m:.2e}", flush=True)
+    assert e_s < 1e-6 and e_o < 1e-6 and e_m < 1e-4
+    mt = pools(workloads.mt_cest(per_offset=True, crop=crop, offset_index=[12])[0], [0, 2], "MT")
+    out = {}
+    for scheme in (1, 2):
+        g = mt.copy()
+        engine.DoMTScanAtGPU(*g.structs, options=engine.make_options(multipool_scheme=scheme))
+        out[scheme] = g
+    ref = oracle.gm_scan(mt, "f64", 1, variant="MT")
+    e_s, e_o = rel_l2(sig(out[1]), sig(out[2])), rel_l2(sig(out[1]), sig(ref))
+    print(f"case MT two pools: folded vs stepwise {e_s:.2e}, vs fp64 oracle {e_o:.2e}", flush=True)
+    assert e_s < 1e-6 and e_o < 1e-6
