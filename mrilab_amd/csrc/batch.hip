@@ -446,9 +446,23 @@ static mrs_status batch_enqueue(mrs_batch *B, int block) {
     return MRS_OK;
 }
 
-static mrs_status batch_time(mrs_batch *B, int block, int reps, float *ms) {
+// Device time of one batch run with this block size: an untimed run when
+// the kernel has not run yet (warm), a timed screening run, then the mean of
+// reps runs unless the screening run already took longer than cutoff ms (a
+// shape that cannot win).
+static mrs_status batch_time(mrs_batch *B, int block, int reps, float cutoff, bool warm, float *ms) {
     mrs_status rc;
-    if ((rc = batch_enqueue(B, block))) return rc;  // warm-up
+    if (warm && (rc = batch_enqueue(B, block))) return rc;
+    HIP_TRY(hipEventRecord(B->ev0, B->stream));
+    if ((rc = batch_enqueue(B, block))) return rc;
+    HIP_TRY(hipEventRecord(B->ev1, B->stream));
+    HIP_TRY(hipEventSynchronize(B->ev1));
+    float w = 0.f;
+    HIP_TRY(hipEventElapsedTime(&w, B->ev0, B->ev1));
+    if (w > cutoff) {
+        *ms = w;
+        return MRS_OK;
+    }
     HIP_TRY(hipEventRecord(B->ev0, B->stream));
     for (int r = 0; r < reps; r++)
         if ((rc = batch_enqueue(B, block))) return rc;
@@ -491,10 +505,25 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
     if (n <= 0) return fail(MRS_ERR_ARG, "n_scans must be positive");
     if (!vobj || !vmag || !vcoi || !vctl || !vseq || !vvar || !vsig) return fail(MRS_ERR_ARG, "NULL array argument");
     mrs_status rc;
+    {
+        // argument checks (they read every time point of every sequence)
+        // spread over host threads; the first failing scan is reported
+        const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<mrs_status> st_(n, MRS_OK);
+        std::vector<std::string> msg(n);
+        std::vector<std::thread> pool;
+        for (int w = 0; w < nth; w++)
+            pool.emplace_back([&, w]() {
+                for (int i = w; i < n; i += nth) {
+                    st_[i] = check_args(vobj[i], vmag[i], vcoi[i], vctl[i], vseq[i], vvar[i], vsig[i]);
+                    if (st_[i]) msg[i] = mrs_last_error();  // the message is thread-local
+                }
+            });
+        for (auto &th : pool) th.join();
+        for (int i = 0; i < n; i++)
+            if (st_[i]) return fail(st_[i], "scan " + std::to_string(i) + ": " + msg[i]);
+    }
     for (int i = 0; i < n; i++) {
-        if ((rc = check_args(vobj[i], vmag[i], vcoi[i], vctl[i], vseq[i], vvar[i], vsig[i]))) {
-            return fail(rc, "scan " + std::to_string(i) + ": " + mrs_last_error());
-        }
         const std::string who = "scan " + std::to_string(i) + ": ";
         if (vobj[i]->TypeNum != 1) return fail(MRS_ERR_UNSUPPORTED, who + "batched scans are single-pool (TypeNum 1)");
         if (vcoi[i]->TxCoilNum != 1) return fail(MRS_ERR_UNSUPPORTED, who + "batched scans have one Tx coil");
@@ -762,7 +791,7 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
             if (s != spts[0] && B->total_iso / s < (int64_t)B->cus * 4 * 64) continue;
             for (int b : blocks) {
                 float ms = 0.f;
-                if ((rc = batch_time(B, b, 2, &ms))) return rc;
+                if ((rc = batch_time(B, b, 2, 1.3f * best, b == blocks[0], &ms))) return rc;
                 B->n_candidates++;
                 worst = std::max(worst, ms);
                 if (ms < best) {
