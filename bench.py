@@ -235,9 +235,9 @@ def multipool_section(threads: int, sat_time: float = 2.0, extras: bool = True) 
                      "launches": info["n_launches"], "voxel_steps_per_s": vs / el}
         if scheme == 1:
             out[name]["note"] = ("the 50 identical 40 ms periods of each train are one loop record: the period "
-                                 "runs once per basis state of the 9-component state (and the zero state), and "
-                                 "its exact affine map is applied once per period; voxel_steps_per_s counts the "
-                                 "sequence's steps")
+                                 "runs for the zero state and the unit states of the 7 state entries that are not "
+                                 "the bound pool's (zero) transverse entries, and its exact affine map is applied "
+                                 "once per period; voxel_steps_per_s counts the sequence's steps")
         log(f"multipool {name}: {info['n_steps']} steps over {n_off} offsets in {el:.2f} s "
             f"({vs / el:.3e} voxel-steps/s, {info['n_launches']} launches)")
     if not extras:

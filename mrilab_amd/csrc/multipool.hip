@@ -105,6 +105,16 @@ __device__ __attribute__((noinline)) void loop_store_column(double *colA, double
     }
 }
 
+// Zero columns of A for the components the block is not run for (bit i
+// of mask clear): their state entries are zero, so the columns only need
+// to be finite.
+template <int D>
+__device__ __attribute__((noinline)) void loop_zero_columns(double *colA, int64_t gs, uint32_t mask) {
+    for (int b = 0; b < D; b++)
+        if (!((mask >> b) & 1u))
+            for (int i = 0; i < D; i++) colA[((int64_t)b * D + i) * gs] = 0.0;
+}
+
 // The waiting state mapped n times by x -> A x + c, into xs.
 template <int D>
 __device__ __attribute__((noinline)) void loop_apply(const double *colA, const double *colC, const double *keep,
@@ -497,10 +507,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
         }
     };
     // loop records (exact scheme): the block [loop_start, loop_end] runs for
-    // basis state loop_b (-1: no loop open); this thread's slots of
+    // basis state loop_b (0 the zero state, b the unit state of component
+    // b - 1; -1: no loop open); this thread's slots of
     // loop_cols hold A ((3 TY)^2 entries), c (3 TY) and the waiting state
     // (3 TY), entry e at e * (threads of the grid)
     int loop_start = 0, loop_end = -1, loop_n = 0, loop_b = -1;
+    uint32_t loop_mask = 0;  // components whose basis states run
+    (void)loop_mask;
     const int64_t lgs = (int64_t)gridDim.x * gridDim.y * blockDim.x;
     double *const lcolA = p.loop_cols ? p.loop_cols + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x +
                                              threadIdx.x
@@ -563,6 +576,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                 loop_end = r + (int)q[6];
                 loop_n = 1 + (int)(flags >> REC_REP_SHIFT);
                 loop_b = 0;
+                // a bound pool's transverse entries have zero rows; where the
+                // whole wave holds zeros there, they stay zero and their
+                // basis states need no run
+                loop_mask = (1u << D) - 1;
+#pragma unroll
+                for (int t = 0; t < TY; t++)
+                    if (p.flag[t] != 0 && __all(st.mx[t] == 0.0 && st.my[t] == 0.0))
+                        loop_mask &= ~(3u << (3 * t));
+                if (loop_mask != (1u << D) - 1) loop_zero_columns<D>(lcolA, lgs, loop_mask);
 #pragma unroll
                 for (int t = 0; t < TY; t++) {
                     lkeep[(int64_t)(3 * t) * lgs] = st.mx[t];
@@ -597,8 +619,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                     sv[3 * t + 2] = st.mz[t];
                 }
                 loop_store_column<D>(lcolA, lcolC, lgs, loop_b, sv);
-                loop_b++;
-                if (loop_b <= D) {
+                const uint32_t rest = loop_mask >> loop_b;  // components from loop_b on
+                if (rest != 0) {
+                    loop_b += __builtin_ctz(rest) + 1;
                     // the next basis state through the block
 #pragma unroll
                     for (int t = 0; t < TY; t++) {
