@@ -729,10 +729,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                         az = az * expf_cr(-wsat[t] * dt);
                     }
                     az = az * e1[t];
+                    // inflow from every pool: fin is 0 where nothing flows
+                    // (adding 0 * B leaves the sum as it is; no per-lane branch)
                     float kx = 0.f, ky = 0.f, kz = 0.f;
 #pragma unroll
                     for (int qq = 0; qq < TY; qq++) {
-                        if (fin[t][qq] == 0.f) continue;
                         if (p.flag[t] == 0) {
                             kx = kx + fin[t][qq] * Bx[qq];
                             ky = ky + fin[t][qq] * By[qq];
@@ -784,13 +785,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                 float ko = 1.f, kx = 0.f, ky = 0.f, kz = 0.f;
 #pragma unroll
                 for (int t = 2; t < TY; t++) {
-                    if (kmat[0][t] != 0.f) ko = ko * fin[0][t];
-                    if (kmat[t][0] != 0.f) {
-                        const float f = 1.f - fin[t][0];
-                        kx = kx + f * Bx[t];
-                        ky = ky + f * By[t];
-                        kz = kz + f * Bz[t];
-                    }
+                    // survival factors are 1 where no rate is set (ko * 1 and
+                    // 0 * B change nothing; no per-lane branches)
+                    ko = ko * fin[0][t];
+                    const float f = 1.f - fin[t][0];
+                    kx = kx + f * Bx[t];
+                    ky = ky + f * By[t];
+                    kz = kz + f * Bz[t];
                 }
                 if constexpr (TY == 2) {
                     st.mx[0] = ox[0] * e2[0];
@@ -805,14 +806,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
 #pragma unroll
                 for (int t = 2; t < TY; t++) {
                     const float bxc = ox[t] * e2[t], byc = oy[t] * e2[t], bzc = oz[t] * e1[t];
-                    const float kt = kmat[t][0] != 0.f ? fin[t][0] : 1.f;
-                    float ix = 0.f, iy = 0.f, iz = 0.f;
-                    if (kmat[0][t] != 0.f) {
-                        const float f = 1.f - fin[0][t];
-                        ix = f * Bx[0];
-                        iy = f * By[0];
-                        iz = f * Bz[0];
-                    }
+                    const float kt = fin[t][0];
+                    const float f = 1.f - fin[0][t];
+                    const float ix = f * Bx[0], iy = f * By[0], iz = f * Bz[0];
                     st.mx[t] = bxc * kt + ix;
                     st.my[t] = byc * kt + iy;
                     st.mz[t] = bzc * kt + iz + m0[t];
