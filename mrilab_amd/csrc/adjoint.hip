@@ -134,9 +134,13 @@ __device__ __forceinline__ void exc_rotate(float bx, float by, float w, float dt
     const float om2 = (bx * bx + by * by) + w * w;
     sn = 0.f;
     cs = 1.f;
-    if (om2 > 0.f) {
-        const float om = sqrtf(om2);
-        const float inv = 1.0f / om;
+    // the hardware reciprocal square root (1 ulp) for the axis and |Omega|,
+    // as the engine's rf step; the backward pass forms |Omega| the same way,
+    // so its (sin, cos) are the forward's. Below |Omega| = 1e-15 rad/s the
+    // rotation is skipped (rsq stays off subnormals).
+    if (om2 > 1e-30f) {
+        const float inv = __builtin_amdgcn_rsqf(om2);
+        const float om = om2 * inv;
         const float nx = bx * inv, ny = by * inv, nz = w * inv;
         sincos_fast(om * dt, &sn, &cs);
         const float ax = mx, ay = my, az = md;
@@ -348,9 +352,9 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                 const float w = wb[s] + gv;
                 const float om2 = b2 + w * w;
                 float nx = 0.f, ny = 0.f, nz = 0.f, th = 0.f;
-                if (om2 > 0.f) {
-                    const float om = sqrtf(om2);
-                    const float inv = 1.0f / om;
+                if (om2 > 1e-30f) {
+                    const float inv = __builtin_amdgcn_rsqf(om2);
+                    const float om = om2 * inv;
                     nx = bx * inv;
                     ny = by * inv;
                     nz = w * inv;
@@ -364,7 +368,7 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                     Md = mz[s] * ie1;
                     sn = 0.f;
                     cs = 1.f;
-                    if (om2 > 0.f) sincos_fast(th, &sn, &cs);
+                    if (om2 > 1e-30f) sincos_fast(th, &sn, &cs);
                 } else {
                     Mx = sc[sidx[s]];
                     My = sc[n_iso + sidx[s]];
@@ -385,9 +389,10 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                     a = 0.5f - t2 * (1.0f / 24) + t2 * t2 * (1.0f / 720);
                     b = (1.0f / 6) - t2 * (1.0f / 120) + t2 * t2 * (1.0f / 5040);
                 } else {
-                    const float i2 = 1.0f / t2;
+                    // gradient factors: the hardware reciprocal (1 ulp) suffices
+                    const float i2 = __builtin_amdgcn_rcpf(t2);
                     a = (1.f - cs) * i2;
-                    b = (th - sn) * i2 / th;
+                    b = (th - sn) * i2 * __builtin_amdgcn_rcpf(th);
                 }
                 // psi = -th * n; c1 = psi x u; c2 = psi x c1
                 const float qx = -th * nx, qy = -th * ny, qz = -th * nz;
@@ -409,7 +414,7 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                     mx[s] = Mx;
                     my[s] = My;
                     mz[s] = Md;
-                    if (om2 > 0.f) exc_unrotate(nx, ny, nz, sn, cs, k.m0s, mx[s], my[s], mz[s]);
+                    if (om2 > 1e-30f) exc_unrotate(nx, ny, nz, sn, cs, k.m0s, mx[s], my[s], mz[s]);
                 }
             }
             // rf gradient of step j: (gx, gy) through B1 = gamma * w_tx * u
