@@ -351,24 +351,17 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
             for (int s = 0; s < SPT; s++) {
                 const float w = wb[s] + gv;
                 const float om2 = b2 + w * w;
-                float nx = 0.f, ny = 0.f, nz = 0.f, th = 0.f;
-                if (om2 > 1e-30f) {
-                    const float inv = __builtin_amdgcn_rsqf(om2);
-                    const float om = om2 * inv;
-                    nx = bx * inv;
-                    ny = by * inv;
-                    nz = w * inv;
-                    th = om * dt;
-                }
+                // below |Omega| = 1e-15 rad/s: a zero axis and angle, for which the
+                // steps below reduce to the identity exactly (no branches)
+                const float inv = om2 > 1e-30f ? __builtin_amdgcn_rsqf(om2) : 0.f;
+                const float nx = bx * inv, ny = by * inv, nz = w * inv, th = (om2 * inv) * dt;
                 float Mx, My, Md, sn, cs;
                 if constexpr (REBUILD) {
                     // M' = E^-1 M_{j+1}; (sin, cos) as the forward computed them
                     Mx = mx[s] * ie2;
                     My = my[s] * ie2;
                     Md = mz[s] * ie1;
-                    sn = 0.f;
-                    cs = 1.f;
-                    if (om2 > 1e-30f) sincos_fast(th, &sn, &cs);
+                    sincos_fast(th, &sn, &cs);
                 } else {
                     Mx = sc[sidx[s]];
                     My = sc[n_iso + sidx[s]];
@@ -383,17 +376,14 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                 // u = lambda' x M'
                 const float ux = py * Mz - pz * My, uy = pz * Mx - px * Mz, uz = px * My - py * Mx;
                 // a = (1 - cos)/th^2, b = (th - sin)/th^3 (series for small th)
-                float a, b;
+                // (selects, both forms computed; the hardware reciprocal, 1 ulp,
+                // suffices for gradient factors)
                 const float t2 = th * th;
-                if (th < 0.05f) {
-                    a = 0.5f - t2 * (1.0f / 24) + t2 * t2 * (1.0f / 720);
-                    b = (1.0f / 6) - t2 * (1.0f / 120) + t2 * t2 * (1.0f / 5040);
-                } else {
-                    // gradient factors: the hardware reciprocal (1 ulp) suffices
-                    const float i2 = __builtin_amdgcn_rcpf(t2);
-                    a = (1.f - cs) * i2;
-                    b = (th - sn) * i2 * __builtin_amdgcn_rcpf(th);
-                }
+                const float i2 = __builtin_amdgcn_rcpf(t2);
+                const bool small = th < 0.05f;
+                const float a = small ? 0.5f - t2 * (1.0f / 24) + t2 * t2 * (1.0f / 720) : (1.f - cs) * i2;
+                const float b = small ? (1.0f / 6) - t2 * (1.0f / 120) + t2 * t2 * (1.0f / 5040)
+                                      : (th - sn) * i2 * __builtin_amdgcn_rcpf(th);
                 // psi = -th * n; c1 = psi x u; c2 = psi x c1
                 const float qx = -th * nx, qy = -th * ny, qz = -th * nz;
                 const float c1x = qy * uz - qz * uy, c1y = qz * ux - qx * uz, c1z = qx * uy - qy * ux;
@@ -414,7 +404,7 @@ __global__ __launch_bounds__(256) void exc_bwd_kernel(ExcParams p) {
                     mx[s] = Mx;
                     my[s] = My;
                     mz[s] = Md;
-                    if (om2 > 1e-30f) exc_unrotate(nx, ny, nz, sn, cs, k.m0s, mx[s], my[s], mz[s]);
+                    exc_unrotate(nx, ny, nz, sn, cs, k.m0s, mx[s], my[s], mz[s]);
                 }
             }
             // rf gradient of step j: (gx, gy) through B1 = gamma * w_tx * u
