@@ -601,6 +601,16 @@ def roofline(prof: dict | None, avg_launch_s: float, ab: float, iso_steps_per_la
         out["valu"] = {"instructions_per_launch": insts, "dual_issue_quad_cycles": dual,
                        "dual_issued_share": 2 * dual / insts,
                        "instructions_over_single_issue_slots": insts / slots}
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in c:
+            # fp32 MFMAs (the coil projection) do not overlap VALU issue on
+            # gfx950 (SQ_VALU_MFMA_COEXEC_CYCLES = 0): SIMD cycles spent
+            # issuing VALU work or executing MFMAs, over the launch's cycles
+            mfma = c["SQ_VALU_MFMA_BUSY_CYCLES"] / SIMDS
+            simd = PEAK_CLOCK_HZ * avg_launch_s
+            out["simd_busy"] = {"valu_frac": (insts - dual) * 4 / SIMDS / simd, "mfma_frac": mfma / simd,
+                                "frac": ((insts - dual) * 4 / SIMDS + mfma) / simd,
+                                "note": "at the peak clock; VALU busy slots x 4 cycles plus MFMA busy cycles, "
+                                        "per SIMD, over the launch's cycles"}
     if "SQ_INSTS_VALU_FLOPS_FP32" in c:
         tf = c["SQ_INSTS_VALU_FLOPS_FP32"] * 64 / avg_launch_s / 1e12
         out["counted_fp32"] = {"TFLOPs": tf, "peak": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS}
