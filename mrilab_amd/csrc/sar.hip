@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "device_common.hpp"
@@ -232,12 +233,6 @@ mrs_status mrs_DoCalSARAtGPU(const mrs_sar_obj *o, const mrs_vseq *vs, mrs_vvar 
     if (vv->utsi < 0 || vv->utsi >= vs->n_uts || vv->rfi < 0 || g->SARi < 0)
         return fail(MRS_ERR_ARG, "VVar.utsi/rfi or SARi out of range");
     const int64_t nvox = (int64_t)o->dims[0] * o->dims[1] * o->dims[2];
-    SarDecoded d;
-    g->n_rf_steps = 0;
-    mrs_status rc = sar_decode(o, vs, vv, g, d);
-    if (rc) return rc;
-    g->n_rf_steps = d.n_steps;
-    if (d.rec.empty()) return MRS_OK;  // nothing reaches the voxels
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (ndev <= 0) return fail(MRS_ERR_DEVICE, "no HIP device visible");
@@ -248,15 +243,39 @@ mrs_status mrs_DoCalSARAtGPU(const mrs_sar_obj *o, const mrs_vseq *vs, mrs_vvar 
         hipStream_t s;
         ~S() { (void)hipStreamDestroy(s); }
     } guard{st};
+    // the event loop (host only: it writes VVar, tRealSample and SARi) runs
+    // on a thread of its own while the per-voxel arrays go to the device
+    SarDecoded d;
+    g->n_rf_steps = 0;
+    mrs_status drc = MRS_OK;
+    std::string dmsg;
+    std::thread decoder([&]() {
+        drc = sar_decode(o, vs, vv, g, d);
+        if (drc) dmsg = mrs_last_error();  // the message is thread-local
+    });
+    struct J {
+        std::thread &t;
+        ~J() {
+            if (t.joinable()) t.join();
+        }
+    } joiner{decoder};
     const int Tx = o->TxCoilNum;
     const size_t eb = (size_t)nvox * Tx * sizeof(float), vb = (size_t)nvox * sizeof(float);
     SDev ex, ey, ez, econ, base, sar, rec;
+    mrs_status rc;
     if ((rc = supload(ex, o->TxE1x, eb, st)) || (rc = supload(ey, o->TxE1y, eb, st)) ||
         (rc = supload(ez, o->TxE1z, eb, st)) || (rc = supload(econ, o->ECon, 3 * vb, st)) ||
         (rc = supload(base, g->SARBase, 3 * vb, st)) ||
-        (rc = supload(sar, g->SAR, (size_t)g->n_samples * vb, st)) ||
-        (rc = supload(rec, d.rec.data(), d.rec.size() * 4, st)))
+        (rc = supload(sar, g->SAR, (size_t)g->n_samples * vb, st)))
         return rc;
+    decoder.join();
+    if (drc) return fail(drc, dmsg);
+    g->n_rf_steps = d.n_steps;
+    if (d.rec.empty()) {  // nothing reaches the voxels
+        HIP_TRY(hipStreamSynchronize(st));
+        return MRS_OK;
+    }
+    if ((rc = supload(rec, d.rec.data(), d.rec.size() * 4, st))) return rc;
     SarParams p;
     p.ex = (const float *)ex.p;
     p.ey = (const float *)ey.p;
