@@ -489,7 +489,7 @@ def coil_sum_section(n_tr: int) -> dict:
 COUNTERS_JSON = os.path.join(ROOT, "profiles", "bloch_fwd_bssfp3d_counters.json")
 KERNEL_SOURCES = ("mrilab_amd/csrc/engine.hip", "mrilab_amd/csrc/spin_step.hpp", "mrilab_amd/csrc/device_common.hpp",
                   "mrilab_amd/csrc/decode.hpp", "mrilab_amd/csrc/decode.cpp")
-FWD_KERNEL = "bloch_fwd_kernel<4, 8, false, 0, true>"
+FWD_KERNEL = "bloch_fwd_kernel<4, 8, false, 0, true, true>"
 SIMDS = 1024                # 256 CUs x 4 SIMDs
 PEAK_CLOCK_HZ = 2.4e9       # MI355X peak engine clock
 

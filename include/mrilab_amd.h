@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define MRS_ABI_VERSION 2
+#define MRS_ABI_VERSION 3
 
 typedef enum mrs_status {
     MRS_OK = 0,
@@ -172,6 +172,21 @@ typedef struct mrs_options {
                                  samples onto the coil maps, 1 = per-lane
                                  coil products and a wavefront butterfly
                                  (other coil counts always take the latter) */
+    int32_t free_steps;       /* single-pool engines, free precession:
+                                 2 = deferred: a free step without an ADC
+                                 sample adds its rotation angle (the
+                                 reference's fp32 angle) to a per-isochromat
+                                 fp64 sum and relaxes Mz; the summed rotation
+                                 is applied once before the next ADC sample,
+                                 rf step, capture or launch end;
+                                 1 = every free step rotates, as
+                                 BlochKernel.h does;
+                                 0 = deferred when the decoded sequence has
+                                 at least 8 such steps per summed rotation,
+                                 else stepwise (the info structs report the
+                                 choice). Both keep each step's angle as the
+                                 reference rounds it and agree to fp32
+                                 rounding. */
 } mrs_options;
 
 /* Thread-local message of the last failed call on this thread. */
@@ -276,6 +291,7 @@ typedef struct mrs_plan_info {
     float   last_run_ms;      /* device time of the last run, 0 if unknown */
     double  fwd_kernel_ms;    /* timing = 1: summed Bloch-kernel time of the last run */
     double  reduce_kernel_ms; /* timing = 1: summed signal-reduction time */
+    int32_t deferred_free_steps; /* 1 if free precession is deferred (mrs_options.free_steps) */
 } mrs_plan_info;
 mrs_status mrs_plan_get_info(const mrs_plan *plan, mrs_plan_info *info);
 void mrs_plan_destroy(mrs_plan *plan);
@@ -350,6 +366,7 @@ typedef struct mrs_batch_info {
     double  decode_ms;        /* host time spent decoding the sequences at create */
     double  tune_ms;          /* host time spent in the autotuner at create */
     double  create_ms;        /* host time of mrs_batch_create as a whole */
+    int32_t deferred_free_steps; /* 1 if free precession is deferred (mrs_options.free_steps) */
 } mrs_batch_info;
 mrs_status mrs_batch_get_info(const mrs_batch *batch, mrs_batch_info *info);
 void mrs_batch_destroy(mrs_batch *batch);

@@ -93,7 +93,7 @@ class OptionsC(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("spins_per_thread", C.c_int32),
                 ("max_adc_per_launch", C.c_int32), ("multipool_scheme", C.c_int32),
                 ("precision_mode", C.c_int32), ("timing", C.c_int32), ("model", C.c_int32),
-                ("coil_sum", C.c_int32)]
+                ("coil_sum", C.c_int32), ("free_steps", C.c_int32)]
 
 
 class PlanInfoC(C.Structure):
@@ -102,7 +102,8 @@ class PlanInfoC(C.Structure):
                 ("n_cached_steps", C.c_int64),
                 ("block_size", C.c_int32), ("spins_per_thread", C.c_int32),
                 ("grid_blocks", C.c_int32), ("last_run_ms", C.c_float),
-                ("fwd_kernel_ms", C.c_double), ("reduce_kernel_ms", C.c_double)]
+                ("fwd_kernel_ms", C.c_double), ("reduce_kernel_ms", C.c_double),
+                ("deferred_free_steps", C.c_int32)]
 
 
 class BatchInfoC(C.Structure):
@@ -111,7 +112,7 @@ class BatchInfoC(C.Structure):
                 ("spins_per_thread", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32),
                 ("tuned", C.c_int32), ("n_candidates", C.c_int32), ("last_run_ms", C.c_float),
                 ("tune_best_ms", C.c_float), ("tune_worst_ms", C.c_float), ("decode_ms", C.c_double),
-                ("tune_ms", C.c_double), ("create_ms", C.c_double)]
+                ("tune_ms", C.c_double), ("create_ms", C.c_double), ("deferred_free_steps", C.c_int32)]
 
 
 class ExcitationC(C.Structure):

@@ -72,7 +72,7 @@ struct BatchParams {
     int32_t n_tiles;
 };
 
-template <int SPT, int NRX, int PREC>
+template <int SPT, int NRX, int PREC, bool DEFER>
 __global__ __launch_bounds__(256) void bloch_batch_kernel(BatchParams p, const uint32_t *__restrict__ rec_g) {
     constexpr int NV = 2 * NRX;
     const int lane = threadIdx.x & 63;
@@ -182,6 +182,7 @@ __global__ __launch_bounds__(256) void bloch_batch_kernel(BatchParams p, const u
                 st.spoil();
             }
             if (flags & REC_ADC) {  // DoScanAtCPU.cpp:445-491, before the step
+                if constexpr (DEFER) apply_pending(st);
                 // lanes past the last voxel carry zero coil weights
                 float sx = st.mx[0], sy = st.my[0];
 #pragma unroll
@@ -200,8 +201,10 @@ __global__ __launch_bounds__(256) void bloch_batch_kernel(BatchParams p, const u
                 if ((lane & (G - 1)) == 0) p.partial[(part0 + adc * ntile) * NV + lane / G] = val[0];
                 adc++;
             }
-            if (!(flags & REC_NOSTEP)) bloch_step<SPT, false, PREC>(st, k, (const uint32_t *)q, flags, txm, rin);
+            if (!(flags & REC_NOSTEP))
+                bloch_step<SPT, false, PREC, DEFER>(st, k, (const uint32_t *)q, flags, txm, rin);
         }
+        if constexpr (DEFER) apply_pending(st);
         if (valid) {
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
@@ -249,51 +252,53 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const float *__restri
 typedef void (*batch_fn)(dim3, dim3, hipStream_t, const BatchParams &, const uint32_t *);
 typedef int (*occ_fn)(int block);
 
-template <int SPT, int NRX, int PREC>
+template <int SPT, int NRX, int PREC, bool DEFER>
 static void launch_batch(dim3 g, dim3 b, hipStream_t st, const BatchParams &p, const uint32_t *rec) {
-    bloch_batch_kernel<SPT, NRX, PREC><<<g, b, 0, st>>>(p, rec);
+    bloch_batch_kernel<SPT, NRX, PREC, DEFER><<<g, b, 0, st>>>(p, rec);
 }
 
-template <int SPT, int NRX, int PREC>
+template <int SPT, int NRX, int PREC, bool DEFER>
 static int occupancy_batch(int block) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bloch_batch_kernel<SPT, NRX, PREC>, block, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, bloch_batch_kernel<SPT, NRX, PREC, DEFER>, block, 0) !=
         hipSuccess)
         return 1;
     return std::max(n, 1);
 }
 
-template <int SPT, int PREC>
+template <int SPT, int PREC, bool D>
 static void pick_batch_nrx(int nrx, batch_fn *f, occ_fn *o) {
     switch (nrx) {
-        case 1: *f = launch_batch<SPT, 1, PREC>; *o = occupancy_batch<SPT, 1, PREC>; return;
-        case 2: *f = launch_batch<SPT, 2, PREC>; *o = occupancy_batch<SPT, 2, PREC>; return;
-        case 4: *f = launch_batch<SPT, 4, PREC>; *o = occupancy_batch<SPT, 4, PREC>; return;
-        case 8: *f = launch_batch<SPT, 8, PREC>; *o = occupancy_batch<SPT, 8, PREC>; return;
-        case 16: *f = launch_batch<SPT, 16, PREC>; *o = occupancy_batch<SPT, 16, PREC>; return;
-        case 32: *f = launch_batch<SPT, 32, PREC>; *o = occupancy_batch<SPT, 32, PREC>; return;
+        case 1: *f = launch_batch<SPT, 1, PREC, D>; *o = occupancy_batch<SPT, 1, PREC, D>; return;
+        case 2: *f = launch_batch<SPT, 2, PREC, D>; *o = occupancy_batch<SPT, 2, PREC, D>; return;
+        case 4: *f = launch_batch<SPT, 4, PREC, D>; *o = occupancy_batch<SPT, 4, PREC, D>; return;
+        case 8: *f = launch_batch<SPT, 8, PREC, D>; *o = occupancy_batch<SPT, 8, PREC, D>; return;
+        case 16: *f = launch_batch<SPT, 16, PREC, D>; *o = occupancy_batch<SPT, 16, PREC, D>; return;
+        case 32: *f = launch_batch<SPT, 32, PREC, D>; *o = occupancy_batch<SPT, 32, PREC, D>; return;
     }
     *f = nullptr;
     *o = nullptr;
 }
 
-template <int PREC>
+template <int PREC, bool D>
 static void pick_batch_spt(int spt, int nrx, batch_fn *f, occ_fn *o) {
     switch (spt) {
-        case 1: pick_batch_nrx<1, PREC>(nrx, f, o); return;
-        case 2: pick_batch_nrx<2, PREC>(nrx, f, o); return;
-        case 4: pick_batch_nrx<4, PREC>(nrx, f, o); return;
-        case 8: pick_batch_nrx<8, PREC>(nrx, f, o); return;
+        case 1: pick_batch_nrx<1, PREC, D>(nrx, f, o); return;
+        case 2: pick_batch_nrx<2, PREC, D>(nrx, f, o); return;
+        case 4: pick_batch_nrx<4, PREC, D>(nrx, f, o); return;
+        case 8: pick_batch_nrx<8, PREC, D>(nrx, f, o); return;
     }
     *f = nullptr;
     *o = nullptr;
 }
 
-static void pick_batch(int spt, int nrx, int prec, batch_fn *f, occ_fn *o) {
+// Batch kernel for spt spins per thread, nrx (padded) coils, relaxation
+// form prec, and deferred or stepwise free steps.
+static void pick_batch(int spt, int nrx, int prec, bool defer, batch_fn *f, occ_fn *o) {
     if (prec)
-        pick_batch_spt<1>(spt, nrx, f, o);
+        defer ? pick_batch_spt<1, true>(spt, nrx, f, o) : pick_batch_spt<1, false>(spt, nrx, f, o);
     else
-        pick_batch_spt<0>(spt, nrx, f, o);
+        defer ? pick_batch_spt<0, true>(spt, nrx, f, o) : pick_batch_spt<0, false>(spt, nrx, f, o);
 }
 
 struct BDev {
@@ -323,7 +328,8 @@ struct mrs_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int n = 0, prec = 0, timing = 0;
+    int n = 0, prec = 0, timing = 0, free_steps = 0;
+    bool defer = false;  // deferred free precession (DEFER kernel)
     int spt = 1, block = 64, grid = 1, nrx = 1, nv = 2, cus = 1;
     bool tuned = false;
     int n_candidates = 0;
@@ -424,7 +430,7 @@ static BatchParams batch_params(mrs_batch *B) {
 static mrs_status batch_enqueue(mrs_batch *B, int block) {
     batch_fn f;
     occ_fn o;
-    pick_batch(B->spt, B->nrx, B->prec, &f, &o);
+    pick_batch(B->spt, B->nrx, B->prec, B->defer, &f, &o);
     if (!f) return fail(MRS_ERR_UNSUPPORTED, "no batched kernel for this shape");
     const int wpb = block / 64;
     const int64_t need = (B->n_tiles + wpb - 1) / wpb;
@@ -544,6 +550,8 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
     B->device = vctl[0]->GPUIndex;
     B->prec = opt ? opt->precision_mode : 0;
     B->timing = opt ? opt->timing : 0;
+    B->free_steps = opt ? opt->free_steps : 0;
+    if (B->free_steps < 0 || B->free_steps > 2) return fail(MRS_ERR_ARG, "free_steps must be 0, 1 or 2");
     HIP_TRY(hipSetDevice(B->device));
     HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&B->ev0));
@@ -576,6 +584,18 @@ mrs_status mrs_batch_create(int32_t n, mrs_vobj *const *vobj, const mrs_vmag *co
         for (auto &th : pool) th.join();
     }
     B->decode_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    {
+        // one kernel runs every scan: deferral as the plan chooses it, over
+        // the steps and pending rotations of all scans
+        int64_t steps = 0, rotations = 0;
+        for (int i = 0; i < n; i++) {
+            int64_t a = 0, b = 0;
+            deferral_counts(B->dec[i], &a, &b);
+            steps += a;
+            rotations += b;
+        }
+        B->defer = B->free_steps == 2 || (B->free_steps == 0 && deferral_pays(steps, rotations));
+    }
 
     // concatenate inputs
     B->hscan.resize(n);
@@ -899,6 +919,7 @@ mrs_status mrs_batch_get_info(const mrs_batch *B, mrs_batch_info *info) {
     info->last_run_ms = B->last_ms;
     info->tune_best_ms = B->tune_best;
     info->tune_worst_ms = B->tune_worst;
+    info->deferred_free_steps = B->defer ? 1 : 0;
     info->decode_ms = B->decode_ms;
     info->tune_ms = B->tune_ms;
     info->create_ms = B->create_ms;

@@ -154,6 +154,25 @@ void compress_runs(Decoded &d, int64_t max_adc_run) {
     }
 }
 
+void deferral_counts(const Decoded &d, int64_t *steps, int64_t *rotations) {
+    int64_t n = 0, k = 0;
+    bool pend = false;
+    for (int64_t r = 0; r < d.n_rec(); r++) {
+        const uint32_t f = d.rec[(size_t)r * d.stride + 5];
+        if (f & REC_SPOIL) pend = false;  // the spoiler discards the pending rotation
+        if (f & (REC_ADC | REC_RF | REC_CAPTURE)) {
+            if (pend) k++;
+            pend = false;
+        }
+        if (!(f & (REC_NOSTEP | REC_ADC | REC_RF))) {
+            n += 1 + (int64_t)(f >> REC_REP_SHIFT);
+            pend = !(f & REC_CAPTURE);
+        }
+    }
+    *steps = n;
+    *rotations = k + (pend ? 1 : 0);
+}
+
 int64_t fold_periods(Decoded &d, int64_t min_repeats, int64_t max_block) {
     const int st = d.stride;
     const int64_t n = d.n_rec();

@@ -71,6 +71,17 @@ void compress_runs(Decoded &d, int64_t max_adc_run = 0);
 // matches except the flag bits that only the split scheme reads (REC_SAME,
 // REC_NEWDT, REC_ONCE). Returns the number of loops made.
 int64_t fold_periods(Decoded &d, int64_t min_repeats, int64_t max_block);
+// Deferred free precession (the single-pool kernels' DEFER form): the free
+// steps that would be deferred (free steps without an ADC sample) and the
+// pending rotations they cause (one at each later ADC sample, rf step or
+// capture, and one at the end of the stream). A pending rotation costs about
+// what kDeferStepsPerRotation deferred steps save, so deferral pays when the
+// steps reach that many per rotation.
+constexpr int64_t kDeferStepsPerRotation = 8;
+void deferral_counts(const Decoded &d, int64_t *steps, int64_t *rotations);
+inline bool deferral_pays(int64_t steps, int64_t rotations) {
+    return steps > 0 && steps >= kDeferStepsPerRotation * rotations;
+}
 // ADC samples a record takes.
 inline int64_t rec_adc_samples(uint32_t flags) {
     if (!(flags & REC_ADC)) return 0;

@@ -14,8 +14,8 @@ constexpr int kRedBatch = 16;          // ADC samples buffered in LDS before a f
 typedef const __attribute__((address_space(4))) uint32_t *crec_t;
 
 
-// sin and cos of an fp32 angle, correctly rounded to fp32 (up to rare
-// near-midpoint cases): the reduction by pi/2 (two-part constant, FMA, exact
+// sin and cos of an fp64 angle (sincos_cr: of an fp32 angle), correctly
+// rounded to fp32 (up to rare near-midpoint cases): the reduction by pi/2 (two-part constant, FMA, exact
 // enough for |x| up to 1e15 rad) and the Taylor polynomials of sin and cos
 // on [-pi/4, pi/4] (truncation error below 3e-14) run in fp64, which on
 // gfx950 issues at the fp32 FMA rate when packed fp32 is off. Beyond 1e15
@@ -24,8 +24,7 @@ typedef const __attribute__((address_space(4))) uint32_t *crec_t;
 // but not its phase. MRiLab's CPU engine takes sin/cos from a
 // 24-bit-accurate vector library; a 1.5 ulp fp32 sincos drifts visibly
 // from it over 1e5 cached rotations, this one does not.
-__device__ __forceinline__ void sincos_cr(float xf, float *sp, float *cp) {
-    double x = (double)xf;
+__device__ __forceinline__ void sincos_cr_d(double x, float *sp, float *cp) {
     if (fabs(x) > 1.0e15) x = fmod(x, 6.283185307179586477);
     const double k = rint(x * 0.63661977236758134308);   // 2/pi
     double r = fma(-k, 1.5707963267948966192, x);        // pi/2, high part
@@ -44,6 +43,21 @@ __device__ __forceinline__ void sincos_cr(float xf, float *sp, float *cp) {
     *sp = (q & 2) ? -so : so;
     *cp = ((q + 1) & 2) ? -co : co;
 }
+__device__ __forceinline__ void sincos_cr(float xf, float *sp, float *cp) { sincos_cr_d((double)xf, sp, cp); }
+
+// The Taylor polynomials of sincos_fast on the reduced angle r in
+// [-pi/4, pi/4] and the quadrant q of the angle (its multiple of pi/2, mod 4).
+__device__ __forceinline__ void sincos_poly(float r, int q, float *sp, float *cp) {
+    const float r2 = r * r;
+    const float s = __builtin_fmaf(r * r2, __builtin_fmaf(r2, __builtin_fmaf(r2, __builtin_fmaf(r2, 2.75573192e-06f,
+                                   -1.98412698e-04f), 8.33333333e-03f), -1.66666667e-01f), r);
+    const float c = __builtin_fmaf(r2, __builtin_fmaf(r2, __builtin_fmaf(r2, __builtin_fmaf(r2, __builtin_fmaf(r2,
+                                   -2.75573192e-07f, 2.48015873e-05f), -1.38888889e-03f), 4.16666667e-02f), -0.5f), 1.0f);
+    const float so = (q & 1) ? c : s;
+    const float co = (q & 1) ? s : c;
+    *sp = (q & 2) ? -so : so;
+    *cp = ((q + 1) & 2) ? -co : co;
+}
 
 // sin and cos of an fp32 angle in fp32, within 1.5 ulp: Cody-Waite
 // reduction by pi/2 (three-part constant, FMA) and the Taylor polynomials of
@@ -59,16 +73,18 @@ __device__ __forceinline__ void sincos_fast(float x, float *sp, float *cp) {
     float r = __builtin_fmaf(-k, 1.57079637050628662109375f, x);
     r = __builtin_fmaf(-k, -4.37113882867379e-08f, r);
     r = __builtin_fmaf(-k, -1.77635684e-15f, r);
-    const float r2 = r * r;
-    const float s = __builtin_fmaf(r * r2, __builtin_fmaf(r2, __builtin_fmaf(r2, __builtin_fmaf(r2, 2.75573192e-06f,
-                                   -1.98412698e-04f), 8.33333333e-03f), -1.66666667e-01f), r);
-    const float c = __builtin_fmaf(r2, __builtin_fmaf(r2, __builtin_fmaf(r2, __builtin_fmaf(r2, __builtin_fmaf(r2,
-                                   -2.75573192e-07f, 2.48015873e-05f), -1.38888889e-03f), 4.16666667e-02f), -0.5f), 1.0f);
-    const int q = ((int)k) & 3;
-    const float so = (q & 1) ? c : s;
-    const float co = (q & 1) ? s : c;
-    *sp = (q & 2) ? -so : so;
-    *cp = ((q + 1) & 2) ? -co : co;
+    sincos_poly(r, ((int)k) & 3, sp, cp);
+}
+
+// sin and cos of an fp64 angle in fp32, within 1.5 ulp: the reduction by
+// pi/2 in fp64 (as sincos_cr_d), then sincos_fast's fp32 polynomials. For
+// rotations applied once, such as the summed angle of deferred free steps.
+__device__ __forceinline__ void sincos_fast_d(double x, float *sp, float *cp) {
+    if (fabs(x) > 1.0e15) x = fmod(x, 6.283185307179586477);
+    const double k = rint(x * 0.63661977236758134308);
+    double r = fma(-k, 1.5707963267948966192, x);
+    r = fma(-k, 6.1232339957367660e-17, r);
+    sincos_poly((float)r, (int)(((long long)k) & 3), sp, cp);
 }
 
 // Lane exchange for one butterfly stage across lane bit O. For O = 32 and

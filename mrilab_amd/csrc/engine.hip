@@ -67,7 +67,9 @@ struct FwdParams {
 // values the limit moves to scratch are spilled before the step loop and
 // reloaded after it.
 // MFC: the MFMA coil projection (4 or 8 coils), else the butterfly.
-template <int SPT, int NRX, bool GEN, int PREC, bool MFC>
+// DEFER: free steps between ADC samples, rf steps and spoilers gather their
+// rotation angles and apply them once (bloch_step), else every step rotates.
+template <int SPT, int NRX, bool GEN, int PREC, bool MFC, bool DEFER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPT <= 4 ? 4 : 1)))
 void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, int r1) {
     constexpr int NV = 2 * NRX;
@@ -196,6 +198,7 @@ void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, i
         }
         // DoScanAtCPU.cpp:445-491, before the step
         auto adc_sample = [&]() __attribute__((always_inline)) {
+            if constexpr (DEFER) apply_pending(st);
             // lanes past the last voxel carry zero coil weights
             float sx = st.mx[0], sy = st.my[0];
 #pragma unroll
@@ -239,7 +242,7 @@ void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, i
         const bool run = (flags & REC_ADCRUN) != 0;
         if (flags & REC_ADC) adc_sample();
         if (!(flags & REC_NOSTEP))
-            bloch_step<SPT, GEN, PREC>(st, k, q, run ? flags & ((1u << REC_REP_SHIFT) - 1) : flags, txm);
+            bloch_step<SPT, GEN, PREC, DEFER>(st, k, q, run ? flags & ((1u << REC_REP_SHIFT) - 1) : flags, txm);
         if (run) {
             int left = (int)(flags >> REC_REP_SHIFT);  // (sample, step) pairs after this step
             if constexpr (MF) {
@@ -279,6 +282,9 @@ void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, i
                 free_step<SPT, PREC>(st, k.m0s);
             }
         }
+        if constexpr (DEFER && GEN) {
+            if (flags & REC_CAPTURE) apply_pending(st);
+        }
         if (GEN && (flags & REC_CAPTURE) && valid) {  // DoScanAtCPU.cpp:544-552
             const int64_t base = (int64_t)p.cap_slot[r] * nvox * p.S * p.Ty;
 #pragma unroll
@@ -295,6 +301,7 @@ void bloch_fwd_kernel(FwdParams p, const uint32_t *__restrict__ rec_g, int r0, i
         __syncthreads();
         flush_partials(red, slot, nwaves, NV, p.partial, adc_batch0, nblk, blk, NV);
     }
+    if constexpr (DEFER) apply_pending(st);
     if (valid) {
 #pragma unroll
         for (int s = 0; s < SPT; s++) {
@@ -383,34 +390,34 @@ mrs_status fail(mrs_status st, const std::string &msg) {
 
 typedef void (*fwd_fn)(FwdParams, const uint32_t *, int64_t, int64_t);
 
-template <int SPT, int NRX, bool GEN, int PREC, bool MFC>
+template <int SPT, int NRX, bool GEN, int PREC, bool MFC, bool DEFER>
 static void launch_fwd(dim3 grid, dim3 block, size_t lds, hipStream_t st, const FwdParams &p,
                        const uint32_t *rec, int r0, int r1) {
-    bloch_fwd_kernel<SPT, NRX, GEN, PREC, MFC><<<grid, block, lds, st>>>(p, rec, r0, r1);
+    bloch_fwd_kernel<SPT, NRX, GEN, PREC, MFC, DEFER><<<grid, block, lds, st>>>(p, rec, r0, r1);
 }
 
 typedef void (*launch_fn)(dim3, dim3, size_t, hipStream_t, const FwdParams &, const uint32_t *, int, int);
 
-template <int SPT, bool GEN, int PREC>
+template <int SPT, bool GEN, int PREC, bool DEFER>
 static launch_fn pick_nrx(int nrx, bool mfma) {
     switch (nrx) {
-        case 1: return launch_fwd<SPT, 1, GEN, PREC, false>;
-        case 2: return launch_fwd<SPT, 2, GEN, PREC, false>;
-        case 4: return mfma ? launch_fwd<SPT, 4, GEN, PREC, true> : launch_fwd<SPT, 4, GEN, PREC, false>;
-        case 8: return mfma ? launch_fwd<SPT, 8, GEN, PREC, true> : launch_fwd<SPT, 8, GEN, PREC, false>;
-        case 16: return launch_fwd<SPT, 16, GEN, PREC, false>;
-        case 32: return launch_fwd<SPT, 32, GEN, PREC, false>;
+        case 1: return launch_fwd<SPT, 1, GEN, PREC, false, DEFER>;
+        case 2: return launch_fwd<SPT, 2, GEN, PREC, false, DEFER>;
+        case 4: return mfma ? launch_fwd<SPT, 4, GEN, PREC, true, DEFER> : launch_fwd<SPT, 4, GEN, PREC, false, DEFER>;
+        case 8: return mfma ? launch_fwd<SPT, 8, GEN, PREC, true, DEFER> : launch_fwd<SPT, 8, GEN, PREC, false, DEFER>;
+        case 16: return launch_fwd<SPT, 16, GEN, PREC, false, DEFER>;
+        case 32: return launch_fwd<SPT, 32, GEN, PREC, false, DEFER>;
     }
     return nullptr;
 }
 
-template <bool GEN, int PREC>
+template <bool GEN, int PREC, bool DEFER>
 static launch_fn pick_spt(int spt, int nrx, bool mfma) {
     switch (spt) {
-        case 1: return pick_nrx<1, GEN, PREC>(nrx, mfma);
-        case 2: return pick_nrx<2, GEN, PREC>(nrx, mfma);
-        case 4: return pick_nrx<4, GEN, PREC>(nrx, mfma);
-        case 8: return pick_nrx<8, GEN, PREC>(nrx, mfma);
+        case 1: return pick_nrx<1, GEN, PREC, DEFER>(nrx, mfma);
+        case 2: return pick_nrx<2, GEN, PREC, DEFER>(nrx, mfma);
+        case 4: return pick_nrx<4, GEN, PREC, DEFER>(nrx, mfma);
+        case 8: return pick_nrx<8, GEN, PREC, DEFER>(nrx, mfma);
     }
     return nullptr;
 }
@@ -419,13 +426,18 @@ static launch_fn pick_spt(int spt, int nrx, bool mfma) {
 // signal with the MFMA projection (coil_sum option 0) or the butterfly (1).
 static bool use_mfma(int nrx, int coil_sum) { return coil_sum == 0 && (nrx == 4 || nrx == 8); }
 
+template <bool GEN, int PREC>
+static launch_fn pick_defer(int spt, int nrx, bool mfma, bool defer) {
+    return defer ? pick_spt<GEN, PREC, true>(spt, nrx, mfma) : pick_spt<GEN, PREC, false>(spt, nrx, mfma);
+}
+
 // Forward kernel for spt spins per thread, nrx (padded) receive coils,
-// the general variant (multi-Tx, capture) or not, relaxation form prec and
-// the coil sum of use_mfma.
-static launch_fn pick_fwd(int spt, int nrx, bool gen, int prec, bool mfma) {
+// the general variant (multi-Tx, capture) or not, relaxation form prec, the
+// coil sum of use_mfma, and deferred or stepwise free steps.
+static launch_fn pick_fwd(int spt, int nrx, bool gen, int prec, bool mfma, bool defer) {
     if (prec)
-        return gen ? pick_spt<true, 1>(spt, nrx, mfma) : pick_spt<false, 1>(spt, nrx, mfma);
-    return gen ? pick_spt<true, 0>(spt, nrx, mfma) : pick_spt<false, 0>(spt, nrx, mfma);
+        return gen ? pick_defer<true, 1>(spt, nrx, mfma, defer) : pick_defer<false, 1>(spt, nrx, mfma, defer);
+    return gen ? pick_defer<true, 0>(spt, nrx, mfma, defer) : pick_defer<false, 0>(spt, nrx, mfma, defer);
 }
 
 void launch_signal_reduce(int nv, dim3 grid, hipStream_t st, const float *partial, int nblk, int bstep,
@@ -484,6 +496,8 @@ struct mrs_plan {
     int timing = 0;
     int model = 0, scheme = 0;        // 0 single pool; 1 GM, 2 MT, 3 CEST, 4 ME multi-pool
     int coil_sum = 0;                 // mrs_options.coil_sum
+    int free_steps = 0;               // mrs_options.free_steps
+    bool defer = false;               // the program's free steps are deferred (DEFER kernels)
     int variant = 0;                  // model - 1 for the multi-pool engines
     DevBuf K;
     int flag[kGmMaxTypes] = {0};
@@ -648,8 +662,10 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     P->model = opt ? opt->model : 0;
     P->scheme = opt ? opt->multipool_scheme : 0;
     P->coil_sum = opt ? opt->coil_sum : 0;
+    P->free_steps = opt ? opt->free_steps : 0;
     if (P->model < 0 || P->model > 4) return fail(MRS_ERR_ARG, "unknown engine model");
     if (P->scheme < 0 || P->scheme > 2) return fail(MRS_ERR_ARG, "multipool_scheme must be 0, 1 or 2");
+    if (P->free_steps < 0 || P->free_steps > 2) return fail(MRS_ERR_ARG, "free_steps must be 0, 1 or 2");
     if (P->model >= 1) {
         P->variant = P->model - 1;
         if (!vo->K) return fail(MRS_ERR_ARG, "multi-pool scan needs VObj.K");
@@ -713,7 +729,7 @@ static mrs_status plan_setup(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *vm
     max_adc = std::max<int64_t>(kRedBatch, std::min<int64_t>(max_adc, 65536));
     if (opt && opt->max_adc_per_launch > 0) max_adc = opt->max_adc_per_launch;
     P->max_adc = max_adc;
-    if (!pick_fwd(P->spt, P->nrx, true, 0, false)) return fail(MRS_ERR_UNSUPPORTED, "no kernel for this coil count");
+    if (!pick_fwd(P->spt, P->nrx, true, 0, false, false)) return fail(MRS_ERR_UNSUPPORTED, "no kernel for this coil count");
 
     hipStream_t st = P->stream;
     const size_t vb = (size_t)P->nvox * sizeof(float);
@@ -955,7 +971,7 @@ static mrs_status execute_group(mrs_plan *P, int64_t adc_base, int g) {
     for (int t = 0; t < P->Ty; t++) fp.cs[t] = P->cs[t];
     if (P->model >= 1) return execute_gm(P, g);
     const bool mfma = use_mfma(P->nrx, P->coil_sum);
-    launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1, P->precision != 0, mfma);
+    launch_fn fn = pick_fwd(P->spt, P->nrx, P->Tx != 1 || P->run_mode == 1, P->precision != 0, mfma, P->defer);
     const dim3 grid(P->gx, P->Ty * P->ngroups), block(P->block);
     const size_t lds = (size_t)kRedBatch * (P->block / 64) * P->nv * sizeof(float) +
                        (mfma ? (size_t)(P->block / 64) * kMfmaWaveFloats * sizeof(float) : 0);
@@ -1141,6 +1157,14 @@ static mrs_status refresh_maps(mrs_plan *P, const mrs_vobj *vo, const mrs_vmag *
 extern "C" {
 
 const char *mrs_last_error(void) { return g_err.c_str(); }
+// free_steps 0 defers free precession when the decoded program has enough
+// deferrable steps per pending rotation (deferral_pays), 2 always, 1 never.
+static void choose_deferral(mrs_plan *P) {
+    int64_t steps = 0, rotations = 0;
+    deferral_counts(P->dec, &steps, &rotations);
+    P->defer = P->free_steps == 2 || (P->free_steps == 0 && deferral_pays(steps, rotations));
+}
+
 int32_t mrs_abi_version(void) { return MRS_ABI_VERSION; }
 int32_t mrs_device_count(void) {
     int n = 0;
@@ -1168,6 +1192,7 @@ mrs_status mrs_plan_create(const mrs_vobj *vobj, const mrs_vmag *vmag, const mrs
     dec.run(P->dec);
     dec.flush_tail(P->dec);
     compress_runs(P->dec, P->model == 0 ? P->max_adc : 0);
+    choose_deferral(P);
     fold_exact(P, P->dec, (int64_t)P->gx * P->ngroups * P->block, &P->n_loops);
     if ((int64_t)P->dec.adc_ref.size() > P->signal_num) {
         delete P;
@@ -1298,6 +1323,7 @@ mrs_status mrs_plan_get_info(const mrs_plan *P, mrs_plan_info *info) {
     info->last_run_ms = P->last_ms;
     info->fwd_kernel_ms = P->fwd_ms;
     info->reduce_kernel_ms = P->red_ms;
+    info->deferred_free_steps = P->defer ? 1 : 0;
     return MRS_OK;
 }
 
@@ -1537,7 +1563,8 @@ static mrs_status scan_with_callbacks(mrs_vobj *vobj, const mrs_vmag *vmag, cons
         if (!all.muts.empty()) P->dec.muts = all.muts;
         const DecodeStop stop = dec.run(P->dec);
         if (stop == DECODE_END) dec.flush_tail(P->dec);
-        compress_runs(P->dec, P->model == 0 ? P->max_adc : 0);
+            compress_runs(P->dec, P->model == 0 ? P->max_adc : 0);
+            choose_deferral(P);
         fold_exact(P, P->dec, (int64_t)P->gx * P->ngroups * P->block, &P->n_loops);
         if (adc_base + (int64_t)P->dec.adc_ref.size() > P->signal_num)
             return fail(MRS_ERR_ARG, "VSig.SignalNum is smaller than the number of ADC samples");

@@ -4,7 +4,8 @@
 // A thread owns SPT isochromats of one voxel and type. bloch_step() applies
 // one step record (BlochKernel.h:7-397): rf rotation or free precession,
 // then T2/T1 relaxation, with the cached free-precession rotation reused for
-// REC_SAME steps and repeated for the folded run length of the record.
+// REC_SAME steps and repeated for the folded run length of the record, or
+// (DEFER) with free precession gathered as a pending rotation angle.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,15 +44,83 @@ struct SpinState {
     float e1 = 1.f, e2 = 1.f, m0e = 0.f;  // reference relaxation factors
     float d1 = 0.f, d2 = 0.f;             // accurate relaxation factors
     bool have_rot = false, have_e = false;
+    // Deferred free precession (DEFER kernels): the summed rotation angle of
+    // the free steps not yet applied to Mx, My, per spin, and the product of
+    // their transverse relaxation factors; apply_pending() applies both.
+    double ph[SPT] = {};
+    float gp = 1.f;
+    bool pend = false;
 
     __device__ __forceinline__ void spoil() {  // Plugin_IdealSpoiler.m:10-11
 #pragma unroll
         for (int s = 0; s < SPT; s++) {
             mx[s] = 0.f;
             my[s] = 0.f;
+            ph[s] = 0.0;
         }
+        gp = 1.f;
+        pend = false;
     }
 };
+
+// The gradient terms of the off-resonance, each rounded as BlochKernel.h:44-51
+// rounds them (gamma*G comes rounded in the record).
+struct GradTerms {
+    float z, y, x;
+};
+template <typename Q>
+__device__ __forceinline__ GradTerms grad_terms(const VoxelConst &k, Q q) {
+#pragma clang fp contract(off)
+    return {k.z * __uint_as_float(q[1]), k.y * __uint_as_float(q[2]), k.x * __uint_as_float(q[3])};
+}
+// Off-resonance of a spin (rad/s) in BlochKernel.h:38-51's order: the spin's
+// field-free term wb, then the z, y and x gradient terms one at a time, each
+// sum rounded. bSSFP makes the signal sensitive to the phase each spin
+// gathers per TR, so the angle of a step is kept bit-equal to the reference's.
+__device__ __forceinline__ float ref_dw(float wb, GradTerms g) {
+#pragma clang fp contract(off)
+    float w = wb + g.z;
+    w = w + g.y;
+    return w + g.x;
+}
+
+// Applies the deferred free precession: rotation of each spin about -z by
+// its summed angle (sin and cos of the fp64 sum within 1.5 ulp: the rotation
+// is applied once, so its rounding does not build up as a cached rotation's
+// would), then the summed transverse relaxation factor.
+template <int SPT>
+__device__ __forceinline__ void apply_pending(SpinState<SPT> &st) {
+    if (!st.pend) return;
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        float sn, cs;
+        sincos_fast_d(st.ph[s], &sn, &cs);
+        const float ax = st.mx[s], ay = st.my[s];
+        const float rx = ax * cs + ay * sn;
+        const float ry = ay * cs - ax * sn;
+        st.mx[s] = rx * st.gp;
+        st.my[s] = ry * st.gp;
+        st.ph[s] = 0.0;
+    }
+    st.gp = 1.f;
+    st.pend = false;
+}
+
+// The relaxation of one deferred free step: Mz as the reference rounds it
+// (PREC 0) or in the accurate form (PREC 1); the transverse factor goes
+// into the pending product.
+template <int SPT, int PREC>
+__device__ __forceinline__ void relax_step(SpinState<SPT> &st, float m0s) {
+    if constexpr (PREC == 0) {
+        st.gp = st.gp * st.e2;
+#pragma unroll
+        for (int s = 0; s < SPT; s++) st.mz[s] = relax_mz_ref(st.mz[s], st.e1, st.m0e);
+    } else {
+        st.gp = st.gp - st.gp * st.d2;
+#pragma unroll
+        for (int s = 0; s < SPT; s++) st.mz[s] = st.mz[s] + (m0s - st.mz[s]) * st.d1;
+    }
+}
 
 // One step of free precession + relaxation with a cached rotation
 // (BlochKernel.h:314-368); PREC selects the relaxation form. In reference
@@ -91,8 +160,15 @@ struct RelaxIn {
 // the caller). q is any word accessor: a scalar-path pointer into the record
 // stream, or a record already held in registers. GEN: any Tx coil count
 // (TxMaps), else one Tx coil (VoxelConst weight). PREC: 0 = the reference's
-// fp32 relaxation rounding, 1 = accurate.
-template <int SPT, bool GEN, int PREC, typename Q>
+// fp32 relaxation rounding, 1 = accurate. DEFER: a free step without an ADC
+// sample only adds its rotation angle (the reference's fp32 angle, times the
+// record's step count) to the spin's pending fp64 sum and relaxes Mz step
+// by step; the caller applies the pending rotation (apply_pending) before
+// anything reads or turns Mx, My (an ADC sample, a capture, the end of the
+// launch); rf steps apply it here. Free precession commutes with itself, so
+// this changes the rounding only: one rotation by the exact sum of the
+// reference's angles instead of one fp32 rotation per step.
+template <int SPT, bool GEN, int PREC, bool DEFER = false, typename Q>
 __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst &k, Q q, uint32_t flags,
                                            const TxMaps &txm, RelaxIn relax = RelaxIn{false, 0.f, 0.f}) {
     const float dt = __uint_as_float(q[0]);
@@ -123,17 +199,22 @@ __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst 
             st.d2 = (float)(-expm1(-(double)dt / (double)k.t2));
             st.d1 = (float)(-expm1(-(double)dt / (double)k.t1));
         }
+        // (deferred kernels rotate per step only at ADC samples, where the
+        // cached path serves single-use rotations too)
+        if constexpr (!DEFER) {
 #pragma unroll
-        for (int s = 1; s < SPT; s++) {
-            sincos_cr((st.wb[s] - st.wb[0]) * dt, &st.sd[s], &st.cd[s]);
-            if constexpr (PREC == 0) {
-                st.sd[s] *= st.e2;
-                st.cd[s] *= st.e2;
+            for (int s = 1; s < SPT; s++) {
+                sincos_cr((st.wb[s] - st.wb[0]) * dt, &st.sd[s], &st.cd[s]);
+                if constexpr (PREC == 0) {
+                    st.sd[s] *= st.e2;
+                    st.cd[s] *= st.e2;
+                }
             }
         }
     }
-    const float gv = k.z * __uint_as_float(q[1]) + k.y * __uint_as_float(q[2]) + k.x * __uint_as_float(q[3]);
+    const GradTerms gt = grad_terms(k, q);
     if (flags & REC_RF) {
+        if constexpr (DEFER) apply_pending(st);
         const float wrf = __uint_as_float(q[4]);
         float bx, by;
         if (!GEN || txm.tx == 1) {
@@ -163,7 +244,7 @@ __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst 
             // step within a few ulp of the correctly rounded one; below
             // |Omega| = 1e-15 rad/s the rotation, under 1e-15 rad per
             // second of dt, is skipped, which keeps rsq off subnormals)
-            const float w = (st.wb[s] + gv) + wrf;
+            const float w = ref_dw(st.wb[s], gt) + wrf;
             const float om2 = b2 + w * w;
             if (om2 > 1e-30f) {
                 const float inv = __builtin_amdgcn_rsqf(om2);
@@ -194,12 +275,27 @@ __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst 
             }
         }
     } else {
-        if (flags & REC_ONCE) {
+        if constexpr (DEFER) {
+            if (!(flags & REC_ADC)) {
+                const int n = 1 + (int)(flags >> REC_REP_SHIFT);
+#pragma unroll
+                for (int s = 0; s < SPT; s++) st.ph[s] += (double)(ref_dw(st.wb[s], gt) * dt) * (double)n;
+                st.pend = true;
+                st.have_rot = false;  // the next REC_SAME step computes its rotation
+                if (n & 1) relax_step<SPT, PREC>(st, k.m0s);
+                for (int r = n & 1; r < n; r += 2) {
+                    relax_step<SPT, PREC>(st, k.m0s);
+                    relax_step<SPT, PREC>(st, k.m0s);
+                }
+                return;
+            }
+        }
+        if (!DEFER && (flags & REC_ONCE)) {
             // rotation used by this step only: spin 0's angle, then angle
             // addition with the cached per-spin offsets (about 2 ulp, not
             // accumulated since no later step reuses it)
             float s0, c0;
-            sincos_fast((st.wb[0] + gv) * dt, &s0, &c0);
+            sincos_fast(ref_dw(st.wb[0], gt) * dt, &s0, &c0);
             if constexpr (PREC == 0) {
                 st.ss[0] = s0 * st.e2;
                 st.cc[0] = c0 * st.e2;
@@ -217,7 +313,7 @@ __device__ __forceinline__ void bloch_step(SpinState<SPT> &st, const VoxelConst 
             // a REC_SAME step has this step's dt, so e2 stays valid with the cache
 #pragma unroll
             for (int s = 0; s < SPT; s++) {
-                sincos_cr((st.wb[s] + gv) * dt, &st.ss[s], &st.cc[s]);
+                sincos_cr(ref_dw(st.wb[s], gt) * dt, &st.ss[s], &st.cc[s]);
                 if constexpr (PREC == 0) {
                     st.ss[s] *= st.e2;
                     st.cc[s] *= st.e2;

@@ -178,10 +178,11 @@ def _check(rc: int) -> None:
 
 
 def make_options(block_size=0, spins_per_thread=0, max_adc_per_launch=0, multipool_scheme=0,
-                 precision_mode=0, timing=0, model=0, coil_sum=0) -> abi.OptionsC:
+                 precision_mode=0, timing=0, model=0, coil_sum=0, free_steps=0) -> abi.OptionsC:
     return abi.OptionsC(block_size=block_size, spins_per_thread=spins_per_thread,
                         max_adc_per_launch=max_adc_per_launch, multipool_scheme=multipool_scheme,
-                        precision_mode=precision_mode, timing=timing, model=model, coil_sum=coil_sum)
+                        precision_mode=precision_mode, timing=timing, model=model, coil_sum=coil_sum,
+                        free_steps=free_steps)
 
 
 def make_callbacks(ext: Optional[Callable] = None, gpu_fetch: Optional[Callable] = None):

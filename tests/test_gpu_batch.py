@@ -40,14 +40,17 @@ def run_batch(scans, **opt):
     return outs
 
 
-def test_batch_matches_single_scans_and_oracle(gpu):
+@pytest.mark.parametrize("free_steps", [2, 1])
+def test_batch_matches_single_scans_and_oracle(gpu, free_steps):
+    """Both free-step modes (2: deferred rotation, 1: every step rotates);
+    each batched scan against the single-scan entry point in the same mode."""
     scans = mixed_batch()
     for s in scans:
-        print(f"case {s.name}: iso={s.n_iso} steps={s.n_steps()}", flush=True)
-    outs = run_batch(scans)
+        print(f"case {s.name}: iso={s.n_iso} steps={s.n_steps()} free_steps {free_steps}", flush=True)
+    outs = run_batch(scans, free_steps=free_steps)
     for s, b in zip(scans, outs):
         single = s.copy()
-        engine.DoScanAtGPU(*single.structs)
+        engine.DoScanAtGPU(*single.structs, options=engine.make_options(free_steps=free_steps))
         ref = oracle.scan(s, "f32")
         e1, e2 = rel_l2(sig(b), sig(single)), rel_l2(mag(b), mag(single))
         e3, e4 = rel_l2(sig(b), sig(ref)), rel_l2(mag(b), mag(ref))
@@ -85,6 +88,9 @@ def test_batch_graph_replays_match_eager(gpu):
         info = b.info()
         print(f"batch info: {info}", flush=True)
         assert info["n_scans"] == 20 and info["tuned"] == 1 and info["n_candidates"] >= 2
+        # fingerprint TRs hold a few free steps per ADC sample and rf pulse:
+        # by default they rotate every step
+        assert info["deferred_free_steps"] == 0
         for rep in range(3):
             b.run()
             b.sync()

@@ -93,18 +93,53 @@ def test_deterministic_and_plan_rerun(gpu):
     np.testing.assert_array_equal(mag(r), mag(a))
 
 
+@pytest.mark.parametrize("free_steps", [2, 1])
 @pytest.mark.parametrize("shape,spins,coils,ntr", [((16, 16, 8), 4, 8, 6), ((12, 10, 4), 2, 3, 5),
                                                    ((8, 8, 2), 8, 1, 4), ((8, 8, 2), 1, 16, 3)])
-def test_bssfp3d_multicoil(gpu, shape, spins, coils, ntr):
+def test_bssfp3d_multicoil(gpu, shape, spins, coils, ntr, free_steps):
     sc = workloads.bssfp3d(shape=shape, n_ro=16, spins=spins, n_coils=coils, n_tr=ntr, tr_steps=120,
                            fov=(0.064, 0.064, 0.032))
-    print(f"case {sc.name}", flush=True)
+    print(f"case {sc.name} free_steps {free_steps}", flush=True)
     o32 = oracle.scan(sc, "f32")
-    g = run_gpu(sc)
+    g = run_gpu(sc, free_steps=free_steps)
     e_sig, e_mag = rel_l2(sig(g), sig(o32)), rel_l2(mag(g), mag(o32))
     print(f"  signal {e_sig:.3e} magnetization {e_mag:.3e}", flush=True)
     assert e_sig < 1e-4 and e_mag < 1e-4
     assert_decoder_equal(g, o32)
+
+
+def test_long_bssfp_free_step_modes(gpu):
+    """128 TRs of the 3-D bSSFP sequence at its full field of view on a
+    16 x 16 x 8 phantom (4 isochromats, 8 coils), both free-step modes
+    against the fp32 oracle. bSSFP turns every isochromat's phase per TR into
+    signal, so fp32 rounding of the step angles builds up over the TRs: both
+    modes compute each step's angle in the reference's order and rounding,
+    and the deferred mode (free_steps 2) applies the fp64 sum of a free
+    stretch's angles as one rotation; it has to stay as close to the oracle
+    as stepping every rotation does. The default (0) defers here."""
+    sc = workloads.bssfp3d(shape=(16, 16, 8), n_pe_y=16, n_pe_z=8, n_tr=128)
+    print(f"case {sc.name}, 128 TRs", flush=True)
+    o32 = oracle.scan(sc, "f32")
+    d = run_gpu(sc, free_steps=2)
+    s = run_gpu(sc, free_steps=1)
+    a = sc.copy()
+    with engine.Plan(*a.structs) as p:
+        assert p.info()["deferred_free_steps"] == 1
+        p.run()
+        p.fetch(VObj=a.VObj, VVar=a.VVar, VSig=a.VSig)
+    np.testing.assert_array_equal(sig(a), sig(d))
+    e = [rel_l2(sig(d), sig(o32)), rel_l2(mag(d), mag(o32)), rel_l2(sig(s), sig(o32)), rel_l2(mag(s), mag(o32)),
+         rel_l2(sig(d), sig(s))]
+    print("  deferred: signal %.3e magnetization %.3e; stepwise: %.3e %.3e; between %.3e" % tuple(e), flush=True)
+    assert max(e) < 1e-4
+    assert e[0] < 1.5 * e[2] + 1e-6
+    assert_decoder_equal(d, o32)
+
+
+def test_free_steps_option_checked(gpu):
+    sc = workloads.gre2d(n_pe=2)
+    with pytest.raises(engine.MRSError, match="free_steps"):
+        run_gpu(sc, free_steps=3)
 
 
 @pytest.mark.parametrize("coils,spins", [(8, 4), (3, 2), (5, 1)])
