@@ -182,6 +182,10 @@ def _scalar(struct, sname, field, dtype):
     if field not in struct:
         raise KeyError(f"{sname}.{field} is missing")
     v = struct[field]
+    if isinstance(v, np.generic):  # the common case, without an array round trip
+        if v.dtype != dtype:
+            raise TypeError(f"{sname}.{field} has dtype {v.dtype}, expected {np.dtype(dtype).name}")
+        return v.item()
     a = np.asarray(v)
     if isinstance(v, np.ndarray) or isinstance(v, np.generic):
         if a.dtype != np.dtype(dtype):
