@@ -87,6 +87,27 @@ __device__ __forceinline__ void sincos_fast_d(double x, float *sp, float *cp) {
     sincos_poly((float)r, (int)(((long long)k) & 3), sp, cp);
 }
 
+// The gradient terms of the off-resonance, position times gamma*G, each
+// rounded as BlochKernel.h:44-51 rounds them (gamma*G comes rounded in the
+// record).
+struct GradTerms {
+    float z, y, x;
+};
+__device__ __forceinline__ GradTerms grad_terms_xyz(float z, float y, float x, float gz, float gy, float gx) {
+#pragma clang fp contract(off)
+    return {z * gz, y * gy, x * gx};
+}
+// Off-resonance of a spin (rad/s) in BlochKernel.h:38-51's order: the spin's
+// field-free term wb, then the z, y and x gradient terms one at a time, each
+// sum rounded. bSSFP makes the signal sensitive to the phase each spin
+// gathers per TR, so the angle of a step is kept bit-equal to the reference's.
+__device__ __forceinline__ float ref_dw(float wb, GradTerms g) {
+#pragma clang fp contract(off)
+    float w = wb + g.z;
+    w = w + g.y;
+    return w + g.x;
+}
+
 // Lane exchange for one butterfly stage across lane bit O. For O = 32 and
 // 16, gfx950's v_permlane{32,16}_swap exchange whole rows; for O <= 8, DPP
 // row_mirror / row_half_mirror / quad_perm pair each lane with one whose bit

@@ -297,11 +297,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
 
     // the step of record q: dt, gradient phase rate, rf frequency, effective
     // B1 (gamma-scaled) and the bound pools' saturation rates
-    auto rec_fields = [&](const crec_t q, uint32_t flags, float &dt, bool &rf, float &gv, float &wrf, float &bx,
-                          float &by, float (&wsat)[TY]) __attribute__((always_inline)) {
+    // (gv: the summed gradient term for the fp64 schemes; gt: the three
+    // terms the split scheme adds in the reference's order)
+    auto rec_fields = [&](const crec_t q, uint32_t flags, float &dt, bool &rf, float &gv, GradTerms &gt, float &wrf,
+                          float &bx, float &by, float (&wsat)[TY]) __attribute__((always_inline)) {
         dt = __uint_as_float(q[0]);
         rf = (flags & REC_RF) != 0;
         gv = z * __uint_as_float(q[1]) + y * __uint_as_float(q[2]) + x * __uint_as_float(q[3]);
+        gt = grad_terms_xyz(z, y, x, __uint_as_float(q[1]), __uint_as_float(q[2]), __uint_as_float(q[3]));
         wrf = __uint_as_float(q[4]);
         // effective B1 (gamma-scaled) and bound-pool saturation rates
         bx = 0.f;
@@ -355,9 +358,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
     auto exact_record = [&](const crec_t q, uint32_t flags, double *sv) __attribute__((always_inline)) {
         if constexpr (SCHEME == 1) {
             float dt, gv, wrf, bx, by;
+            GradTerms gt;
             bool rf;
             float wsat[TY];
-            rec_fields(q, flags, dt, rf, gv, wrf, bx, by, wsat);
+            rec_fields(q, flags, dt, rf, gv, gt, wrf, bx, by, wsat);
             const int nrep = 1 + (int)(flags >> REC_REP_SHIFT);
             // exact step: x <- exp(A*dt) x with the affine M0 term; per step
             // the rotation of each free pool (zero for others) and the bound
@@ -646,16 +650,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
         }
         if (flags & REC_NOSTEP) continue;
         float dt, gv, wrf, bx, by;
+        GradTerms gt;
         bool rf;
         float wsat[TY];
-        rec_fields(q, flags, dt, rf, gv, wrf, bx, by, wsat);
+        rec_fields(q, flags, dt, rf, gv, gt, wrf, bx, by, wsat);
         const int nrep = 1 + (int)(flags >> REC_REP_SHIFT);
         if constexpr (SCHEME == 0) {
             if (rf) {
                 have_frot = false;
             } else if (!((flags & REC_SAME) && have_frot)) {
 #pragma unroll
-                for (int t = 0; t < TY; t++) sincos_cr((wb[t] + gv) * dt, &fsn[t], &fcs[t]);
+                for (int t = 0; t < TY; t++) sincos_cr(ref_dw(wb[t], gt) * dt, &fsn[t], &fcs[t]);
                 have_frot = true;
             }
         }
@@ -663,7 +668,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
         // (pool values passed in, so the arrays stay in registers)
         auto rotate_free = [&](float wbt, float fst, float fct, float &ax, float &ay,
                                float &az) __attribute__((always_inline)) {
-            const float w = rf ? (wbt + gv) + wrf : wbt + gv;
+            const float w = rf ? ref_dw(wbt, gt) + wrf : ref_dw(wbt, gt);
             if (rf) {
                 // applied once: the hardware reciprocal square root and the
                 // 1.5 ulp fp32 sincos (as the single-pool rf step); below
@@ -684,7 +689,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gm_waves_mi
                     az = rz;
                 }
             } else {
-                // the cached sincos of (wb[t] + gv) * dt
+                // the cached sincos of the step's angle
                 const float rx = ax * fct + ay * fst;
                 const float ry = ay * fct - ax * fst;
                 ax = rx;

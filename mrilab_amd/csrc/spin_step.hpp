@@ -63,27 +63,11 @@ struct SpinState {
     }
 };
 
-// The gradient terms of the off-resonance, each rounded as BlochKernel.h:44-51
-// rounds them (gamma*G comes rounded in the record).
-struct GradTerms {
-    float z, y, x;
-};
+// The gradient terms of record q for this voxel.
 template <typename Q>
 __device__ __forceinline__ GradTerms grad_terms(const VoxelConst &k, Q q) {
-#pragma clang fp contract(off)
-    return {k.z * __uint_as_float(q[1]), k.y * __uint_as_float(q[2]), k.x * __uint_as_float(q[3])};
+    return grad_terms_xyz(k.z, k.y, k.x, __uint_as_float(q[1]), __uint_as_float(q[2]), __uint_as_float(q[3]));
 }
-// Off-resonance of a spin (rad/s) in BlochKernel.h:38-51's order: the spin's
-// field-free term wb, then the z, y and x gradient terms one at a time, each
-// sum rounded. bSSFP makes the signal sensitive to the phase each spin
-// gathers per TR, so the angle of a step is kept bit-equal to the reference's.
-__device__ __forceinline__ float ref_dw(float wb, GradTerms g) {
-#pragma clang fp contract(off)
-    float w = wb + g.z;
-    w = w + g.y;
-    return w + g.x;
-}
-
 // Applies the deferred free precession: rotation of each spin about -z by
 // its summed angle (sin and cos of the fp64 sum within 1.5 ulp: the rotation
 // is applied once, so its rounding does not build up as a cached rotation's
