@@ -48,6 +48,8 @@ def test_batch_matches_single_scans_and_oracle(gpu, free_steps):
     for s in scans:
         print(f"case {s.name}: iso={s.n_iso} steps={s.n_steps()} free_steps {free_steps}", flush=True)
     outs = run_batch(scans, free_steps=free_steps)
+    with engine.Batch([s.copy().structs for s in scans], options=engine.make_options(free_steps=free_steps)) as bb:
+        assert bb.info()["deferred_free_steps"] == (1 if free_steps == 2 else 0)
     for s, b in zip(scans, outs):
         single = s.copy()
         engine.DoScanAtGPU(*single.structs, options=engine.make_options(free_steps=free_steps))
