@@ -1,7 +1,7 @@
 // Host check of fold_periods (mrilab_amd/csrc/decode.cpp): a record stream
 // with loops unfolds to the stream it came from, and only blocks that may
-// fold (no ADC, spoiler, capture; enough repeats) do. Prints "ok" or the
-// first failure and exits non-zero.
+// fold (no ADC, spoiler, capture; enough repeats) do; and of deferral_counts
+// on a mixed stream. Prints "ok" or the first failure and exits non-zero.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -153,6 +153,27 @@ int main() {
                 }
             r += w[6];
         }
+    }
+    // deferral_counts: deferred free steps (their step counts) and the pending
+    // rotations they cause (at an ADC sample, rf step or capture, and at the
+    // end); ADC and rf steps are not deferred, a spoiler drops what is pending
+    {
+        Decoded d;
+        d.stride = kStride;
+        push(d, 1, 4u << REC_REP_SHIFT);  // 5 free steps
+        push(d, 2, REC_ADC);              // rotation 1; a sample, not deferred
+        push(d, 3, 0);                    // 1 free step
+        push(d, 4, REC_RF);               // rotation 2
+        push(d, 5, REC_RF);               // nothing pending
+        push(d, 6, 2u << REC_REP_SHIFT);  // 3 free steps
+        push(d, 7, REC_NOSTEP | REC_SPOIL); // spoiler event: the pending rotation is dropped
+        push(d, 8, REC_NOSTEP | REC_ADC);   // a sample with nothing pending
+        push(d, 9, 9u << REC_REP_SHIFT);    // 10 free steps, rotation 3 at the end
+        int64_t steps = 0, rot = 0;
+        deferral_counts(d, &steps, &rot);
+        check(steps == 5 + 1 + 3 + 10 && rot == 3, "deferral counts of a mixed stream");
+        check(deferral_pays(20, 2) && !deferral_pays(15, 2) && !deferral_pays(0, 0),
+              "deferral pays from 8 steps per rotation");
     }
     if (fails) return 1;
     std::printf("ok\n");

@@ -3,7 +3,9 @@ mrilab_amd/csrc/decode.cpp), checked on the host: a C++ driver compiled
 with g++ against decode.cpp folds synthetic record streams (a saturation
 train with a symmetric pulse, rectangular rf, blocks holding ADC samples,
 spoilers or captures, random streams) and checks that every stream unfolds
-to its input and that only foldable blocks fold."""
+to its input and that only foldable blocks fold. The same driver checks
+deferral_counts, which decides whether free precession is deferred, on a
+stream with ADC samples, rf steps and a spoiler."""
 import os
 import shutil
 import subprocess
