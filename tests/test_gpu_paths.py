@@ -70,12 +70,14 @@ MTX = [
 ]
 
 
+@pytest.mark.parametrize("free_steps", [0, 2])
 @pytest.mark.parametrize("name,make", MTX, ids=[m[0] for m in MTX])
-def test_multi_transmit_scan(gpu, name, make):
+def test_multi_transmit_scan(gpu, name, make, free_steps):
+    """free_steps 2 forces the deferred kernel with any Tx coil count."""
     sc = make()
-    print(f"case {name}: {sc.name}, Tx {int(sc.VCoi['TxCoilNum'])}", flush=True)
+    print(f"case {name}: {sc.name}, Tx {int(sc.VCoi['TxCoilNum'])}, free_steps {free_steps}", flush=True)
     o32 = oracle.scan(sc, "f32")
-    g = run_gpu(sc)
+    g = run_gpu(sc, free_steps=free_steps)
     check(g, o32, name)
     if name.startswith("bssfp"):
         # Ext 7 took the phase of coil 2 (rf phase alternates 0/pi, coil 2 adds 0.6)
@@ -101,13 +103,17 @@ def test_two_species_chemical_shift(gpu):
     ("gre_ty2", lambda: workloads.two_species(
         workloads.gre2d(n_pe=3, n_ro=8, spins=1, tr_steps=300, te_steps=150, rf_steps=60))),
 ])
-def test_run_mode1_capture(gpu, name, make):
+@pytest.mark.parametrize("free_steps", [0, 2])
+def test_run_mode1_capture(gpu, name, make, free_steps):
     """RunMode 1: the state after every step lands in VSig.Mx/My/Mz[..., utsi]
-    and the elapsed time in VSig.Muts, as the oracle writes them."""
+    and the elapsed time in VSig.Muts, as the oracle writes them. With
+    free_steps 2 the deferred kernel applies the pending rotation before
+    each capture."""
     sc = workloads.with_run_mode1(make())
-    print(f"case {name} RunMode 1: {sc.name}, n_uts {sc.VSeq['utsLine'].size}", flush=True)
+    print(f"case {name} RunMode 1: {sc.name}, n_uts {sc.VSeq['utsLine'].size}, free_steps {free_steps}",
+          flush=True)
     o32 = oracle.scan(sc, "f32")
-    g = run_gpu(sc)
+    g = run_gpu(sc, free_steps=free_steps)
     check(g, o32, name)
     cap = lambda s: np.concatenate([s.VSig[k].ravel() for k in ("Mx", "My", "Mz")])  # noqa: E731
     e = rel_l2(cap(g), cap(o32))
