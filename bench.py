@@ -718,7 +718,8 @@ def main() -> int:
         "config": {"workload": "3-D bSSFP 128x128x64 voxels x 4 isochromats, 8 Rx coils, 45 deg hard rf, "
                                "TR 5 ms = 500 x 10 us steps, 128x64 phase encodes x 128 readout",
                    "isochromats": info["n_iso"], "steps": info["n_steps"], "adc_samples": info["n_adc"],
-                   "launches_per_step": n_launch, "precision_mode": "reference"},
+                   "launches_per_step": n_launch, "precision_mode": "reference",
+                   "free_steps": "deferred" if info["deferred_free_steps"] else "stepwise"},
         "roofline": roofline(load_profile(), avg_launch_s, ab, iso_steps / n_launch, flop),
         "kernel_ms_per_step": {"bloch_fwd": fwd_ms / args.steps, "signal_reduce": red_ms / args.steps},
     }
