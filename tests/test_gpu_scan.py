@@ -108,17 +108,18 @@ def test_bssfp3d_multicoil(gpu, shape, spins, coils, ntr, free_steps):
     assert_decoder_equal(g, o32)
 
 
-def test_long_bssfp_free_step_modes(gpu):
-    """128 TRs of the 3-D bSSFP sequence at its full field of view on a
-    16 x 16 x 8 phantom (4 isochromats, 8 coils), both free-step modes
+@pytest.mark.parametrize("pe,ntr", [((16, 8), 128), ((64, 32), 512)])
+def test_long_bssfp_free_step_modes(gpu, pe, ntr):
+    """128 and 512 TRs of the 3-D bSSFP sequence at its full field of view on
+    a 16 x 16 x 8 phantom (4 isochromats, 8 coils), both free-step modes
     against the fp32 oracle. bSSFP turns every isochromat's phase per TR into
     signal, so fp32 rounding of the step angles builds up over the TRs: both
     modes compute each step's angle in the reference's order and rounding,
     and the deferred mode (free_steps 2) applies the fp64 sum of a free
     stretch's angles as one rotation; it has to stay as close to the oracle
     as stepping every rotation does. The default (0) defers here."""
-    sc = workloads.bssfp3d(shape=(16, 16, 8), n_pe_y=16, n_pe_z=8, n_tr=128)
-    print(f"case {sc.name}, 128 TRs", flush=True)
+    sc = workloads.bssfp3d(shape=(16, 16, 8), n_pe_y=pe[0], n_pe_z=pe[1], n_tr=ntr)
+    print(f"case {sc.name}, {ntr} TRs", flush=True)
     o32 = oracle.scan(sc, "f32")
     d = run_gpu(sc, free_steps=2)
     s = run_gpu(sc, free_steps=1)
